@@ -1,0 +1,177 @@
+/*
+ * rahman_hip.h -- C ABI of the MI355X-native Rahman molecular-dynamics engine.
+ *
+ * The boundary mirrors the reference's Rust engine API (kshitijl/micrographia):
+ *   - rh_system3 / rh_system2 are POD images of md_core::three::System3 and md_core::System
+ *     (crates/md-core/src/three.rs:13-31, crates/md-core/src/system.rs:26-61);
+ *   - a state is passed as three flat arrays, positions i64[D*n], velocities i32[D*n] and
+ *     species u32[n], in particle-id order: md_core::three::State3 (three.rs:267-273);
+ *   - rh_engine3_* replaces md::GpuEngine3 (crates/md/src/engine3.rs:307-648) and rh_engine2_*
+ *     md::GpuEngine's periodic / hard-walled subset (crates/md/src/engine.rs:567-1098).
+ * Every call returns RH_OK or a negative status; rh_last_error() describes the last failure on
+ * the calling thread.  Nothing unwinds or aborts across the boundary.  An engine handle belongs
+ * to one host thread; it owns one HIP stream; step() returns once the steps are done on the
+ * device, reads synchronise.
+ */
+#ifndef RAHMAN_HIP_H
+#define RAHMAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RH_OK 0
+#define RH_ERR_INVALID (-1)   /* a contract violation (the reference's assert!) */
+#define RH_ERR_HIP (-2)       /* a HIP runtime failure */
+#define RH_ERR_NOMEM (-3)     /* allocation failed or over budget */
+#define RH_ERR_UNSUPPORTED (-4)
+
+#define RH_MAX_SPECIES 4
+
+/* md_core::Pair (crates/md-core/src/pair.rs:3-8) */
+typedef struct rh_pair {
+    double eps, sigma, rc;
+} rh_pair;
+
+/* md_core::Wall (crates/md-core/src/system.rs:12-24) */
+enum { RH_WALL_HARD = 0, RH_WALL_SOFT = 1, RH_WALL_THERMAL = 2 };
+
+/* md_core::three::System3 (crates/md-core/src/three.rs:13-31).  pairs are indexed [a*4 + b]
+ * (symmetric); walled[a] = 0 for a periodic axis; bath fields are the Andersen thermostat. */
+typedef struct rh_system3 {
+    int64_t box_len[3];
+    double dt;
+    uint32_t nspecies;
+    uint32_t pinned;
+    double masses[4];
+    rh_pair pairs[16];
+    uint32_t walled[3];
+    uint32_t wall_kind[3][2];
+    double wall_t[3][2];
+    double gravity[3];
+    uint64_t seed;
+    uint32_t has_bath;
+    uint32_t bath_species;
+    double bath_t, bath_rate;
+} rh_system3;
+
+/* md_core::System, periodic / hard-walled subset (crates/md-core/src/system.rs:26-61) */
+typedef struct rh_system2 {
+    int64_t box_len[2];
+    double dt;
+    uint32_t nspecies;
+    uint32_t pinned;
+    double masses[4];
+    rh_pair pairs[16];
+    uint32_t walled[2];
+    double gravity[2];
+    uint64_t seed;
+} rh_system2;
+
+/* md::ForceMode (crates/md/src/engine.rs:104-110): kind 0 = CellList, 1 = Verlet{skin, cap}.
+ * cap 0 asks for the engine's own capacity rule. */
+typedef struct rh_mode {
+    uint32_t kind;
+    uint32_t cap;
+    double skin;
+} rh_mode;
+
+/* md::engine3::Ledger3 (crates/md/src/engine3.rs:199-205) */
+typedef struct rh_ledger3 {
+    double energy[7], impulse[7];
+    uint64_t hits[7];
+} rh_ledger3;
+
+/* Counters of the list machinery (the reference's read_list_stats, engine3.rs:272-279, plus
+ * this engine's own). */
+typedef struct rh_list_stats {
+    uint32_t rebuilds;
+    uint32_t overflowed;  /* a list ever overflowed (it was then regrown and rebuilt) */
+    uint32_t cap;         /* current capacity per particle */
+    uint32_t max_count;   /* largest list at the last build */
+    uint64_t steps_done;
+    uint64_t launches_skipped; /* gated step launches that found the list stale */
+} rh_list_stats;
+
+const char* rh_last_error(void);
+const char* rh_version(void);
+int rh_device_count(int* count);
+
+/* ---- md-core initial conditions (crates/md-core/src/three.rs:311-404, init.rs:15-121) ---- */
+int rh_cube_for(uint64_t n, double density, int64_t out[3]);
+int rh_fcc_cells(uint64_t n, const int64_t box_len[3], uint64_t out[3]);
+int rh_fcc(uint64_t n, const int64_t box_len[3], const uint64_t cells[3], double jitter, uint64_t seed, int64_t* pos);
+int rh_maxwell3(const rh_system3* sys, uint64_t n, const uint32_t* species, double t, uint64_t seed, int32_t* vel);
+int rh_set_temperature3(const rh_system3* sys, uint64_t n, const uint32_t* species, int32_t* vel, double t);
+uint64_t rh_hash3(uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species);
+int rh_box_for(uint64_t n, double density, double aspect, int64_t out[2]);
+int rh_lattice(uint64_t n, const int64_t box_len[2], double jitter, uint64_t seed, int64_t* pos);
+int rh_maxwell2(const rh_system2* sys, uint64_t n, const uint32_t* species, double t, uint64_t seed, int32_t* vel);
+int rh_fluid2(const rh_system2* sys, uint64_t n, double t, uint64_t seed, int64_t* pos, int32_t* vel, uint32_t* species);
+uint64_t rh_hash2(uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species);
+/* ForceMode::verlet_with_skin (engine3.rs:61-68 / engine.rs:129-141) without the 128 MiB
+ * WebGPU binding clamp. */
+int rh_verlet_mode3(const rh_system3* sys, uint64_t n, double skin, rh_mode* out);
+int rh_verlet_mode2(const rh_system2* sys, uint64_t n, double skin, rh_mode* out);
+
+/* ---- the 3D engine: md::GpuEngine3 ---- */
+typedef struct rh_engine3 rh_engine3;
+
+/* GpuEngine3::with_mode (engine3.rs:342-457); mode NULL = GpuEngine3::new (Verlet, skin 0.4). */
+int rh_engine3_create(const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode* mode, rh_engine3** out);
+int rh_engine3_destroy(rh_engine3* e);
+int rh_engine3_n(const rh_engine3* e, uint64_t* n);
+int rh_engine3_steps(const rh_engine3* e, int64_t* steps);
+int rh_engine3_step(rh_engine3* e, uint64_t steps);                     /* engine3.rs:491-522 */
+int rh_engine3_reverse(rh_engine3* e);                                  /* engine3.rs:524-531 */
+int rh_engine3_update_system(rh_engine3* e, const rh_system3* sys);     /* engine3.rs:533-559 */
+int rh_engine3_kick_region(rh_engine3* e, const double lo[3], const double hi[3], double scale, const double add[3]); /* :561-568 */
+int rh_engine3_set_species(rh_engine3* e, const uint32_t* ids, uint64_t count, uint32_t species);             /* :570-581 */
+int rh_engine3_set_species_velocity(rh_engine3* e, uint32_t species, const double v[3]);                      /* :583-589 */
+int rh_engine3_upload(rh_engine3* e, const int64_t* pos, const int32_t* vel, const uint32_t* species);          /* :591-602 */
+int rh_engine3_read_state(rh_engine3* e, int64_t* pos, int32_t* vel, uint32_t* species);                        /* :604-606 */
+int rh_engine3_kicks(rh_engine3* e, int32_t* kicks, double* potential);                                         /* :608-616 */
+int rh_engine3_read_kinetic_total(rh_engine3* e, double* ke);                                                   /* :633-637 */
+int rh_engine3_read_potential(rh_engine3* e, double* pe);                                                       /* :640-643 */
+int rh_engine3_read_ledger(rh_engine3* e, rh_ledger3* out);                                                     /* :645-647 */
+int rh_engine3_list_stats(rh_engine3* e, rh_list_stats* out);                                                   /* :272-279 */
+int rh_engine3_set_sort_interval(rh_engine3* e, uint32_t interval);                                             /* sort_interval, :327-329 */
+/* The engine's HIP stream (hipStream_t), for callers that time or order work around it. */
+int rh_engine3_stream(rh_engine3* e, void** stream);
+/* Mean duration (ms) of the force+integrate launches of the last step() call, measured with HIP
+ * events on the engine's stream (0 if not measured); enable with rh_engine3_set_timing(e, 1). */
+int rh_engine3_set_timing(rh_engine3* e, int on);
+int rh_engine3_kernel_ms(rh_engine3* e, double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds);
+
+/* ---- the 2D engine: md::GpuEngine (periodic / hard walls, pair forces in f32) ---- */
+typedef struct rh_engine2 rh_engine2;
+int rh_engine2_create(const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode* mode, rh_engine2** out);
+int rh_engine2_destroy(rh_engine2* e);
+int rh_engine2_step(rh_engine2* e, uint64_t steps);
+int rh_engine2_reverse(rh_engine2* e);
+int rh_engine2_upload(rh_engine2* e, const int64_t* pos, const int32_t* vel, const uint32_t* species);
+int rh_engine2_read_state(rh_engine2* e, int64_t* pos, int32_t* vel, uint32_t* species);
+int rh_engine2_kicks(rh_engine2* e, int32_t* kicks, double* potential);
+int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke);
+int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out);
+int rh_engine2_stream(rh_engine2* e, void** stream);
+
+/* ---- gpu-prims on device buffers (crates/gpu-prims/src/prims.rs:116-312, threefry.rs) ----
+ * Pointers are device memory; stream may be NULL (the null stream).  These run synchronously
+ * with respect to the host (they synchronise the stream before returning). */
+int rh_prims_exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
+int rh_prims_radix_sort_u32(uint32_t* d_keys, uint32_t* d_vals, uint64_t n, uint32_t key_bits, void* stream);
+int rh_prims_reduce_sum_f64(const double* d_in, double* d_out, uint64_t n, void* stream);
+int rh_prims_threefry(uint32_t* d_out, uint64_t n, uint32_t ctr0, uint32_t ctr1, uint32_t key0, uint32_t key1, void* stream);
+int rh_prims_atomic_add_i64(const int64_t* d_vals, uint64_t n, int64_t* d_acc, void* stream);
+/* The engine's float -> integer kick conversion on device: out[i] = i32(rint(x[i])), saturating,
+ * NaN -> 0 (Rust `as i32` after round_ties_even). */
+int rh_prims_kick_convert(const float* d_x, int32_t* d_out, uint64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,44 @@
+"""micrographia_amd: an MI355X-native Rahman molecular-dynamics engine.
+
+The hot path (cell binning, Verlet lists, Lennard-Jones pair kicks and the integer leapfrog) is
+hand-written HIP for gfx950 in ``csrc/``, exposed through the C ABI of ``include/rahman_hip.h``
+(``librahman_hip.so``).  This package is the Python host side of that ABI, mirroring the
+reference's md-core / md engine API.
+"""
+from ._lib import RahmanError, load
+from .core import (
+    ONE,
+    UNIT,
+    Bath,
+    ForceMode,
+    Pair,
+    State,
+    State3,
+    System,
+    System3,
+    Wall,
+    box_for,
+    cube_for,
+    fcc,
+    fcc_cells,
+    fluid,
+    lj_liquid,
+    maxwell,
+    maxwell2,
+    set_temperature,
+    thermal_wall,
+    to_f64,
+    to_fixed,
+    verlet3,
+    verlet3_with_skin,
+    verlet_for,
+    verlet_with_skin,
+)
+from .engine import GpuEngine, GpuEngine3, ListStats
+
+__all__ = [
+    "ONE", "UNIT", "Bath", "ForceMode", "Pair", "State", "State3", "System", "System3", "Wall",
+    "box_for", "cube_for", "fcc", "fcc_cells", "fluid", "lj_liquid", "maxwell", "maxwell2",
+    "set_temperature", "thermal_wall", "to_f64", "to_fixed", "verlet3", "verlet3_with_skin",
+    "verlet_for", "verlet_with_skin", "GpuEngine", "GpuEngine3", "ListStats", "RahmanError", "load",
+]

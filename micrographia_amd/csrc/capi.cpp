@@ -1,0 +1,324 @@
+// The C ABI (include/rahman_hip.h): status codes, a thread-local last error, and thin wrappers
+// over rh::Engine<3> / rh::Engine<2> and the gpu-prims launchers.
+#include <exception>
+#include <new>
+#include <string>
+
+#include "engine.h"
+
+namespace rh {
+void kick_convert(const float* x, int32_t* out, uint64_t n, hipStream_t s);
+}
+
+struct rh_engine3 {
+    rh::Engine<3>* e;
+};
+struct rh_engine2 {
+    rh::Engine<2>* e;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return RH_OK;
+    } catch (const rh::Error& err) {
+        g_last_error = err.what();
+        std::string m = err.what();
+        if (m.find("HIP error") != std::string::npos) return RH_ERR_HIP;
+        if (m.find("allocation") != std::string::npos) return RH_ERR_NOMEM;
+        if (m.find("not supported") != std::string::npos) return RH_ERR_UNSUPPORTED;
+        return RH_ERR_INVALID;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return RH_ERR_NOMEM;
+    } catch (const std::exception& err) {
+        g_last_error = err.what();
+        return RH_ERR_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return RH_ERR_INVALID;
+    }
+}
+
+rh::SystemDesc desc3(const rh_system3* s) {
+    RH_REQUIRE(s != nullptr, "null system");
+    RH_REQUIRE(!s->has_bath, "the Andersen heat bath is not supported by this engine yet");
+    rh::SystemDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.dim = 3;
+    for (int a = 0; a < 3; a++) {
+        RH_REQUIRE(s->box_len[a] > 0, "box lengths must be positive");
+        d.box_len[a] = s->box_len[a];
+        d.walled[a] = s->walled[a];
+        d.wall_kind[a][0] = s->wall_kind[a][0];
+        d.wall_kind[a][1] = s->wall_kind[a][1];
+        d.gravity[a] = s->gravity[a];
+    }
+    d.dt = s->dt;
+    d.nspecies = s->nspecies;
+    d.pinned = s->pinned;
+    for (int k = 0; k < 4; k++) d.masses[k] = k < (int)s->nspecies ? s->masses[k] : 1.0;
+    for (int k = 0; k < 16; k++) d.pairs[k] = s->pairs[k];
+    return d;
+}
+
+rh::SystemDesc desc2(const rh_system2* s) {
+    RH_REQUIRE(s != nullptr, "null system");
+    rh::SystemDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.dim = 2;
+    for (int a = 0; a < 2; a++) {
+        RH_REQUIRE(s->box_len[a] > 0, "box lengths must be positive");
+        d.box_len[a] = s->box_len[a];
+        d.walled[a] = s->walled[a];
+        d.wall_kind[a][0] = RH_WALL_HARD;
+        d.wall_kind[a][1] = RH_WALL_HARD;
+        d.gravity[a] = s->gravity[a];
+    }
+    d.box_len[2] = 1;
+    d.dt = s->dt;
+    d.nspecies = s->nspecies;
+    d.pinned = s->pinned;
+    for (int k = 0; k < 4; k++) d.masses[k] = k < (int)s->nspecies ? s->masses[k] : 1.0;
+    for (int k = 0; k < 16; k++) d.pairs[k] = s->pairs[k];
+    return d;
+}
+
+rh_mode default_mode3(const rh_system3* s, uint64_t n) {
+    rh_mode m;
+    rh_verlet_mode3(s, n, 0.4, &m);
+    return m;
+}
+
+rh_mode default_mode2(const rh_system2* s, uint64_t n) {
+    rh_mode m;
+    rh_verlet_mode2(s, n, 0.4, &m);
+    return m;
+}
+
+template <class H>
+auto* eng(H* h) {
+    RH_REQUIRE(h != nullptr && h->e != nullptr, "null engine");
+    return h->e;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rh_last_error(void) { return g_last_error.c_str(); }
+const char* rh_version(void) { return "rahman-hip 0.1 (gfx950)"; }
+
+int rh_device_count(int* count) {
+    return guarded([&] { RH_HIP(hipGetDeviceCount(count)); });
+}
+
+int rh_engine3_create(const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode* mode, rh_engine3** out) {
+    return guarded([&] {
+        RH_REQUIRE(out && pos && vel && species, "null argument");
+        *out = nullptr;
+        rh_mode m = mode ? *mode : default_mode3(sys, n);
+        auto* h = new rh_engine3{nullptr};
+        try {
+            h->e = new rh::Engine<3>(desc3(sys), n, pos, vel, species, m);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int rh_engine3_destroy(rh_engine3* e) {
+    return guarded([&] {
+        if (!e) return;
+        delete e->e;
+        delete e;
+    });
+}
+
+int rh_engine3_n(const rh_engine3* e, uint64_t* n) {
+    return guarded([&] { *n = eng(e)->n(); });
+}
+int rh_engine3_steps(const rh_engine3* e, int64_t* steps) {
+    return guarded([&] { *steps = eng(e)->steps(); });
+}
+int rh_engine3_step(rh_engine3* e, uint64_t steps) {
+    return guarded([&] { eng(e)->step(steps); });
+}
+int rh_engine3_reverse(rh_engine3* e) {
+    return guarded([&] { eng(e)->reverse(); });
+}
+int rh_engine3_update_system(rh_engine3* e, const rh_system3* sys) {
+    return guarded([&] { eng(e)->update_system(desc3(sys)); });
+}
+int rh_engine3_kick_region(rh_engine3* e, const double lo[3], const double hi[3], double scale, const double add[3]) {
+    return guarded([&] { eng(e)->kick_region(lo, hi, scale, add); });
+}
+int rh_engine3_set_species(rh_engine3* e, const uint32_t* ids, uint64_t count, uint32_t species) {
+    return guarded([&] { eng(e)->set_species(ids, count, species); });
+}
+int rh_engine3_set_species_velocity(rh_engine3* e, uint32_t species, const double v[3]) {
+    return guarded([&] { eng(e)->set_species_velocity(species, v); });
+}
+int rh_engine3_upload(rh_engine3* e, const int64_t* pos, const int32_t* vel, const uint32_t* species) {
+    return guarded([&] { eng(e)->upload(pos, vel, species); });
+}
+int rh_engine3_read_state(rh_engine3* e, int64_t* pos, int32_t* vel, uint32_t* species) {
+    return guarded([&] { eng(e)->read_state(pos, vel, species); });
+}
+int rh_engine3_kicks(rh_engine3* e, int32_t* kicks, double* potential) {
+    return guarded([&] { eng(e)->kicks(kicks, potential); });
+}
+int rh_engine3_read_kinetic_total(rh_engine3* e, double* ke) {
+    return guarded([&] { *ke = eng(e)->kinetic_total(); });
+}
+int rh_engine3_read_potential(rh_engine3* e, double* pe) {
+    return guarded([&] { *pe = eng(e)->potential(); });
+}
+int rh_engine3_read_ledger(rh_engine3* e, rh_ledger3* out) {
+    return guarded([&] {
+        eng(e);
+        std::memset(out, 0, sizeof *out);
+    });
+}
+int rh_engine3_list_stats(rh_engine3* e, rh_list_stats* out) {
+    return guarded([&] { *out = eng(e)->list_stats(); });
+}
+int rh_engine3_set_sort_interval(rh_engine3* e, uint32_t interval) {
+    return guarded([&] { eng(e)->set_sort_interval(interval); });
+}
+int rh_engine3_stream(rh_engine3* e, void** stream) {
+    return guarded([&] { *stream = (void*)eng(e)->stream(); });
+}
+int rh_engine3_set_timing(rh_engine3* e, int on) {
+    return guarded([&] { eng(e)->set_timing(on != 0); });
+}
+int rh_engine3_kernel_ms(rh_engine3* e, double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds) {
+    return guarded([&] { eng(e)->kernel_ms(force_ms, force_launches, rebuild_ms, rebuilds); });
+}
+
+int rh_engine2_create(const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode* mode, rh_engine2** out) {
+    return guarded([&] {
+        RH_REQUIRE(out && pos && vel && species, "null argument");
+        *out = nullptr;
+        rh_mode m = mode ? *mode : default_mode2(sys, n);
+        auto* h = new rh_engine2{nullptr};
+        try {
+            h->e = new rh::Engine<2>(desc2(sys), n, pos, vel, species, m);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+int rh_engine2_destroy(rh_engine2* e) {
+    return guarded([&] {
+        if (!e) return;
+        delete e->e;
+        delete e;
+    });
+}
+int rh_engine2_step(rh_engine2* e, uint64_t steps) {
+    return guarded([&] { eng(e)->step(steps); });
+}
+int rh_engine2_reverse(rh_engine2* e) {
+    return guarded([&] { eng(e)->reverse(); });
+}
+int rh_engine2_upload(rh_engine2* e, const int64_t* pos, const int32_t* vel, const uint32_t* species) {
+    return guarded([&] { eng(e)->upload(pos, vel, species); });
+}
+int rh_engine2_read_state(rh_engine2* e, int64_t* pos, int32_t* vel, uint32_t* species) {
+    return guarded([&] { eng(e)->read_state(pos, vel, species); });
+}
+int rh_engine2_kicks(rh_engine2* e, int32_t* kicks, double* potential) {
+    return guarded([&] { eng(e)->kicks(kicks, potential); });
+}
+int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke) {
+    return guarded([&] { *ke = eng(e)->kinetic_total(); });
+}
+int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out) {
+    return guarded([&] { *out = eng(e)->list_stats(); });
+}
+int rh_engine2_stream(rh_engine2* e, void** stream) {
+    return guarded([&] { *stream = (void*)eng(e)->stream(); });
+}
+
+// ---- gpu-prims ----
+
+int rh_prims_exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t)stream;
+        void* temp = nullptr;
+        RH_HIP(hipMalloc(&temp, rh::scan_temp_bytes(n)));
+        try {
+            rh::exclusive_scan_u32(d_in, d_out, n, temp, nullptr, s);
+            RH_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipFree(temp);
+            throw;
+        }
+        RH_HIP(hipFree(temp));
+    });
+}
+
+int rh_prims_radix_sort_u32(uint32_t* d_keys, uint32_t* d_vals, uint64_t n, uint32_t key_bits, void* stream) {
+    return guarded([&] {
+        RH_REQUIRE(key_bits >= 1 && key_bits <= 32, "key_bits must be 1..32");
+        if (n == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        rh::RadixSort rs;
+        rs.reserve(n);
+        RH_HIP(hipMemcpyAsync(rs.keys[0], d_keys, n * 4, hipMemcpyDeviceToDevice, s));
+        RH_HIP(hipMemcpyAsync(rs.vals[0], d_vals, n * 4, hipMemcpyDeviceToDevice, s));
+        int r = rs.sort(n, key_bits, nullptr, s);
+        RH_HIP(hipMemcpyAsync(d_keys, rs.keys[r], n * 4, hipMemcpyDeviceToDevice, s));
+        RH_HIP(hipMemcpyAsync(d_vals, rs.vals[r], n * 4, hipMemcpyDeviceToDevice, s));
+        RH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int rh_prims_reduce_sum_f64(const double* d_in, double* d_out, uint64_t n, void* stream) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t)stream;
+        void* temp = nullptr;
+        RH_HIP(hipMalloc(&temp, rh::reduce_temp_bytes(n)));
+        try {
+            rh::reduce_sum_f64(d_in, d_out, n, temp, s);
+            RH_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipFree(temp);
+            throw;
+        }
+        RH_HIP(hipFree(temp));
+    });
+}
+
+int rh_prims_threefry(uint32_t* d_out, uint64_t n, uint32_t ctr0, uint32_t ctr1, uint32_t key0, uint32_t key1, void* stream) {
+    return guarded([&] {
+        rh::threefry_fill(d_out, n, ctr0, ctr1, key0, key1, (hipStream_t)stream);
+        RH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+int rh_prims_atomic_add_i64(const int64_t* d_vals, uint64_t n, int64_t* d_acc, void* stream) {
+    return guarded([&] {
+        rh::atomic_add_i64(d_vals, n, d_acc, (hipStream_t)stream);
+        RH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+int rh_prims_kick_convert(const float* d_x, int32_t* d_out, uint64_t n, void* stream) {
+    return guarded([&] {
+        rh::kick_convert(d_x, d_out, n, (hipStream_t)stream);
+        RH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+}  // extern "C"

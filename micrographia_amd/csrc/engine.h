@@ -1,0 +1,166 @@
+// The Rahman engine on one GPU: integer positions (i64 per axis, 2^-32 sigma units), integer
+// velocities (i32 displacement per step), f32 Lennard-Jones pair forces rounded per pair to
+// integer kicks, summed with wrapping i32 adds (equal to the low 32 bits of an i64 sum, so the
+// total does not depend on summation order).
+#pragma once
+
+#include <vector>
+
+#include "core.h"
+#include "prims.h"
+
+namespace rh {
+
+// Control words shared by the device-gated launches (all u32).
+enum : uint32_t {
+    CTL_STALE = 0,       // the Verlet list must be rebuilt before the next step
+    CTL_STALE_NEXT = 1,  // set by the running step when some particle moved half the skin
+    CTL_OVERFLOW = 2,    // a list build overflowed its capacity (steps stop until regrown)
+    CTL_CLOCK = 3,       // steps taken in either direction (stochastic boundaries)
+    CTL_REBUILDS = 4,
+    CTL_REMAINING = 5,   // steps still to take in the current step() call
+    CTL_PPOS = 6,        // which position buffer is current (they alternate every step)
+    CTL_PMETA = 7,       // which velocity/species/id buffer is current (alternate per rebuild)
+    CTL_DO_REBUILD = 8,  // the gate of the rebuild chain launched at the head of a chunk
+    CTL_MAXCOUNT = 9,    // largest list at the last build
+    CTL_DONE_LO = 10,
+    CTL_DONE_HI = 11,
+    CTL_SKIPPED = 12,    // gated step launches that found the list stale
+    CTL_WORDS = 32
+};
+
+// f32 pair constants per species pair [a*4 + b] and kick factors per species, kept in device
+// memory so per-lane species indexing needs no private copy.
+struct PairTable {
+    float rc2[16], eps4[16], six_eps4[16], sigma2[16], shift[16], list_r2[16];
+    float kick_c[4];
+};
+
+template <int D>
+struct Params {
+    uint32_t n;
+    uint32_t nc[3];
+    uint32_t mbits;      // bits per axis of the Morton cell key
+    uint32_t periodic;   // bit a: axis a periodic
+    uint32_t nspecies;
+    uint32_t pinned;
+    uint32_t soft_walls; // bit 2a+side: a soft wall
+    uint32_t cap;
+    int64_t box[3];
+    int64_t half[3];
+    double cell_scale[3];  // cells per position unit
+    float kick_c[4];
+    int32_t gravity[3];
+    float half_skin2;
+    PairTable pt;
+};
+
+template <int D>
+struct Bufs {
+    int64_t* pos[2][D];  // [parity][axis]
+    int32_t* vel[2][D];
+    uint32_t* species[2];
+    uint32_t* ids[2];
+    int64_t* ref[D];     // positions at the last list build
+    uint32_t* nbr;       // column-major: entry k of slot i at k*n + i
+    uint32_t* nbr_count;
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    int32_t* kicks[D];   // kicks of the last kicks() / cell-list step (slot order)
+    double* pe;          // per-slot potential energy (halved pair sums + walls)
+    double* ke;          // per-slot kinetic energy
+    uint32_t* ctl;
+    const PairTable* pt;
+};
+
+struct SystemDesc {
+    int dim;
+    int64_t box_len[3];
+    double dt;
+    uint32_t nspecies, pinned;
+    double masses[4];
+    rh_pair pairs[16];
+    uint32_t walled[3];
+    uint32_t wall_kind[3][2];
+    double gravity[3];
+};
+
+template <int D>
+class Engine {
+   public:
+    Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode& mode);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    uint64_t n() const { return n_; }
+    int64_t steps() const { return steps_; }
+    void step(uint64_t steps);
+    void reverse();
+    void update_system(const SystemDesc& sys);
+    void upload(const int64_t* pos, const int32_t* vel, const uint32_t* species);
+    void read_state(int64_t* pos, int32_t* vel, uint32_t* species);
+    void kicks(int32_t* kicks, double* potential);
+    double kinetic_total();
+    double potential();
+    void kick_region(const double lo[3], const double hi[3], double scale, const double add[3]);
+    void set_species(const uint32_t* ids, uint64_t count, uint32_t species);
+    void set_species_velocity(uint32_t species, const double v[3]);
+    rh_list_stats list_stats();
+    hipStream_t stream() const { return stream_; }
+    void set_timing(bool on) { timing_ = on; }
+    void kernel_ms(double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds) const;
+    void set_sort_interval(uint32_t v) { sort_interval_ = v; }
+
+   private:
+    void build_params(const SystemDesc& sys, const rh_mode& mode);
+    void alloc_list(uint32_t cap);
+    void enqueue_rebuild(bool gated);
+    void enqueue_cells_at_current(const uint32_t* gate);  // keys + sort + ranges of current slots
+    void enqueue_force_cells(const uint32_t* gate);       // kicks/pe of current slots
+    void step_list(uint64_t steps);
+    void step_cells(uint64_t steps);
+    void read_ctl(uint32_t* out);
+    void write_ctl(uint32_t word, uint32_t value);
+    void handle_overflow();
+    void upload_tables();
+    void record_timer(bool rebuild, hipEvent_t start);
+    hipEvent_t timer_event();
+    void collect_timers();
+
+    SystemDesc sys_;
+    rh_mode mode_;
+    Params<D> P_;
+    Bufs<D> B_;
+    uint64_t n_ = 0;
+    int64_t steps_ = 0;
+    uint32_t key_bits_ = 0;
+    uint64_t ncell_keys_ = 0;
+    RadixSort sort_;
+    hipStream_t stream_ = nullptr;
+    uint32_t* h_ctl_ = nullptr;  // pinned readback slots
+    std::vector<hipEvent_t> ev_;
+    void* reduce_temp_ = nullptr;
+    double* d_scalar_ = nullptr;
+    uint32_t* d_set_list_ = nullptr;
+    uint64_t set_list_cap_ = 0;
+    double lifetime_ = 8.0;  // running estimate of steps per list
+    bool timing_ = false;
+    // Pairs of events around timed launches, resolved after the stream drains.
+    std::vector<hipEvent_t> ev_pool_;
+    size_t ev_used_ = 0;
+    struct Timed {
+        hipEvent_t a, b;
+        bool rebuild;
+    };
+    std::vector<Timed> timed_;
+    uint32_t since_sort_ = 100;
+    PairTable* d_pt_ = nullptr;
+    double force_ms_ = 0.0, rebuild_ms_ = 0.0;
+    uint64_t force_launches_ = 0, rebuild_launches_ = 0;
+    uint32_t sort_interval_ = 100;
+    std::vector<void*> allocs_;
+    void* dalloc(size_t bytes);
+};
+
+}  // namespace rh

@@ -1,0 +1,1139 @@
+// Device kernels and the host driver of the Rahman engine (see engine.h).
+//
+// One step, Verlet-list path (crates/md/shaders/md3.wgsl force_list + integrate_list, restated):
+//   k_step_list   one lane per particle, Morton-sorted slots: gather the listed neighbours'
+//                 i64 positions, f32 pair math in md-ref's operation order, per-pair rint and a
+//                 saturating convert to an i32 kick, wrapping i32 sum; then the leapfrog
+//                 (v += kick + g; x += v; wrap / mirror) and the half-skin check, fused in one
+//                 launch.  Positions alternate between two buffers (read one, write the other).
+//   k_step_commit one lane: advance the control words (parity, remaining, stale flag).
+// A chunk of K steps is preceded by a rebuild chain gated on the device's own stale flag:
+//   keys (Morton cell) -> stable radix sort -> gather (reorder storage) -> cell ranges ->
+//   k_build_list -> commit.  Steps after the list went stale are skipped on the device and
+//   re-issued by the host, so every step runs with a valid list and the trajectory is the same
+//   bit for bit whatever the chunking.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+
+namespace rh {
+
+constexpr int TPB = 256;
+constexpr float SOFT_RC_F = 1.122462048309373f;
+
+// ---------------------------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    uint32_t x = v & 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {
+    uint32_t x = v & 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+template <int D>
+__device__ __forceinline__ uint32_t morton(const uint32_t c[D]) {
+    if constexpr (D == 3) return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+    else return spread2(c[0]) | (spread2(c[1]) << 1);
+}
+
+template <int D>
+__device__ __forceinline__ void cell_of(const Params<D>& P, const int64_t x[D], uint32_t c[D]) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double f = (double)x[a] * P.cell_scale[a];
+        uint32_t ci = f > 0.0 ? (uint32_t)f : 0u;
+        c[a] = ci < P.nc[a] - 1 ? ci : P.nc[a] - 1;
+    }
+}
+
+// Minimum image (periodic axes) of b - a, exactly fixed::min_image (fixed.rs:31-40).
+// Branch-free: the two corrections exclude each other, as the reference's if / else-if do.
+template <int D>
+__device__ __forceinline__ int64_t separation(const Params<D>& P, int64_t a, int64_t b, int ax) {
+    int64_t d = b - a;
+    if ((P.periodic >> ax) & 1u) {
+        int64_t down = d >= P.half[ax] ? -P.box[ax] : int64_t(0);
+        int64_t up = d < -P.half[ax] ? P.box[ax] : int64_t(0);
+        d += down + up;
+    }
+    return d;
+}
+
+// fixed::sep_f32 (fixed.rs:55-61): |d| >> 4 as f32 times 2^-28, with the sign of d; false if
+// |d| >= 8 sigma.  Truncating d/16 toward zero and converting the signed value rounds
+// symmetrically, so this equals the reference bit for bit (a tiny negative d gives +0 instead
+// of -0, which no kick or energy can tell apart).
+__device__ __forceinline__ bool sep_f(int64_t d, float& f) {
+    bool ok = (uint64_t)(d + (int64_t(8) << 32)) < (uint64_t(16) << 32);
+    int64_t q = (d + ((d >> 63) & 15)) >> 4;
+    f = (float)(int32_t)q * 0x1p-28f;
+    return ok;
+}
+
+// One pair's kick on particle i, in PairConsts::eval3's operation order (real.rs:145-156);
+// the factor 6*eps4 is the same f32 product md-ref forms, tabulated per species pair.
+template <int D>
+__device__ __forceinline__ void pair_kick(const float f[D], float rc2, float sigma2, float six_eps4, float c, uint32_t k[D]) {
+    float r2 = f[0] * f[0] + f[1] * f[1];
+    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+    if (r2 < rc2) {
+        float s2 = sigma2 / r2;
+        float s6 = s2 * s2 * s2;
+        // 2*s6 is exact, so one fused op rounds (2*s6 - 1) exactly once, as the separate ops do.
+        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float a = -fr * c;
+#pragma unroll
+        for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
+    }
+}
+
+// The same pair with its energy u = eps4*s6*(s6-1) - shift.
+template <int D>
+__device__ __forceinline__ void pair_kick_pe(const float f[D], const PairTable* __restrict__ pt, int p, float c, uint32_t k[D], double& pe) {
+    float r2 = f[0] * f[0] + f[1] * f[1];
+    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+    if (r2 < pt->rc2[p]) {
+        float s2 = pt->sigma2[p] / r2;
+        float s6 = s2 * s2 * s2;
+        float fr = pt->six_eps4[p] * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float a = -fr * c;
+#pragma unroll
+        for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
+        float u = pt->eps4[p] * s6 * (s6 - 1.0f) - pt->shift[p];
+        pe += (double)u;
+    }
+}
+
+// Soft (WCA) walls on the centre's distance from each wall (int_engine3.rs:106-126).
+template <int D>
+__device__ __forceinline__ void soft_walls(const Params<D>& P, const int64_t x[D], float c, uint32_t k[D], double* wall_pe) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        for (int side = 0; side < 2; side++) {
+            if (!((P.soft_walls >> (2 * a + side)) & 1u)) continue;
+            int64_t d = side == 0 ? x[a] : P.box[a] - x[a];
+            float df;
+            if (!sep_f(d, df)) continue;
+            if (df < SOFT_RC_F) {
+                float s2 = 1.0f / (df * df);
+                float s6 = s2 * s2 * s2;
+                float fw = 24.0f * s6 * (2.0f * s6 - 1.0f) / df;
+                int32_t kw = cvt_sat_i32(__builtin_rintf(fw * c));
+                k[a] += side == 0 ? (uint32_t)kw : (uint32_t)0 - (uint32_t)kw;
+                if (wall_pe) *wall_pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
+            }
+        }
+    }
+}
+
+// Wrap (periodic) or mirror about -1/2 and L-1/2 (walls), walls::mirror (walls.rs:27-35);
+// a mirrored particle's velocity on that axis is reversed (hard and soft walls).
+template <int D>
+__device__ __forceinline__ void bound(const Params<D>& P, int64_t x[D], int32_t v[D]) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int64_t l = P.box[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+        } else if (x[a] < 0) {
+            x[a] = -1 - x[a];
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        } else if (x[a] >= l) {
+            x[a] = 2 * l - 1 - x[a];
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void stencil_cell(const Params<D>& P, const uint32_t c[D], int o, uint32_t out[D]) {
+    int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int v = (int)c[a] + off[a];
+        if (v < 0) v += (int)P.nc[a];
+        else if (v >= (int)P.nc[a]) v -= (int)P.nc[a];
+        out[a] = (uint32_t)v;
+    }
+}
+
+constexpr int stencil_size(int d) { return d == 3 ? 27 : 9; }
+
+// ---------------------------------------------------------------------------------------------
+// Control kernels (one lane)
+// ---------------------------------------------------------------------------------------------
+
+__global__ void k_add_remaining(uint32_t* ctl, uint32_t add) { ctl[CTL_REMAINING] += add; }
+
+__global__ void k_set_word(uint32_t* ctl, uint32_t word, uint32_t value) { ctl[word] = value; }
+
+__global__ void k_rebuild_decide(uint32_t* ctl) {
+    uint32_t go = (ctl[CTL_STALE] != 0u && ctl[CTL_REMAINING] != 0u && ctl[CTL_OVERFLOW] == 0u) ? 1u : 0u;
+    ctl[CTL_DO_REBUILD] = go;
+    if (go) ctl[CTL_MAXCOUNT] = 0u;
+}
+
+// Unconditional reorder (cell-list mode's periodic Morton sort).
+__global__ void k_rebuild_force(uint32_t* ctl) {
+    ctl[CTL_DO_REBUILD] = 1u;
+    ctl[CTL_MAXCOUNT] = 0u;
+}
+
+__global__ void k_rebuild_commit(uint32_t* ctl) {
+    if (ctl[CTL_DO_REBUILD]) {
+        ctl[CTL_STALE] = 0u;
+        ctl[CTL_STALE_NEXT] = 0u;
+        ctl[CTL_PPOS] ^= 1u;
+        ctl[CTL_PMETA] ^= 1u;
+        ctl[CTL_REBUILDS] += 1u;
+        ctl[CTL_DO_REBUILD] = 0u;
+    }
+}
+
+__device__ __forceinline__ bool step_runs(const uint32_t* ctl) {
+    return (ctl[CTL_STALE] | ctl[CTL_OVERFLOW]) == 0u && ctl[CTL_REMAINING] != 0u;
+}
+
+__global__ void k_step_commit(uint32_t* ctl) {
+    if (step_runs(ctl)) {
+        ctl[CTL_PPOS] ^= 1u;
+        ctl[CTL_REMAINING] -= 1u;
+        uint32_t lo = ctl[CTL_DONE_LO] + 1u;
+        ctl[CTL_DONE_LO] = lo;
+        if (lo == 0u) ctl[CTL_DONE_HI] += 1u;
+        ctl[CTL_CLOCK] += 1u;
+        ctl[CTL_STALE] = ctl[CTL_STALE_NEXT];
+        ctl[CTL_STALE_NEXT] = 0u;
+    } else if (ctl[CTL_REMAINING] != 0u) {
+        ctl[CTL_SKIPPED] += 1u;
+    }
+}
+
+// Cell-list steps are host-counted: advance clock and done by `steps`.
+__global__ void k_cells_commit(uint32_t* ctl, uint32_t steps) {
+    uint32_t lo = ctl[CTL_DONE_LO] + steps;
+    if (lo < ctl[CTL_DONE_LO]) ctl[CTL_DONE_HI] += 1u;
+    ctl[CTL_DONE_LO] = lo;
+    ctl[CTL_CLOCK] += steps;
+}
+
+__global__ void k_flip_pp_pm(uint32_t* ctl) {
+    ctl[CTL_PPOS] ^= 1u;
+    ctl[CTL_PMETA] ^= 1u;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Rebuild chain
+// ---------------------------------------------------------------------------------------------
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_keys(Params<D> P, Bufs<D> B, uint32_t* keys, uint32_t* vals, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t pp = B.ctl[CTL_PPOS];
+    int64_t x[D];
+    for (int a = 0; a < D; a++) x[a] = B.pos[pp][a][i];
+    uint32_t c[D];
+    cell_of<D>(P, x, c);
+    keys[i] = morton<D>(c);
+    vals[i] = i;
+}
+
+__global__ void k_ranges(const uint32_t* keys, uint32_t n, uint32_t* start, uint32_t* end, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= n) return;
+    uint32_t key = keys[k];
+    if (k == 0 || keys[k - 1] != key) start[key] = k;
+    if (k == n - 1 || keys[k + 1] != key) end[key] = k + 1;
+}
+
+// Reorder storage by the sorted permutation into the other buffers; positions also become the
+// list's reference positions.
+template <int D>
+__global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const uint32_t* perm, int write_ref, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= P.n) return;
+    uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
+    uint32_t i = perm[k];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int64_t x = B.pos[pp][a][i];
+        B.pos[pp ^ 1][a][k] = x;
+        if (write_ref) B.ref[a][k] = x;
+        B.vel[pm ^ 1][a][k] = B.vel[pm][a][i];
+    }
+    B.species[pm ^ 1][k] = B.species[pm][i];
+    B.ids[pm ^ 1][k] = B.ids[pm][i];
+}
+
+// Full Verlet list within each species pair's (rc + skin)(1 + 1e-4) (md3.wgsl build_list), over
+// the 3^D cells around the particle's, in the freshly gathered (not yet committed) buffers.
+template <int D, bool MULTI>
+__global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
+    const int64_t* px[D];
+    for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
+    const uint32_t* sp = B.species[pm];
+    int64_t xi[D];
+    for (int a = 0; a < D; a++) xi[a] = px[a][i];
+    uint32_t si = MULTI ? sp[i] : 0u;
+    const float list_r2 = B.pt->list_r2[0];
+    uint32_t c[D];
+    cell_of<D>(P, xi, c);
+    uint32_t count = 0;
+    const uint32_t cap = P.cap, n = P.n;
+    for (int o = 0; o < stencil_size(D); o++) {
+        uint32_t cc[D];
+        stencil_cell<D>(P, c, o, cc);
+        uint32_t key = morton<D>(cc);
+        uint32_t s = B.cell_start[key], e = B.cell_end[key];
+        for (uint32_t j = s; j < e; j++) {
+            if (j == i) continue;
+            float f[D];
+            bool ok = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], px[a][j], a), f[a]);
+            float r2 = f[0] * f[0] + f[1] * f[1];
+            if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+            float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
+            if (ok && r2 < lr2) {
+                if (count < cap) B.nbr[(size_t)count * n + i] = j;
+                count++;
+            }
+        }
+    }
+    if (count > cap) {
+        atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
+    }
+    atomicMax(&B.ctl[CTL_MAXCOUNT], count);
+    B.nbr_count[i] = count < cap ? count : cap;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The step: forces over the list, leapfrog, half-skin check (one launch)
+// ---------------------------------------------------------------------------------------------
+
+template <int D, bool MULTI>
+__global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
+    const uint32_t* ctl = B.ctl;
+    if (!step_runs(ctl)) return;
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
+    const int64_t* px[D];
+    for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
+    const uint32_t* sp = B.species[pm];
+    int64_t xi[D];
+    for (int a = 0; a < D; a++) xi[a] = px[a][i];
+    uint32_t si = MULTI ? sp[i] : 0u;
+    const PairTable* __restrict__ pt = B.pt;
+    float c = MULTI ? pt->kick_c[si] : P.kick_c[0];
+    const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0];
+    uint32_t k[D];
+    for (int a = 0; a < D; a++) k[a] = 0u;
+    const uint32_t m = B.nbr_count[i];
+    const uint32_t* nb = B.nbr + i;
+    const size_t stride = P.n;
+    uint32_t j_next = m ? nb[0] : 0u;
+    for (uint32_t e = 0; e < m; e++) {
+        const uint32_t j = j_next;
+        if (e + 1 < m) j_next = nb[(size_t)(e + 1) * stride];
+        int64_t xj[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) xj[a] = px[a][j];
+        float f[D];
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
+        if constexpr (MULTI) {
+            uint32_t p = si * 4 + sp[j];
+            if (ok) pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], c, k);
+        } else {
+            if (ok) pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, c, k);
+        }
+    }
+    if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
+    // leapfrog: v += kick + g, x += v
+    const bool pin = (P.pinned >> si) & 1u;
+    int32_t v[D];
+    int64_t x[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + k[a] + (pin ? 0u : (uint32_t)P.gravity[a]));
+        x[a] = xi[a] + (int64_t)v[a];
+    }
+    bound<D>(P, x, v);
+    float r2 = 0.0f;
+    bool far = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        B.pos[pp ^ 1][a][i] = x[a];
+        B.vel[pm][a][i] = v[a];
+        float d;
+        far |= !sep_f(separation<D>(P, B.ref[a][i], x[a], a), d);
+        r2 += d * d;
+    }
+    bool stale = far || r2 >= P.half_skin2;
+    uint64_t bal = ballot64(stale);
+    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cell-list forces (ForceMode::CellList, and kicks()/reverse()/energy readouts)
+// ---------------------------------------------------------------------------------------------
+
+// items: slot of each sorted position (null: slots are already sorted, items[q] = q).
+template <int D, bool MULTI>
+__global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, const uint32_t* items, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
+    const int64_t* px[D];
+    for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
+    const uint32_t* sp = B.species[pm];
+    int64_t xi[D];
+    for (int a = 0; a < D; a++) xi[a] = px[a][i];
+    uint32_t si = MULTI ? sp[i] : 0u;
+    float c = B.pt->kick_c[si];
+    uint32_t k[D];
+    for (int a = 0; a < D; a++) k[a] = 0u;
+    double pe = 0.0, wall_pe = 0.0;
+    uint32_t cell[D];
+    cell_of<D>(P, xi, cell);
+    for (int o = 0; o < stencil_size(D); o++) {
+        uint32_t cc[D];
+        stencil_cell<D>(P, cell, o, cc);
+        uint32_t key = morton<D>(cc);
+        uint32_t s = B.cell_start[key], e = B.cell_end[key];
+        for (uint32_t q = s; q < e; q++) {
+            uint32_t j = items ? items[q] : q;
+            if (j == i) continue;
+            float f[D];
+            bool ok = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], px[a][j], a), f[a]);
+            uint32_t p = MULTI ? si * 4 + sp[j] : 0u;
+            if (ok) pair_kick_pe<D>(f, B.pt, (int)p, c, k, pe);
+        }
+    }
+    if (P.soft_walls) soft_walls<D>(P, xi, c, k, &wall_pe);
+    for (int a = 0; a < D; a++) B.kicks[a][i] = (int32_t)k[a];
+    B.pe[i] = 0.5 * pe + wall_pe;
+}
+
+// Leapfrog in place from the stored kicks (cell-list mode).
+template <int D>
+__global__ __launch_bounds__(TPB) void k_integrate(Params<D> P, Bufs<D> B) {
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
+    uint32_t si = B.species[pm][i];
+    const bool pin = (P.pinned >> si) & 1u;
+    int32_t v[D];
+    int64_t x[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + (uint32_t)B.kicks[a][i] + (pin ? 0u : (uint32_t)P.gravity[a]));
+        x[a] = B.pos[pp][a][i] + (int64_t)v[a];
+    }
+    bound<D>(P, x, v);
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        B.pos[pp][a][i] = x[a];
+        B.vel[pm][a][i] = v[a];
+    }
+}
+
+// v <- -(v + kick + g) (int_engine3.rs:209-219)
+template <int D>
+__global__ __launch_bounds__(TPB) void k_reverse(Params<D> P, Bufs<D> B) {
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pm = B.ctl[CTL_PMETA];
+    uint32_t si = B.species[pm][i];
+    const bool pin = (P.pinned >> si) & 1u;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        uint32_t v = (uint32_t)B.vel[pm][a][i] + (uint32_t)B.kicks[a][i] + (pin ? 0u : (uint32_t)P.gravity[a]);
+        B.vel[pm][a][i] = (int32_t)((uint32_t)0 - v);
+    }
+}
+
+// Half-step kinetic energy per slot: 0.5 m sum (v UNIT / dt)^2, zero when pinned.
+template <int D>
+__global__ __launch_bounds__(TPB) void k_kinetic(Params<D> P, Bufs<D> B, double m0, double m1, double m2, double m3, double dt) {
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pm = B.ctl[CTL_PMETA];
+    uint32_t si = B.species[pm][i];
+    double m = si == 0 ? m0 : si == 1 ? m1 : si == 2 ? m2 : m3;
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double u = (double)B.vel[pm][a][i] * (1.0 / 4294967296.0) / dt;
+        s += u * u;
+    }
+    B.ke[i] = ((P.pinned >> si) & 1u) ? 0.0 : 0.5 * m * s;
+}
+
+// md3.wgsl kick_region (a GPU-only command; f32 coordinates and round-half-even as there).
+template <int D>
+__global__ __launch_bounds__(TPB) void k_kick_region(Params<D> P, Bufs<D> B, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2, float scale, int32_t a0, int32_t a1, int32_t a2) {
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
+    if ((P.pinned >> B.species[pm][i]) & 1u) return;
+    float lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    int32_t add[3] = {a0, a1, a2};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int64_t x = B.pos[pp][a][i];
+        float xf = (float)(uint32_t)((uint64_t)x >> 32) + __builtin_ldexpf((float)(uint32_t)x, -32);
+        if (xf < lo[a] || xf >= hi[a]) return;
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        float v = (float)B.vel[pm][a][i] * scale;
+        B.vel[pm][a][i] = (int32_t)((uint32_t)cvt_sat_i32(__builtin_rintf(v)) + (uint32_t)add[a]);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_set_species(Params<D> P, Bufs<D> B, const uint32_t* list, uint32_t count, uint32_t to) {
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n || count == 0) return;
+    const uint32_t pm = B.ctl[CTL_PMETA];
+    uint32_t id = B.ids[pm][i];
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (list[mid] < id) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < count && list[lo] == id) B.species[pm][i] = to;
+}
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_set_species_velocity(Params<D> P, Bufs<D> B, uint32_t s, int32_t a0, int32_t a1, int32_t a2) {
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pm = B.ctl[CTL_PMETA];
+    if (B.species[pm][i] != s) return;
+    int32_t add[3] = {a0, a1, a2};
+#pragma unroll
+    for (int a = 0; a < D; a++) B.vel[pm][a][i] = add[a];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+
+template <int D>
+void* Engine<D>::dalloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e != hipSuccess) throw Error(std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    allocs_.push_back(p);
+    return p;
+}
+
+template <int D>
+void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
+    RH_REQUIRE(sys.nspecies >= 1 && sys.nspecies <= 4, "between 1 and 4 species");
+    Params<D>& P = P_;
+    std::memset(&P, 0, sizeof P);
+    P.n = (uint32_t)n_;
+    P.nspecies = sys.nspecies;
+    P.pinned = sys.pinned;
+    double skin = mode.kind == 1 ? mode.skin : 0.0;
+    double rcm = core::rc_max(sys.pairs, sys.nspecies);
+    RH_REQUIRE(rcm < 7.9, "cutoff must be below 8 sigma");
+    // Cells at least (rc + skin) wide plus a rounding margin (three.rs:151-157).
+    double b[3] = {0, 0, 0}, bmax = 0.0;
+    for (int a = 0; a < D; a++) {
+        b[a] = core::to_f64(sys.box_len[a]);
+        bmax = std::max(bmax, b[a]);
+    }
+    double margin = 1e-3 + 1e-6 * bmax;
+    uint32_t ncmax = 1;
+    for (int a = 0; a < D; a++) {
+        P.nc[a] = core::sat_u32(std::floor(b[a] / (rcm + skin + margin)));
+        RH_REQUIRE(P.nc[a] >= 3, "box too small for the cutoff (fewer than 3 cells on an axis)");
+        ncmax = std::max(ncmax, P.nc[a]);
+        P.box[a] = sys.box_len[a];
+        P.half[a] = sys.box_len[a] / 2;
+        P.cell_scale[a] = (double)P.nc[a] / (double)sys.box_len[a];
+        if (!sys.walled[a]) P.periodic |= 1u << a;
+        else
+            for (int side = 0; side < 2; side++) {
+                RH_REQUIRE(sys.wall_kind[a][side] != RH_WALL_THERMAL, "heat-bath walls are not supported by this engine yet");
+                if (sys.wall_kind[a][side] == RH_WALL_SOFT) P.soft_walls |= 1u << (2 * a + side);
+            }
+        P.gravity[a] = core::gravity_kick(sys.dt, sys.gravity[a]);
+    }
+    P.mbits = 32 - __builtin_clz(std::max(ncmax - 1, 1u));
+    RH_REQUIRE((D == 3 && P.mbits <= 10) || (D == 2 && P.mbits <= 15), "too many cells for Morton keys");
+    key_bits_ = D * P.mbits;
+    ncell_keys_ = uint64_t(1) << key_bits_;
+    for (uint32_t s = 0; s < 4; s++) P.kick_c[s] = s < sys.nspecies ? (float)core::kick_coeff(sys.dt, sys.masses[s], (sys.pinned >> s) & 1u) : 0.0f;
+    double hs = 0.5 * skin * (1.0 - 1e-3);
+    P.half_skin2 = (float)(hs * hs);
+    for (uint32_t a = 0; a < 4; a++)
+        for (uint32_t bb = 0; bb < 4; bb++) {
+            int k = a * 4 + bb;
+            if (a >= sys.nspecies || bb >= sys.nspecies) continue;
+            const rh_pair& p = sys.pairs[k];
+            RH_REQUIRE(p.rc < 7.9, "cutoff must be below 8 sigma");
+            // Pair::gpu_constants (pair.rs:53-56): f64 constants rounded once to f32.
+            P.pt.rc2[k] = (float)(p.rc * p.rc);
+            P.pt.eps4[k] = (float)(4.0 * p.eps);
+            P.pt.sigma2[k] = (float)(p.sigma * p.sigma);
+            P.pt.shift[k] = (float)core::pair_shift(p);
+            P.pt.six_eps4[k] = 6.0f * P.pt.eps4[k];
+            double r = (p.rc + skin) * (1.0 + 1e-4);
+            P.pt.list_r2[k] = (float)(r * r);
+        }
+    for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
+    P.cap = mode.kind == 1 ? mode.cap : 0;
+}
+
+template <int D>
+void Engine<D>::upload_tables() {
+    RH_HIP(hipMemcpyAsync(d_pt_, &P_.pt, sizeof(PairTable), hipMemcpyHostToDevice, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+    B_.pt = d_pt_;
+}
+
+template <int D>
+hipEvent_t Engine<D>::timer_event() {
+    if (ev_used_ == ev_pool_.size()) {
+        hipEvent_t e;
+        RH_HIP(hipEventCreate(&e));
+        ev_pool_.push_back(e);
+    }
+    return ev_pool_[ev_used_++];
+}
+
+template <int D>
+void Engine<D>::record_timer(bool rebuild, hipEvent_t start) {
+    hipEvent_t end = timer_event();
+    RH_HIP(hipEventRecord(end, stream_));
+    timed_.push_back({start, end, rebuild});
+}
+
+template <int D>
+void Engine<D>::collect_timers() {
+    if (timed_.empty()) return;
+    RH_HIP(hipStreamSynchronize(stream_));
+    for (auto& t : timed_) {
+        float ms = 0.f;
+        RH_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+        if (t.rebuild) {
+            rebuild_ms_ += ms;
+            rebuild_launches_++;
+        } else {
+            force_ms_ += ms;
+            force_launches_++;
+        }
+    }
+    timed_.clear();
+    ev_used_ = 0;
+}
+
+template <int D>
+void Engine<D>::alloc_list(uint32_t cap) {
+    if (B_.nbr) {
+        auto it = std::find(allocs_.begin(), allocs_.end(), (void*)B_.nbr);
+        if (it != allocs_.end()) allocs_.erase(it);
+        RH_HIP(hipFree(B_.nbr));
+        B_.nbr = nullptr;
+    }
+    P_.cap = cap;
+    B_.nbr = (uint32_t*)dalloc((size_t)cap * n_ * sizeof(uint32_t));
+}
+
+template <int D>
+Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode& mode)
+    : sys_(sys), mode_(mode), n_(n) {
+    RH_REQUIRE(n >= 1, "an engine needs at least one particle");
+    RH_REQUIRE(n < (uint64_t(1) << 31), "too many particles for one engine");
+    RH_REQUIRE(mode.kind <= 1, "unknown force mode");
+    if (mode.kind == 1) RH_REQUIRE(mode.skin >= 0.0 && mode.cap >= 1, "Verlet mode needs skin >= 0 and cap >= 1");
+    for (uint64_t i = 0; i < n; i++) RH_REQUIRE(species[i] < sys.nspecies, "species out of range");
+    std::memset(&B_, 0, sizeof B_);
+    build_params(sys, mode);
+    RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (int p = 0; p < 2; p++) {
+        for (int a = 0; a < D; a++) {
+            B_.pos[p][a] = (int64_t*)dalloc(n * sizeof(int64_t));
+            B_.vel[p][a] = (int32_t*)dalloc(n * sizeof(int32_t));
+        }
+        B_.species[p] = (uint32_t*)dalloc(n * sizeof(uint32_t));
+        B_.ids[p] = (uint32_t*)dalloc(n * sizeof(uint32_t));
+    }
+    for (int a = 0; a < D; a++) {
+        B_.ref[a] = (int64_t*)dalloc(n * sizeof(int64_t));
+        B_.kicks[a] = (int32_t*)dalloc(n * sizeof(int32_t));
+    }
+    B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
+    B_.cell_start = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
+    B_.cell_end = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
+    B_.pe = (double*)dalloc(n * sizeof(double));
+    B_.ke = (double*)dalloc(n * sizeof(double));
+    B_.ctl = (uint32_t*)dalloc(CTL_WORDS * sizeof(uint32_t));
+    reduce_temp_ = dalloc(reduce_temp_bytes(n));
+    d_scalar_ = (double*)dalloc(64);
+    if (mode.kind == 1) alloc_list(mode.cap);
+    sort_.reserve(n);
+    constexpr int NSLOTS = 8;
+    RH_HIP(hipHostMalloc((void**)&h_ctl_, NSLOTS * CTL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+    ev_.resize(NSLOTS);
+    for (auto& e : ev_) RH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    d_pt_ = (PairTable*)dalloc(sizeof(PairTable));
+    upload_tables();
+    since_sort_ = sort_interval_;
+    upload(pos, vel, species);
+}
+
+template <int D>
+Engine<D>::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocs_) (void)hipFree(p);
+    for (auto& e : ev_) (void)hipEventDestroy(e);
+    for (auto& e : ev_pool_) (void)hipEventDestroy(e);
+    if (h_ctl_) (void)hipHostFree(h_ctl_);
+    sort_.release();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <int D>
+void Engine<D>::write_ctl(uint32_t word, uint32_t value) {
+    k_set_word<<<1, 1, 0, stream_>>>(B_.ctl, word, value);
+    RH_HIP(hipGetLastError());
+}
+
+template <int D>
+void Engine<D>::read_ctl(uint32_t* out) {
+    RH_HIP(hipMemcpyAsync(h_ctl_, B_.ctl, CTL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(out, h_ctl_, CTL_WORDS * sizeof(uint32_t));
+}
+
+template <int D>
+void Engine<D>::upload(const int64_t* pos, const int32_t* vel, const uint32_t* species) {
+    RH_HIP(hipStreamSynchronize(stream_));
+    for (uint64_t i = 0; i < n_; i++) {
+        RH_REQUIRE(species[i] < P_.nspecies, "species out of range");
+        for (int a = 0; a < D; a++)
+            RH_REQUIRE(pos[D * i + a] >= 0 && pos[D * i + a] < P_.box[a], "position outside the box");
+    }
+    std::vector<int64_t> px(n_);
+    std::vector<int32_t> vx(n_);
+    std::vector<uint32_t> ids(n_);
+    for (int a = 0; a < D; a++) {
+        for (uint64_t i = 0; i < n_; i++) px[i] = pos[D * i + a];
+        RH_HIP(hipMemcpy(B_.pos[0][a], px.data(), n_ * sizeof(int64_t), hipMemcpyHostToDevice));
+        for (uint64_t i = 0; i < n_; i++) vx[i] = vel[D * i + a];
+        RH_HIP(hipMemcpy(B_.vel[0][a], vx.data(), n_ * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    RH_HIP(hipMemcpy(B_.species[0], species, n_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (uint64_t i = 0; i < n_; i++) ids[i] = (uint32_t)i;
+    RH_HIP(hipMemcpy(B_.ids[0], ids.data(), n_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    std::vector<uint32_t> ctl(CTL_WORDS, 0u);
+    ctl[CTL_STALE] = 1u;
+    RH_HIP(hipMemcpy(B_.ctl, ctl.data(), CTL_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice));
+    steps_ = 0;
+    since_sort_ = sort_interval_;
+}
+
+template <int D>
+void Engine<D>::enqueue_cells_at_current(const uint32_t* gate) {
+    uint32_t nb = blocks_for(n_, TPB);
+    k_keys<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.keys[0], sort_.vals[0], gate);
+    RH_HIP(hipGetLastError());
+    int r = sort_.sort(n_, key_bits_, gate, stream_);
+    fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
+    fill_u32(B_.cell_end, 0u, ncell_keys_, gate, stream_);
+    k_ranges<<<nb, TPB, 0, stream_>>>(sort_.keys[r], (uint32_t)n_, B_.cell_start, B_.cell_end, gate);
+    RH_HIP(hipGetLastError());
+}
+
+template <int D>
+void Engine<D>::enqueue_force_cells(const uint32_t* gate) {
+    int r = RadixSort::result_index(key_bits_);
+    uint32_t nb = blocks_for(n_, TPB);
+    if (P_.nspecies > 1) k_force_cells<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], gate);
+    else k_force_cells<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], gate);
+    RH_HIP(hipGetLastError());
+}
+
+// The gated rebuild chain: keys, sort, reorder, cell ranges of the new order, list build.
+template <int D>
+void Engine<D>::enqueue_rebuild(bool gated) {
+    uint32_t* gate = B_.ctl + CTL_DO_REBUILD;
+    if (gated) k_rebuild_decide<<<1, 1, 0, stream_>>>(B_.ctl);
+    else k_rebuild_force<<<1, 1, 0, stream_>>>(B_.ctl);
+    RH_HIP(hipGetLastError());
+    hipEvent_t t0 = nullptr;
+    if (timing_) {
+        t0 = timer_event();
+        RH_HIP(hipEventRecord(t0, stream_));
+    }
+    uint32_t nb = blocks_for(n_, TPB);
+    k_keys<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.keys[0], sort_.vals[0], gate);
+    RH_HIP(hipGetLastError());
+    int r = sort_.sort(n_, key_bits_, gate, stream_);
+    k_gather<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], 1, gate);
+    RH_HIP(hipGetLastError());
+    fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
+    fill_u32(B_.cell_end, 0u, ncell_keys_, gate, stream_);
+    k_ranges<<<nb, TPB, 0, stream_>>>(sort_.keys[r], (uint32_t)n_, B_.cell_start, B_.cell_end, gate);
+    RH_HIP(hipGetLastError());
+    if (mode_.kind == 1) {
+        if (P_.nspecies > 1) k_build_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+        else k_build_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+        RH_HIP(hipGetLastError());
+    }
+    if (timing_) record_timer(true, t0);
+    k_rebuild_commit<<<1, 1, 0, stream_>>>(B_.ctl);
+    RH_HIP(hipGetLastError());
+}
+
+template <int D>
+void Engine<D>::handle_overflow() {
+    uint32_t ctl[CTL_WORDS];
+    read_ctl(ctl);
+    uint32_t want = std::max<uint32_t>(ctl[CTL_MAXCOUNT] + ctl[CTL_MAXCOUNT] / 4 + 8, P_.cap + P_.cap / 2);
+    alloc_list(want);
+    write_ctl(CTL_OVERFLOW, 0u);
+    write_ctl(CTL_STALE, 1u);
+}
+
+template <int D>
+void Engine<D>::step_list(uint64_t steps) {
+    constexpr int NSLOTS = 8, DEPTH = 3;
+    uint64_t left = steps;
+    uint32_t nb = blocks_for(n_, TPB);
+    while (left > 0) {
+        uint32_t add = (uint32_t)std::min<uint64_t>(left, 1u << 30);
+        left -= add;
+        k_add_remaining<<<1, 1, 0, stream_>>>(B_.ctl, add);
+        RH_HIP(hipGetLastError());
+        struct Flight {
+            int slot;
+            uint32_t k;
+        };
+        std::deque<Flight> flight;
+        uint64_t remaining = add;
+        uint32_t last_done = 0, last_rebuilds = 0;
+        bool have_last = false;
+        int next_slot = 0;
+        bool done = false;
+        while (!done) {
+            // Chunk length: a little over the expected list lifetime, no more than is left.
+            double expect_inflight = 0.0;
+            for (auto& f : flight) expect_inflight += std::min<double>(f.k, lifetime_);
+            double want = std::ceil(lifetime_ * 1.25) + 1.0;
+            double room = (double)remaining - expect_inflight;
+            if (room >= 1.0) {
+                uint32_t k = (uint32_t)std::max(1.0, std::min({want, room + 1.0, 512.0}));
+                enqueue_rebuild(true);
+                for (uint32_t s = 0; s < k; s++) {
+                    hipEvent_t t0 = nullptr;
+                    if (timing_) {
+                        t0 = timer_event();
+                        RH_HIP(hipEventRecord(t0, stream_));
+                    }
+                    if (P_.nspecies > 1) k_step_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_);
+                    else k_step_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_);
+                    RH_HIP(hipGetLastError());
+                    if (timing_) record_timer(false, t0);
+                    k_step_commit<<<1, 1, 0, stream_>>>(B_.ctl);
+                    RH_HIP(hipGetLastError());
+                }
+                int slot = next_slot++ % NSLOTS;
+                RH_HIP(hipMemcpyAsync(h_ctl_ + slot * CTL_WORDS, B_.ctl, CTL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+                RH_HIP(hipEventRecord(ev_[slot], stream_));
+                flight.push_back({slot, k});
+            }
+            bool must_wait = flight.size() >= (size_t)DEPTH || room < 1.0;
+            while (!flight.empty() && (must_wait || hipEventQuery(ev_[flight.front().slot]) == hipSuccess)) {
+                must_wait = false;
+                Flight f = flight.front();
+                flight.pop_front();
+                RH_HIP(hipEventSynchronize(ev_[f.slot]));
+                const uint32_t* c = h_ctl_ + f.slot * CTL_WORDS;
+                remaining = c[CTL_REMAINING];
+                uint32_t dn = c[CTL_DONE_LO], rb = c[CTL_REBUILDS];
+                if (have_last && rb > last_rebuilds) {
+                    double life = (double)(dn - last_done) / (double)(rb - last_rebuilds);
+                    lifetime_ = 0.5 * lifetime_ + 0.5 * std::max(1.0, life);
+                } else if (have_last && rb == last_rebuilds && dn > last_done) {
+                    lifetime_ = std::min(512.0, std::max(lifetime_, (double)(dn - last_done) * 1.5));
+                }
+                last_done = dn;
+                last_rebuilds = rb;
+                have_last = true;
+                if (c[CTL_OVERFLOW]) {
+                    RH_HIP(hipStreamSynchronize(stream_));
+                    flight.clear();
+                    handle_overflow();
+                    uint32_t now[CTL_WORDS];
+                    read_ctl(now);
+                    remaining = now[CTL_REMAINING];
+                    have_last = false;
+                    break;
+                }
+                if (remaining == 0) {
+                    done = true;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+template <int D>
+void Engine<D>::step_cells(uint64_t steps) {
+    uint32_t nb = blocks_for(n_, TPB);
+    uint32_t& since = since_sort_;
+    for (uint64_t s = 0; s < steps; s++) {
+        if (sort_interval_ > 0 && since >= sort_interval_) {
+            // Morton reorder of storage (engine3.rs:502-507), then forces on the sorted slots.
+            enqueue_rebuild(false);
+            since = 0;
+        }
+        enqueue_cells_at_current(nullptr);
+        enqueue_force_cells(nullptr);
+        k_integrate<D><<<nb, TPB, 0, stream_>>>(P_, B_);
+        RH_HIP(hipGetLastError());
+        since++;
+    }
+    k_cells_commit<<<1, 1, 0, stream_>>>(B_.ctl, (uint32_t)steps);
+    RH_HIP(hipGetLastError());
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+void Engine<D>::step(uint64_t steps) {
+    if (steps == 0) return;
+    if (mode_.kind == 1) step_list(steps);
+    else step_cells(steps);
+    collect_timers();
+    steps_ += (int64_t)steps;
+}
+
+template <int D>
+void Engine<D>::kicks(int32_t* kicks, double* potential) {
+    enqueue_cells_at_current(nullptr);
+    enqueue_force_cells(nullptr);
+    uint32_t ctl[CTL_WORDS];
+    read_ctl(ctl);
+    uint32_t pm = ctl[CTL_PMETA];
+    std::vector<int32_t> k(n_);
+    std::vector<uint32_t> ids(n_);
+    std::vector<double> pe(n_);
+    RH_HIP(hipMemcpy(ids.data(), B_.ids[pm], n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int a = 0; a < D; a++) {
+        RH_HIP(hipMemcpy(k.data(), B_.kicks[a], n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (kicks)
+            for (uint64_t s = 0; s < n_; s++) kicks[D * (uint64_t)ids[s] + a] = k[s];
+    }
+    RH_HIP(hipMemcpy(pe.data(), B_.pe, n_ * sizeof(double), hipMemcpyDeviceToHost));
+    double total = 0.0;
+    std::vector<double> by_id(n_);
+    for (uint64_t s = 0; s < n_; s++) by_id[ids[s]] = pe[s];
+    for (uint64_t i = 0; i < n_; i++) total += by_id[i];
+    if (potential) *potential = total;
+}
+
+template <int D>
+double Engine<D>::potential() {
+    double pe = 0.0;
+    kicks(nullptr, &pe);
+    return pe;
+}
+
+template <int D>
+void Engine<D>::reverse() {
+    enqueue_cells_at_current(nullptr);
+    enqueue_force_cells(nullptr);
+    k_reverse<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_);
+    RH_HIP(hipGetLastError());
+    RH_HIP(hipStreamSynchronize(stream_));
+    steps_ = -steps_;
+}
+
+template <int D>
+void Engine<D>::read_state(int64_t* pos, int32_t* vel, uint32_t* species) {
+    uint32_t ctl[CTL_WORDS];
+    read_ctl(ctl);
+    uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
+    std::vector<uint32_t> ids(n_), sp(n_);
+    std::vector<int64_t> px(n_);
+    std::vector<int32_t> vx(n_);
+    RH_HIP(hipMemcpy(ids.data(), B_.ids[pm], n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int a = 0; a < D; a++) {
+        RH_HIP(hipMemcpy(px.data(), B_.pos[pp][a], n_ * sizeof(int64_t), hipMemcpyDeviceToHost));
+        RH_HIP(hipMemcpy(vx.data(), B_.vel[pm][a], n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (uint64_t s = 0; s < n_; s++) {
+            if (pos) pos[D * (uint64_t)ids[s] + a] = px[s];
+            if (vel) vel[D * (uint64_t)ids[s] + a] = vx[s];
+        }
+    }
+    if (species) {
+        RH_HIP(hipMemcpy(sp.data(), B_.species[pm], n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint64_t s = 0; s < n_; s++) species[ids[s]] = sp[s];
+    }
+}
+
+template <int D>
+double Engine<D>::kinetic_total() {
+    k_kinetic<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, sys_.masses[0], sys_.masses[1], sys_.masses[2], sys_.masses[3], sys_.dt);
+    RH_HIP(hipGetLastError());
+    reduce_sum_f64(B_.ke, d_scalar_, n_, reduce_temp_, stream_);
+    double out = 0.0;
+    RH_HIP(hipMemcpyAsync(&out, d_scalar_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+    return out;
+}
+
+template <int D>
+void Engine<D>::update_system(const SystemDesc& sys) {
+    // engine3.rs:536-559: same box, species count and wall layout; cutoffs may not grow.
+    for (int a = 0; a < D; a++) {
+        RH_REQUIRE(sys.box_len[a] == sys_.box_len[a], "update_system: the box must stay the same");
+        RH_REQUIRE((sys.walled[a] != 0) == (sys_.walled[a] != 0), "update_system: walls cannot be added or removed");
+    }
+    RH_REQUIRE(sys.nspecies == sys_.nspecies, "update_system: the species count must stay the same");
+    RH_REQUIRE(core::rc_max(sys.pairs, sys.nspecies) <= core::rc_max(sys_.pairs, sys_.nspecies) + 1e-12, "cutoffs cannot grow at run time");
+    Params<D> old = P_;
+    uint32_t old_bits = key_bits_;
+    uint64_t old_keys = ncell_keys_;
+    build_params(sys, mode_);
+    key_bits_ = old_bits;
+    ncell_keys_ = old_keys;
+    // Keep the cell geometry the buffers were sized for; a pair whose list radius grew is listed
+    // further out from the next rebuild on.
+    for (int a = 0; a < 3; a++) {
+        P_.nc[a] = old.nc[a];
+        P_.cell_scale[a] = old.cell_scale[a];
+    }
+    P_.mbits = old.mbits;
+    P_.cap = old.cap;
+    P_.half_skin2 = old.half_skin2;
+    bool grew = false;
+    for (int k = 0; k < 16; k++) {
+        grew |= P_.pt.list_r2[k] > old.pt.list_r2[k];
+        P_.pt.list_r2[k] = std::max(P_.pt.list_r2[k], old.pt.list_r2[k]);
+    }
+    if (grew) write_ctl(CTL_STALE, 1u);
+    upload_tables();
+    sys_ = sys;
+}
+
+template <int D>
+void Engine<D>::kick_region(const double lo[3], const double hi[3], double scale, const double add[3]) {
+    int32_t a[3] = {0, 0, 0};
+    for (int k = 0; k < D; k++) a[k] = core::to_step_displacement(sys_.dt, add[k]);
+    k_kick_region<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, (float)lo[0], (float)lo[1], D == 3 ? (float)lo[2] : 0.f, (float)hi[0], (float)hi[1],
+                                                                D == 3 ? (float)hi[2] : 0.f, (float)scale, a[0], a[1], a[2]);
+    RH_HIP(hipGetLastError());
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+void Engine<D>::set_species(const uint32_t* ids, uint64_t count, uint32_t species) {
+    RH_REQUIRE(species < P_.nspecies, "species out of range");
+    std::vector<uint32_t> list(ids, ids + count);
+    std::sort(list.begin(), list.end());
+    list.erase(std::unique(list.begin(), list.end()), list.end());
+    if (list.empty()) return;
+    if (list.size() > set_list_cap_) {
+        if (d_set_list_) {
+            auto it = std::find(allocs_.begin(), allocs_.end(), (void*)d_set_list_);
+            if (it != allocs_.end()) allocs_.erase(it);
+            RH_HIP(hipFree(d_set_list_));
+        }
+        d_set_list_ = (uint32_t*)dalloc(list.size() * sizeof(uint32_t));
+        set_list_cap_ = list.size();
+    }
+    RH_HIP(hipMemcpyAsync(d_set_list_, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+    k_set_species<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, d_set_list_, (uint32_t)list.size(), species);
+    RH_HIP(hipGetLastError());
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+void Engine<D>::set_species_velocity(uint32_t species, const double v[3]) {
+    int32_t a[3] = {0, 0, 0};
+    for (int k = 0; k < D; k++) a[k] = core::to_step_displacement(sys_.dt, v[k]);
+    k_set_species_velocity<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, species, a[0], a[1], a[2]);
+    RH_HIP(hipGetLastError());
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+rh_list_stats Engine<D>::list_stats() {
+    uint32_t ctl[CTL_WORDS];
+    read_ctl(ctl);
+    rh_list_stats s;
+    s.rebuilds = ctl[CTL_REBUILDS];
+    s.overflowed = ctl[CTL_OVERFLOW];
+    s.cap = P_.cap;
+    s.max_count = ctl[CTL_MAXCOUNT];
+    s.steps_done = ((uint64_t)ctl[CTL_DONE_HI] << 32) | ctl[CTL_DONE_LO];
+    s.launches_skipped = ctl[CTL_SKIPPED];
+    return s;
+}
+
+template <int D>
+void Engine<D>::kernel_ms(double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds) const {
+    if (force_ms) *force_ms = force_ms_;
+    if (force_launches) *force_launches = force_launches_;
+    if (rebuild_ms) *rebuild_ms = rebuild_ms_;
+    if (rebuilds) *rebuilds = rebuild_launches_;
+}
+
+template class Engine<2>;
+template class Engine<3>;
+
+// ---------------------------------------------------------------------------------------------
+// Device-side conversion check (used by the tests to pin the kick conversion on the hardware).
+// ---------------------------------------------------------------------------------------------
+
+__global__ void k_kick_convert(const float* x, int32_t* out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) out[i] = cvt_sat_i32(__builtin_rintf(x[i]));
+}
+
+void kick_convert(const float* x, int32_t* out, uint64_t n, hipStream_t s) {
+    if (n == 0) return;
+    k_kick_convert<<<blocks_for(n, TPB), TPB, 0, s>>>(x, out, n);
+    RH_HIP(hipGetLastError());
+}
+
+}  // namespace rh

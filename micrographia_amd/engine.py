@@ -1,0 +1,239 @@
+"""GpuEngine3 / GpuEngine: the MI355X engines behind the reference's engine API.
+
+Method names and meanings follow md::GpuEngine3 (crates/md/src/engine3.rs:307-648) and
+md::GpuEngine (crates/md/src/engine.rs:567-1098).  The reference's readbacks are futures; here
+they return once the device has answered.  Errors raise RahmanError (the reference panics).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, load, ptr
+from .core import ForceMode, State, State3, System, System3, verlet3, verlet_for
+
+
+@dataclass
+class Ledger3:
+    energy: np.ndarray
+    impulse: np.ndarray
+    hits: np.ndarray
+
+
+@dataclass
+class ListStats:
+    rebuilds: int
+    overflowed: bool
+    cap: int
+    max_count: int
+    steps_done: int
+    launches_skipped: int
+
+
+def _stats(s: _lib.CListStats) -> ListStats:
+    return ListStats(int(s.rebuilds), bool(s.overflowed), int(s.cap), int(s.max_count), int(s.steps_done), int(s.launches_skipped))
+
+
+class GpuEngine3:
+    """md::GpuEngine3 on one MI355X."""
+
+    def __init__(self, sys: System3, state: State3, mode: Optional[ForceMode] = None):
+        self.sys = sys
+        self.mode = mode if mode is not None else verlet3(sys, state.n())
+        self._n = state.n()
+        h = C.c_void_p()
+        cs = sys.to_c()
+        cm = self.mode.to_c()
+        check(load().rh_engine3_create(C.byref(cs), self._n, ptr(state.pos), ptr(state.vel), ptr(state.species), C.byref(cm), C.byref(h)))
+        self._h = h
+        self._sort_interval = 100
+
+    @staticmethod
+    def new(sys: System3, state: State3) -> "GpuEngine3":
+        """GpuEngine3::new: Verlet lists with a 0.4 sigma skin (engine3.rs:337-340)."""
+        return GpuEngine3(sys, state)
+
+    @staticmethod
+    def with_mode(sys: System3, state: State3, mode: ForceMode) -> "GpuEngine3":
+        return GpuEngine3(sys, state, mode)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().rh_engine3_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n(self) -> int:
+        return self._n
+
+    @property
+    def steps(self) -> int:
+        v = C.c_int64()
+        check(load().rh_engine3_steps(self._h, C.byref(v)))
+        return int(v.value)
+
+    @property
+    def sort_interval(self) -> int:
+        return self._sort_interval
+
+    @sort_interval.setter
+    def sort_interval(self, v: int) -> None:
+        check(load().rh_engine3_set_sort_interval(self._h, int(v)))
+        self._sort_interval = int(v)
+
+    def step(self, steps: int) -> None:
+        check(load().rh_engine3_step(self._h, int(steps)))
+
+    def reverse(self) -> None:
+        check(load().rh_engine3_reverse(self._h))
+
+    def update_system(self, sys: System3) -> None:
+        cs = sys.to_c()
+        check(load().rh_engine3_update_system(self._h, C.byref(cs)))
+        self.sys = sys
+
+    def kick_region(self, lo: Sequence[float], hi: Sequence[float], scale: float, add: Sequence[float]) -> None:
+        lo_a, hi_a, add_a = (np.ascontiguousarray(x, np.float64) for x in (lo, hi, add))
+        check(load().rh_engine3_kick_region(self._h, ptr(lo_a), ptr(hi_a), float(scale), ptr(add_a)))
+
+    def set_species(self, ids: Sequence[int], species: int) -> None:
+        a = np.ascontiguousarray(ids, np.uint32)
+        check(load().rh_engine3_set_species(self._h, ptr(a), len(a), int(species)))
+
+    def set_species_velocity(self, species: int, v: Sequence[float]) -> None:
+        a = np.ascontiguousarray(v, np.float64)
+        check(load().rh_engine3_set_species_velocity(self._h, int(species), ptr(a)))
+
+    def upload(self, state: State3) -> None:
+        if state.n() != self._n:
+            raise ValueError("upload needs the same particle count")
+        check(load().rh_engine3_upload(self._h, ptr(state.pos), ptr(state.vel), ptr(state.species)))
+
+    def read_state(self) -> State3:
+        pos = np.zeros((self._n, 3), np.int64)
+        vel = np.zeros((self._n, 3), np.int32)
+        sp = np.zeros(self._n, np.uint32)
+        check(load().rh_engine3_read_state(self._h, ptr(pos), ptr(vel), ptr(sp)))
+        return State3(pos, vel, sp)
+
+    def kicks(self) -> Tuple[np.ndarray, float]:
+        k = np.zeros((self._n, 3), np.int32)
+        pe = C.c_double()
+        check(load().rh_engine3_kicks(self._h, ptr(k), C.byref(pe)))
+        return k, float(pe.value)
+
+    def read_kinetic_total(self) -> float:
+        v = C.c_double()
+        check(load().rh_engine3_read_kinetic_total(self._h, C.byref(v)))
+        return float(v.value)
+
+    def read_potential(self) -> float:
+        v = C.c_double()
+        check(load().rh_engine3_read_potential(self._h, C.byref(v)))
+        return float(v.value)
+
+    def read_ledger(self) -> Ledger3:
+        l = _lib.CLedger3()
+        check(load().rh_engine3_read_ledger(self._h, C.byref(l)))
+        return Ledger3(np.array(l.energy[:]), np.array(l.impulse[:]), np.array(l.hits[:], dtype=np.uint64))
+
+    def read_list_stats(self) -> ListStats:
+        s = _lib.CListStats()
+        check(load().rh_engine3_list_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def stream(self) -> int:
+        v = C.c_void_p()
+        check(load().rh_engine3_stream(self._h, C.byref(v)))
+        return int(v.value or 0)
+
+    def set_timing(self, on: bool) -> None:
+        check(load().rh_engine3_set_timing(self._h, 1 if on else 0))
+
+    def kernel_ms(self):
+        f, r = C.c_double(), C.c_double()
+        fl, rl = C.c_uint64(), C.c_uint64()
+        check(load().rh_engine3_kernel_ms(self._h, C.byref(f), C.byref(fl), C.byref(r), C.byref(rl)))
+        return float(f.value), int(fl.value), float(r.value), int(rl.value)
+
+
+class GpuEngine:
+    """md::GpuEngine (2D): periodic or hard-walled axes, gravity, pinned species, f32 pair forces."""
+
+    def __init__(self, sys: System, state: State, mode: Optional[ForceMode] = None):
+        self.sys = sys
+        self.mode = mode if mode is not None else verlet_for(sys, state.n())
+        self._n = state.n()
+        h = C.c_void_p()
+        cs = sys.to_c()
+        cm = self.mode.to_c()
+        check(load().rh_engine2_create(C.byref(cs), self._n, ptr(state.pos), ptr(state.vel), ptr(state.species), C.byref(cm), C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def new(sys: System, state: State) -> "GpuEngine":
+        return GpuEngine(sys, state)
+
+    @staticmethod
+    def with_mode(sys: System, state: State, mode: ForceMode) -> "GpuEngine":
+        return GpuEngine(sys, state, mode)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().rh_engine2_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n(self) -> int:
+        return self._n
+
+    def step(self, steps: int) -> None:
+        check(load().rh_engine2_step(self._h, int(steps)))
+
+    def reverse(self) -> None:
+        check(load().rh_engine2_reverse(self._h))
+
+    def upload(self, state: State) -> None:
+        check(load().rh_engine2_upload(self._h, ptr(state.pos), ptr(state.vel), ptr(state.species)))
+
+    def read_state(self) -> State:
+        pos = np.zeros((self._n, 2), np.int64)
+        vel = np.zeros((self._n, 2), np.int32)
+        sp = np.zeros(self._n, np.uint32)
+        check(load().rh_engine2_read_state(self._h, ptr(pos), ptr(vel), ptr(sp)))
+        return State(pos, vel, sp)
+
+    def kicks(self) -> Tuple[np.ndarray, float]:
+        k = np.zeros((self._n, 2), np.int32)
+        pe = C.c_double()
+        check(load().rh_engine2_kicks(self._h, ptr(k), C.byref(pe)))
+        return k, float(pe.value)
+
+    def read_kinetic_total(self) -> float:
+        v = C.c_double()
+        check(load().rh_engine2_read_kinetic_total(self._h, C.byref(v)))
+        return float(v.value)
+
+    def read_list_stats(self) -> ListStats:
+        s = _lib.CListStats()
+        check(load().rh_engine2_list_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def stream(self) -> int:
+        v = C.c_void_p()
+        check(load().rh_engine2_stream(self._h, C.byref(v)))
+        return int(v.value or 0)

@@ -1,0 +1,261 @@
+"""The 3D engine against the oracle (IntEngine3<f32> bit for bit, IntEngine3<f64> within the
+reference's tolerances) and its exactness properties, mirroring crates/md/tests/three.rs."""
+import math
+
+import numpy as np
+import pytest
+
+import micrographia_amd as rh
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ONE = 1 << 32
+
+
+def liquid(n, density, t, walls=False, seed_pos=7, seed_vel=3):
+    """crates/md/tests/three.rs:15-31 (product md-core; equal to the oracle's, see test_host)."""
+    sys = rh.System3.new(rh.cube_for(n, density), 0.005, rh.Pair.lj(2.5), [1.0])
+    if walls:
+        sys = sys.with_walls(2, rh.Wall.Hard, rh.Wall.Hard)
+    pos = rh.fcc(n, sys.box_len, rh.fcc_cells(n, sys.box_len), 0.05, seed_pos)
+    if walls:
+        pos[:, 2] = np.clip(pos[:, 2], ONE // 2, sys.box_len[2] - ONE // 2)
+    sp = np.zeros(n, np.uint32)
+    vel = rh.maxwell(sys, sp, t, seed_vel)
+    return sys, rh.State3(pos, vel, sp)
+
+
+def oracle_system(sys: rh.System3) -> O.System3:
+    o = O.System3()
+    c = sys.to_c()
+    for a in range(3):
+        o.box_len[a] = c.box_len[a]
+        o.walled[a] = c.walled[a]
+        o.gravity[a] = c.gravity[a]
+        for s in range(2):
+            o.wall_kind[a][s] = c.wall_kind[a][s]
+            o.wall_t[a][s] = c.wall_t[a][s]
+    o.dt = c.dt
+    o.nspecies = c.nspecies
+    o.pinned = c.pinned
+    for k in range(4):
+        o.masses[k] = c.masses[k]
+    for k in range(16):
+        o.pairs[k] = O.Pair(c.pairs[k].eps, c.pairs[k].sigma, c.pairs[k].rc)
+    o.seed = c.seed
+    return o
+
+
+def oracle_run(sys, st: rh.State3, steps, prec=32):
+    pos, vel, sp = st.pos.copy(), st.vel.copy(), st.species.copy()
+    O.step3(oracle_system(sys), pos, vel, sp, steps, prec)
+    return rh.State3(pos, vel, sp)
+
+
+MODES = {
+    "cells": lambda sys, n: rh.ForceMode.cell_list(),
+    "verlet": lambda sys, n: rh.verlet3_with_skin(sys, n, 0.3),
+}
+
+
+@pytest.mark.parametrize("mode", ["cells", "verlet"])
+def test_kicks_are_bit_exact_with_the_f32_twin(mode):
+    """three.rs:33-64, strengthened: every kick identical to IntEngine3<f32>, rms vs f64 < 1e-5."""
+    sys, s0 = liquid(4000, 0.8, 1.0)
+    st = oracle_run(sys, s0, 100)
+    gpu = rh.GpuEngine3(sys, st, MODES[mode](sys, st.n()))
+    kg, peg = gpu.kicks()
+    osys = oracle_system(sys)
+    kc, pec = O.kicks3(osys, st.pos, st.species, 32)
+    k64, pe64 = O.kicks3(osys, st.pos, st.species, 64)
+    print(f"identical kicks: {(kg == kc).all(axis=1).sum()}/{len(kg)}")
+    assert np.array_equal(kg, kc)
+    rel = math.sqrt(((kg - k64).astype(np.float64) ** 2).sum() / (k64.astype(np.float64) ** 2).sum())
+    assert rel < 1e-5
+    assert abs((peg - pec) / pec) < 1e-5
+    assert abs((peg - pe64) / pe64) < 1e-5
+
+
+@pytest.mark.parametrize("mode", ["cells", "verlet"])
+def test_trajectory_is_bit_exact_with_the_f32_twin(mode):
+    """three.rs:66-80 asks for 1e-5 sigma after 50 steps; integer state here is identical."""
+    sys, s0 = liquid(2000, 0.8, 1.0, walls=True)
+    gpu = rh.GpuEngine3(sys, s0, MODES[mode](sys, s0.n()))
+    gpu.step(200)
+    want = oracle_run(sys, s0, 200)
+    got = gpu.read_state()
+    assert got == want
+    assert got.hash() == want.hash()
+
+
+def test_periodic_liquid_bit_exact_over_many_rebuilds():
+    sys, s0 = liquid(4000, 0.8442, 1.2)
+    gpu = rh.GpuEngine3(sys, s0, rh.verlet3_with_skin(sys, 4000, 0.3))
+    gpu.step(300)
+    stats = gpu.read_list_stats()
+    assert stats.rebuilds >= 10 and not stats.overflowed
+    assert gpu.read_state() == oracle_run(sys, s0, 300)
+
+
+def test_reversal_restores_the_start_bit_for_bit():
+    """three.rs:82-93"""
+    sys, s = liquid(4000, 0.8, 1.0, walls=True)
+    gpu = rh.GpuEngine3(sys, s, rh.verlet3(sys, s.n()))
+    gpu.step(1000)
+    gpu.reverse()
+    gpu.step(1000)
+    gpu.reverse()
+    back = gpu.read_state()
+    assert back.hash() == s.hash()
+
+
+def test_energy_is_conserved():
+    """three.rs:95-111, energies from the oracle's f64 evaluation of GPU states."""
+    sys, s = liquid(8000, 0.8, 1.0)
+    n = s.n()
+    gpu = rh.GpuEngine3(sys, s, rh.verlet3_with_skin(sys, n, 0.3))
+    osys = oracle_system(sys)
+    energies = []
+    for _ in range(10):
+        gpu.step(500)
+        st = gpu.read_state()
+        ke, pe = O.energy3(osys, st.pos, st.vel, st.species, 64)
+        energies.append((ke + pe) / n)
+    drift = max(energies[2:]) - min(energies[2:])
+    print("energy per particle:", [f"{e:.5f}" for e in energies])
+    assert drift < 2e-3
+    # relative total-energy tolerance over the run
+    assert abs(energies[-1] - energies[2]) / abs(energies[2]) < 1e-3
+
+
+def test_verlet_lists_and_cell_list_give_the_same_trajectory():
+    """three.rs:135-151"""
+    sys, s = liquid(6000, 0.8, 1.0, walls=True)
+    cells = rh.GpuEngine3(sys, s, rh.ForceMode.cell_list())
+    lists = rh.GpuEngine3(sys, s, rh.verlet3(sys, s.n()))
+    cells.step(2000)
+    lists.step(2000)
+    a, b = cells.read_state(), lists.read_state()
+    stats = lists.read_list_stats()
+    print(f"3D: {stats.rebuilds} list rebuilds in 2000 steps")
+    assert not stats.overflowed
+    assert stats.rebuilds > 10
+    assert a.hash() == b.hash()
+
+
+def test_run_to_run_identical_and_momentum_exact():
+    sys, s = liquid(5000, 0.7, 1.0)
+    a = rh.GpuEngine3(sys, s, rh.verlet3_with_skin(sys, 5000, 0.3))
+    b = rh.GpuEngine3(sys, s, rh.verlet3_with_skin(sys, 5000, 0.3))
+    a.step(700)
+    b.step(300)
+    b.step(400)
+    sa, sb = a.read_state(), b.read_state()
+    assert sa != s and sa == sb
+    assert (sa.vel.astype(np.int64).sum(0) == 0).all()
+
+
+def test_storage_permutation_does_not_change_the_trajectory():
+    """gpu_engine.rs:107-123 in 3D"""
+    sys, s = liquid(3000, 0.8, 1.0)
+    perm = np.random.default_rng(4).permutation(3000)
+    sp = rh.State3(s.pos[perm], s.vel[perm], s.species[perm])
+    a = rh.GpuEngine3(sys, s)
+    b = rh.GpuEngine3(sys, sp)
+    a.step(400)
+    b.step(400)
+    ra, rb = a.read_state(), b.read_state()
+    assert np.array_equal(ra.pos[perm], rb.pos) and np.array_equal(ra.vel[perm], rb.vel)
+
+
+def test_a_tiny_list_capacity_regrows_and_stays_exact():
+    sys, s = liquid(3000, 0.8442, 1.2)
+    gpu = rh.GpuEngine3(sys, s, rh.ForceMode(1, 0.3, 12))
+    gpu.step(150)
+    stats = gpu.read_list_stats()
+    assert stats.cap > 12
+    assert gpu.read_state() == oracle_run(sys, s, 150)
+
+
+def test_mixture_pinned_gravity_soft_walls_match_the_twin():
+    n = 3000
+    sys, s = liquid(n, 0.7, 1.0, walls=True)
+    sys = rh.System3.new(sys.box_len, 0.005, rh.Pair.lj(2.5), [1.0, 3.0, 1.0])
+    sys = sys.with_pair(0, 1, rh.Pair(0.5, 1.2, 3.0)).with_pair(1, 2, rh.Pair(1.0, 0.9, 2.2))
+    sys = sys.with_walls(2, rh.Wall.Soft, rh.Wall.Hard).with_gravity((0.0, 0.0, -0.1)).with_pinned(2)
+    s.species[::3] = 1
+    s.species[::7] = 2
+    s.pos[:, 2] = np.clip(s.pos[:, 2], ONE, sys.box_len[2] - ONE)
+    s.vel = rh.maxwell(sys, s.species, 1.0, 5)
+    for mode in (rh.ForceMode.cell_list(), rh.verlet3_with_skin(sys, n, 0.3)):
+        gpu = rh.GpuEngine3(sys, s, mode)
+        gpu.step(150)
+        assert gpu.read_state() == oracle_run(sys, s, 150)
+
+
+def test_set_species_changes_the_chosen_particles():
+    """three.rs:153-166"""
+    sys, s = liquid(2000, 0.8, 1.0)
+    sys = rh.System3.new(sys.box_len, 0.005, rh.Pair.lj(2.5), [1.0, 1.0])
+    gpu = rh.GpuEngine3(sys, s)
+    gpu.step(300)
+    gpu.set_species([5, 17, 1999], 1)
+    gpu.step(2000)
+    st = gpu.read_state()
+    assert list(np.nonzero(st.species == 1)[0]) == [5, 17, 1999]
+
+
+def test_upload_round_trips_and_kick_region():
+    sys, s = liquid(2000, 0.8, 1.0)
+    gpu = rh.GpuEngine3(sys, s)
+    gpu.step(50)
+    mid = gpu.read_state()
+    gpu.upload(s)
+    assert gpu.read_state() == s
+    gpu.upload(mid)
+    assert gpu.read_state() == mid
+    b = sys.box_f64()
+    gpu.kick_region([0, 0, 0], [b[0] / 2, b[1], b[2]], 0.0, [0.0, 0.0, 0.0])
+    st = gpu.read_state()
+    x = mid.pos[:, 0] / ONE
+    left, right = x < b[0] / 2 - 1e-4, x >= b[0] / 2 + 1e-4
+    assert (st.vel[left] == 0).all() and (st.vel[right] == mid.vel[right]).all()
+    assert left.sum() > 500 and right.sum() > 500
+
+
+def test_kinetic_energy_readout():
+    sys, s = liquid(4000, 0.8, 1.0)
+    gpu = rh.GpuEngine3(sys, s)
+    assert abs(gpu.read_kinetic_total() - s.half_step_kinetic(sys)) < 1e-9 * s.half_step_kinetic(sys)
+
+
+def test_benchmark_size_short_run_bit_exact():
+    """262,144 atoms (the single-GPU benchmark's liquid) for a few steps against the oracle."""
+    n = 262144
+    sys = rh.System3.new(rh.cube_for(n, 0.8442), 0.005, rh.Pair.lj(2.5), [1.0])
+    pos = rh.fcc(n, sys.box_len, rh.fcc_cells(n, sys.box_len), 0.05, 1)
+    sp = np.zeros(n, np.uint32)
+    s = rh.State3(pos, rh.maxwell(sys, sp, 0.72, 1), sp)
+    gpu = rh.GpuEngine3(sys, s, rh.verlet3_with_skin(sys, n, 0.3))
+    gpu.step(3)
+    assert gpu.read_state() == oracle_run(sys, s, 3)
+
+
+def test_large_system_properties():
+    """4,194,304 atoms at T = 1.2: zero integer momentum and exact reversal at full size."""
+    n = 4194304
+    sys = rh.System3.new(rh.cube_for(n, 0.8442), 0.005, rh.Pair.lj(2.5), [1.0])
+    pos = rh.fcc(n, sys.box_len, rh.fcc_cells(n, sys.box_len), 0.05, 2)
+    sp = np.zeros(n, np.uint32)
+    s = rh.State3(pos, rh.maxwell(sys, sp, 1.2, 2), sp)
+    gpu = rh.GpuEngine3(sys, s, rh.verlet3_with_skin(sys, n, 0.3))
+    gpu.step(60)
+    mid = gpu.read_state()
+    assert (mid.vel.astype(np.int64).sum(0) == 0).all()
+    stats = gpu.read_list_stats()
+    assert stats.rebuilds >= 3 and not stats.overflowed
+    gpu.reverse()
+    gpu.step(60)
+    gpu.reverse()
+    assert gpu.read_state().hash() == s.hash()
