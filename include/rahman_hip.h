@@ -139,6 +139,11 @@ int rh_engine3_read_potential(rh_engine3* e, double* pe);                       
 int rh_engine3_read_ledger(rh_engine3* e, rh_ledger3* out);                                                     /* :645-647 */
 int rh_engine3_list_stats(rh_engine3* e, rh_list_stats* out);                                                   /* :272-279 */
 int rh_engine3_set_sort_interval(rh_engine3* e, uint32_t interval);                                             /* sort_interval, :327-329 */
+/* Kernel choice (this engine's own tuning knob, no reference counterpart): variant 0 = full
+ * Verlet list, one lane per particle; 1 = half list inside work-group blocks of `block` particles
+ * (256, 512 or 1024; the partner's kick accumulated in LDS) and full lists across blocks.  Both
+ * give identical trajectories.  Forces a list rebuild. */
+int rh_engine3_tune(rh_engine3* e, int variant, uint32_t block);
 /* The engine's HIP stream (hipStream_t), for callers that time or order work around it. */
 int rh_engine3_stream(rh_engine3* e, void** stream);
 /* Mean duration (ms) of the force+integrate launches of the last step() call, measured with HIP
@@ -157,6 +162,7 @@ int rh_engine2_read_state(rh_engine2* e, int64_t* pos, int32_t* vel, uint32_t* s
 int rh_engine2_kicks(rh_engine2* e, int32_t* kicks, double* potential);
 int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke);
 int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out);
+int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block);
 int rh_engine2_stream(rh_engine2* e, void** stream);
 
 /* ---- gpu-prims on device buffers (crates/gpu-prims/src/prims.rs:116-312, threefry.rs) ----

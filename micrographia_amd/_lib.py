@@ -117,6 +117,7 @@ SIGNATURES = {
     "rh_engine3_read_ledger": ([P, P], i32),
     "rh_engine3_list_stats": ([P, P], i32),
     "rh_engine3_set_sort_interval": ([P, u32], i32),
+    "rh_engine3_tune": ([P, i32, u32], i32),
     "rh_engine3_stream": ([P, P], i32),
     "rh_engine3_set_timing": ([P, i32], i32),
     "rh_engine3_kernel_ms": ([P, P, P, P, P], i32),
@@ -129,6 +130,7 @@ SIGNATURES = {
     "rh_engine2_kicks": ([P, P, P], i32),
     "rh_engine2_read_kinetic_total": ([P, P], i32),
     "rh_engine2_list_stats": ([P, P], i32),
+    "rh_engine2_tune": ([P, i32, u32], i32),
     "rh_engine2_stream": ([P, P], i32),
     "rh_prims_exclusive_scan_u32": ([P, P, u64, P], i32),
     "rh_prims_radix_sort_u32": ([P, P, u64, u32, P], i32),
@@ -150,6 +152,14 @@ def load() -> C.CDLL:
                 f"{LIB_PATH} is missing: build it with `python -m micrographia_amd.build` "
                 "(the engine has no CPU fallback)"
             )
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (same SONAME), and a
+        # process that loads this library first and torch later ends up with two runtimes, the
+        # second of which sees no device.  Loading torch first makes this library bind to the
+        # runtime torch already mapped.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -193,6 +193,9 @@ int rh_engine3_list_stats(rh_engine3* e, rh_list_stats* out) {
 int rh_engine3_set_sort_interval(rh_engine3* e, uint32_t interval) {
     return guarded([&] { eng(e)->set_sort_interval(interval); });
 }
+int rh_engine3_tune(rh_engine3* e, int variant, uint32_t block) {
+    return guarded([&] { eng(e)->tune(variant, block); });
+}
 int rh_engine3_stream(rh_engine3* e, void** stream) {
     return guarded([&] { *stream = (void*)eng(e)->stream(); });
 }
@@ -245,6 +248,9 @@ int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke) {
 }
 int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out) {
     return guarded([&] { *out = eng(e)->list_stats(); });
+}
+int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block) {
+    return guarded([&] { eng(e)->tune(variant, block); });
 }
 int rh_engine2_stream(rh_engine2* e, void** stream) {
     return guarded([&] { *stream = (void*)eng(e)->stream(); });

@@ -52,6 +52,10 @@ struct Params {
     float kick_c[4];
     int32_t gravity[3];
     float half_skin2;
+    uint32_t block;        // half-list variant: particles per work-group block
+    uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
+    float cell_w[3];       // cell width (sigma), f32
+    double cell_w64[3];    // cell width (sigma)
     PairTable pt;
 };
 
@@ -62,6 +66,7 @@ struct Bufs {
     uint32_t* species[2];
     uint32_t* ids[2];
     int64_t* ref[D];     // positions at the last list build
+    float* rel[D];       // position relative to its cell's corner (sigma, f32), for list builds
     uint32_t* nbr;       // column-major: entry k of slot i at k*n + i
     uint32_t* nbr_count;
     uint32_t* cell_start;
@@ -111,6 +116,12 @@ class Engine {
     void set_timing(bool on) { timing_ = on; }
     void kernel_ms(double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds) const;
     void set_sort_interval(uint32_t v) { sort_interval_ = v; }
+    // Kernel variant: 0 full Verlet list (one lane per particle, both ends of every pair), 1 half
+    // list within work-group blocks of `block` particles (pairs in a block once, the partner's kick
+    // accumulated in LDS) plus the full list across blocks.
+    void tune(int variant, uint32_t block);
+    int variant() const { return variant_; }
+    uint32_t block() const { return P_.block; }
 
    private:
     void build_params(const SystemDesc& sys, const rh_mode& mode);
@@ -127,6 +138,7 @@ class Engine {
     void record_timer(bool rebuild, hipEvent_t start);
     hipEvent_t timer_event();
     void collect_timers();
+    void launch_step();
 
     SystemDesc sys_;
     rh_mode mode_;
@@ -159,6 +171,8 @@ class Engine {
     double force_ms_ = 0.0, rebuild_ms_ = 0.0;
     uint64_t force_launches_ = 0, rebuild_launches_ = 0;
     uint32_t sort_interval_ = 100;
+    int variant_ = 1;
+    uint32_t block_ = 512;
     std::vector<void*> allocs_;
     void* dalloc(size_t bytes);
 };

@@ -274,13 +274,19 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
     if (k >= P.n) return;
     uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
     uint32_t i = perm[k];
+    int64_t xs[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
         int64_t x = B.pos[pp][a][i];
+        xs[a] = x;
         B.pos[pp ^ 1][a][k] = x;
         if (write_ref) B.ref[a][k] = x;
         B.vel[pm ^ 1][a][k] = B.vel[pm][a][i];
     }
+    uint32_t c[D];
+    cell_of<D>(P, xs, c);
+#pragma unroll
+    for (int a = 0; a < D; a++) B.rel[a][k] = (float)((double)xs[a] * 0x1p-32 - (double)c[a] * P.cell_w64[a]);
     B.species[pm ^ 1][k] = B.species[pm][i];
     B.ids[pm ^ 1][k] = B.ids[pm][i];
 }
@@ -376,6 +382,237 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     }
     if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
     // leapfrog: v += kick + g, x += v
+    const bool pin = (P.pinned >> si) & 1u;
+    int32_t v[D];
+    int64_t x[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + k[a] + (pin ? 0u : (uint32_t)P.gravity[a]));
+        x[a] = xi[a] + (int64_t)v[a];
+    }
+    bound<D>(P, x, v);
+    float r2 = 0.0f;
+    bool far = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        B.pos[pp ^ 1][a][i] = x[a];
+        B.vel[pm][a][i] = v[a];
+        float d;
+        far |= !sep_f(separation<D>(P, B.ref[a][i], x[a], a), d);
+        r2 += d * d;
+    }
+    bool stale = far || r2 >= P.half_skin2;
+    uint64_t bal = ballot64(stale);
+    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Half-list variant
+// ---------------------------------------------------------------------------------------------
+
+// Which end of an in-block pair lists it: alternate by the parity of i + j so every particle
+// owns about half of its in-block neighbours whatever its place in the block.
+__device__ __forceinline__ bool owns(uint32_t i, uint32_t j) {
+    uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    return (((i ^ j) & 1u) ? lo : hi) == i;
+}
+
+// List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
+// ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
+// every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
+// not wrap the stencil.  In-block pairs go to the owner's list from the front; pairs across
+// blocks go to both ends' lists from the back.
+template <int D, bool MULTI>
+__global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
+    const uint32_t* sp = B.species[pm];
+    int64_t xi[D];
+    float ri[D];
+    for (int a = 0; a < D; a++) {
+        xi[a] = B.pos[pp][a][i];
+        ri[a] = B.rel[a][i];
+    }
+    uint32_t si = MULTI ? sp[i] : 0u;
+    const float list_r2 = B.pt->list_r2[0];
+    uint32_t c[D];
+    cell_of<D>(P, xi, c);
+    const uint32_t cap = P.cap, n = P.n, blk = i / P.block;
+    uint32_t na = 0, nb = 0;
+    for (int o = 0; o < stencil_size(D); o++) {
+        int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+        uint32_t cc[D];
+        float shift[D];
+        bool skip = false;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            int v = (int)c[a] + off[a];
+            if (v < 0 || v >= (int)P.nc[a]) {
+                if (!((P.periodic >> a) & 1u)) skip = true;
+                v = v < 0 ? v + (int)P.nc[a] : v - (int)P.nc[a];
+            }
+            cc[a] = (uint32_t)v;
+            shift[a] = (float)off[a] * P.cell_w[a] - ri[a];
+        }
+        if (skip) continue;
+        uint32_t key = morton<D>(cc);
+        uint32_t s = B.cell_start[key], e = B.cell_end[key];
+        for (uint32_t j = s; j < e; j++) {
+            if (j == i) continue;
+            float r2 = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                float d = B.rel[a][j] + shift[a];
+                r2 = __builtin_fmaf(d, d, r2);
+            }
+            float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
+            if (r2 < lr2) {
+                if (j / P.block == blk) {
+                    if (owns(i, j)) {
+                        if (na + nb < cap) B.nbr[(size_t)na * n + i] = j;
+                        na++;
+                    }
+                } else {
+                    if (na + nb < cap) B.nbr[(size_t)(cap - 1 - nb) * n + i] = j;
+                    nb++;
+                }
+            }
+        }
+    }
+    uint32_t count = na + nb;
+    if (count > cap) {
+        atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
+        na = nb = 0;
+    }
+    atomicMax(&B.ctl[CTL_MAXCOUNT], count);
+    B.nbr_count[i] = na | (nb << 16);
+}
+
+// Kicks of one pair on both ends: returns k_i, adds k_j into kj.  Same operations as pair_kick;
+// the partner's kick is rounded separately with its own factor (int_engine.rs:17-22): for equal
+// factors it is the conversion of -rint(a f), the exact negation of this end's rounded value.
+template <int D, bool MULTI>
+__device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
+    float r2 = f[0] * f[0] + f[1] * f[1];
+    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+#pragma unroll
+    for (int ax = 0; ax < D; ax++) kj[ax] = 0;
+    if (r2 < rc2) {
+        float s2 = sigma2 / r2;
+        float s6 = s2 * s2 * s2;
+        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float a = -fr * ci;
+        if constexpr (MULTI) {
+            float aj = -fr * cj;
+#pragma unroll
+            for (int ax = 0; ax < D; ax++) {
+                ki[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
+                kj[ax] = cvt_sat_i32(__builtin_rintf(aj * -f[ax]));
+            }
+        } else {
+#pragma unroll
+            for (int ax = 0; ax < D; ax++) {
+                float r = __builtin_rintf(a * f[ax]);
+                ki[ax] += (uint32_t)cvt_sat_i32(r);
+                kj[ax] = cvt_sat_i32(-r);
+            }
+        }
+    }
+}
+
+template <int D, bool MULTI, int BS>
+__global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
+    const uint32_t* ctl = B.ctl;
+    if (!step_runs(ctl)) return;
+    __shared__ uint32_t acc[D][BS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t base = blockIdx.x * BS;
+    const uint32_t i = base + t;
+#pragma unroll
+    for (int a = 0; a < D; a++) acc[a][t] = 0u;
+    __syncthreads();
+    const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
+    const int64_t* px[D];
+    for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
+    const uint32_t* sp = B.species[pm];
+    const PairTable* __restrict__ pt = B.pt;
+    const bool live = i < P.n;
+    int64_t xi[D];
+    uint32_t k[D];
+    uint32_t si = 0;
+    float c = P.kick_c[0];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        xi[a] = live ? px[a][i] : 0;
+        k[a] = 0u;
+    }
+    if (live) {
+        if (MULTI) {
+            si = sp[i];
+            c = pt->kick_c[si];
+        }
+        const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0];
+        const uint32_t cnt = B.nbr_count[i];
+        const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
+        const uint32_t* col = B.nbr + i;
+        const size_t stride = P.n;
+        const bool rc = P.range_check != 0;
+        // in-block pairs: both ends
+        uint32_t j_next = na ? col[0] : 0u;
+        for (uint32_t e = 0; e < na; e++) {
+            const uint32_t j = j_next;
+            if (e + 1 < na) j_next = col[(size_t)(e + 1) * stride];
+            int64_t xj[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) xj[a] = px[a][j];
+            float f[D];
+            bool ok = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
+            if (rc && !ok) continue;
+            int32_t kj[D];
+            if constexpr (MULTI) {
+                uint32_t sj = sp[j];
+                uint32_t p = si * 4 + sj;
+                pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], c, pt->kick_c[sj], k, kj);
+            } else {
+                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, c, c, k, kj);
+            }
+            const uint32_t lj = j - base;
+#pragma unroll
+            for (int a = 0; a < D; a++)
+                if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+        }
+        // pairs across blocks: this end only
+        const uint32_t* colb = col + (size_t)(P.cap - 1) * stride;
+        j_next = nb ? colb[0] : 0u;
+        for (uint32_t e = 0; e < nb; e++) {
+            const uint32_t j = j_next;
+            if (e + 1 < nb) j_next = colb[-(ptrdiff_t)((size_t)(e + 1) * stride)];
+            int64_t xj[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) xj[a] = px[a][j];
+            float f[D];
+            bool ok = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
+            if (rc && !ok) continue;
+            if constexpr (MULTI) {
+                uint32_t p = si * 4 + sp[j];
+                pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], c, k);
+            } else {
+                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, c, k);
+            }
+        }
+        if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
+    }
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int a = 0; a < D; a++) k[a] += acc[a][t];
     const bool pin = (P.pinned >> si) & 1u;
     int32_t v[D];
     int64_t x[D];
@@ -618,6 +855,15 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         }
     for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
     P.cap = mode.kind == 1 ? mode.cap : 0;
+    RH_REQUIRE(P.cap < 65536, "list capacity must be below 65536");
+    for (int a = 0; a < D; a++) {
+        P.cell_w64[a] = b[a] / (double)P.nc[a];
+        P.cell_w[a] = (float)P.cell_w64[a];
+    }
+    P.block = block_;
+    // Listed pairs stay within (list radius + skin) of each other while the list is valid; below
+    // 8 sigma the reference's range test cannot fail and is skipped.
+    P.range_check = ((rcm + skin) * (1.0 + 1e-4) + skin >= 7.99) ? 1u : 0u;
 }
 
 template <int D>
@@ -697,6 +943,7 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     for (int a = 0; a < D; a++) {
         B_.ref[a] = (int64_t*)dalloc(n * sizeof(int64_t));
         B_.kicks[a] = (int32_t*)dalloc(n * sizeof(int32_t));
+        B_.rel[a] = (float*)dalloc(n * sizeof(float));
     }
     B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
     B_.cell_start = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
@@ -813,8 +1060,14 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     k_ranges<<<nb, TPB, 0, stream_>>>(sort_.keys[r], (uint32_t)n_, B_.cell_start, B_.cell_end, gate);
     RH_HIP(hipGetLastError());
     if (mode_.kind == 1) {
-        if (P_.nspecies > 1) k_build_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
-        else k_build_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+        bool multi = P_.nspecies > 1;
+        if (variant_ == 1) {
+            if (multi) k_build_half<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            else k_build_half<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+        } else {
+            if (multi) k_build_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            else k_build_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+        }
         RH_HIP(hipGetLastError());
     }
     if (timing_) record_timer(true, t0);
@@ -867,9 +1120,7 @@ void Engine<D>::step_list(uint64_t steps) {
                         t0 = timer_event();
                         RH_HIP(hipEventRecord(t0, stream_));
                     }
-                    if (P_.nspecies > 1) k_step_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_);
-                    else k_step_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_);
-                    RH_HIP(hipGetLastError());
+                    launch_step();
                     if (timing_) record_timer(false, t0);
                     k_step_commit<<<1, 1, 0, stream_>>>(B_.ctl);
                     RH_HIP(hipGetLastError());
@@ -914,6 +1165,42 @@ void Engine<D>::step_list(uint64_t steps) {
             }
         }
     }
+}
+
+template <int D>
+void Engine<D>::launch_step() {
+    const bool multi = P_.nspecies > 1;
+    if (variant_ == 0) {
+        uint32_t nb = blocks_for(n_, TPB);
+        if (multi) k_step_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_);
+        else k_step_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_);
+    } else {
+        uint32_t bs = P_.block;
+        uint32_t nb = blocks_for(n_, bs);
+#define RH_STEP_HALF(BS)                                                              \
+    if (multi) k_step_half<D, true, BS><<<nb, BS, 0, stream_>>>(P_, B_);              \
+    else k_step_half<D, false, BS><<<nb, BS, 0, stream_>>>(P_, B_);
+        if (bs == 256) {
+            RH_STEP_HALF(256)
+        } else if (bs == 512) {
+            RH_STEP_HALF(512)
+        } else {
+            RH_STEP_HALF(1024)
+        }
+#undef RH_STEP_HALF
+    }
+    RH_HIP(hipGetLastError());
+}
+
+template <int D>
+void Engine<D>::tune(int variant, uint32_t block) {
+    RH_REQUIRE(variant == 0 || variant == 1, "kernel variant must be 0 (full list) or 1 (half list)");
+    RH_REQUIRE(block == 256 || block == 512 || block == 1024, "block must be 256, 512 or 1024");
+    RH_HIP(hipStreamSynchronize(stream_));
+    variant_ = variant;
+    block_ = block;
+    P_.block = block;
+    write_ctl(CTL_STALE, 1u);
 }
 
 template <int D>
@@ -1041,6 +1328,8 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     for (int a = 0; a < 3; a++) {
         P_.nc[a] = old.nc[a];
         P_.cell_scale[a] = old.cell_scale[a];
+        P_.cell_w[a] = old.cell_w[a];
+        P_.cell_w64[a] = old.cell_w64[a];
     }
     P_.mbits = old.mbits;
     P_.cap = old.cap;

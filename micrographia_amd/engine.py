@@ -151,6 +151,10 @@ class GpuEngine3:
         check(load().rh_engine3_list_stats(self._h, C.byref(s)))
         return _stats(s)
 
+    def tune(self, variant: int, block: int = 512) -> None:
+        """Kernel variant (0 full list, 1 half list in blocks of `block`); trajectories are identical."""
+        check(load().rh_engine3_tune(self._h, int(variant), int(block)))
+
     def stream(self) -> int:
         v = C.c_void_p()
         check(load().rh_engine3_stream(self._h, C.byref(v)))
@@ -232,6 +236,9 @@ class GpuEngine:
         s = _lib.CListStats()
         check(load().rh_engine2_list_stats(self._h, C.byref(s)))
         return _stats(s)
+
+    def tune(self, variant: int, block: int = 512) -> None:
+        check(load().rh_engine2_tune(self._h, int(variant), int(block)))
 
     def stream(self) -> int:
         v = C.c_void_p()

@@ -1,6 +1,9 @@
 import os
 import sys
 
+# Import torch before the engine library so both share torch's HIP runtime (see _lib.load).
+import torch  # noqa: F401,E402
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

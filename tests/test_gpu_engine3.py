@@ -58,6 +58,16 @@ MODES = {
     "verlet": lambda sys, n: rh.verlet3_with_skin(sys, n, 0.3),
 }
 
+# (kernel variant, block): the full list, and the half list at each block size
+VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024)]
+
+
+def engine(sys, st, mode, variant=1, block=512):
+    e = rh.GpuEngine3(sys, st, mode)
+    if mode.kind == 1:
+        e.tune(variant, block)
+    return e
+
 
 @pytest.mark.parametrize("mode", ["cells", "verlet"])
 def test_kicks_are_bit_exact_with_the_f32_twin(mode):
@@ -78,10 +88,13 @@ def test_kicks_are_bit_exact_with_the_f32_twin(mode):
 
 
 @pytest.mark.parametrize("mode", ["cells", "verlet"])
-def test_trajectory_is_bit_exact_with_the_f32_twin(mode):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_trajectory_is_bit_exact_with_the_f32_twin(mode, variant):
     """three.rs:66-80 asks for 1e-5 sigma after 50 steps; integer state here is identical."""
+    if mode == "cells" and variant != VARIANTS[0]:
+        pytest.skip("the variant applies to Verlet lists only")
     sys, s0 = liquid(2000, 0.8, 1.0, walls=True)
-    gpu = rh.GpuEngine3(sys, s0, MODES[mode](sys, s0.n()))
+    gpu = engine(sys, s0, MODES[mode](sys, s0.n()), *variant)
     gpu.step(200)
     want = oracle_run(sys, s0, 200)
     got = gpu.read_state()
@@ -89,9 +102,10 @@ def test_trajectory_is_bit_exact_with_the_f32_twin(mode):
     assert got.hash() == want.hash()
 
 
-def test_periodic_liquid_bit_exact_over_many_rebuilds():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_periodic_liquid_bit_exact_over_many_rebuilds(variant):
     sys, s0 = liquid(4000, 0.8442, 1.2)
-    gpu = rh.GpuEngine3(sys, s0, rh.verlet3_with_skin(sys, 4000, 0.3))
+    gpu = engine(sys, s0, rh.verlet3_with_skin(sys, 4000, 0.3), *variant)
     gpu.step(300)
     stats = gpu.read_list_stats()
     assert stats.rebuilds >= 10 and not stats.overflowed
@@ -169,9 +183,10 @@ def test_storage_permutation_does_not_change_the_trajectory():
     assert np.array_equal(ra.pos[perm], rb.pos) and np.array_equal(ra.vel[perm], rb.vel)
 
 
-def test_a_tiny_list_capacity_regrows_and_stays_exact():
+@pytest.mark.parametrize("variant", [(0, 256), (1, 512)])
+def test_a_tiny_list_capacity_regrows_and_stays_exact(variant):
     sys, s = liquid(3000, 0.8442, 1.2)
-    gpu = rh.GpuEngine3(sys, s, rh.ForceMode(1, 0.3, 12))
+    gpu = engine(sys, s, rh.ForceMode(1, 0.3, 12), *variant)
     gpu.step(150)
     stats = gpu.read_list_stats()
     assert stats.cap > 12
@@ -188,10 +203,12 @@ def test_mixture_pinned_gravity_soft_walls_match_the_twin():
     s.species[::7] = 2
     s.pos[:, 2] = np.clip(s.pos[:, 2], ONE, sys.box_len[2] - ONE)
     s.vel = rh.maxwell(sys, s.species, 1.0, 5)
-    for mode in (rh.ForceMode.cell_list(), rh.verlet3_with_skin(sys, n, 0.3)):
-        gpu = rh.GpuEngine3(sys, s, mode)
+    want = oracle_run(sys, s, 150)
+    for mode, variant in ((rh.ForceMode.cell_list(), (0, 256)), (rh.verlet3_with_skin(sys, n, 0.3), (0, 256)),
+                          (rh.verlet3_with_skin(sys, n, 0.3), (1, 256)), (rh.verlet3_with_skin(sys, n, 0.3), (1, 1024))):
+        gpu = engine(sys, s, mode, *variant)
         gpu.step(150)
-        assert gpu.read_state() == oracle_run(sys, s, 150)
+        assert gpu.read_state() == want
 
 
 def test_set_species_changes_the_chosen_particles():
