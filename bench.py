@@ -57,15 +57,15 @@ def pairs_per_atom(density: float, rc: float = 2.5) -> float:
     return 4.0 / 3.0 * math.pi * rc ** 3 * density
 
 
-def cpu_baseline(sys_, state, budget_s: float = 20.0):
-    """The oracle (C restatement of md-ref's IntEngine3<f32>), one thread, on this host: as many
-    whole steps of the same 4M-atom system as fit the time budget (at least one)."""
-    import numpy as np
-
+def cpu_baseline(n: int, density: float, t: float, seed: int, budget_s: float = 20.0):
+    """The oracle (C restatement of md-ref's IntEngine3<f32>), one thread, on this host, on the
+    same liquid (density, temperature, cutoff) at n atoms: as many whole steps as fit the time
+    budget (at least one).  One step of the 4M-atom system alone takes ~50 s on one core."""
     from oracle import oracle as O
 
     if not os.path.exists(O.LIB):
         O.build()
+    sys_, state = make_liquid(n, density, t, seed)
     osys = O.system3(sys_.box_len)
     pos, vel, sp = state.pos.copy(), state.vel.copy(), state.species.copy()
     steps, t0 = 0, time.perf_counter()
@@ -81,7 +81,7 @@ def cpu_baseline(sys_, state, budget_s: float = 20.0):
         "unit": "atom-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} step(s) of the same N={n} system with the oracle's IntEngine3<f32> (cell list, one thread), {el:.1f} s",
+        "sample": f"{steps} step(s) of the same liquid at N={n} (rho {density}, T {t}, rc 2.5) with the oracle's IntEngine3<f32> (md-ref's cell-list loop, one thread), {el:.1f} s",
         "ns_per_particle_step": el / (n * steps) * 1e9,
         "host_cpu": _cpu_model(),
         "host_nproc": os.cpu_count(),
@@ -110,6 +110,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-n", type=int, default=262144, help="atoms in the CPU-baseline sample")
+    ap.add_argument("--variant", type=int, default=1, help="0 full Verlet list, 1 half list in blocks")
+    ap.add_argument("--block", type=int, default=512, help="half-list block size (256, 512, 1024)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,9 +120,9 @@ def main() -> int:
               "independent systems across GPUs by construction and is measured per GPU.", file=sys.stderr)
         return 2
 
-    ensure_built()
-    import torch
+    import torch  # before the engine library: one HIP runtime per process
 
+    ensure_built()
     import micrographia_amd as rh
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -129,7 +132,8 @@ def main() -> int:
     log(f"[bench] init {time.perf_counter() - t0:.1f} s; creating engine")
     mode = rh.verlet3_with_skin(sys_, args.n, args.skin)
     eng = rh.GpuEngine3(sys_, state, mode)
-    log(f"[bench] warmup {args.warmup} steps")
+    eng.tune(args.variant, args.block)
+    log(f"[bench] warmup {args.warmup} steps (variant {args.variant}, block {args.block})")
     eng.step(args.warmup)
     torch.cuda.synchronize()
     s0 = eng.read_list_stats()
@@ -181,10 +185,13 @@ def main() -> int:
             "rc": 2.5,
             "skin": args.skin,
             "dt": 0.005,
+            "kernel_variant": args.variant,
+            "block": args.block,
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "k_step_list (forces over the list + leapfrog + skin check)",
+            "kernel": ("k_step_half (half list in blocks of %d, LDS partner kicks; leapfrog; skin check)" % args.block)
+            if args.variant == 1 else "k_step_list (full list; leapfrog; skin check)",
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -204,7 +211,7 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, one thread)")
-        out["cpu_baseline"] = cpu_baseline(sys_, state, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.density, args.temperature, args.seed, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -33,7 +33,8 @@ def test_exclusive_scan(n):
     x = torch.from_numpy(a.view(np.int32)).cuda()
     y = torch.empty_like(x)
     _lib.check(_lib.load().rh_prims_exclusive_scan_u32(dptr(x), dptr(y), n, None))
-    want = (np.concatenate([[0], np.cumsum(a.astype(np.uint64))[:-1]]) & 0xFFFFFFFF).astype(np.uint32)
+    csum = np.cumsum(a.astype(np.uint64))
+    want = (np.concatenate([np.zeros(1, np.uint64), csum[:-1]]) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     assert np.array_equal(y.cpu().numpy().view(np.uint32), want)
 
 
