@@ -165,6 +165,36 @@ int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out);
 int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block);
 int rh_engine2_stream(rh_engine2* e, void** stream);
 
+/* ---- the batched engine (no reference counterpart: the reference runs one system per engine;
+ * its 2D "compartments", crates/md-core/src/system.rs:41-48, are the nearest prior art) ----
+ * Independent 2D and 3D systems of any sizes share one device arena and advance together, one
+ * launch per kernel for all of them, in hipGraph-captured chunks.  Systems may be added and
+ * removed between runs (continuous batching).  Each system's trajectory is bit-identical to
+ * running it alone in rh_engine3 / rh_engine2. */
+typedef struct rh_batch rh_batch;
+/* budget_bytes: device memory for the arenas (0: half the free memory); max_systems: slots. */
+int rh_batch_create(uint64_t budget_bytes, uint32_t max_systems, rh_batch** out);
+int rh_batch_destroy(rh_batch* b);
+/* Admits a system with `steps` steps to run, Verlet skin `skin`; *id = -1 when it does not fit
+ * in the free arena space now (the admission rule). */
+int rh_batch_add3(rh_batch* b, const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id);
+int rh_batch_add2(rh_batch* b, const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id);
+int rh_batch_remove(rh_batch* b, int32_t id);
+int rh_batch_set_steps(rh_batch* b, int32_t id, uint64_t steps);
+/* Enqueues `iterations` iterations (every unfinished system takes one step per iteration). */
+int rh_batch_run(rh_batch* b, uint64_t iterations);
+int rh_batch_sync(rh_batch* b);
+/* Synchronises; remaining[id] = steps left for each id < count (0 for unused ids). */
+int rh_batch_poll(rh_batch* b, uint64_t* remaining, uint32_t count);
+int rh_batch_read_state(rh_batch* b, int32_t id, int64_t* pos, int32_t* vel, uint32_t* species);
+int rh_batch_list_stats(rh_batch* b, int32_t id, rh_list_stats* out);
+/* Iterations per captured hipGraph (0: launch directly). */
+int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk);
+int rh_batch_stream(rh_batch* b, void** stream);
+int rh_batch_capacity(rh_batch* b, uint64_t* slots, uint64_t* list_entries, uint64_t* cells, uint32_t* active);
+int rh_batch_bytes_needed3(rh_batch* b, const rh_system3* sys, uint64_t n, double skin, uint64_t* bytes);
+int rh_batch_bytes_needed2(rh_batch* b, const rh_system2* sys, uint64_t n, double skin, uint64_t* bytes);
+
 /* ---- gpu-prims on device buffers (crates/gpu-prims/src/prims.rs:116-312, threefry.rs) ----
  * Pointers are device memory; stream may be NULL (the null stream).  These run synchronously
  * with respect to the host (they synchronise the stream before returning). */

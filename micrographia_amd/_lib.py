@@ -132,6 +132,22 @@ SIGNATURES = {
     "rh_engine2_list_stats": ([P, P], i32),
     "rh_engine2_tune": ([P, i32, u32], i32),
     "rh_engine2_stream": ([P, P], i32),
+    "rh_batch_create": ([u64, u32, P], i32),
+    "rh_batch_destroy": ([P], i32),
+    "rh_batch_add3": ([P, P, u64, P, P, P, dbl, u64, P], i32),
+    "rh_batch_add2": ([P, P, u64, P, P, P, dbl, u64, P], i32),
+    "rh_batch_remove": ([P, C.c_int32], i32),
+    "rh_batch_set_steps": ([P, C.c_int32, u64], i32),
+    "rh_batch_run": ([P, u64], i32),
+    "rh_batch_sync": ([P], i32),
+    "rh_batch_poll": ([P, P, u32], i32),
+    "rh_batch_read_state": ([P, C.c_int32, P, P, P], i32),
+    "rh_batch_list_stats": ([P, C.c_int32, P], i32),
+    "rh_batch_set_graph_chunk": ([P, u32], i32),
+    "rh_batch_stream": ([P, P], i32),
+    "rh_batch_capacity": ([P, P, P, P, P], i32),
+    "rh_batch_bytes_needed3": ([P, P, u64, dbl, P], i32),
+    "rh_batch_bytes_needed2": ([P, P, u64, dbl, P], i32),
     "rh_prims_exclusive_scan_u32": ([P, P, u64, P], i32),
     "rh_prims_radix_sort_u32": ([P, P, u64, u32, P], i32),
     "rh_prims_reduce_sum_f64": ([P, P, u64, P], i32),

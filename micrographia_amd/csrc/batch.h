@@ -1,0 +1,134 @@
+// Batched engine: many independent systems (2D and 3D, any sizes) share one device arena and
+// advance together, one launch per kernel for all of them.  Systems join and leave between runs
+// (continuous batching): their slot, list and cell ranges come from first-fit allocators over
+// arenas sized by a device-memory budget, which is also the admission rule.
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "engine.h"
+
+namespace rh {
+
+constexpr uint32_t BATCH_WG = 256;  // slots per work-group in every batched kernel (and the half-list block)
+
+// Per-system device descriptor.
+struct BSys {
+    uint32_t active, dim, n, slot0, nslots;
+    uint32_t nc[3];
+    uint32_t ncells, cell0;
+    uint32_t periodic, nspecies, pinned, soft_walls, range_check, cap;
+    uint64_t nbr0;  // list entry k of local slot i at nbr0 + k * nslots + i
+    int64_t box[3], half[3];
+    double cell_scale[3], cell_w64[3];
+    float cell_w[3];
+    float kick_c[4];
+    int32_t gravity[3];
+    float half_skin2;
+    PairTable pt;
+};
+
+// Per-system control words (device).
+struct BCtl {
+    uint32_t stale, stale_next, overflow, remaining, ppos, pmeta, do_rebuild, maxcount;
+    uint32_t clock, rebuilds, done_lo, done_hi, skipped, pad0, pad1, pad2;
+};
+
+struct BBufs {
+    int64_t* pos[2][3];
+    int32_t* vel[2][3];
+    uint32_t* species[2];
+    uint32_t* ids[2];
+    int64_t* ref[3];
+    float* rel[3];
+    uint32_t* key;
+    uint32_t* rank;
+    uint32_t* nbr;
+    uint32_t* nbr_count;
+    uint32_t* cell_count;
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    const uint32_t* wg_sys;  // system of each BATCH_WG-slot work-group (~0u: none)
+    const BSys* sys;
+    BCtl* ctl;
+};
+
+// First-fit allocator over [0, capacity) in units; ranges coalesce on free.
+class RangeAlloc {
+   public:
+    explicit RangeAlloc(uint64_t capacity = 0) { reset(capacity); }
+    void reset(uint64_t capacity);
+    bool alloc(uint64_t size, uint64_t* start);
+    void free(uint64_t start, uint64_t size);
+    uint64_t capacity() const { return cap_; }
+    uint64_t used() const { return used_; }
+    uint64_t high_water() const;  // end of the highest allocated range
+
+   private:
+    std::map<uint64_t, uint64_t> free_;  // start -> size
+    std::map<uint64_t, uint64_t> taken_;
+    uint64_t cap_ = 0, used_ = 0;
+};
+
+struct BatchMember {
+    SystemDesc sys;
+    uint64_t n;
+    uint32_t slot0, nslots, cell0, ncells, cap;
+    uint64_t nbr0, nbr_entries;
+    double skin;
+};
+
+class Batch {
+   public:
+    // budget_bytes: device memory the arenas may use (0: a share of the free memory).
+    Batch(uint64_t budget_bytes, uint32_t max_systems);
+    ~Batch();
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    // Admits a system if its ranges fit now; returns its id, or -1 (not admitted).
+    int add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps);
+    void remove(int id);
+    void set_steps(int id, uint64_t steps);
+    void run(uint64_t iterations);
+    void sync();
+    // Copies every system's control words to the host (synchronises); returns remaining steps.
+    void poll(uint64_t* remaining, uint32_t count);
+    void read_state(int id, int64_t* pos, int32_t* vel, uint32_t* species);
+    rh_list_stats stats(int id);
+    void set_graph_chunk(uint32_t chunk) { graph_chunk_ = chunk; }
+    hipStream_t stream() const { return stream_; }
+    uint64_t bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double skin) const;
+    uint64_t free_slots() const { return slots_.capacity() - slots_.used(); }
+    uint32_t active() const { return (uint32_t)members_.size(); }
+    void budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) const;
+
+   private:
+    void plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const;
+    void upload_tables();
+    void enqueue_iteration();
+    void handle_overflow();
+    void invalidate_graph();
+
+    hipStream_t stream_ = nullptr;
+    BBufs B_;
+    BSys* d_sys_ = nullptr;
+    BCtl* d_ctl_ = nullptr;
+    uint32_t* d_wg_ = nullptr;
+    uint32_t max_sys_;
+    RangeAlloc slots_, lists_, cells_;  // units: BATCH_WG slots, LIST_UNIT entries, CELL_UNIT cells
+    std::map<int, BatchMember> members_;
+    std::vector<BSys> h_sys_;
+    std::vector<int> free_ids_;
+    uint32_t grid_wgs_ = 0;  // work-groups covering the slot high-water mark
+    bool tables_dirty_ = true;
+    uint32_t graph_chunk_ = 100;
+    hipGraphExec_t graph_ = nullptr;
+    uint32_t graph_wgs_ = 0, graph_len_ = 0;
+    BCtl* h_ctl_ = nullptr;
+    std::vector<void*> allocs_;
+    void* dalloc(size_t bytes);
+};
+
+}  // namespace rh

@@ -1,0 +1,862 @@
+// Batched engine (see batch.h).  One iteration, for every active system at once:
+//   k_bb_decide          per system: rebuild now? (stale list, steps left, no overflow)
+//   k_bb_count           per slot of a rebuilding system: linear cell, rank in the cell (atomic)
+//   k_bb_scan            per system (one work-group): cell counts -> cell ranges, counts cleared
+//   k_bb_scatter         per slot: move to its cell-sorted slot in the other buffers
+//   k_bb_build           per slot: half list inside 256-slot blocks + full list across blocks
+//   k_bb_commit_rebuild  per system
+//   k_bb_step            per slot: forces, leapfrog, half-skin check (every active system)
+//   k_bb_commit_step     per system
+// Every system takes exactly one step per iteration (its rebuild, when due, runs first in the
+// same iteration), so the host enqueues iterations without reading anything back and a chunk of
+// iterations can be captured once as a hipGraph and replayed.  Systems never interact, and each
+// particle's kick is the same integer sum as in the single-system engine, so every system's
+// trajectory is bit-identical to running it alone.
+#include "batch.h"
+
+#include <algorithm>
+#include <cmath>
+
+#include "md_device.h"
+
+namespace rh {
+
+constexpr uint32_t LIST_UNIT = 4096;  // list-arena allocation unit (entries)
+constexpr uint32_t CELL_UNIT = 256;   // cell-arena allocation unit (cells)
+constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 12 + 8 + 4;
+
+// ---------------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------------
+
+template <int D>
+__device__ __forceinline__ uint32_t lin_cell(const BSys& S, const uint32_t c[D]) {
+    if constexpr (D == 3) return (c[2] * S.nc[1] + c[1]) * S.nc[0] + c[0];
+    else return c[1] * S.nc[0] + c[0];
+}
+
+__global__ void k_bb_decide(const BSys* S, BCtl* C, uint32_t nsys) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    BCtl& c = C[s];
+    uint32_t go = (S[s].active && c.stale && c.remaining && !c.overflow) ? 1u : 0u;
+    c.do_rebuild = go;
+    if (go) c.maxcount = 0u;
+}
+
+template <int D>
+__device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C, uint32_t slot) {
+    int64_t x[D];
+    for (int a = 0; a < D; a++) x[a] = B.pos[C.ppos][a][slot];
+    uint32_t c[D];
+    cell_of<D>(S, x, c);
+    uint32_t key = lin_cell<D>(S, c);
+    B.key[slot] = key;
+    B.rank[slot] = atomicAdd(&B.cell_count[S.cell0 + key], 1u);
+}
+
+__global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B) {
+    uint32_t s = B.wg_sys[blockIdx.x];
+    if (s == ~0u) return;
+    const BSys& S = B.sys[s];
+    const BCtl& C = B.ctl[s];
+    if (!C.do_rebuild) return;
+    uint32_t slot = blockIdx.x * BATCH_WG + threadIdx.x;
+    if (slot - S.slot0 >= S.n) return;
+    if (S.dim == 3) bb_count<3>(B, S, C, slot);
+    else bb_count<2>(B, S, C, slot);
+}
+
+__device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, WAVE);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == WAVE - 1) lds[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < (int)(BATCH_WG / WAVE); k++) {
+        uint32_t s = lds[k];
+        pre += (k < (int)w) ? s : 0u;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(BATCH_WG) void k_bb_scan(BBufs B, uint32_t nsys) {
+    uint32_t s = blockIdx.x;
+    if (s >= nsys) return;
+    const BSys& S = B.sys[s];
+    if (!S.active || !B.ctl[s].do_rebuild) return;
+    __shared__ uint32_t lds[BATCH_WG / WAVE];
+    uint32_t carry = 0;
+    uint32_t* cnt = B.cell_count + S.cell0;
+    uint32_t* st = B.cell_start + S.cell0;
+    uint32_t* en = B.cell_end + S.cell0;
+    for (uint32_t base = 0; base < S.ncells; base += 4 * BATCH_WG) {
+        uint32_t c0 = base + 4 * threadIdx.x;
+        uint32_t v[4];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = c0 + k < S.ncells ? cnt[c0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t ex = carry + wg_exclusive_scan(sum, lds, &total);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (c0 + k < S.ncells) {
+                st[c0 + k] = ex;
+                en[c0 + k] = ex + v[k];
+                cnt[c0 + k] = 0u;
+            }
+            ex += v[k];
+        }
+        carry += total;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& C, uint32_t slot) {
+    const uint32_t pp = C.ppos, pm = C.pmeta;
+    uint32_t key = B.key[slot];
+    uint32_t ns = S.slot0 + B.cell_start[S.cell0 + key] + B.rank[slot];
+    uint32_t c[3] = {key % S.nc[0], (key / S.nc[0]) % S.nc[1], D == 3 ? key / (S.nc[0] * S.nc[1]) : 0u};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int64_t x = B.pos[pp][a][slot];
+        B.pos[pp ^ 1][a][ns] = x;
+        B.ref[a][ns] = x;
+        B.vel[pm ^ 1][a][ns] = B.vel[pm][a][slot];
+        B.rel[a][ns] = (float)((double)x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
+    }
+    B.species[pm ^ 1][ns] = B.species[pm][slot];
+    B.ids[pm ^ 1][ns] = B.ids[pm][slot];
+}
+
+__global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B) {
+    uint32_t s = B.wg_sys[blockIdx.x];
+    if (s == ~0u) return;
+    const BSys& S = B.sys[s];
+    const BCtl& C = B.ctl[s];
+    if (!C.do_rebuild) return;
+    uint32_t slot = blockIdx.x * BATCH_WG + threadIdx.x;
+    if (slot - S.slot0 >= S.n) return;
+    if (S.dim == 3) bb_scatter<3>(B, S, C, slot);
+    else bb_scatter<2>(B, S, C, slot);
+}
+
+// The list of local slot i (in the freshly scattered buffers): f32 cell-relative separations
+// against each species pair's (rc + skin)(1 + 1e-4), in-block pairs (same 256-slot block) owned by
+// one end, pairs across blocks listed at both ends.  variant 0 lists every pair at both ends.
+template <int D, bool MULTI>
+__device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local, uint32_t variant) {
+    const uint32_t pm = C.pmeta ^ 1u;
+    const uint32_t slot = S.slot0 + local;
+    const uint32_t* sp = B.species[pm] + S.slot0;
+    float ri[D];
+    int64_t xi[D];
+    for (int a = 0; a < D; a++) {
+        ri[a] = B.rel[a][slot];
+        xi[a] = B.pos[C.ppos ^ 1u][a][slot];
+    }
+    uint32_t si = MULTI ? sp[local] : 0u;
+    const float list_r2 = S.pt.list_r2[0];
+    uint32_t c[D];
+    cell_of<D>(S, xi, c);
+    const uint32_t cap = S.cap, stride = S.nslots, blk = local / BATCH_WG;
+    uint32_t* col = B.nbr + S.nbr0 + local;
+    const uint32_t* cs = B.cell_start + S.cell0;
+    const uint32_t* ce = B.cell_end + S.cell0;
+    const float* rel[D];
+    for (int a = 0; a < D; a++) rel[a] = B.rel[a] + S.slot0;
+    uint32_t na = 0, nb = 0;
+    for (int o = 0; o < stencil_size(D); o++) {
+        int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+        uint32_t cc[D];
+        float shift[D];
+        bool skip = false;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            int v = (int)c[a] + off[a];
+            if (S.nc[a] == 1u && off[a] != 0) skip = true;
+            if (v < 0 || v >= (int)S.nc[a]) {
+                if (!((S.periodic >> a) & 1u)) skip = true;
+                v = v < 0 ? v + (int)S.nc[a] : v - (int)S.nc[a];
+            }
+            cc[a] = (uint32_t)v;
+            shift[a] = (float)off[a] * S.cell_w[a] - ri[a];
+        }
+        if (skip) continue;
+        uint32_t key = lin_cell<D>(S, cc);
+        uint32_t s0 = cs[key], e0 = ce[key];
+        for (uint32_t j = s0; j < e0; j++) {
+            if (j == local) continue;
+            float r2 = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                float d = rel[a][j] + shift[a];
+                if (S.nc[a] == 1u && ((S.periodic >> a) & 1u)) {
+                    if (d > 0.5f * S.cell_w[a]) d -= S.cell_w[a];
+                    else if (d < -0.5f * S.cell_w[a]) d += S.cell_w[a];
+                }
+                r2 = __builtin_fmaf(d, d, r2);
+            }
+            float lr2 = MULTI ? S.pt.list_r2[si * 4 + sp[j]] : list_r2;
+            if (r2 < lr2) {
+                if (variant == 1 && j / BATCH_WG == blk) {
+                    if (owns(local, j)) {
+                        if (na + nb < cap) col[(size_t)na * stride] = j;
+                        na++;
+                    }
+                } else {
+                    if (na + nb < cap) col[(size_t)(cap - 1 - nb) * stride] = j;
+                    nb++;
+                }
+            }
+        }
+    }
+    uint32_t count = na + nb;
+    if (count > cap) {
+        atomicOr(&C.overflow, 1u);
+        na = nb = 0;
+    }
+    atomicMax(&C.maxcount, count);
+    B.nbr_count[slot] = na | (nb << 16);
+}
+
+__global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t variant) {
+    uint32_t s = B.wg_sys[blockIdx.x];
+    if (s == ~0u) return;
+    const BSys& S = B.sys[s];
+    BCtl& C = B.ctl[s];
+    if (!C.do_rebuild) return;
+    uint32_t local = blockIdx.x * BATCH_WG + threadIdx.x - S.slot0;
+    if (local >= S.n) return;
+    bool multi = S.nspecies > 1;
+    if (S.dim == 3) {
+        if (multi) bb_build<3, true>(B, S, C, local, variant);
+        else bb_build<3, false>(B, S, C, local, variant);
+    } else {
+        if (multi) bb_build<2, true>(B, S, C, local, variant);
+        else bb_build<2, false>(B, S, C, local, variant);
+    }
+}
+
+__global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    BCtl& c = C[s];
+    if (c.do_rebuild) {
+        c.stale = 0u;
+        c.stale_next = 0u;
+        c.ppos ^= 1u;
+        c.pmeta ^= 1u;
+        c.rebuilds += 1u;
+        c.do_rebuild = 0u;
+    }
+}
+
+__device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S.active && (C.stale | C.overflow) == 0u && C.remaining != 0u; }
+
+template <int D, bool MULTI>
+__device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C) {
+    __shared__ uint32_t acc[D][BATCH_WG];
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < D; a++) acc[a][t] = 0u;
+    __syncthreads();
+    const uint32_t slot = blockIdx.x * BATCH_WG + t;
+    const uint32_t local = slot - S.slot0;
+    const uint32_t base = blockIdx.x * BATCH_WG - S.slot0;  // first local slot of this block
+    const bool live = local < S.n;
+    const uint32_t pp = C.ppos, pm = C.pmeta;
+    const int64_t* px[D];
+    for (int a = 0; a < D; a++) px[a] = B.pos[pp][a] + S.slot0;
+    const uint32_t* sp = B.species[pm] + S.slot0;
+    const PairTable& pt = S.pt;
+    int64_t xi[D];
+    uint32_t k[D];
+    uint32_t si = 0;
+    float c = S.kick_c[0];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        xi[a] = live ? px[a][local] : 0;
+        k[a] = 0u;
+    }
+    if (live) {
+        if (MULTI) {
+            si = sp[local];
+            c = pt.kick_c[si];
+        }
+        const float rc2_0 = pt.rc2[0], sigma2_0 = pt.sigma2[0], six_eps4_0 = pt.six_eps4[0];
+        const uint32_t cnt = B.nbr_count[slot];
+        const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
+        const uint32_t* col = B.nbr + S.nbr0 + local;
+        const size_t stride = S.nslots;
+        const bool rc = S.range_check != 0;
+        for (uint32_t e = 0; e < na; e++) {
+            const uint32_t j = col[(size_t)e * stride];
+            int64_t xj[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) xj[a] = px[a][j];
+            float f[D];
+            bool ok = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xj[a], a), f[a]);
+            if (rc && !ok) continue;
+            int32_t kj[D];
+            if constexpr (MULTI) {
+                uint32_t sj = sp[j];
+                uint32_t p = si * 4 + sj;
+                pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], c, pt.kick_c[sj], k, kj);
+            } else {
+                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, c, c, k, kj);
+            }
+            const uint32_t lj = j - base;
+#pragma unroll
+            for (int a = 0; a < D; a++)
+                if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+        }
+        const uint32_t* colb = col + (size_t)(S.cap - 1) * stride;
+        for (uint32_t e = 0; e < nb; e++) {
+            const uint32_t j = colb[-(ptrdiff_t)((size_t)e * stride)];
+            int64_t xj[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) xj[a] = px[a][j];
+            float f[D];
+            bool ok = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xj[a], a), f[a]);
+            if (rc && !ok) continue;
+            if constexpr (MULTI) {
+                uint32_t p = si * 4 + sp[j];
+                pair_kick<D>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], c, k);
+            } else {
+                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, c, k);
+            }
+        }
+        if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
+    }
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int a = 0; a < D; a++) k[a] += acc[a][t];
+    const bool pin = (S.pinned >> si) & 1u;
+    int32_t v[D];
+    int64_t x[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        v[a] = (int32_t)((uint32_t)B.vel[pm][a][slot] + k[a] + (pin ? 0u : (uint32_t)S.gravity[a]));
+        x[a] = xi[a] + (int64_t)v[a];
+    }
+    bound<D>(S, x, v);
+    float r2 = 0.0f;
+    bool far = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        B.pos[pp ^ 1][a][slot] = x[a];
+        B.vel[pm][a][slot] = v[a];
+        float d;
+        far |= !sep_f(separation<D>(S, B.ref[a][slot], x[a], a), d);
+        r2 += d * d;
+    }
+    bool stale = far || r2 >= S.half_skin2;
+    uint64_t bal = ballot64(stale);
+    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&C.stale_next, 1u);
+}
+
+__global__ __launch_bounds__(BATCH_WG) void k_bb_step(BBufs B) {
+    uint32_t s = B.wg_sys[blockIdx.x];
+    if (s == ~0u) return;
+    const BSys& S = B.sys[s];
+    BCtl& C = B.ctl[s];
+    if (!bb_runs(S, C)) return;
+    bool multi = S.nspecies > 1;
+    if (S.dim == 3) {
+        if (multi) bb_step<3, true>(B, S, C);
+        else bb_step<3, false>(B, S, C);
+    } else {
+        if (multi) bb_step<2, true>(B, S, C);
+        else bb_step<2, false>(B, S, C);
+    }
+}
+
+__global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    BCtl& c = C[s];
+    if (bb_runs(S[s], c)) {
+        c.ppos ^= 1u;
+        c.remaining -= 1u;
+        uint32_t lo = c.done_lo + 1u;
+        c.done_lo = lo;
+        if (lo == 0u) c.done_hi += 1u;
+        c.clock += 1u;
+        c.stale = c.stale_next;
+        c.stale_next = 0u;
+    } else if (S[s].active && c.remaining) {
+        c.skipped += 1u;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// RangeAlloc
+// ---------------------------------------------------------------------------------------------
+
+void RangeAlloc::reset(uint64_t capacity) {
+    free_.clear();
+    taken_.clear();
+    cap_ = capacity;
+    used_ = 0;
+    if (capacity) free_[0] = capacity;
+}
+
+bool RangeAlloc::alloc(uint64_t size, uint64_t* start) {
+    if (size == 0) size = 1;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second >= size) {
+            uint64_t s = it->first, rest = it->second - size;
+            free_.erase(it);
+            if (rest) free_[s + size] = rest;
+            taken_[s] = size;
+            used_ += size;
+            *start = s;
+            return true;
+        }
+    }
+    return false;
+}
+
+void RangeAlloc::free(uint64_t start, uint64_t size) {
+    if (size == 0) size = 1;
+    auto t = taken_.find(start);
+    if (t == taken_.end()) return;
+    taken_.erase(t);
+    used_ -= size;
+    auto it = free_.emplace(start, size).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+        it->second += next->second;
+        free_.erase(next);
+    }
+    if (it != free_.begin()) {
+        auto prev = std::prev(it);
+        if (prev->first + prev->second == it->first) {
+            prev->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+uint64_t RangeAlloc::high_water() const {
+    if (taken_.empty()) return 0;
+    auto last = std::prev(taken_.end());
+    return last->first + last->second;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batch host side
+// ---------------------------------------------------------------------------------------------
+
+void* Batch::dalloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e != hipSuccess) throw Error(std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    allocs_.push_back(p);
+    return p;
+}
+
+Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
+    RH_REQUIRE(max_systems >= 1 && max_systems <= (1u << 20), "max_systems must be 1..2^20");
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        RH_HIP(hipMemGetInfo(&fr, &tot));
+        budget = fr / 2;
+    }
+    uint64_t slot_cap = (uint64_t)(budget * 0.16 / SLOT_BYTES) / BATCH_WG * BATCH_WG;
+    uint64_t list_cap = (uint64_t)(budget * 0.80 / 4) / LIST_UNIT * LIST_UNIT;
+    uint64_t cell_cap = (uint64_t)(budget * 0.04 / 12) / CELL_UNIT * CELL_UNIT;
+    slot_cap = std::min<uint64_t>(slot_cap, (uint64_t(1) << 31) - BATCH_WG);
+    RH_REQUIRE(slot_cap >= BATCH_WG && list_cap >= LIST_UNIT && cell_cap >= CELL_UNIT, "batch memory budget too small");
+    slots_.reset(slot_cap / BATCH_WG);
+    lists_.reset(list_cap / LIST_UNIT);
+    cells_.reset(cell_cap / CELL_UNIT);
+    RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    std::memset(&B_, 0, sizeof B_);
+    for (int p = 0; p < 2; p++) {
+        for (int a = 0; a < 3; a++) {
+            B_.pos[p][a] = (int64_t*)dalloc(slot_cap * 8);
+            B_.vel[p][a] = (int32_t*)dalloc(slot_cap * 4);
+        }
+        B_.species[p] = (uint32_t*)dalloc(slot_cap * 4);
+        B_.ids[p] = (uint32_t*)dalloc(slot_cap * 4);
+    }
+    for (int a = 0; a < 3; a++) {
+        B_.ref[a] = (int64_t*)dalloc(slot_cap * 8);
+        B_.rel[a] = (float*)dalloc(slot_cap * 4);
+    }
+    B_.key = (uint32_t*)dalloc(slot_cap * 4);
+    B_.rank = (uint32_t*)dalloc(slot_cap * 4);
+    B_.nbr_count = (uint32_t*)dalloc(slot_cap * 4);
+    B_.nbr = (uint32_t*)dalloc(list_cap * 4);
+    B_.cell_count = (uint32_t*)dalloc(cell_cap * 4);
+    B_.cell_start = (uint32_t*)dalloc(cell_cap * 4);
+    B_.cell_end = (uint32_t*)dalloc(cell_cap * 4);
+    RH_HIP(hipMemset(B_.cell_count, 0, cell_cap * 4));
+    d_sys_ = (BSys*)dalloc(sizeof(BSys) * max_sys_);
+    d_ctl_ = (BCtl*)dalloc(sizeof(BCtl) * max_sys_);
+    RH_HIP(hipMemset(d_sys_, 0, sizeof(BSys) * max_sys_));
+    RH_HIP(hipMemset(d_ctl_, 0, sizeof(BCtl) * max_sys_));
+    d_wg_ = (uint32_t*)dalloc(sizeof(uint32_t) * (slot_cap / BATCH_WG));
+    RH_HIP(hipMemset(d_wg_, 0xff, sizeof(uint32_t) * (slot_cap / BATCH_WG)));
+    B_.wg_sys = d_wg_;
+    B_.sys = d_sys_;
+    B_.ctl = d_ctl_;
+    h_sys_.assign(max_sys_, BSys{});
+    for (int i = (int)max_sys_ - 1; i >= 0; i--) free_ids_.push_back(i);
+    RH_HIP(hipHostMalloc((void**)&h_ctl_, sizeof(BCtl) * max_sys_, hipHostMallocDefault));
+    RH_HIP(hipDeviceSynchronize());
+}
+
+Batch::~Batch() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (void* p : allocs_) (void)hipFree(p);
+    if (h_ctl_) (void)hipHostFree(h_ctl_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) const {
+    if (slot_cap) *slot_cap = slots_.capacity() * BATCH_WG;
+    if (list_cap) *list_cap = lists_.capacity() * LIST_UNIT;
+    if (cell_cap) *cell_cap = cells_.capacity() * CELL_UNIT;
+}
+
+// Cells, list capacity and constants of one system, as Params::new (engine3.rs:70-162).
+void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const {
+    const int D = sys.dim;
+    RH_REQUIRE(D == 2 || D == 3, "dimension must be 2 or 3");
+    RH_REQUIRE(sys.nspecies >= 1 && sys.nspecies <= 4, "between 1 and 4 species");
+    RH_REQUIRE(n >= 1 && n < (1u << 30), "particle count out of range");
+    RH_REQUIRE(skin >= 0.0, "skin must be >= 0");
+    std::memset(&d, 0, sizeof d);
+    d.dim = D;
+    d.n = (uint32_t)n;
+    d.nspecies = sys.nspecies;
+    d.pinned = sys.pinned;
+    double rcm = core::rc_max(sys.pairs, sys.nspecies);
+    RH_REQUIRE(rcm < 7.9, "cutoff must be below 8 sigma");
+    double b[3] = {0, 0, 0}, bmax = 0, vol = 1;
+    for (int a = 0; a < D; a++) {
+        b[a] = core::to_f64(sys.box_len[a]);
+        bmax = std::max(bmax, b[a]);
+        vol *= b[a];
+    }
+    double margin = 1e-3 + 1e-6 * bmax;
+    uint64_t ncells = 1;
+    for (int a = 0; a < D; a++) {
+        d.nc[a] = core::sat_u32(std::floor(b[a] / (rcm + skin + margin)));
+        if (d.nc[a] < 3) {
+            RH_REQUIRE(b[a] > 2.0 * (rcm + skin) * (1.0 + 1e-4) + margin, "box too small: shorter than twice the cutoff plus skin");
+            d.nc[a] = 1;
+        }
+        ncells *= d.nc[a];
+        d.box[a] = sys.box_len[a];
+        d.half[a] = sys.box_len[a] / 2;
+        d.cell_scale[a] = (double)d.nc[a] / (double)sys.box_len[a];
+        d.cell_w64[a] = b[a] / (double)d.nc[a];
+        d.cell_w[a] = (float)d.cell_w64[a];
+        if (!sys.walled[a]) d.periodic |= 1u << a;
+        else
+            for (int side = 0; side < 2; side++) {
+                RH_REQUIRE(sys.wall_kind[a][side] != RH_WALL_THERMAL, "heat-bath walls are not supported by this engine yet");
+                if (sys.wall_kind[a][side] == RH_WALL_SOFT) d.soft_walls |= 1u << (2 * a + side);
+            }
+        d.gravity[a] = core::gravity_kick(sys.dt, sys.gravity[a]);
+    }
+    RH_REQUIRE(ncells < (uint64_t(1) << 31), "too many cells");
+    d.ncells = (uint32_t)ncells;
+    for (uint32_t s = 0; s < 4; s++) {
+        float kc = s < sys.nspecies ? (float)core::kick_coeff(sys.dt, sys.masses[s], (sys.pinned >> s) & 1u) : 0.0f;
+        d.kick_c[s] = kc;
+        d.pt.kick_c[s] = kc;
+    }
+    for (uint32_t a = 0; a < sys.nspecies; a++)
+        for (uint32_t bb = 0; bb < sys.nspecies; bb++) {
+            int k = a * 4 + bb;
+            const rh_pair& p = sys.pairs[k];
+            RH_REQUIRE(p.rc < 7.9, "cutoff must be below 8 sigma");
+            d.pt.rc2[k] = (float)(p.rc * p.rc);
+            d.pt.eps4[k] = (float)(4.0 * p.eps);
+            d.pt.sigma2[k] = (float)(p.sigma * p.sigma);
+            d.pt.shift[k] = (float)core::pair_shift(p);
+            d.pt.six_eps4[k] = 6.0f * d.pt.eps4[k];
+            double r = (p.rc + skin) * (1.0 + 1e-4);
+            d.pt.list_r2[k] = (float)(r * r);
+        }
+    double hs = 0.5 * skin * (1.0 - 1e-3);
+    d.half_skin2 = (float)(hs * hs);
+    d.range_check = ((rcm + skin) * (1.0 + 1e-4) + skin >= 7.99) ? 1u : 0u;
+    const double pi = 3.141592653589793;
+    double r = rcm + skin;
+    double dens = (double)n / vol;
+    double c = D == 3 ? std::ceil(4.0 / 3.0 * pi * r * r * r * std::fmax(2.0 * dens, 1.2)) : std::ceil(pi * r * r * std::fmax(2.0 * dens, 1.2));
+    *cap = (uint32_t)std::min(c + 8.0, 65535.0);
+}
+
+uint64_t Batch::bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double skin) const {
+    BSys d;
+    uint32_t cap;
+    SystemDesc s2 = sys;
+    s2.dim = dim;
+    plan(s2, n, skin, d, &cap);
+    uint64_t nslots = (n + BATCH_WG - 1) / BATCH_WG * BATCH_WG;
+    return nslots * SLOT_BYTES + (uint64_t)cap * nslots * 4 + (uint64_t)d.ncells * 12;
+}
+
+int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps) {
+    RH_REQUIRE(steps < (uint64_t(1) << 32), "steps per run must fit 32 bits");
+    const int D = sys.dim;
+    for (uint64_t i = 0; i < n; i++) {
+        RH_REQUIRE(species[i] < sys.nspecies, "species out of range");
+        for (int a = 0; a < D; a++) RH_REQUIRE(pos[D * i + a] >= 0 && pos[D * i + a] < sys.box_len[a], "position outside the box");
+    }
+    if (free_ids_.empty()) return -1;
+    BSys d;
+    uint32_t cap;
+    plan(sys, n, skin, d, &cap);
+    uint64_t nslots = (n + BATCH_WG - 1) / BATCH_WG * BATCH_WG;
+    uint64_t s0, l0, c0;
+    uint64_t lunits = ((uint64_t)cap * nslots + LIST_UNIT - 1) / LIST_UNIT;
+    uint64_t cunits = (d.ncells + CELL_UNIT - 1) / CELL_UNIT;
+    if (!slots_.alloc(nslots / BATCH_WG, &s0)) return -1;
+    if (!lists_.alloc(lunits, &l0)) {
+        slots_.free(s0, nslots / BATCH_WG);
+        return -1;
+    }
+    if (!cells_.alloc(cunits, &c0)) {
+        slots_.free(s0, nslots / BATCH_WG);
+        lists_.free(l0, lunits);
+        return -1;
+    }
+    int id = free_ids_.back();
+    free_ids_.pop_back();
+    d.active = 1;
+    d.slot0 = (uint32_t)(s0 * BATCH_WG);
+    d.nslots = (uint32_t)nslots;
+    d.nbr0 = l0 * LIST_UNIT;
+    d.cap = cap;
+    d.cell0 = (uint32_t)(c0 * CELL_UNIT);
+    BatchMember m;
+    m.sys = sys;
+    m.n = n;
+    m.slot0 = d.slot0;
+    m.nslots = d.nslots;
+    m.cell0 = d.cell0;
+    m.ncells = d.ncells;
+    m.cap = cap;
+    m.nbr0 = d.nbr0;
+    m.nbr_entries = lunits * LIST_UNIT;
+    m.skin = skin;
+    members_[id] = m;
+    h_sys_[id] = d;
+    // Upload the state into parity 0 of the new range (nothing in flight touches it).
+    RH_HIP(hipStreamSynchronize(stream_));
+    std::vector<int64_t> px(n);
+    std::vector<int32_t> vx(n);
+    std::vector<uint32_t> ids(n);
+    for (int a = 0; a < D; a++) {
+        for (uint64_t i = 0; i < n; i++) px[i] = pos[D * i + a];
+        RH_HIP(hipMemcpy(B_.pos[0][a] + d.slot0, px.data(), n * 8, hipMemcpyHostToDevice));
+        for (uint64_t i = 0; i < n; i++) vx[i] = vel[D * i + a];
+        RH_HIP(hipMemcpy(B_.vel[0][a] + d.slot0, vx.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    RH_HIP(hipMemcpy(B_.species[0] + d.slot0, species, n * 4, hipMemcpyHostToDevice));
+    for (uint64_t i = 0; i < n; i++) ids[i] = (uint32_t)i;
+    RH_HIP(hipMemcpy(B_.ids[0] + d.slot0, ids.data(), n * 4, hipMemcpyHostToDevice));
+    BCtl c;
+    std::memset(&c, 0, sizeof c);
+    c.stale = 1u;
+    c.remaining = (uint32_t)steps;
+    RH_HIP(hipMemcpy(d_ctl_ + id, &c, sizeof c, hipMemcpyHostToDevice));
+    tables_dirty_ = true;
+    return id;
+}
+
+void Batch::remove(int id) {
+    auto it = members_.find(id);
+    RH_REQUIRE(it != members_.end(), "no such system in the batch");
+    RH_HIP(hipStreamSynchronize(stream_));
+    const BatchMember& m = it->second;
+    slots_.free(m.slot0 / BATCH_WG, m.nslots / BATCH_WG);
+    lists_.free(m.nbr0 / LIST_UNIT, m.nbr_entries / LIST_UNIT);
+    cells_.free(m.cell0 / CELL_UNIT, (m.ncells + CELL_UNIT - 1) / CELL_UNIT);
+    h_sys_[id].active = 0;
+    members_.erase(it);
+    free_ids_.push_back(id);
+    tables_dirty_ = true;
+}
+
+void Batch::set_steps(int id, uint64_t steps) {
+    RH_REQUIRE(members_.count(id), "no such system in the batch");
+    RH_REQUIRE(steps < (uint64_t(1) << 32), "steps per run must fit 32 bits");
+    RH_HIP(hipStreamSynchronize(stream_));
+    BCtl c;
+    RH_HIP(hipMemcpy(&c, d_ctl_ + id, sizeof c, hipMemcpyDeviceToHost));
+    c.remaining = (uint32_t)steps;
+    RH_HIP(hipMemcpy(d_ctl_ + id, &c, sizeof c, hipMemcpyHostToDevice));
+}
+
+void Batch::upload_tables() {
+    RH_HIP(hipStreamSynchronize(stream_));
+    RH_HIP(hipMemcpy(d_sys_, h_sys_.data(), sizeof(BSys) * max_sys_, hipMemcpyHostToDevice));
+    uint64_t hw = slots_.high_water();  // units of BATCH_WG
+    std::vector<uint32_t> wg(std::max<uint64_t>(hw, 1), ~0u);
+    for (auto& kv : members_) {
+        const BatchMember& m = kv.second;
+        for (uint32_t w = m.slot0 / BATCH_WG; w < (m.slot0 + m.nslots) / BATCH_WG; w++) wg[w] = (uint32_t)kv.first;
+    }
+    RH_HIP(hipMemcpy(d_wg_, wg.data(), wg.size() * 4, hipMemcpyHostToDevice));
+    uint32_t new_wgs = (uint32_t)std::max<uint64_t>(hw, 1);
+    if (new_wgs != grid_wgs_) invalidate_graph();
+    grid_wgs_ = new_wgs;
+    tables_dirty_ = false;
+}
+
+void Batch::invalidate_graph() {
+    if (graph_) {
+        RH_HIP(hipGraphExecDestroy(graph_));
+        graph_ = nullptr;
+    }
+}
+
+void Batch::enqueue_iteration() {
+    const uint32_t sb = blocks_for(max_sys_, 256);
+    k_bb_decide<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
+    k_bb_count<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
+    k_bb_scatter<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    k_bb_build<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_, 1u);
+    k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
+    k_bb_step<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
+}
+
+void Batch::run(uint64_t iterations) {
+    if (tables_dirty_) upload_tables();
+    if (members_.empty() || iterations == 0) return;
+    uint64_t left = iterations;
+    if (graph_chunk_ > 0 && iterations >= graph_chunk_) {
+        if (!graph_ || graph_wgs_ != grid_wgs_ || graph_len_ != graph_chunk_) {
+            invalidate_graph();
+            hipGraph_t g;
+            RH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (uint32_t k = 0; k < graph_chunk_; k++) enqueue_iteration();
+            RH_HIP(hipStreamEndCapture(stream_, &g));
+            RH_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+            RH_HIP(hipGraphDestroy(g));
+            graph_wgs_ = grid_wgs_;
+            graph_len_ = graph_chunk_;
+        }
+        while (left >= graph_chunk_) {
+            RH_HIP(hipGraphLaunch(graph_, stream_));
+            left -= graph_chunk_;
+        }
+    }
+    for (uint64_t k = 0; k < left; k++) enqueue_iteration();
+    RH_HIP(hipGetLastError());
+}
+
+void Batch::sync() { RH_HIP(hipStreamSynchronize(stream_)); }
+
+void Batch::handle_overflow() {
+    for (auto& kv : members_) {
+        int id = kv.first;
+        BCtl& c = h_ctl_[id];
+        if (!c.overflow) continue;
+        BatchMember& m = kv.second;
+        uint32_t want = std::max<uint32_t>(c.maxcount + c.maxcount / 4 + 8, m.cap + m.cap / 2);
+        want = std::min<uint32_t>(want, 65535u);
+        uint64_t lunits = ((uint64_t)want * m.nslots + LIST_UNIT - 1) / LIST_UNIT;
+        uint64_t l0;
+        lists_.free(m.nbr0 / LIST_UNIT, m.nbr_entries / LIST_UNIT);
+        if (!lists_.alloc(lunits, &l0)) {
+            // No room to grow now: keep the old range; the system waits (its steps are gated).
+            lists_.alloc(m.nbr_entries / LIST_UNIT, &l0);
+            m.nbr0 = l0 * LIST_UNIT;
+            h_sys_[id].nbr0 = m.nbr0;
+            tables_dirty_ = true;
+            continue;
+        }
+        m.cap = want;
+        m.nbr0 = l0 * LIST_UNIT;
+        m.nbr_entries = lunits * LIST_UNIT;
+        h_sys_[id].cap = want;
+        h_sys_[id].nbr0 = m.nbr0;
+        c.overflow = 0u;
+        c.stale = 1u;
+        RH_HIP(hipMemcpy(d_ctl_ + id, &c, sizeof c, hipMemcpyHostToDevice));
+        tables_dirty_ = true;
+    }
+}
+
+void Batch::poll(uint64_t* remaining, uint32_t count) {
+    RH_HIP(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(BCtl) * max_sys_, hipMemcpyDeviceToHost, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+    handle_overflow();
+    if (remaining)
+        for (uint32_t i = 0; i < count && i < max_sys_; i++) remaining[i] = members_.count((int)i) ? h_ctl_[i].remaining : 0;
+}
+
+void Batch::read_state(int id, int64_t* pos, int32_t* vel, uint32_t* species) {
+    auto it = members_.find(id);
+    RH_REQUIRE(it != members_.end(), "no such system in the batch");
+    const BatchMember& m = it->second;
+    RH_HIP(hipStreamSynchronize(stream_));
+    BCtl c;
+    RH_HIP(hipMemcpy(&c, d_ctl_ + id, sizeof c, hipMemcpyDeviceToHost));
+    const int D = m.sys.dim;
+    const uint64_t n = m.n;
+    std::vector<uint32_t> ids(n), sp(n);
+    std::vector<int64_t> px(n);
+    std::vector<int32_t> vx(n);
+    RH_HIP(hipMemcpy(ids.data(), B_.ids[c.pmeta] + m.slot0, n * 4, hipMemcpyDeviceToHost));
+    for (int a = 0; a < D; a++) {
+        RH_HIP(hipMemcpy(px.data(), B_.pos[c.ppos][a] + m.slot0, n * 8, hipMemcpyDeviceToHost));
+        RH_HIP(hipMemcpy(vx.data(), B_.vel[c.pmeta][a] + m.slot0, n * 4, hipMemcpyDeviceToHost));
+        for (uint64_t s = 0; s < n; s++) {
+            if (pos) pos[D * (uint64_t)ids[s] + a] = px[s];
+            if (vel) vel[D * (uint64_t)ids[s] + a] = vx[s];
+        }
+    }
+    if (species) {
+        RH_HIP(hipMemcpy(sp.data(), B_.species[c.pmeta] + m.slot0, n * 4, hipMemcpyDeviceToHost));
+        for (uint64_t s = 0; s < n; s++) species[ids[s]] = sp[s];
+    }
+}
+
+rh_list_stats Batch::stats(int id) {
+    auto it = members_.find(id);
+    RH_REQUIRE(it != members_.end(), "no such system in the batch");
+    RH_HIP(hipStreamSynchronize(stream_));
+    BCtl c;
+    RH_HIP(hipMemcpy(&c, d_ctl_ + id, sizeof c, hipMemcpyDeviceToHost));
+    rh_list_stats s;
+    s.rebuilds = c.rebuilds;
+    s.overflowed = c.overflow;
+    s.cap = it->second.cap;
+    s.max_count = c.maxcount;
+    s.steps_done = ((uint64_t)c.done_hi << 32) | c.done_lo;
+    s.launches_skipped = c.skipped;
+    return s;
+}
+
+}  // namespace rh

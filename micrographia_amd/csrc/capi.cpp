@@ -4,6 +4,7 @@
 #include <new>
 #include <string>
 
+#include "batch.h"
 #include "engine.h"
 
 namespace rh {
@@ -15,6 +16,9 @@ struct rh_engine3 {
 };
 struct rh_engine2 {
     rh::Engine<2>* e;
+};
+struct rh_batch {
+    rh::Batch* b;
 };
 
 namespace {
@@ -105,6 +109,11 @@ template <class H>
 auto* eng(H* h) {
     RH_REQUIRE(h != nullptr && h->e != nullptr, "null engine");
     return h->e;
+}
+
+rh::Batch* eng(rh_batch* h) {
+    RH_REQUIRE(h != nullptr && h->b != nullptr, "null batch");
+    return h->b;
 }
 
 }  // namespace
@@ -254,6 +263,81 @@ int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block) {
 }
 int rh_engine2_stream(rh_engine2* e, void** stream) {
     return guarded([&] { *stream = (void*)eng(e)->stream(); });
+}
+
+// ---- batched engine ----
+
+int rh_batch_create(uint64_t budget_bytes, uint32_t max_systems, rh_batch** out) {
+    return guarded([&] {
+        RH_REQUIRE(out, "null argument");
+        *out = nullptr;
+        auto* h = new rh_batch{nullptr};
+        try {
+            h->b = new rh::Batch(budget_bytes, max_systems);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+int rh_batch_destroy(rh_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        delete b->b;
+        delete b;
+    });
+}
+int rh_batch_add3(rh_batch* b, const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id) {
+    return guarded([&] {
+        RH_REQUIRE(pos && vel && species && id, "null argument");
+        *id = eng(b)->add(desc3(sys), n, pos, vel, species, skin, steps);
+    });
+}
+int rh_batch_add2(rh_batch* b, const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id) {
+    return guarded([&] {
+        RH_REQUIRE(pos && vel && species && id, "null argument");
+        *id = eng(b)->add(desc2(sys), n, pos, vel, species, skin, steps);
+    });
+}
+int rh_batch_remove(rh_batch* b, int32_t id) {
+    return guarded([&] { eng(b)->remove(id); });
+}
+int rh_batch_set_steps(rh_batch* b, int32_t id, uint64_t steps) {
+    return guarded([&] { eng(b)->set_steps(id, steps); });
+}
+int rh_batch_run(rh_batch* b, uint64_t iterations) {
+    return guarded([&] { eng(b)->run(iterations); });
+}
+int rh_batch_sync(rh_batch* b) {
+    return guarded([&] { eng(b)->sync(); });
+}
+int rh_batch_poll(rh_batch* b, uint64_t* remaining, uint32_t count) {
+    return guarded([&] { eng(b)->poll(remaining, count); });
+}
+int rh_batch_read_state(rh_batch* b, int32_t id, int64_t* pos, int32_t* vel, uint32_t* species) {
+    return guarded([&] { eng(b)->read_state(id, pos, vel, species); });
+}
+int rh_batch_list_stats(rh_batch* b, int32_t id, rh_list_stats* out) {
+    return guarded([&] { *out = eng(b)->stats(id); });
+}
+int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk) {
+    return guarded([&] { eng(b)->set_graph_chunk(chunk); });
+}
+int rh_batch_stream(rh_batch* b, void** stream) {
+    return guarded([&] { *stream = (void*)eng(b)->stream(); });
+}
+int rh_batch_capacity(rh_batch* b, uint64_t* slots, uint64_t* list_entries, uint64_t* cells, uint32_t* active) {
+    return guarded([&] {
+        eng(b)->budget(slots, list_entries, cells);
+        if (active) *active = eng(b)->active();
+    });
+}
+int rh_batch_bytes_needed3(rh_batch* b, const rh_system3* sys, uint64_t n, double skin, uint64_t* bytes) {
+    return guarded([&] { *bytes = eng(b)->bytes_needed(3, n, desc3(sys), skin); });
+}
+int rh_batch_bytes_needed2(rh_batch* b, const rh_system2* sys, uint64_t n, double skin, uint64_t* bytes) {
+    return guarded([&] { *bytes = eng(b)->bytes_needed(2, n, desc2(sys), skin); });
 }
 
 // ---- gpu-prims ----

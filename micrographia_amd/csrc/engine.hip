@@ -13,6 +13,7 @@
 //   re-issued by the host, so every step runs with a valid list and the trajectory is the same
 //   bit for bit whatever the chunking.
 #include "engine.h"
+#include "md_device.h"
 
 #include <algorithm>
 #include <cmath>
@@ -21,159 +22,12 @@
 namespace rh {
 
 constexpr int TPB = 256;
-constexpr float SOFT_RC_F = 1.122462048309373f;
 
 // ---------------------------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t spread3(uint32_t v) {
-    uint32_t x = v & 0x3ffu;
-    x = (x | (x << 16)) & 0x030000ffu;
-    x = (x | (x << 8)) & 0x0300f00fu;
-    x = (x | (x << 4)) & 0x030c30c3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
 
-__device__ __forceinline__ uint32_t spread2(uint32_t v) {
-    uint32_t x = v & 0xffffu;
-    x = (x | (x << 8)) & 0x00ff00ffu;
-    x = (x | (x << 4)) & 0x0f0f0f0fu;
-    x = (x | (x << 2)) & 0x33333333u;
-    x = (x | (x << 1)) & 0x55555555u;
-    return x;
-}
-
-template <int D>
-__device__ __forceinline__ uint32_t morton(const uint32_t c[D]) {
-    if constexpr (D == 3) return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
-    else return spread2(c[0]) | (spread2(c[1]) << 1);
-}
-
-template <int D>
-__device__ __forceinline__ void cell_of(const Params<D>& P, const int64_t x[D], uint32_t c[D]) {
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        double f = (double)x[a] * P.cell_scale[a];
-        uint32_t ci = f > 0.0 ? (uint32_t)f : 0u;
-        c[a] = ci < P.nc[a] - 1 ? ci : P.nc[a] - 1;
-    }
-}
-
-// Minimum image (periodic axes) of b - a, exactly fixed::min_image (fixed.rs:31-40).
-// Branch-free: the two corrections exclude each other, as the reference's if / else-if do.
-template <int D>
-__device__ __forceinline__ int64_t separation(const Params<D>& P, int64_t a, int64_t b, int ax) {
-    int64_t d = b - a;
-    if ((P.periodic >> ax) & 1u) {
-        int64_t down = d >= P.half[ax] ? -P.box[ax] : int64_t(0);
-        int64_t up = d < -P.half[ax] ? P.box[ax] : int64_t(0);
-        d += down + up;
-    }
-    return d;
-}
-
-// fixed::sep_f32 (fixed.rs:55-61): |d| >> 4 as f32 times 2^-28, with the sign of d; false if
-// |d| >= 8 sigma.  Truncating d/16 toward zero and converting the signed value rounds
-// symmetrically, so this equals the reference bit for bit (a tiny negative d gives +0 instead
-// of -0, which no kick or energy can tell apart).
-__device__ __forceinline__ bool sep_f(int64_t d, float& f) {
-    bool ok = (uint64_t)(d + (int64_t(8) << 32)) < (uint64_t(16) << 32);
-    int64_t q = (d + ((d >> 63) & 15)) >> 4;
-    f = (float)(int32_t)q * 0x1p-28f;
-    return ok;
-}
-
-// One pair's kick on particle i, in PairConsts::eval3's operation order (real.rs:145-156);
-// the factor 6*eps4 is the same f32 product md-ref forms, tabulated per species pair.
-template <int D>
-__device__ __forceinline__ void pair_kick(const float f[D], float rc2, float sigma2, float six_eps4, float c, uint32_t k[D]) {
-    float r2 = f[0] * f[0] + f[1] * f[1];
-    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
-    if (r2 < rc2) {
-        float s2 = sigma2 / r2;
-        float s6 = s2 * s2 * s2;
-        // 2*s6 is exact, so one fused op rounds (2*s6 - 1) exactly once, as the separate ops do.
-        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
-        float a = -fr * c;
-#pragma unroll
-        for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
-    }
-}
-
-// The same pair with its energy u = eps4*s6*(s6-1) - shift.
-template <int D>
-__device__ __forceinline__ void pair_kick_pe(const float f[D], const PairTable* __restrict__ pt, int p, float c, uint32_t k[D], double& pe) {
-    float r2 = f[0] * f[0] + f[1] * f[1];
-    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
-    if (r2 < pt->rc2[p]) {
-        float s2 = pt->sigma2[p] / r2;
-        float s6 = s2 * s2 * s2;
-        float fr = pt->six_eps4[p] * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
-        float a = -fr * c;
-#pragma unroll
-        for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
-        float u = pt->eps4[p] * s6 * (s6 - 1.0f) - pt->shift[p];
-        pe += (double)u;
-    }
-}
-
-// Soft (WCA) walls on the centre's distance from each wall (int_engine3.rs:106-126).
-template <int D>
-__device__ __forceinline__ void soft_walls(const Params<D>& P, const int64_t x[D], float c, uint32_t k[D], double* wall_pe) {
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        for (int side = 0; side < 2; side++) {
-            if (!((P.soft_walls >> (2 * a + side)) & 1u)) continue;
-            int64_t d = side == 0 ? x[a] : P.box[a] - x[a];
-            float df;
-            if (!sep_f(d, df)) continue;
-            if (df < SOFT_RC_F) {
-                float s2 = 1.0f / (df * df);
-                float s6 = s2 * s2 * s2;
-                float fw = 24.0f * s6 * (2.0f * s6 - 1.0f) / df;
-                int32_t kw = cvt_sat_i32(__builtin_rintf(fw * c));
-                k[a] += side == 0 ? (uint32_t)kw : (uint32_t)0 - (uint32_t)kw;
-                if (wall_pe) *wall_pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
-            }
-        }
-    }
-}
-
-// Wrap (periodic) or mirror about -1/2 and L-1/2 (walls), walls::mirror (walls.rs:27-35);
-// a mirrored particle's velocity on that axis is reversed (hard and soft walls).
-template <int D>
-__device__ __forceinline__ void bound(const Params<D>& P, int64_t x[D], int32_t v[D]) {
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        int64_t l = P.box[a];
-        if ((P.periodic >> a) & 1u) {
-            if (x[a] < 0) x[a] += l;
-            else if (x[a] >= l) x[a] -= l;
-        } else if (x[a] < 0) {
-            x[a] = -1 - x[a];
-            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
-        } else if (x[a] >= l) {
-            x[a] = 2 * l - 1 - x[a];
-            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
-        }
-    }
-}
-
-template <int D>
-__device__ __forceinline__ void stencil_cell(const Params<D>& P, const uint32_t c[D], int o, uint32_t out[D]) {
-    int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        int v = (int)c[a] + off[a];
-        if (v < 0) v += (int)P.nc[a];
-        else if (v >= (int)P.nc[a]) v -= (int)P.nc[a];
-        out[a] = (uint32_t)v;
-    }
-}
-
-constexpr int stencil_size(int d) { return d == 3 ? 27 : 9; }
 
 // ---------------------------------------------------------------------------------------------
 // Control kernels (one lane)
@@ -293,6 +147,16 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
 
 // Full Verlet list within each species pair's (rc + skin)(1 + 1e-4) (md3.wgsl build_list), over
 // the 3^D cells around the particle's, in the freshly gathered (not yet committed) buffers.
+// Offsets of the 3^D stencil that revisit a cell: on an axis with a single cell only offset 0.
+template <int D, class PT>
+__device__ __forceinline__ bool stencil_dup(const PT& P, int o) {
+    int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+    bool dup = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) dup |= (P.nc[a] == 1u && off[a] != 0);
+    return dup;
+}
+
 template <int D, bool MULTI>
 __global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
@@ -311,6 +175,7 @@ __global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, cons
     uint32_t count = 0;
     const uint32_t cap = P.cap, n = P.n;
     for (int o = 0; o < stencil_size(D); o++) {
+        if (stencil_dup<D>(P, o)) continue;
         uint32_t cc[D];
         stencil_cell<D>(P, c, o, cc);
         uint32_t key = morton<D>(cc);
@@ -411,13 +276,6 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // Half-list variant
 // ---------------------------------------------------------------------------------------------
 
-// Which end of an in-block pair lists it: alternate by the parity of i + j so every particle
-// owns about half of its in-block neighbours whatever its place in the block.
-__device__ __forceinline__ bool owns(uint32_t i, uint32_t j) {
-    uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-    return (((i ^ j) & 1u) ? lo : hi) == i;
-}
-
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
@@ -450,6 +308,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
 #pragma unroll
         for (int a = 0; a < D; a++) {
             int v = (int)c[a] + off[a];
+            if (P.nc[a] == 1u && off[a] != 0) skip = true;
             if (v < 0 || v >= (int)P.nc[a]) {
                 if (!((P.periodic >> a) & 1u)) skip = true;
                 v = v < 0 ? v + (int)P.nc[a] : v - (int)P.nc[a];
@@ -466,6 +325,11 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
 #pragma unroll
             for (int a = 0; a < D; a++) {
                 float d = B.rel[a][j] + shift[a];
+                if (P.nc[a] == 1u && ((P.periodic >> a) & 1u)) {
+                    // one cell spans the axis: the nearest image (cell_w is the box length)
+                    if (d > 0.5f * P.cell_w[a]) d -= P.cell_w[a];
+                    else if (d < -0.5f * P.cell_w[a]) d += P.cell_w[a];
+                }
                 r2 = __builtin_fmaf(d, d, r2);
             }
             float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
@@ -489,38 +353,6 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     }
     atomicMax(&B.ctl[CTL_MAXCOUNT], count);
     B.nbr_count[i] = na | (nb << 16);
-}
-
-// Kicks of one pair on both ends: returns k_i, adds k_j into kj.  Same operations as pair_kick;
-// the partner's kick is rounded separately with its own factor (int_engine.rs:17-22): for equal
-// factors it is the conversion of -rint(a f), the exact negation of this end's rounded value.
-template <int D, bool MULTI>
-__device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
-    float r2 = f[0] * f[0] + f[1] * f[1];
-    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
-#pragma unroll
-    for (int ax = 0; ax < D; ax++) kj[ax] = 0;
-    if (r2 < rc2) {
-        float s2 = sigma2 / r2;
-        float s6 = s2 * s2 * s2;
-        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
-        float a = -fr * ci;
-        if constexpr (MULTI) {
-            float aj = -fr * cj;
-#pragma unroll
-            for (int ax = 0; ax < D; ax++) {
-                ki[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
-                kj[ax] = cvt_sat_i32(__builtin_rintf(aj * -f[ax]));
-            }
-        } else {
-#pragma unroll
-            for (int ax = 0; ax < D; ax++) {
-                float r = __builtin_rintf(a * f[ax]);
-                ki[ax] += (uint32_t)cvt_sat_i32(r);
-                kj[ax] = cvt_sat_i32(-r);
-            }
-        }
-    }
 }
 
 template <int D, bool MULTI, int BS>
@@ -661,6 +493,7 @@ __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, con
     uint32_t cell[D];
     cell_of<D>(P, xi, cell);
     for (int o = 0; o < stencil_size(D); o++) {
+        if (stencil_dup<D>(P, o)) continue;
         uint32_t cc[D];
         stencil_cell<D>(P, cell, o, cc);
         uint32_t key = morton<D>(cc);
@@ -818,7 +651,12 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
     uint32_t ncmax = 1;
     for (int a = 0; a < D; a++) {
         P.nc[a] = core::sat_u32(std::floor(b[a] / (rcm + skin + margin)));
-        RH_REQUIRE(P.nc[a] >= 3, "box too small for the cutoff (fewer than 3 cells on an axis)");
+        if (P.nc[a] < 3) {
+            // Fewer than three cells fit: one cell spans the axis (every pair along it is a
+            // candidate).  The minimum image stays unique while L exceeds twice the list radius.
+            RH_REQUIRE(b[a] > 2.0 * (rcm + skin) * (1.0 + 1e-4) + margin, "box too small: shorter than twice the cutoff plus skin");
+            P.nc[a] = 1;
+        }
         ncmax = std::max(ncmax, P.nc[a]);
         P.box[a] = sys.box_len[a];
         P.half[a] = sys.box_len[a] / 2;

@@ -1,0 +1,201 @@
+// Device arithmetic shared by the single-system and batched engines: Morton keys, cell
+// assignment, the exact minimum image, the reference's f32 separation, the per-pair integer kick
+// (md-ref's operation order), soft walls and the wrap / mirror of the leapfrog.  Functions take
+// any parameter struct with the fields they read (box, half, periodic, nc, cell_scale,
+// soft_walls).
+#pragma once
+
+#include "rh_common.h"
+
+namespace rh {
+
+constexpr float SOFT_RC_F = 1.122462048309373f;
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    uint32_t x = v & 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {
+    uint32_t x = v & 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+template <int D>
+__device__ __forceinline__ uint32_t morton(const uint32_t c[D]) {
+    if constexpr (D == 3) return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+    else return spread2(c[0]) | (spread2(c[1]) << 1);
+}
+
+template <int D, class PT>
+__device__ __forceinline__ void cell_of(const PT& P, const int64_t x[D], uint32_t c[D]) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double f = (double)x[a] * P.cell_scale[a];
+        uint32_t ci = f > 0.0 ? (uint32_t)f : 0u;
+        c[a] = ci < P.nc[a] - 1 ? ci : P.nc[a] - 1;
+    }
+}
+
+// Minimum image (periodic axes) of b - a, exactly fixed::min_image (fixed.rs:31-40).
+// Branch-free: the two corrections exclude each other, as the reference's if / else-if do.
+template <int D, class PT>
+__device__ __forceinline__ int64_t separation(const PT& P, int64_t a, int64_t b, int ax) {
+    int64_t d = b - a;
+    if ((P.periodic >> ax) & 1u) {
+        int64_t down = d >= P.half[ax] ? -P.box[ax] : int64_t(0);
+        int64_t up = d < -P.half[ax] ? P.box[ax] : int64_t(0);
+        d += down + up;
+    }
+    return d;
+}
+
+// fixed::sep_f32 (fixed.rs:55-61): |d| >> 4 as f32 times 2^-28, with the sign of d; false if
+// |d| >= 8 sigma.  Truncating d/16 toward zero and converting the signed value rounds
+// symmetrically, so this equals the reference bit for bit (a tiny negative d gives +0 instead
+// of -0, which no kick or energy can tell apart).
+__device__ __forceinline__ bool sep_f(int64_t d, float& f) {
+    bool ok = (uint64_t)(d + (int64_t(8) << 32)) < (uint64_t(16) << 32);
+    int64_t q = (d + ((d >> 63) & 15)) >> 4;
+    f = (float)(int32_t)q * 0x1p-28f;
+    return ok;
+}
+
+// One pair's kick on particle i, in PairConsts::eval3's operation order (real.rs:145-156);
+// the factor 6*eps4 is the same f32 product md-ref forms, tabulated per species pair.
+template <int D>
+__device__ __forceinline__ void pair_kick(const float f[D], float rc2, float sigma2, float six_eps4, float c, uint32_t k[D]) {
+    float r2 = f[0] * f[0] + f[1] * f[1];
+    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+    if (r2 < rc2) {
+        float s2 = sigma2 / r2;
+        float s6 = s2 * s2 * s2;
+        // 2*s6 is exact, so one fused op rounds (2*s6 - 1) exactly once, as the separate ops do.
+        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float a = -fr * c;
+#pragma unroll
+        for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
+    }
+}
+
+// The same pair with its energy u = eps4*s6*(s6-1) - shift.
+template <int D>
+__device__ __forceinline__ void pair_kick_pe(const float f[D], const PairTable* __restrict__ pt, int p, float c, uint32_t k[D], double& pe) {
+    float r2 = f[0] * f[0] + f[1] * f[1];
+    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+    if (r2 < pt->rc2[p]) {
+        float s2 = pt->sigma2[p] / r2;
+        float s6 = s2 * s2 * s2;
+        float fr = pt->six_eps4[p] * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float a = -fr * c;
+#pragma unroll
+        for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
+        float u = pt->eps4[p] * s6 * (s6 - 1.0f) - pt->shift[p];
+        pe += (double)u;
+    }
+}
+
+// Soft (WCA) walls on the centre's distance from each wall (int_engine3.rs:106-126).
+template <int D, class PT>
+__device__ __forceinline__ void soft_walls(const PT& P, const int64_t x[D], float c, uint32_t k[D], double* wall_pe) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        for (int side = 0; side < 2; side++) {
+            if (!((P.soft_walls >> (2 * a + side)) & 1u)) continue;
+            int64_t d = side == 0 ? x[a] : P.box[a] - x[a];
+            float df;
+            if (!sep_f(d, df)) continue;
+            if (df < SOFT_RC_F) {
+                float s2 = 1.0f / (df * df);
+                float s6 = s2 * s2 * s2;
+                float fw = 24.0f * s6 * (2.0f * s6 - 1.0f) / df;
+                int32_t kw = cvt_sat_i32(__builtin_rintf(fw * c));
+                k[a] += side == 0 ? (uint32_t)kw : (uint32_t)0 - (uint32_t)kw;
+                if (wall_pe) *wall_pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
+            }
+        }
+    }
+}
+
+// Wrap (periodic) or mirror about -1/2 and L-1/2 (walls), walls::mirror (walls.rs:27-35);
+// a mirrored particle's velocity on that axis is reversed (hard and soft walls).
+template <int D, class PT>
+__device__ __forceinline__ void bound(const PT& P, int64_t x[D], int32_t v[D]) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int64_t l = P.box[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+        } else if (x[a] < 0) {
+            x[a] = -1 - x[a];
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        } else if (x[a] >= l) {
+            x[a] = 2 * l - 1 - x[a];
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        }
+    }
+}
+
+template <int D, class PT>
+__device__ __forceinline__ void stencil_cell(const PT& P, const uint32_t c[D], int o, uint32_t out[D]) {
+    int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int v = (int)c[a] + off[a];
+        if (v < 0) v += (int)P.nc[a];
+        else if (v >= (int)P.nc[a]) v -= (int)P.nc[a];
+        out[a] = (uint32_t)v;
+    }
+}
+
+constexpr int stencil_size(int d) { return d == 3 ? 27 : 9; }
+
+// Which end of an in-block pair lists it: alternate by the parity of i + j so every particle
+// owns about half of its in-block neighbours whatever its place in the block.
+__device__ __forceinline__ bool owns(uint32_t i, uint32_t j) {
+    uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    return (((i ^ j) & 1u) ? lo : hi) == i;
+}
+
+// Kicks of one pair on both ends: returns k_i, adds k_j into kj.  Same operations as pair_kick;
+// the partner's kick is rounded separately with its own factor (int_engine.rs:17-22): for equal
+// factors it is the conversion of -rint(a f), the exact negation of this end's rounded value.
+template <int D, bool MULTI>
+__device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
+    float r2 = f[0] * f[0] + f[1] * f[1];
+    if constexpr (D == 3) r2 = r2 + f[2] * f[2];
+#pragma unroll
+    for (int ax = 0; ax < D; ax++) kj[ax] = 0;
+    if (r2 < rc2) {
+        float s2 = sigma2 / r2;
+        float s6 = s2 * s2 * s2;
+        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float a = -fr * ci;
+        if constexpr (MULTI) {
+            float aj = -fr * cj;
+#pragma unroll
+            for (int ax = 0; ax < D; ax++) {
+                ki[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
+                kj[ax] = cvt_sat_i32(__builtin_rintf(aj * -f[ax]));
+            }
+        } else {
+#pragma unroll
+            for (int ax = 0; ax < D; ax++) {
+                float r = __builtin_rintf(a * f[ax]);
+                ki[ax] += (uint32_t)cvt_sat_i32(r);
+                kj[ax] = cvt_sat_i32(-r);
+            }
+        }
+    }
+}
+
+}  // namespace rh

@@ -635,7 +635,12 @@ static void free_cells3(cells3 *c) {
         cells3 c = {{0, 0, 0}, 0, 0, 0};                                                                      \
         if (!brute) {                                                                                         \
             int rc_ = build_cells3(s, n, pos, &c);                                                            \
-            if (rc_) {                                                                                        \
+            if (rc_ == -1) {                                                                                  \
+                /* fewer than 3 cells: md-ref asserts; every pair is a candidate instead */                  \
+                free_cells3(&c);                                                                              \
+                memset(&c, 0, sizeof c);                                                                      \
+                brute = 1;                                                                                    \
+            } else if (rc_) {                                                                                 \
                 free_cells3(&c);                                                                              \
                 return rc_;                                                                                   \
             }                                                                                                 \
@@ -1136,8 +1141,9 @@ static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *po
     uint32_t nc[2];
     present_cells2(s, n, sp, nc);
     if (nc[0] < 3 || nc[1] < 3) {
-        free(tab);
-        return -1;
+        /* fewer than 3 cells: md-ref asserts; every pair is a candidate instead */
+        brute = 1;
+        nc[0] = nc[1] = 1;
     }
     uint64_t ncell = (uint64_t)nc[0] * nc[1];
     uint64_t *start = calloc(ncell + 1, sizeof(uint64_t));
