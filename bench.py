@@ -112,7 +112,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-n", type=int, default=262144, help="atoms in the CPU-baseline sample")
     ap.add_argument("--variant", type=int, default=1, help="0 full Verlet list, 1 half list in blocks")
-    ap.add_argument("--block", type=int, default=512, help="half-list block size (256, 512, 1024)")
+    ap.add_argument("--block", type=int, default=1024, help="half-list block size (256, 512, 1024)")
     args = ap.parse_args()
 
     if args.gpus != 1:

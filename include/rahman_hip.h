@@ -206,6 +206,12 @@ int rh_prims_atomic_add_i64(const int64_t* d_vals, uint64_t n, int64_t* d_acc, v
 /* The engine's float -> integer kick conversion on device: out[i] = i32(rint(x[i])), saturating,
  * NaN -> 0 (Rust `as i32` after round_ties_even). */
 int rh_prims_kick_convert(const float* d_x, int32_t* d_out, uint64_t n, void* stream);
+/* The pair kick divides by r2 twice with one shared, refined reciprocal when r2 >= r2_fast
+ * (derived per species pair); below it, IEEE division.  rh_pair_division_floor gives r2_fast for
+ * a pair; rh_prims_check_pair_division compares the shared-reciprocal result with IEEE division
+ * for EVERY f32 value of r2 in [r2_fast, rc^2) on the device and counts differing results. */
+int rh_pair_division_floor(double eps, double sigma, double rc, float* r2_fast);
+int rh_prims_check_pair_division(double eps, double sigma, double rc, uint64_t* checked, uint64_t* mismatches, float* first_bad, void* stream);
 
 #ifdef __cplusplus
 }

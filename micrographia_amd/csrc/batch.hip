@@ -296,7 +296,7 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C) {
             si = sp[local];
             c = pt.kick_c[si];
         }
-        const float rc2_0 = pt.rc2[0], sigma2_0 = pt.sigma2[0], six_eps4_0 = pt.six_eps4[0];
+        const float rc2_0 = pt.rc2[0], sigma2_0 = pt.sigma2[0], six_eps4_0 = pt.six_eps4[0], r2f_0 = pt.r2_fast[0];
         const uint32_t cnt = B.nbr_count[slot];
         const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
         const uint32_t* col = B.nbr + S.nbr0 + local;
@@ -316,9 +316,9 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C) {
             if constexpr (MULTI) {
                 uint32_t sj = sp[j];
                 uint32_t p = si * 4 + sj;
-                pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], c, pt.kick_c[sj], k, kj);
+                pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
             } else {
-                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, c, c, k, kj);
+                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
             }
             const uint32_t lj = j - base;
 #pragma unroll
@@ -338,9 +338,9 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C) {
             if (rc && !ok) continue;
             if constexpr (MULTI) {
                 uint32_t p = si * 4 + sp[j];
-                pair_kick<D>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], c, k);
+                pair_kick<D>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, k);
             } else {
-                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, c, k);
+                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
             }
         }
         if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
@@ -601,6 +601,7 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
             d.pt.six_eps4[k] = 6.0f * d.pt.eps4[k];
             double r = (p.rc + skin) * (1.0 + 1e-4);
             d.pt.list_r2[k] = (float)(r * r);
+            d.pt.r2_fast[k] = fast_division_floor(d.pt.rc2[k], d.pt.sigma2[k], d.pt.six_eps4[k]);
         }
     double hs = 0.5 * skin * (1.0 - 1e-3);
     d.half_skin2 = (float)(hs * hs);

@@ -9,6 +9,7 @@
 
 namespace rh {
 void kick_convert(const float* x, int32_t* out, uint64_t n, hipStream_t s);
+void check_division(float r2_lo, float r2_hi, float sigma2, float six_eps4, uint64_t* checked, uint64_t* mismatches, float* first_bad, hipStream_t s);
 }
 
 struct rh_engine3 {
@@ -401,6 +402,22 @@ int rh_prims_atomic_add_i64(const int64_t* d_vals, uint64_t n, int64_t* d_acc, v
     return guarded([&] {
         rh::atomic_add_i64(d_vals, n, d_acc, (hipStream_t)stream);
         RH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+int rh_pair_division_floor(double eps, double sigma, double rc, float* r2_fast) {
+    return guarded([&] {
+        float rc2 = (float)(rc * rc), sigma2 = (float)(sigma * sigma), six_eps4 = 6.0f * (float)(4.0 * eps);
+        *r2_fast = rh::fast_division_floor(rc2, sigma2, six_eps4);
+    });
+}
+
+int rh_prims_check_pair_division(double eps, double sigma, double rc, uint64_t* checked, uint64_t* mismatches, float* first_bad, void* stream) {
+    return guarded([&] {
+        float rc2 = (float)(rc * rc), sigma2 = (float)(sigma * sigma), six_eps4 = 6.0f * (float)(4.0 * eps);
+        float lo = rh::fast_division_floor(rc2, sigma2, six_eps4);
+        RH_REQUIRE(lo < rc2, "this pair never takes the fast division");
+        rh::check_division(lo, rc2, sigma2, six_eps4, checked, mismatches, first_bad, (hipStream_t)stream);
     });
 }
 

@@ -6,6 +6,7 @@
 #include <limits>
 #include <vector>
 
+#include "engine.h"
 #include "prims.h"
 
 namespace rh::core {
@@ -292,6 +293,32 @@ int32_t to_step_displacement(double dt, double v) { return sat_i32(std::round(v 
 int32_t gravity_kick(double dt, double g) { return sat_i32(std::round(g * dt * dt * ONE_F)); }
 
 }  // namespace rh::core
+
+namespace rh {
+
+// Bounds for pair_fr's fast path (all quantities f32, as the kernel forms them):
+//   s2 = sigma2 / r2 needs no scaling when sigma2 is 0 or in [2^-60, 2^30] and r2 in
+//   [2^-24, 64]: quotient and reciprocal normal, exponent gap below 96;
+//   X = six_eps4 s6 (2 s6 - 1) has |X| <= 2 |six_eps4| max(s6, s6^2); with |X| < 2^64 and
+//   r2 >= 2^-24 the gap stays below 96 and X / r2 below 2^88;
+//   a nonzero X is at least |six_eps4| (sigma2/rc2)^3 2^-24 (2 s6 - 1 is a multiple of 2^-24
+//   or exceeds 1/2 in magnitude), which must stay above 2^-100 (numerator and quotient normal).
+float fast_division_floor(float rc2, float sigma2, float six_eps4) {
+    const double inf = std::numeric_limits<double>::infinity();
+    double s = sigma2, e = std::fabs((double)six_eps4), rc = rc2;
+    if (!(rc > 0.0) || rc > 64.0) return (float)inf;
+    if (s != 0.0 && (s < std::ldexp(1.0, -60) || s > std::ldexp(1.0, 30))) return (float)inf;
+    if (e != 0.0 && s != 0.0 && e * std::pow(s / rc, 3.0) * std::ldexp(1.0, -24) < std::ldexp(1.0, -99)) return (float)inf;
+    double floor2 = std::ldexp(1.0, -24);
+    if (e != 0.0 && s != 0.0) {
+        // 2 e s6max^2 < 2^63 (a factor two of margin) with s6max = (s / r2)^3
+        double r2min = s * std::pow(2.0 * e / std::ldexp(1.0, 63), 1.0 / 6.0);
+        floor2 = std::max(floor2, r2min * 1.01);
+    }
+    return (float)floor2;
+}
+
+}  // namespace rh
 
 // ---------------------------------------------------------------------------------------------
 // Initial conditions behind the C ABI.

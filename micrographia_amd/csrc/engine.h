@@ -34,6 +34,7 @@ enum : uint32_t {
 struct PairTable {
     float rc2[16], eps4[16], six_eps4[16], sigma2[16], shift[16], list_r2[16];
     float kick_c[4];
+    float r2_fast[16];  // pairs with r2 >= this use the shared-reciprocal division (0: never)
 };
 
 template <int D>
@@ -66,7 +67,7 @@ struct Bufs {
     uint32_t* species[2];
     uint32_t* ids[2];
     int64_t* ref[D];     // positions at the last list build
-    float* rel[D];       // position relative to its cell's corner (sigma, f32), for list builds
+    float4* rel;         // position relative to its cell's corner (sigma, f32; w unused), for list builds
     uint32_t* nbr;       // column-major: entry k of slot i at k*n + i
     uint32_t* nbr_count;
     uint32_t* cell_start;
@@ -77,6 +78,10 @@ struct Bufs {
     uint32_t* ctl;
     const PairTable* pt;
 };
+
+// The smallest r2 of a species pair from which the shared-reciprocal division of the pair kick
+// provably equals IEEE division (see pair_fr); +inf if none.
+float fast_division_floor(float rc2, float sigma2, float six_eps4);
 
 struct SystemDesc {
     int dim;

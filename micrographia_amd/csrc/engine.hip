@@ -139,8 +139,10 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
     }
     uint32_t c[D];
     cell_of<D>(P, xs, c);
+    float r[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; a++) B.rel[a][k] = (float)((double)xs[a] * 0x1p-32 - (double)c[a] * P.cell_w64[a]);
+    for (int a = 0; a < D; a++) r[a] = (float)((double)xs[a] * 0x1p-32 - (double)c[a] * P.cell_w64[a]);
+    B.rel[k] = make_float4(r[0], r[1], r[2], 0.f);
     B.species[pm ^ 1][k] = B.species[pm][i];
     B.ids[pm ^ 1][k] = B.ids[pm][i];
 }
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     uint32_t si = MULTI ? sp[i] : 0u;
     const PairTable* __restrict__ pt = B.pt;
     float c = MULTI ? pt->kick_c[si] : P.kick_c[0];
-    const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0];
+    const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
     uint32_t k[D];
     for (int a = 0; a < D; a++) k[a] = 0u;
     const uint32_t m = B.nbr_count[i];
@@ -240,9 +242,9 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
         for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
         if constexpr (MULTI) {
             uint32_t p = si * 4 + sp[j];
-            if (ok) pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], c, k);
+            if (ok) pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
         } else {
-            if (ok) pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, c, k);
+            if (ok) pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
         }
     }
     if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
@@ -288,22 +290,56 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     if (i >= P.n) return;
     uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
     const uint32_t* sp = B.species[pm];
+    const float4* __restrict__ rel = B.rel;
     int64_t xi[D];
-    float ri[D];
-    for (int a = 0; a < D; a++) {
-        xi[a] = B.pos[pp][a][i];
-        ri[a] = B.rel[a][i];
-    }
+    for (int a = 0; a < D; a++) xi[a] = B.pos[pp][a][i];
+    const float4 me = rel[i];
+    const float ri[3] = {me.x, me.y, me.z};
     uint32_t si = MULTI ? sp[i] : 0u;
     const float list_r2 = B.pt->list_r2[0];
     uint32_t c[D];
     cell_of<D>(P, xi, c);
     const uint32_t cap = P.cap, n = P.n, blk = i / P.block;
+    uint32_t* col = B.nbr + i;
+    bool wrap[3] = {false, false, false};
+    bool any_wrap = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        wrap[a] = P.nc[a] == 1u && ((P.periodic >> a) & 1u);
+        any_wrap |= wrap[a];
+    }
     uint32_t na = 0, nb = 0;
+    auto consider = [&](uint32_t j, float4 q, const float* shift) {
+        float d[3] = {q.x + shift[0], q.y + shift[1], q.z + shift[2]};
+        if (any_wrap) {
+#pragma unroll
+            for (int a = 0; a < D; a++)
+                if (wrap[a]) {
+                    // one cell spans the axis: the nearest image (cell_w is the box length)
+                    if (d[a] > 0.5f * P.cell_w[a]) d[a] -= P.cell_w[a];
+                    else if (d[a] < -0.5f * P.cell_w[a]) d[a] += P.cell_w[a];
+                }
+        }
+        float r2 = d[0] * d[0];
+#pragma unroll
+        for (int a = 1; a < D; a++) r2 = __builtin_fmaf(d[a], d[a], r2);
+        float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
+        if (r2 < lr2 && j != i) {
+            if (j / P.block == blk) {
+                if (owns(i, j)) {
+                    if (na + nb < cap) col[(size_t)na * n] = j;
+                    na++;
+                }
+            } else {
+                if (na + nb < cap) col[(size_t)(cap - 1 - nb) * n] = j;
+                nb++;
+            }
+        }
+    };
     for (int o = 0; o < stencil_size(D); o++) {
         int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
         uint32_t cc[D];
-        float shift[D];
+        float shift[3] = {0.f, 0.f, 0.f};
         bool skip = false;
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -319,32 +355,13 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         if (skip) continue;
         uint32_t key = morton<D>(cc);
         uint32_t s = B.cell_start[key], e = B.cell_end[key];
-        for (uint32_t j = s; j < e; j++) {
-            if (j == i) continue;
-            float r2 = 0.0f;
-#pragma unroll
-            for (int a = 0; a < D; a++) {
-                float d = B.rel[a][j] + shift[a];
-                if (P.nc[a] == 1u && ((P.periodic >> a) & 1u)) {
-                    // one cell spans the axis: the nearest image (cell_w is the box length)
-                    if (d > 0.5f * P.cell_w[a]) d -= P.cell_w[a];
-                    else if (d < -0.5f * P.cell_w[a]) d += P.cell_w[a];
-                }
-                r2 = __builtin_fmaf(d, d, r2);
-            }
-            float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
-            if (r2 < lr2) {
-                if (j / P.block == blk) {
-                    if (owns(i, j)) {
-                        if (na + nb < cap) B.nbr[(size_t)na * n + i] = j;
-                        na++;
-                    }
-                } else {
-                    if (na + nb < cap) B.nbr[(size_t)(cap - 1 - nb) * n + i] = j;
-                    nb++;
-                }
-            }
+        uint32_t j = s;
+        for (; j + 1 < e; j += 2) {
+            float4 q0 = rel[j], q1 = rel[j + 1];
+            consider(j, q0, shift);
+            consider(j + 1, q1, shift);
         }
+        if (j < e) consider(j, rel[j], shift);
     }
     uint32_t count = na + nb;
     if (count > cap) {
@@ -386,7 +403,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             si = sp[i];
             c = pt->kick_c[si];
         }
-        const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0];
+        const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
         const uint32_t cnt = B.nbr_count[i];
         const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
         const uint32_t* col = B.nbr + i;
@@ -409,9 +426,9 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             if constexpr (MULTI) {
                 uint32_t sj = sp[j];
                 uint32_t p = si * 4 + sj;
-                pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], c, pt->kick_c[sj], k, kj);
+                pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
             } else {
-                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, c, c, k, kj);
+                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
             }
             const uint32_t lj = j - base;
 #pragma unroll
@@ -434,9 +451,9 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             if (rc && !ok) continue;
             if constexpr (MULTI) {
                 uint32_t p = si * 4 + sp[j];
-                pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], c, k);
+                pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
             } else {
-                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, c, k);
+                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
             }
         }
         if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
@@ -690,6 +707,7 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
             P.pt.six_eps4[k] = 6.0f * P.pt.eps4[k];
             double r = (p.rc + skin) * (1.0 + 1e-4);
             P.pt.list_r2[k] = (float)(r * r);
+            P.pt.r2_fast[k] = fast_division_floor(P.pt.rc2[k], P.pt.sigma2[k], P.pt.six_eps4[k]);
         }
     for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
     P.cap = mode.kind == 1 ? mode.cap : 0;
@@ -781,9 +799,9 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     for (int a = 0; a < D; a++) {
         B_.ref[a] = (int64_t*)dalloc(n * sizeof(int64_t));
         B_.kicks[a] = (int32_t*)dalloc(n * sizeof(int32_t));
-        B_.rel[a] = (float*)dalloc(n * sizeof(float));
     }
     B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
+    B_.rel = (float4*)dalloc(n * sizeof(float4));
     B_.cell_start = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.cell_end = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.pe = (double*)dalloc(n * sizeof(double));
@@ -1255,6 +1273,49 @@ template class Engine<3>;
 __global__ void k_kick_convert(const float* x, int32_t* out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
     if (i < n) out[i] = cvt_sat_i32(__builtin_rintf(x[i]));
+}
+
+// Every f32 r2 in [r2_lo, r2_hi): the shared-reciprocal path of pair_fr against IEEE division,
+// bit for bit (fr depends only on r2 for a given species pair, so this is exhaustive).
+__global__ void k_check_division(uint32_t lo_bits, uint32_t count, float sigma2, float six_eps4, unsigned long long* mism, uint32_t* first) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        float r2 = __uint_as_float(lo_bits + k);
+        float y = rcp_refined(r2);
+        float s2f = div_refined(sigma2, r2, y);
+        float s6f = s2f * s2f * s2f;
+        float frf = div_refined(six_eps4 * s6f * __builtin_fmaf(2.0f, s6f, -1.0f), r2, y);
+        float s2 = sigma2 / r2;
+        float s6 = s2 * s2 * s2;
+        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        if (__float_as_uint(frf) != __float_as_uint(fr) || __float_as_uint(s2f) != __float_as_uint(s2)) {
+            atomicAdd(mism, 1ull);
+            atomicMin(first, lo_bits + k);
+        }
+    }
+}
+
+void check_division(float r2_lo, float r2_hi, float sigma2, float six_eps4, uint64_t* checked, uint64_t* mismatches, float* first_bad, hipStream_t s) {
+    uint32_t lo = __builtin_bit_cast(uint32_t, r2_lo), hi = __builtin_bit_cast(uint32_t, r2_hi);
+    RH_REQUIRE(r2_lo > 0.0f && r2_hi > r2_lo, "need 0 < r2_lo < r2_hi");
+    uint32_t count = hi - lo;
+    unsigned long long* d_m = nullptr;
+    uint32_t* d_f = nullptr;
+    RH_HIP(hipMalloc(&d_m, 8));
+    RH_HIP(hipMalloc(&d_f, 4));
+    RH_HIP(hipMemsetAsync(d_m, 0, 8, s));
+    RH_HIP(hipMemsetAsync(d_f, 0xff, 4, s));
+    k_check_division<<<4096, 256, 0, s>>>(lo, count, sigma2, six_eps4, d_m, d_f);
+    RH_HIP(hipGetLastError());
+    unsigned long long m = 0;
+    uint32_t f = 0;
+    RH_HIP(hipMemcpyAsync(&m, d_m, 8, hipMemcpyDeviceToHost, s));
+    RH_HIP(hipMemcpyAsync(&f, d_f, 4, hipMemcpyDeviceToHost, s));
+    RH_HIP(hipStreamSynchronize(s));
+    (void)hipFree(d_m);
+    (void)hipFree(d_f);
+    *checked = count;
+    *mismatches = m;
+    *first_bad = m ? __builtin_bit_cast(float, f) : 0.0f;
 }
 
 void kick_convert(const float* x, int32_t* out, uint64_t n, hipStream_t s) {

@@ -69,17 +69,51 @@ __device__ __forceinline__ bool sep_f(int64_t d, float& f) {
     return ok;
 }
 
-// One pair's kick on particle i, in PairConsts::eval3's operation order (real.rs:145-156);
-// the factor 6*eps4 is the same f32 product md-ref forms, tabulated per species pair.
+// The reciprocal and the division steps of the hardware's correctly rounded f32 division
+// (v_rcp, one refinement of the reciprocal, then two correction steps), without its operand
+// scaling and special-case fixup.  For operands where the hardware sequence scales nothing
+// (normal divisor and quotient, exponent gap below 96, numerator above 2^-103) and the quotient
+// is normal, this IS that sequence, so the result is the correctly rounded quotient.  The
+// reciprocal is shared by the two divisions by r2 of a pair.
+__device__ __forceinline__ float rcp_refined(float b) {
+    float y0 = __builtin_amdgcn_rcpf(b);
+    float e = __builtin_fmaf(-b, y0, 1.0f);
+    return __builtin_fmaf(e, y0, y0);
+}
+
+__device__ __forceinline__ float div_refined(float a, float b, float y) {
+    float q0 = a * y;
+    float r0 = __builtin_fmaf(-b, q0, a);
+    float q1 = __builtin_fmaf(r0, y, q0);
+    float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y, q1);
+}
+
+// fr = 6 eps4 s6 (2 s6 - 1) / r2 with s2 = sigma2 / r2, in PairConsts::eval3's operation order
+// (real.rs:145-156); 2*s6 is exact, so one fused op rounds (2*s6 - 1) once, as the separate ops
+// do.  r2 >= r2_fast (checked for the whole wave) guarantees the shared-reciprocal division
+// equals IEEE division (the host derives r2_fast per species pair); otherwise IEEE '/'.
+__device__ __forceinline__ float pair_fr(float r2, float sigma2, float six_eps4, float r2_fast) {
+    bool slow = !(r2 >= r2_fast);
+    if (ballot64(slow) == 0ull) {
+        float y = rcp_refined(r2);
+        float s2 = div_refined(sigma2, r2, y);
+        float s6 = s2 * s2 * s2;
+        return div_refined(six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f), r2, y);
+    }
+    float s2 = sigma2 / r2;
+    float s6 = s2 * s2 * s2;
+    return six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+}
+
+// One pair's kick on particle i: per-axis rint and the saturating conversion, summed with
+// wrapping adds (int_engine3.rs:94-101); the factor 6*eps4 is the same f32 product md-ref forms.
 template <int D>
-__device__ __forceinline__ void pair_kick(const float f[D], float rc2, float sigma2, float six_eps4, float c, uint32_t k[D]) {
+__device__ __forceinline__ void pair_kick(const float f[D], float rc2, float sigma2, float six_eps4, float r2_fast, float c, uint32_t k[D]) {
     float r2 = f[0] * f[0] + f[1] * f[1];
     if constexpr (D == 3) r2 = r2 + f[2] * f[2];
     if (r2 < rc2) {
-        float s2 = sigma2 / r2;
-        float s6 = s2 * s2 * s2;
-        // 2*s6 is exact, so one fused op rounds (2*s6 - 1) exactly once, as the separate ops do.
-        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float fr = pair_fr(r2, sigma2, six_eps4, r2_fast);
         float a = -fr * c;
 #pragma unroll
         for (int ax = 0; ax < D; ax++) k[ax] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * f[ax]));
@@ -166,19 +200,17 @@ __device__ __forceinline__ bool owns(uint32_t i, uint32_t j) {
     return (((i ^ j) & 1u) ? lo : hi) == i;
 }
 
-// Kicks of one pair on both ends: returns k_i, adds k_j into kj.  Same operations as pair_kick;
-// the partner's kick is rounded separately with its own factor (int_engine.rs:17-22): for equal
-// factors it is the conversion of -rint(a f), the exact negation of this end's rounded value.
+// Kicks of one pair on both ends: adds k_i into ki and returns k_j in kj.  The partner's kick is
+// rounded separately with its own factor (int_engine.rs:17-22): for equal factors it is the
+// conversion of -rint(a f), the exact negation of this end's rounded value.
 template <int D, bool MULTI>
-__device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
+__device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float r2_fast, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
     float r2 = f[0] * f[0] + f[1] * f[1];
     if constexpr (D == 3) r2 = r2 + f[2] * f[2];
 #pragma unroll
     for (int ax = 0; ax < D; ax++) kj[ax] = 0;
     if (r2 < rc2) {
-        float s2 = sigma2 / r2;
-        float s6 = s2 * s2 * s2;
-        float fr = six_eps4 * s6 * __builtin_fmaf(2.0f, s6, -1.0f) / r2;
+        float fr = pair_fr(r2, sigma2, six_eps4, r2_fast);
         float a = -fr * ci;
         if constexpr (MULTI) {
             float aj = -fr * cj;

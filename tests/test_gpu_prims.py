@@ -122,3 +122,17 @@ def test_kick_conversion_on_the_hardware():
     got = out.cpu().numpy()
     want = np.array([rust_kick(float(v)) for v in xs], np.int64)
     assert np.array_equal(got.astype(np.int64), want)
+
+
+@pytest.mark.parametrize("eps,sigma,rc", [(1.0, 1.0, 2.5), (1.0, 1.0, 2.0 ** (1 / 6)), (0.5, 1.2, 3.0), (0.7, 1.05, 2.5), (1.0, 0.9, 2.2)])
+def test_shared_reciprocal_division_is_ieee_for_every_r2(eps, sigma, rc):
+    """Exhaustive: for every f32 r2 the fast path handles, the pair's two divisions equal IEEE
+    division bit for bit (so the kicks equal md-ref's)."""
+    torch_cuda()
+    import ctypes
+
+    checked, mism = ctypes.c_uint64(), ctypes.c_uint64()
+    first = ctypes.c_float()
+    _lib.check(_lib.load().rh_prims_check_pair_division(eps, sigma, rc, ctypes.byref(checked), ctypes.byref(mism), ctypes.byref(first), None))
+    assert checked.value > 10_000_000
+    assert mism.value == 0, f"first differing r2 {first.value!r}"

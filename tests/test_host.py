@@ -102,3 +102,13 @@ def test_verlet_capacity_rule():
     r = 2.8
     want = int(np.ceil(4 / 3 * np.pi * r ** 3 * max(2 * 0.8442, 1.2))) + 8
     assert m.kind == 1 and m.cap in (want, want + 1) and abs(m.skin - 0.3) < 1e-15
+
+
+def test_division_floor_bounds():
+    import ctypes
+
+    f = ctypes.c_float()
+    _lib.check(_lib.load().rh_pair_division_floor(1.0, 1.0, 2.5, ctypes.byref(f)))
+    assert 0 < f.value < 0.01  # r below 0.1 sigma falls back to IEEE division
+    _lib.check(_lib.load().rh_pair_division_floor(1e-35, 1.0, 2.5, ctypes.byref(f)))
+    assert f.value == float("inf")  # tiny forces: never the fast path
