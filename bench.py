@@ -88,6 +88,97 @@ def cpu_baseline(n: int, density: float, t: float, seed: int, budget_s: float = 
     }
 
 
+def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool):
+    """The batched ensemble: `count` independent LJ systems (2D/3D 50/50, N log-uniform in
+    [256, 65536], density U[0.3, 0.9], T U[0.5, 2.0], own seeds) advanced together by the batched
+    engine in hipGraph chunks of `chunk` steps, against the same systems stepped one at a time by
+    the single-system engines (one system per launch)."""
+    import torch
+
+    import micrographia_amd as rh
+    from micrographia_amd.batch import BatchEngine
+    from micrographia_amd.workloads import ensemble
+
+    specs = ensemble(count, meta_seed=meta_seed)
+    t0 = time.perf_counter()
+    built = [s.build() for s in specs]
+    atoms = sum(st.n() for _, st in built)
+    log(f"[bench] batched: {count} systems, {atoms} atoms, host init {time.perf_counter() - t0:.1f} s")
+    b = BatchEngine(budget_bytes=0, max_systems=max(count, 1), graph_chunk=chunk)
+    need = sum(b.bytes_needed(sys_, st.n()) for sys_, st in built)
+    ids = [b.add(sys_, st, steps + chunk) for sys_, st in built]
+    assert all(i >= 0 for i in ids), "the ensemble does not fit the arena"
+    b.step_all(chunk)  # warm-up chunk: first lists, graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step_all(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    left = b.poll()
+    assert all(v == 0 for v in left.values())
+    out = {
+        "workload": f"{count} independent LJ systems (2D/3D 50/50, N log-uniform [256, 65536], rho U[0.3,0.9], T U[0.5,2.0], rc 2.5, skin 0.3, dt 0.005), meta-seed {meta_seed}, hipGraph chunks of {chunk} steps",
+        "systems": count,
+        "atoms": atoms,
+        "steps": steps,
+        "seconds": el,
+        "atom_steps_per_s": atoms * steps / el,
+        "system_steps_per_s": count * steps / el,
+        "systems_per_s": count * steps / chunk / el,
+        "systems_per_s_unit": f"systems advanced by one {chunk}-step chunk per second",
+        "arena_bytes_needed": need,
+    }
+    b.close()
+    if plain:
+        # the same systems, one engine at a time: stepping timed, engine creation not
+        tot = 0.0
+        for sys_, st in built:
+            if isinstance(sys_, rh.System3):
+                e = rh.GpuEngine3(sys_, st, rh.verlet3_with_skin(sys_, st.n(), 0.3))
+            else:
+                e = rh.GpuEngine(sys_, st, rh.verlet_with_skin(sys_, st.n(), 0.3))
+            e.step(chunk)  # same warm-up as the batch
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            e.step(steps)
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t1
+            e.close()
+        out["plain_seconds"] = tot
+        out["plain_atom_steps_per_s"] = atoms * steps / tot
+        out["speedup_vs_plain"] = tot / el
+    log(f"[bench] batched: {out['atom_steps_per_s']:.4g} atom-steps/s" + (f", {out['speedup_vs_plain']:.1f}x plain" if plain else ""))
+    return out
+
+
+def serving_leg(rate: float, duration: float, meta_seed: int, plain: bool):
+    """The serving loop: Poisson arrivals at `rate`/s over `duration` s of LJ requests (N
+    log-uniform in [1k, 1M], 2D/3D 50/50, steps U[100, 10k], density U[0.3, 0.9], T U[0.5, 2.0],
+    own seeds), admitted against the device-memory budget, continuously batched."""
+    import torch
+
+    from micrographia_amd.serve import Server, serve_plain
+    from micrographia_amd.workloads import request_stream
+
+    reqs = request_stream(rate, duration, meta_seed=meta_seed)
+    free, total = torch.cuda.mem_get_info()
+    budget = int(free * 0.85)
+    log(f"[bench] serving: {len(reqs)} requests, budget {budget / 2**30:.1f} GiB")
+    srv = Server(budget_bytes=budget, max_systems=4096, chunk_s=0.02)
+    rep = srv.serve(reqs, realtime=True)
+    srv.close()
+    out = {"workload": f"Poisson {rate}/s over {duration} s (meta-seed {meta_seed}): N log-uniform [1k, 1M], 2D/3D 50/50, steps U[100, 10k], rho U[0.3, 0.9], T U[0.5, 2.0]; admission budget {budget} B of device memory",
+           **rep.summary()}
+    if plain:
+        p = serve_plain(reqs, realtime=True)
+        out["plain"] = p.summary()
+        for c, d in zip(sorted(rep.completed, key=lambda c: c.request.index), sorted(p.completed, key=lambda c: c.request.index)):
+            assert c.state_hash == d.state_hash, f"request {c.request.index} differs between the server and the plain engine"
+        out["identical_to_plain"] = True
+    log(f"[bench] serving: {out['atom_steps_per_s']:.4g} atom-steps/s, p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s")
+    return out
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -113,6 +204,12 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=262144, help="atoms in the CPU-baseline sample")
     ap.add_argument("--variant", type=int, default=1, help="0 full Verlet list, 1 half list in blocks")
     ap.add_argument("--block", type=int, default=1024, help="half-list block size (256, 512, 1024)")
+    ap.add_argument("--batch-systems", type=int, default=512, help="systems in the batched leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--serve-duration", type=float, default=0.2, help="seconds of Poisson arrivals in the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-system-at-a-time comparisons")
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -209,6 +306,19 @@ def main() -> int:
         "rebuild_ms_per_rebuild": rebuild_ms / max(rebuilds, 1),
         "pair_interactions_per_s": value * pairs_per_atom(args.density),
     }
+    eng.close()
+    del eng
+    if args.calibrate_copy:
+        x = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = args.calibrate_copy << 20
+    if args.batch_systems:
+        out["batched"] = batched_leg(args.batch_systems, args.batch_steps, 100, 2026, not args.no_plain)
+    if args.serve_duration > 0:
+        out["serving"] = serving_leg(args.serve_rate, args.serve_duration, 7, not args.no_plain)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, one thread)")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.density, args.temperature, args.seed, args.cpu_budget)

@@ -233,12 +233,13 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
 }
 
 __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t variant) {
-    uint32_t s = B.wg_sys[blockIdx.x];
+    const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t s = B.wg_sys[wg];
     if (s == ~0u) return;
     const BSys& S = B.sys[s];
     BCtl& C = B.ctl[s];
     if (!C.do_rebuild) return;
-    uint32_t local = blockIdx.x * BATCH_WG + threadIdx.x - S.slot0;
+    uint32_t local = wg * BATCH_WG + threadIdx.x - S.slot0;
     if (local >= S.n) return;
     bool multi = S.nspecies > 1;
     if (S.dim == 3) {
@@ -267,15 +268,15 @@ __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
 __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S.active && (C.stale | C.overflow) == 0u && C.remaining != 0u; }
 
 template <int D, bool MULTI>
-__device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C) {
+__device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t wg) {
     __shared__ uint32_t acc[D][BATCH_WG];
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int a = 0; a < D; a++) acc[a][t] = 0u;
     __syncthreads();
-    const uint32_t slot = blockIdx.x * BATCH_WG + t;
+    const uint32_t slot = wg * BATCH_WG + t;
     const uint32_t local = slot - S.slot0;
-    const uint32_t base = blockIdx.x * BATCH_WG - S.slot0;  // first local slot of this block
+    const uint32_t base = wg * BATCH_WG - S.slot0;  // first local slot of this block
     const bool live = local < S.n;
     const uint32_t pp = C.ppos, pm = C.pmeta;
     const int64_t* px[D];
@@ -374,18 +375,19 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C) {
 }
 
 __global__ __launch_bounds__(BATCH_WG) void k_bb_step(BBufs B) {
-    uint32_t s = B.wg_sys[blockIdx.x];
+    const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t s = B.wg_sys[wg];
     if (s == ~0u) return;
     const BSys& S = B.sys[s];
     BCtl& C = B.ctl[s];
     if (!bb_runs(S, C)) return;
     bool multi = S.nspecies > 1;
     if (S.dim == 3) {
-        if (multi) bb_step<3, true>(B, S, C);
-        else bb_step<3, false>(B, S, C);
+        if (multi) bb_step<3, true>(B, S, C, wg);
+        else bb_step<3, false>(B, S, C, wg);
     } else {
-        if (multi) bb_step<2, true>(B, S, C);
-        else bb_step<2, false>(B, S, C);
+        if (multi) bb_step<2, true>(B, S, C, wg);
+        else bb_step<2, false>(B, S, C, wg);
     }
 }
 

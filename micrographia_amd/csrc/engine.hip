@@ -162,7 +162,7 @@ __device__ __forceinline__ bool stencil_dup(const PT& P, int o) {
 template <int D, bool MULTI>
 __global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
-    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    uint32_t i = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= P.n) return;
     uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
     const int64_t* px[D];
@@ -212,7 +212,7 @@ template <int D, bool MULTI>
 __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     const uint32_t* ctl = B.ctl;
     if (!step_runs(ctl)) return;
-    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    uint32_t i = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
     const int64_t* px[D];
@@ -283,61 +283,33 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
 // not wrap the stencil.  In-block pairs go to the owner's list from the front; pairs across
 // blocks go to both ends' lists from the back.
-template <int D, bool MULTI>
+template <int D, bool MULTI, bool WRAP>
 __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
-    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= P.n) return;
-    uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
+    const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
     const uint32_t* sp = B.species[pm];
     const float4* __restrict__ rel = B.rel;
     int64_t xi[D];
+#pragma unroll
     for (int a = 0; a < D; a++) xi[a] = B.pos[pp][a][i];
     const float4 me = rel[i];
     const float ri[3] = {me.x, me.y, me.z};
-    uint32_t si = MULTI ? sp[i] : 0u;
+    const uint32_t si = MULTI ? sp[i] : 0u;
     const float list_r2 = B.pt->list_r2[0];
     uint32_t c[D];
     cell_of<D>(P, xi, c);
-    const uint32_t cap = P.cap, n = P.n, blk = i / P.block;
+    const uint32_t cap = P.cap, n = P.n;
+    const uint32_t bshift = __ffs(P.block) - 1;  // the block is a power of two
     uint32_t* col = B.nbr + i;
-    bool wrap[3] = {false, false, false};
-    bool any_wrap = false;
+    // single-cell periodic axes: the nearest image of the one cell (cell_w is the box length)
+    float wl[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-        wrap[a] = P.nc[a] == 1u && ((P.periodic >> a) & 1u);
-        any_wrap |= wrap[a];
-    }
+    for (int a = 0; a < D; a++) wl[a] = (P.nc[a] == 1u && ((P.periodic >> a) & 1u)) ? P.cell_w[a] : 0.f;
     uint32_t na = 0, nb = 0;
-    auto consider = [&](uint32_t j, float4 q, const float* shift) {
-        float d[3] = {q.x + shift[0], q.y + shift[1], q.z + shift[2]};
-        if (any_wrap) {
-#pragma unroll
-            for (int a = 0; a < D; a++)
-                if (wrap[a]) {
-                    // one cell spans the axis: the nearest image (cell_w is the box length)
-                    if (d[a] > 0.5f * P.cell_w[a]) d[a] -= P.cell_w[a];
-                    else if (d[a] < -0.5f * P.cell_w[a]) d[a] += P.cell_w[a];
-                }
-        }
-        float r2 = d[0] * d[0];
-#pragma unroll
-        for (int a = 1; a < D; a++) r2 = __builtin_fmaf(d[a], d[a], r2);
-        float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
-        if (r2 < lr2 && j != i) {
-            if (j / P.block == blk) {
-                if (owns(i, j)) {
-                    if (na + nb < cap) col[(size_t)na * n] = j;
-                    na++;
-                }
-            } else {
-                if (na + nb < cap) col[(size_t)(cap - 1 - nb) * n] = j;
-                nb++;
-            }
-        }
-    };
     for (int o = 0; o < stencil_size(D); o++) {
-        int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+        const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
         uint32_t cc[D];
         float shift[3] = {0.f, 0.f, 0.f};
         bool skip = false;
@@ -353,17 +325,36 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             shift[a] = (float)off[a] * P.cell_w[a] - ri[a];
         }
         if (skip) continue;
-        uint32_t key = morton<D>(cc);
-        uint32_t s = B.cell_start[key], e = B.cell_end[key];
-        uint32_t j = s;
-        for (; j + 1 < e; j += 2) {
-            float4 q0 = rel[j], q1 = rel[j + 1];
-            consider(j, q0, shift);
-            consider(j + 1, q1, shift);
+        const uint32_t key = morton<D>(cc);
+        const uint32_t s0 = B.cell_start[key], e0 = B.cell_end[key];
+        const float sx = shift[0], sy = shift[1], sz = shift[2];
+        float4 q_next = s0 < e0 ? rel[s0] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t j = s0; j < e0; j++) {
+            const float4 q = q_next;
+            if (j + 1 < e0) q_next = rel[j + 1];
+            float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
+            if constexpr (WRAP) {
+                if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
+                if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
+                if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
+            }
+            float r2 = dx * dx;
+            r2 = __builtin_fmaf(dy, dy, r2);
+            if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
+            const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
+            const bool hit = r2 < lr2 && j != i;
+            const bool inb = ((i ^ j) >> bshift) == 0u;
+            const bool take_a = hit && inb && owns(i, j);
+            const bool take_b = hit && !inb;
+            if (take_a || take_b) {
+                const uint32_t e = take_a ? na : cap - 1 - nb;
+                if (na + nb < cap) col[(size_t)e * n] = j;
+                na += take_a ? 1u : 0u;
+                nb += take_b ? 1u : 0u;
+            }
         }
-        if (j < e) consider(j, rel[j], shift);
     }
-    uint32_t count = na + nb;
+    const uint32_t count = na + nb;
     if (count > cap) {
         atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         na = nb = 0;
@@ -378,7 +369,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     if (!step_runs(ctl)) return;
     __shared__ uint32_t acc[D][BS];
     const uint32_t t = threadIdx.x;
-    const uint32_t base = blockIdx.x * BS;
+    const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
     const uint32_t i = base + t;
 #pragma unroll
     for (int a = 0; a < D; a++) acc[a][t] = 0u;
@@ -409,51 +400,75 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
         const uint32_t* col = B.nbr + i;
         const size_t stride = P.n;
         const bool rc = P.range_check != 0;
+        // Both loops are software-pipelined: the neighbour index two entries ahead and the
+        // neighbour position one entry ahead are in flight while a pair is evaluated.
         // in-block pairs: both ends
-        uint32_t j_next = na ? col[0] : 0u;
-        for (uint32_t e = 0; e < na; e++) {
-            const uint32_t j = j_next;
-            if (e + 1 < na) j_next = col[(size_t)(e + 1) * stride];
-            int64_t xj[D];
+        {
+            uint32_t j_next = na > 0 ? col[0] : 0u;
+            int64_t x_next[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) xj[a] = px[a][j];
-            float f[D];
-            bool ok = true;
+            for (int a = 0; a < D; a++) x_next[a] = na > 0 ? px[a][j_next] : 0;
+            uint32_t j_after = na > 1 ? col[stride] : 0u;
+            for (uint32_t e = 0; e < na; e++) {
+                const uint32_t j = j_next;
+                int64_t xj[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
-            if (rc && !ok) continue;
-            int32_t kj[D];
-            if constexpr (MULTI) {
-                uint32_t sj = sp[j];
-                uint32_t p = si * 4 + sj;
-                pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
-            } else {
-                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                for (int a = 0; a < D; a++) xj[a] = x_next[a];
+                if (e + 1 < na) {
+                    j_next = j_after;
+#pragma unroll
+                    for (int a = 0; a < D; a++) x_next[a] = px[a][j_next];
+                    if (e + 2 < na) j_after = col[(size_t)(e + 2) * stride];
+                }
+                float f[D];
+                bool ok = true;
+#pragma unroll
+                for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
+                if (rc && !ok) continue;
+                int32_t kj[D];
+                if constexpr (MULTI) {
+                    uint32_t sj = sp[j];
+                    uint32_t p = si * 4 + sj;
+                    pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+                } else {
+                    pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                }
+                const uint32_t lj = j - base;
+#pragma unroll
+                for (int a = 0; a < D; a++)
+                    if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
             }
-            const uint32_t lj = j - base;
-#pragma unroll
-            for (int a = 0; a < D; a++)
-                if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
         }
-        // pairs across blocks: this end only
-        const uint32_t* colb = col + (size_t)(P.cap - 1) * stride;
-        j_next = nb ? colb[0] : 0u;
-        for (uint32_t e = 0; e < nb; e++) {
-            const uint32_t j = j_next;
-            if (e + 1 < nb) j_next = colb[-(ptrdiff_t)((size_t)(e + 1) * stride)];
-            int64_t xj[D];
+        // pairs across blocks: this end only (filled from the back of the row)
+        {
+            const uint32_t* colb = col + (size_t)(P.cap - 1) * stride;
+            uint32_t j_next = nb > 0 ? colb[0] : 0u;
+            int64_t x_next[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) xj[a] = px[a][j];
-            float f[D];
-            bool ok = true;
+            for (int a = 0; a < D; a++) x_next[a] = nb > 0 ? px[a][j_next] : 0;
+            uint32_t j_after = nb > 1 ? colb[-(ptrdiff_t)stride] : 0u;
+            for (uint32_t e = 0; e < nb; e++) {
+                const uint32_t j = j_next;
+                int64_t xj[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
-            if (rc && !ok) continue;
-            if constexpr (MULTI) {
-                uint32_t p = si * 4 + sp[j];
-                pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
-            } else {
-                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
+                for (int a = 0; a < D; a++) xj[a] = x_next[a];
+                if (e + 1 < nb) {
+                    j_next = j_after;
+#pragma unroll
+                    for (int a = 0; a < D; a++) x_next[a] = px[a][j_next];
+                    if (e + 2 < nb) j_after = colb[-(ptrdiff_t)((size_t)(e + 2) * stride)];
+                }
+                float f[D];
+                bool ok = true;
+#pragma unroll
+                for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
+                if (rc && !ok) continue;
+                if constexpr (MULTI) {
+                    uint32_t p = si * 4 + sp[j];
+                    pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
+                } else {
+                    pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
+                }
             }
         }
         if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
@@ -918,8 +933,15 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
         if (variant_ == 1) {
-            if (multi) k_build_half<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
-            else k_build_half<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            bool wrap = false;
+            for (int a = 0; a < D; a++) wrap |= P_.nc[a] == 1u && ((P_.periodic >> a) & 1u);
+            if (wrap) {
+                if (multi) k_build_half<D, true, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+                else k_build_half<D, false, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            } else {
+                if (multi) k_build_half<D, true, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+                else k_build_half<D, false, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            }
         } else {
             if (multi) k_build_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
             else k_build_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);

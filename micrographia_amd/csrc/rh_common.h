@@ -57,4 +57,14 @@ __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 // gated kernel writes it, so a launch runs entirely or not at all.
 __device__ __forceinline__ bool gate_closed(const uint32_t* gate) { return gate != nullptr && *(volatile const uint32_t*)gate == 0u; }
 
+// The dispatcher hands work-groups to the chip's 8 XCDs round-robin (hardware id mod 8), and each
+// XCD has its own L2.  Renumbering so that XCD x runs one contiguous range of logical blocks puts
+// blocks that are neighbours in cell order, and so share most of their neighbour atoms, behind the
+// same L2.  A bijection of [0, nblk) for any nblk.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t nblk) {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t q = nblk / XCDS, r = nblk % XCDS, x = hw % XCDS, k = hw / XCDS;
+    return x * q + (x < r ? x : r) + k;
+}
+
 }  // namespace rh
