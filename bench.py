@@ -24,6 +24,12 @@ FLOP_PER_PAIR = 23  # real.rs:145-156 counted: r2 5, s2 1, s6 2, fr 6, u 4, a 1,
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_ATOM_STEP = 72  # read+write 3 x i64 positions and 3 x i32 velocities
+# HBM bytes per k_step_half launch at N = 4,194,304 measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each calibrated against a plain 1 GiB copy (profiles/ keeps the summary and the
+# commands).  Keyed by (variant, block); a counter pass cannot run inside this process.
+MEASURED_TRAFFIC = {
+    (1, 1024): (2.084e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
+}
 
 
 def log(msg: str) -> None:
@@ -293,7 +299,9 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.variant, args.block), (None,))[0] if n == 4194304 else None,
+            "traffic_unit": "bytes per launch (HBM fetch + write)",
+            "traffic_source": MEASURED_TRAFFIC.get((args.variant, args.block), (None, None))[1] if n == 4194304 else None,
             "flop_per_atom_step": pairs_per_atom(args.density) * FLOP_PER_PAIR,
             "avg_launch_ms": per_step_ms,
             "avg_launch_ms_all_launches": force_ms / max(launches, 1),

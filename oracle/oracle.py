@@ -96,6 +96,8 @@ def lib():
         L.orc_wrap.restype = i64
         L.orc_sep_f32.argtypes = [i64, P]
         L.orc_sep_f32.restype = C.c_int
+        L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_set_mutation.restype = None
         for f in ("orc_pair_shift",):
             getattr(L, f).argtypes = [P]
             getattr(L, f).restype = dbl
@@ -231,6 +233,11 @@ def hash3(pos, vel, species) -> int:
     vel = np.ascontiguousarray(vel, np.int32)
     sp = np.ascontiguousarray(species, np.uint32)
     return int(lib().orc3_hash(len(sp), ptr(pos), ptr(vel), ptr(sp)))
+
+
+def set_mutation(m: int) -> None:
+    """Plant mistake m in the 3D pair loop (0 restores the true law); tests only."""
+    lib().orc_set_mutation(int(m))
 
 
 def cells_for3(sys, r):

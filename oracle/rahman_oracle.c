@@ -7,6 +7,12 @@
  */
 #include "rahman_oracle.h"
 
+/* Planted mistakes, for the tests that show the parity pins catch them (0 = none):
+ * 1 wrong sign of the pair force, 2 dropped "-1" of (2 s6 - 1), 3 wrong axis index in the kick,
+ * 4 dropped energy shift, 5 dropped z term of r2.  Applies to the 3D pair loop only. */
+int orc_mutation = 0;
+void orc_set_mutation(int m) { orc_mutation = m; }
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -677,14 +683,14 @@ static void free_cells3(cells3 *c) {
                     for (int a = 0; a < 3; a++) in &= SEP(separation3(s, pi[a], pj[a], a), &f[a]);           \
                     if (!in) continue;                                                                        \
                     CONSTS pc = consts[si * 4 + sp[j]];                                                       \
-                    F r2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];                                           \
+                    F r2 = f[0] * f[0] + f[1] * f[1] + (orc_mutation == 5 ? (F)0 : f[2] * f[2]);             \
                     if (!(r2 < pc.rc2)) continue;                                                             \
                     F s2 = pc.sigma2 / r2;                                                                    \
                     F s6 = s2 * s2 * s2;                                                                      \
-                    F fr = six * pc.eps4 * s6 * (two * s6 - one) / r2;                                        \
-                    F u = pc.eps4 * s6 * (s6 - one) - pc.shift;                                               \
-                    F aa = -fr * cf;                                                                          \
-                    for (int a = 0; a < 3; a++) k[a] = (int32_t)((uint32_t)k[a] + (uint32_t)sat_i32((double)RINT(aa * f[a]))); \
+                    F fr = six * pc.eps4 * s6 * (two * s6 - (orc_mutation == 2 ? (F)0 : one)) / r2;           \
+                    F u = pc.eps4 * s6 * (s6 - one) - (orc_mutation == 4 ? (F)0 : pc.shift);                  \
+                    F aa = orc_mutation == 1 ? fr * cf : -fr * cf;                                            \
+                    for (int a = 0; a < 3; a++) k[a] = (int32_t)((uint32_t)k[a] + (uint32_t)sat_i32((double)RINT(aa * f[orc_mutation == 3 ? (a + 1) % 3 : a]))); \
                     pe_i += u;                                                                                \
                 }                                                                                             \
             }                                                                                                 \

@@ -36,6 +36,9 @@
 extern "C" {
 #endif
 
+/* Test hook: plant a known mistake in the 3D pair loop (see rahman_oracle.c; 0 = none). */
+void orc_set_mutation(int m);
+
 /* ---- gpu-prims: Threefry-2x32 and its counter-based streams ---- */
 void orc_threefry2x32_rounds(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1, uint32_t rounds, uint32_t out[2]);
 double orc_to_unit_f64(uint32_t r0, uint32_t r1);

@@ -1,6 +1,8 @@
 """The oracle (C restatement of md-ref / md-core) pinned against the reference's golden vectors,
 known-answer tests and the identities its own unit tests assert.  CPU only."""
+import json
 import math
+import os
 import random
 
 import numpy as np
@@ -16,18 +18,11 @@ def _build():
     O.build()
 
 
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+
 # crates/gpu-prims/src/threefry.rs:53-70 (Random123 kat_vectors)
-KAT = [
-    (13, (0x00000000, 0x00000000), (0x00000000, 0x00000000), (0x9D1C5EC6, 0x8BD50731)),
-    (13, (0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF, 0xFFFFFFFF), (0xFD36D048, 0x2D17272C)),
-    (13, (0x243F6A88, 0x85A308D3), (0x13198A2E, 0x03707344), (0xBA3E4725, 0xF27D669E)),
-    (20, (0x00000000, 0x00000000), (0x00000000, 0x00000000), (0x6B200159, 0x99BA4EFE)),
-    (20, (0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF, 0xFFFFFFFF), (0x1CB996FC, 0xBB002BE7)),
-    (20, (0x243F6A88, 0x85A308D3), (0x13198A2E, 0x03707344), (0xC4923A9C, 0x483DF7A0)),
-    (32, (0x00000000, 0x00000000), (0x00000000, 0x00000000), (0xCEE3D47E, 0xA23DFD5C)),
-    (32, (0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF, 0xFFFFFFFF), (0x6E2FE0D0, 0xB1B76F82)),
-    (32, (0x243F6A88, 0x85A308D3), (0x13198A2E, 0x03707344), (0xE2827716, 0xC3C05CDF)),
-]
+KAT = [(r, tuple(int(x, 16) for x in c), tuple(int(x, 16) for x in k), tuple(int(x, 16) for x in o))
+       for r, c, k, o in GOLDEN["threefry2x32_known_answers"]["vectors"]]
 
 
 @pytest.mark.parametrize("rounds,ctr,key,want", KAT)
@@ -40,7 +35,7 @@ def test_golden_all_integer_hash():
     (lattice, Threefry, libm Box-Muller), integer tables, the 2D leapfrog and the state hash."""
     sys, pos, vel, sp = O.golden_setup()
     O.step2(sys, pos, vel, sp, 1000, prec=0)
-    assert O.hash2(pos, vel, sp) == O.GOLDEN_INT_HASH
+    assert O.hash2(pos, vel, sp) == O.GOLDEN_INT_HASH == int(GOLDEN["golden_int_hash"]["hash"], 16)
 
 
 def _ulps(a: float, b: float) -> int:
@@ -280,3 +275,58 @@ def test_planted_mistake_is_caught():
     O.reverse3(sys, pos, vel, sp)
     assert O.hash3(pos, vel, sp) != h0
     assert not np.array_equal(pos, p0)
+
+
+# ---- planted mistakes: each plausible error in the pair loop fails at least one pin ----------
+
+def _lj_textbook(r, eps=1.0, sigma=1.0, rc=2.5):
+    """U(r) = 4 eps ((s/r)^12 - (s/r)^6) - U(rc), the truncated and shifted LJ law."""
+    def u(x):
+        return 4.0 * eps * ((sigma / x) ** 12 - (sigma / x) ** 6)
+    return u(r) - u(rc)
+
+
+def _two_body_pin():
+    """Two atoms at a separation with three unequal components: each kick equals
+    -dU/dr * (d/r) * dt^2/m * 2^32 from a central difference of the textbook law, and the
+    potential equals U(r)."""
+    box = [40 * ONE] * 3
+    sys = O.system3(box)
+    d = np.array([1.03, -0.41, 0.27])
+    pos = np.array([[10 * ONE] * 3, [10 * ONE + int(round(x * ONE)) for x in d]], np.int64)
+    sep = (pos[1] - pos[0]) / ONE
+    r = float(np.sqrt((sep ** 2).sum()))
+    h = 1e-6
+    du = (_lj_textbook(r + h) - _lj_textbook(r - h)) / (2 * h)
+    c = 0.005 ** 2 * ONE
+    want = du / r * c * sep  # kick on atom 0: along +sep when attracted (du > 0)
+    k, pe = O.kicks3(sys, pos, np.zeros(2, np.uint32), prec=64)
+    kicks_ok = all(abs(k[0, a] - want[a]) <= 1 + 1e-6 * abs(want[a]) for a in range(3)) and np.array_equal(k[0], -k[1])
+    pe_ok = abs(pe - _lj_textbook(r)) <= 1e-9 * max(1.0, abs(_lj_textbook(r)))
+    return kicks_ok and pe_ok
+
+
+def _energy_pin():
+    """Total energy of a small liquid is conserved to 1e-3 over 200 steps."""
+    sys, pos, vel, sp = _liquid3(500, 0.8, 1.0)
+    e0 = sum(O.energy3(sys, pos, vel, sp))
+    O.step3(sys, pos, vel, sp, 200, 64)
+    e1 = sum(O.energy3(sys, pos, vel, sp))
+    return abs(e1 - e0) <= 1e-3 * abs(e0)
+
+
+def test_true_pair_law_passes_the_pins():
+    O.set_mutation(0)
+    assert _two_body_pin()
+    assert _energy_pin()
+
+
+@pytest.mark.parametrize("mutation", [1, 2, 3, 4, 5])
+def test_planted_mistake_fails_a_pin(mutation):
+    """1 wrong sign, 2 dropped term, 3 wrong axis index, 4 dropped shift, 5 dropped r2 term."""
+    O.set_mutation(mutation)
+    try:
+        caught = not _two_body_pin() or not _energy_pin()
+    finally:
+        O.set_mutation(0)
+    assert caught, f"planted mistake {mutation} passed every pin"
