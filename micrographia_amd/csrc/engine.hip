@@ -278,6 +278,15 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // Half-list variant
 // ---------------------------------------------------------------------------------------------
 
+// Entry k & 3 of a four-entry chunk, without dynamic register indexing.
+__device__ __forceinline__ void put4(uint4& q, uint32_t k, uint32_t j) {
+    const uint32_t s = k & 3u;
+    q.x = s == 0u ? j : q.x;
+    q.y = s == 1u ? j : q.y;
+    q.z = s == 2u ? j : q.z;
+    q.w = s == 3u ? j : q.w;
+}
+
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
@@ -300,9 +309,14 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     const float list_r2 = B.pt->list_r2[0];
     uint32_t c[D];
     cell_of<D>(P, xi, c);
-    const uint32_t cap = P.cap, n = P.n;
+    const uint32_t cap4 = P.cap / 4u, n = P.n;
     const uint32_t bshift = __ffs(P.block) - 1;  // the block is a power of two
-    uint32_t* col = B.nbr + i;
+    // The list is stored in chunks of four entries: chunk c of particle i is the uint4 at
+    // c * n + i.  In-block entries fill chunks 0, 1, ... from the front; cross-block entries fill
+    // chunks cap/4 - 1, cap/4 - 2, ... from the back.  A lane keeps the chunk it is filling in
+    // registers and stores it whole (16 B) when it is full.
+    uint4* row = reinterpret_cast<uint4*>(B.nbr) + i;
+    uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     // single-cell periodic axes: the nearest image of the one cell (cell_w is the box length)
     float wl[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -346,20 +360,30 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             const bool inb = ((i ^ j) >> bshift) == 0u;
             const bool take_a = hit && inb && owns(i, j);
             const bool take_b = hit && !inb;
-            if (take_a || take_b) {
-                const uint32_t e = take_a ? na : cap - 1 - nb;
-                if (na + nb < cap) col[(size_t)e * n] = j;
-                na += take_a ? 1u : 0u;
-                nb += take_b ? 1u : 0u;
+            if (take_a) {
+                put4(fa, na, j);
+                na++;
+                if ((na & 3u) == 0u && na / 4u <= cap4) row[(size_t)(na / 4u - 1u) * n] = fa;
+            }
+            if (take_b) {
+                put4(fb, nb, j);
+                nb++;
+                if ((nb & 3u) == 0u && nb / 4u <= cap4) row[(size_t)(cap4 - nb / 4u) * n] = fb;
             }
         }
     }
-    const uint32_t count = na + nb;
-    if (count > cap) {
+    // the partial last chunks; a list whose chunks do not fit is flagged and regrown
+    const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
+    const bool over = ca + cb > cap4;
+    if (!over) {
+        if (na & 3u) row[(size_t)(ca - 1u) * n] = fa;
+        if (nb & 3u) row[(size_t)(cap4 - cb) * n] = fb;
+    }
+    if (over) {
         atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         na = nb = 0;
     }
-    atomicMax(&B.ctl[CTL_MAXCOUNT], count);
+    atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
     B.nbr_count[i] = na | (nb << 16);
 }
 
@@ -397,77 +421,80 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
         const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
         const uint32_t cnt = B.nbr_count[i];
         const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
-        const uint32_t* col = B.nbr + i;
         const size_t stride = P.n;
         const bool rc = P.range_check != 0;
-        // Both loops are software-pipelined: the neighbour index two entries ahead and the
-        // neighbour position one entry ahead are in flight while a pair is evaluated.
-        // in-block pairs: both ends
+        // The list is read a four-entry chunk (16 B) at a time, the next chunk in flight while
+        // this one is evaluated; the positions of a chunk's four neighbours are loaded together
+        // before any of its pairs is evaluated.
+        const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr) + i;
+        const uint32_t cap4 = P.cap / 4u;
+        // in-block pairs, both ends: chunks 0, 1, ...
         {
-            uint32_t j_next = na > 0 ? col[0] : 0u;
-            int64_t x_next[D];
+            const uint32_t nch = (na + 3u) / 4u;
+            uint4 q_next = nch > 0 ? rowc[0] : make_uint4(i, i, i, i);
+            for (uint32_t ch = 0; ch < nch; ch++) {
+                const uint4 q = q_next;
+                if (ch + 1 < nch) q_next = rowc[(size_t)(ch + 1) * stride];
+                const uint32_t m = na - 4u * ch;  // entries in this chunk: min(m, 4)
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+                int64_t xs[4][D];
 #pragma unroll
-            for (int a = 0; a < D; a++) x_next[a] = na > 0 ? px[a][j_next] : 0;
-            uint32_t j_after = na > 1 ? col[stride] : 0u;
-            for (uint32_t e = 0; e < na; e++) {
-                const uint32_t j = j_next;
-                int64_t xj[D];
+                for (int e = 0; e < 4; e++)
 #pragma unroll
-                for (int a = 0; a < D; a++) xj[a] = x_next[a];
-                if (e + 1 < na) {
-                    j_next = j_after;
+                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
 #pragma unroll
-                    for (int a = 0; a < D; a++) x_next[a] = px[a][j_next];
-                    if (e + 2 < na) j_after = col[(size_t)(e + 2) * stride];
+                for (int e = 0; e < 4; e++) {
+                    if ((uint32_t)e >= m) break;
+                    const uint32_t j = js[e];
+                    float f[D];
+                    bool ok = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xs[e][a], a), f[a]);
+                    if (rc && !ok) continue;
+                    int32_t kj[D];
+                    if constexpr (MULTI) {
+                        uint32_t sj = sp[j];
+                        uint32_t p = si * 4 + sj;
+                        pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+                    } else {
+                        pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                    }
+                    const uint32_t lj = j - base;
+#pragma unroll
+                    for (int a = 0; a < D; a++)
+                        if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
                 }
-                float f[D];
-                bool ok = true;
-#pragma unroll
-                for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
-                if (rc && !ok) continue;
-                int32_t kj[D];
-                if constexpr (MULTI) {
-                    uint32_t sj = sp[j];
-                    uint32_t p = si * 4 + sj;
-                    pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
-                } else {
-                    pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                }
-                const uint32_t lj = j - base;
-#pragma unroll
-                for (int a = 0; a < D; a++)
-                    if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
             }
         }
-        // pairs across blocks: this end only (filled from the back of the row)
+        // pairs across blocks, this end only: chunks cap/4 - 1, cap/4 - 2, ...
         {
-            const uint32_t* colb = col + (size_t)(P.cap - 1) * stride;
-            uint32_t j_next = nb > 0 ? colb[0] : 0u;
-            int64_t x_next[D];
+            const uint32_t nch = (nb + 3u) / 4u;
+            const uint4* rowb = rowc + (size_t)(cap4 - 1u) * stride;
+            uint4 q_next = nch > 0 ? rowb[0] : make_uint4(i, i, i, i);
+            for (uint32_t ch = 0; ch < nch; ch++) {
+                const uint4 q = q_next;
+                if (ch + 1 < nch) q_next = rowb[-(ptrdiff_t)((size_t)(ch + 1) * stride)];
+                const uint32_t m = nb - 4u * ch;
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+                int64_t xs[4][D];
 #pragma unroll
-            for (int a = 0; a < D; a++) x_next[a] = nb > 0 ? px[a][j_next] : 0;
-            uint32_t j_after = nb > 1 ? colb[-(ptrdiff_t)stride] : 0u;
-            for (uint32_t e = 0; e < nb; e++) {
-                const uint32_t j = j_next;
-                int64_t xj[D];
+                for (int e = 0; e < 4; e++)
 #pragma unroll
-                for (int a = 0; a < D; a++) xj[a] = x_next[a];
-                if (e + 1 < nb) {
-                    j_next = j_after;
+                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
 #pragma unroll
-                    for (int a = 0; a < D; a++) x_next[a] = px[a][j_next];
-                    if (e + 2 < nb) j_after = colb[-(ptrdiff_t)((size_t)(e + 2) * stride)];
-                }
-                float f[D];
-                bool ok = true;
+                for (int e = 0; e < 4; e++) {
+                    if ((uint32_t)e >= m) break;
+                    float f[D];
+                    bool ok = true;
 #pragma unroll
-                for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xj[a], a), f[a]);
-                if (rc && !ok) continue;
-                if constexpr (MULTI) {
-                    uint32_t p = si * 4 + sp[j];
-                    pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
-                } else {
-                    pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
+                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xs[e][a], a), f[a]);
+                    if (rc && !ok) continue;
+                    if constexpr (MULTI) {
+                        uint32_t p = si * 4 + sp[js[e]];
+                        pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
+                    } else {
+                        pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
+                    }
                 }
             }
         }
@@ -788,6 +815,7 @@ void Engine<D>::alloc_list(uint32_t cap) {
         RH_HIP(hipFree(B_.nbr));
         B_.nbr = nullptr;
     }
+    cap = (cap + 3u) & ~3u;  // whole four-entry chunks (the half list stores uint4 chunks)
     P_.cap = cap;
     B_.nbr = (uint32_t*)dalloc((size_t)cap * n_ * sizeof(uint32_t));
 }

@@ -46,3 +46,16 @@ def test_small_budget_queues_and_stays_exact():
         assert c.state_hash == solo_hash(c.request)
     plain = serve_plain([c.request for c in rep.completed], realtime=False)
     assert sorted(c.state_hash for c in plain.completed) == sorted(c.state_hash for c in rep.completed)
+
+
+def test_both_lanes_together_stay_exact():
+    """Requests above big_n run on the single-system lane while the rest are batched; both
+    lanes share the GPU at once."""
+    reqs = request_stream(600.0, 0.03, meta_seed=21, n_lo=500, n_hi=40000, steps_lo=40, steps_hi=200)
+    srv = Server(budget_bytes=1 << 30, max_systems=64, chunk_s=0.005, big_n=8000)
+    rep = srv.serve(reqs, realtime=True)
+    srv.close()
+    assert len(rep.completed) == len(reqs)
+    assert any(r.n >= 8000 for r in reqs) and any(r.n < 8000 for r in reqs)
+    for c in rep.completed:
+        assert c.state_hash == solo_hash(c.request), f"request {c.request.index} (n={c.request.n})"
