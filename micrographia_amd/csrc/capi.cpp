@@ -52,7 +52,6 @@ int guarded(F&& f) {
 
 rh::SystemDesc desc3(const rh_system3* s) {
     RH_REQUIRE(s != nullptr, "null system");
-    RH_REQUIRE(!s->has_bath, "the Andersen heat bath is not supported by this engine yet");
     rh::SystemDesc d;
     std::memset(&d, 0, sizeof d);
     d.dim = 3;
@@ -60,9 +59,22 @@ rh::SystemDesc desc3(const rh_system3* s) {
         RH_REQUIRE(s->box_len[a] > 0, "box lengths must be positive");
         d.box_len[a] = s->box_len[a];
         d.walled[a] = s->walled[a];
-        d.wall_kind[a][0] = s->wall_kind[a][0];
-        d.wall_kind[a][1] = s->wall_kind[a][1];
+        for (int side = 0; side < 2; side++) {
+            RH_REQUIRE(s->wall_kind[a][side] <= RH_WALL_THERMAL, "unknown wall kind");
+            d.wall_kind[a][side] = s->wall_kind[a][side];
+            d.wall_t[a][side] = s->wall_t[a][side];
+            if (s->walled[a] && s->wall_kind[a][side] == RH_WALL_THERMAL)
+                RH_REQUIRE(s->wall_t[a][side] >= 0.0, "heat-bath wall temperature must be >= 0");
+        }
         d.gravity[a] = s->gravity[a];
+    }
+    d.seed = s->seed;
+    d.has_bath = s->has_bath ? 1u : 0u;
+    if (d.has_bath) {
+        RH_REQUIRE(s->bath_t >= 0.0 && s->bath_rate >= 0.0, "bath temperature and rate must be >= 0");
+        d.bath_species = s->bath_species;
+        d.bath_t = s->bath_t;
+        d.bath_rate = s->bath_rate;
     }
     d.dt = s->dt;
     d.nspecies = s->nspecies;
@@ -193,8 +205,8 @@ int rh_engine3_read_potential(rh_engine3* e, double* pe) {
 }
 int rh_engine3_read_ledger(rh_engine3* e, rh_ledger3* out) {
     return guarded([&] {
-        eng(e);
-        std::memset(out, 0, sizeof *out);
+        RH_REQUIRE(out != nullptr, "null ledger");
+        eng(e)->read_ledger(out);
     });
 }
 int rh_engine3_list_stats(rh_engine3* e, rh_list_stats* out) {

@@ -57,6 +57,17 @@ struct Params {
     uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
     float cell_w[3];       // cell width (sigma), f32
     double cell_w64[3];    // cell width (sigma)
+    // Stochastic boundaries and the ledger (int_engine3.rs:138-206, three.rs:232-265)
+    uint32_t bath;           // Andersen heat bath on
+    uint32_t bath_species;   // bit s: species s is coupled to the bath
+    uint32_t thermal_walls;  // bit 2a+side: a heat-bath wall
+    uint32_t ledger;         // book wall and bath exchanges (some axis walled, or a bath)
+    uint32_t seed_lo, seed_hi;
+    double bath_p;           // redraw probability per particle per step (rate * dt)
+    double bath_sd[4];       // sqrt(T / m) of the bath per species (sigma/tau)
+    double wall_sd[6][4];    // sqrt(T / m) per heat-bath wall and species
+    double dt;
+    double mass[4];
     PairTable pt;
 };
 
@@ -77,6 +88,7 @@ struct Bufs {
     double* ke;          // per-slot kinetic energy
     uint32_t* ctl;
     const PairTable* pt;
+    unsigned long long* ledger;  // [7][3]: energy, impulse (2^-32 units), hits per wall; 6 = bath
 };
 
 // The smallest r2 of a species pair from which the shared-reciprocal division of the pair kick
@@ -92,7 +104,11 @@ struct SystemDesc {
     rh_pair pairs[16];
     uint32_t walled[3];
     uint32_t wall_kind[3][2];
+    double wall_t[3][2];
     double gravity[3];
+    uint64_t seed;
+    uint32_t has_bath, bath_species;
+    double bath_t, bath_rate;
 };
 
 template <int D>
@@ -117,6 +133,7 @@ class Engine {
     void set_species(const uint32_t* ids, uint64_t count, uint32_t species);
     void set_species_velocity(uint32_t species, const double v[3]);
     rh_list_stats list_stats();
+    void read_ledger(rh_ledger3* out);
     hipStream_t stream() const { return stream_; }
     void set_timing(bool on) { timing_ = on; }
     void kernel_ms(double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds) const;

@@ -93,6 +93,172 @@ __global__ void k_flip_pp_pm(uint32_t* ctl) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// The leapfrog with its boundaries: Andersen bath, walls (hard, soft, heat bath), ledger
+// (IntEngine3::step, int_engine3.rs:138-206; three.rs:232-265)
+// ---------------------------------------------------------------------------------------------
+
+constexpr uint32_t STREAM_WALL = 0x57414c4cu;    // walls::STREAM_WALL
+constexpr uint32_t STREAM_THERMO = 0x5448524du;  // walls::STREAM_THERMO
+constexpr double LEDGER_ONE = 4294967296.0;      // ledger entries in 2^-32 units
+constexpr int LEDGER_WORDS = 21;                 // 7 entries x (energy, impulse, hits)
+constexpr double TAU_D = 6.283185307179586;
+
+// The ledger sums fixed-point values with 64-bit integer atomics, so its totals do not depend on
+// the order of the additions.
+__device__ __forceinline__ void ledger_add(unsigned long long* L, int w, int field, double value) {
+    double q = rint(value * LEDGER_ONE);
+    q = q != q ? 0.0 : fmin(fmax(q, -9.2e18), 9.2e18);  // saturate (a runaway wall force)
+    atomicAdd(&L[3 * w + field], (unsigned long long)(long long)q);
+}
+
+// threefry::to_unit_f64 (threefry.rs:27-33): 53 random bits in [0, 1)
+__device__ __forceinline__ double unit_f64(uint32_t r0, uint32_t r1) {
+    const uint64_t bits = (((uint64_t)r0 << 21) ^ (uint64_t)(r1 >> 11)) & ((uint64_t(1) << 53) - 1u);
+    return (double)bits * (1.0 / 9007199254740992.0);
+}
+
+// Rust `as i32` of round(v dt 2^32): a velocity (sigma/tau) as a displacement per step.
+template <int D>
+__device__ __forceinline__ int32_t to_step(const Params<D>& P, double v) {
+    const double x = round(v * P.dt * 4294967296.0);
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return INT32_MAX;
+    if (x <= -2147483648.0) return INT32_MIN;
+    return (int32_t)x;
+}
+
+template <int D>
+__device__ __forceinline__ double velocity(const Params<D>& P, int32_t v) {
+    return (double)v * (1.0 / 4294967296.0) / P.dt;
+}
+
+// md_core::three::andersen: with probability rate dt a fresh Maxwell velocity, booked as the
+// bath's (ledger entry 6) energy and hit.
+template <int D>
+__device__ void bath_redraw(const Params<D>& P, unsigned long long* L, uint32_t id, uint32_t clock, uint32_t s, int32_t v[D]) {
+    if constexpr (D == 3) {
+        if (!((P.bath_species >> s) & 1u) || ((P.pinned >> s) & 1u)) return;
+        const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_THERMO;
+        uint32_t r[2], r2[2], r3[2];
+        threefry2x32(id, clock, k0, k1, r);
+        if (unit_f64(r[0], 0u) >= P.bath_p) return;
+        threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+        threefry2x32(id, clock, k0, k1 ^ 2u, r3);
+        const double sd = P.bath_sd[s];
+        const double rad = sqrt(-2.0 * log(1.0 - unit_f64(r[1], 0u)));
+        const double a = TAU_D * unit_f64(r2[0], 0u);
+        const double rad3 = sqrt(-2.0 * log(1.0 - unit_f64(r3[0], 0u)));
+        const double a3 = TAU_D * unit_f64(r3[1], 0u);
+        const double u[3] = {sd * rad * cos(a), sd * rad * sin(a), sd * rad3 * cos(a3)};
+        double before = 0.0, after = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double w = velocity(P, v[b]);
+            before += w * w;
+        }
+#pragma unroll
+        for (int b = 0; b < 3; b++) v[b] = to_step(P, u[b]);
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double w = velocity(P, v[b]);
+            after += w * w;
+        }
+        ledger_add(L, 6, 0, 0.5 * P.mass[s] * (after - before));
+        atomicAdd(&L[3 * 6 + 2], 1ull);
+    }
+}
+
+// Wrap (periodic axes) or mirror about -1/2 and L - 1/2 (walls, walls.rs:27-35); a particle that
+// crossed a wall leaves it with a reversed normal velocity (hard and soft) or a velocity drawn
+// from the wall's bath (three.rs:232-243), and the exchange is booked in the ledger
+// (int_engine3.rs:186-206).
+template <int D>
+__device__ void bound_booked(const Params<D>& P, unsigned long long* L, int64_t x[D], int32_t v[D], uint32_t id, uint32_t clock, uint32_t s) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int64_t l = P.box[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+            continue;
+        }
+        int side = -1;
+        if (x[a] < 0) {
+            x[a] = -1 - x[a];
+            side = 0;
+        } else if (x[a] >= l) {
+            x[a] = 2 * l - 1 - x[a];
+            side = 1;
+        }
+        if (side < 0) continue;
+        const int w = 2 * a + side;
+        double before[D], after[D];
+#pragma unroll
+        for (int b = 0; b < D; b++) before[b] = velocity(P, v[b]);
+        bool thermal = false;
+        if constexpr (D == 3) thermal = (P.thermal_walls >> w) & 1u;
+        if (thermal) {
+            if constexpr (D == 3) {
+                const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_WALL;
+                uint32_t r[2], r2[2];
+                threefry2x32(id, clock, k0, k1, r);
+                threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+                const double u1 = 1.0 - unit_f64(r[0], r[1]);
+                const double u2 = 1.0 - unit_f64(r2[0], 0u), u3 = unit_f64(r2[1], 0u);
+                const double sd = P.wall_sd[w][s];
+                const double inward = side == 0 ? 1.0 : -1.0;
+                const double vn = inward * sd * sqrt(-2.0 * log(u1));
+                const double rad = sd * sqrt(-2.0 * log(u2));
+                const double ang = TAU_D * u3;
+                v[a] = to_step(P, vn);
+                v[(a + 1) % 3] = to_step(P, rad * cos(ang));
+                v[(a + 2) % 3] = to_step(P, rad * sin(ang));
+            }
+        } else {
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        }
+#pragma unroll
+        for (int b = 0; b < D; b++) after[b] = velocity(P, v[b]);
+        const double m = P.mass[s];
+        double ka = after[0] * after[0], kb = before[0] * before[0];
+#pragma unroll
+        for (int b = 1; b < D; b++) {
+            ka += after[b] * after[b];
+            kb += before[b] * before[b];
+        }
+        ledger_add(L, w, 0, 0.5 * m * ka - 0.5 * m * kb);
+        ledger_add(L, w, 1, fabs(m * (before[a] - after[a])));
+        atomicAdd(&L[3 * w + 2], 1ull);
+    }
+}
+
+// v += kick + g, the bath, x += v, then the boundaries.  Soft-wall impulses (wf, the force of
+// each soft wall on this particle) go to the ledger as force x dt.  BOOK = false compiles the
+// plain periodic / hard-wall leapfrog only (no ledger, no stochastic boundaries), so the hot
+// kernels of systems without them carry none of that code's registers.
+template <int D, bool BOOK>
+__device__ __forceinline__ void advance(const Params<D>& P, const Bufs<D>& B, uint32_t i, uint32_t pm, uint32_t si, const uint32_t k[D], const int64_t xi[D],
+                                        uint32_t clock, const float* wf, int64_t x[D], int32_t v[D]) {
+    const bool pin = (P.pinned >> si) & 1u;
+#pragma unroll
+    for (int a = 0; a < D; a++) v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + k[a] + (pin ? 0u : (uint32_t)P.gravity[a]));
+    if (!BOOK || !P.ledger) {
+#pragma unroll
+        for (int a = 0; a < D; a++) x[a] = xi[a] + (int64_t)v[a];
+        bound<D>(P, x, v);
+        return;
+    }
+    const uint32_t id = B.ids[pm][i];
+#pragma unroll
+    for (int w = 0; w < 2 * D; w++)
+        if (wf[w] != 0.0f) ledger_add(B.ledger, w, 1, (double)wf[w] * P.dt);
+    if (P.bath) bath_redraw<D>(P, B.ledger, id, clock, si, v);
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = xi[a] + (int64_t)v[a];
+    bound_booked<D>(P, B.ledger, x, v, id, clock, si);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Rebuild chain
 // ---------------------------------------------------------------------------------------------
 
@@ -208,7 +374,7 @@ __global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, cons
 // The step: forces over the list, leapfrog, half-skin check (one launch)
 // ---------------------------------------------------------------------------------------------
 
-template <int D, bool MULTI>
+template <int D, bool MULTI, bool BOOK>
 __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     const uint32_t* ctl = B.ctl;
     if (!step_runs(ctl)) return;
@@ -247,17 +413,12 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
             if (ok) pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
         }
     }
-    if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
-    // leapfrog: v += kick + g, x += v
-    const bool pin = (P.pinned >> si) & 1u;
+    float wf[2 * D] = {};
+    if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
+    // leapfrog: v += kick + g, x += v, boundaries
     int32_t v[D];
     int64_t x[D];
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + k[a] + (pin ? 0u : (uint32_t)P.gravity[a]));
-        x[a] = xi[a] + (int64_t)v[a];
-    }
-    bound<D>(P, x, v);
+    advance<D, BOOK>(P, B, i, pm, si, k, xi, ctl[CTL_CLOCK], wf, x, v);
     float r2 = 0.0f;
     bool far = false;
 #pragma unroll
@@ -277,6 +438,11 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // ---------------------------------------------------------------------------------------------
 // Half-list variant
 // ---------------------------------------------------------------------------------------------
+
+// Bits u < t of a 32-bit mask (t may be out of [0, 32]).
+__device__ __forceinline__ uint32_t bits_below(int t) {
+    return t <= 0 ? 0u : (t >= 32 ? 0xffffffffu : ((1u << t) - 1u));
+}
 
 // Entry k & 3 of a four-entry chunk, without dynamic register indexing.
 __device__ __forceinline__ void put4(uint4& q, uint32_t k, uint32_t j) {
@@ -342,33 +508,50 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         const uint32_t key = morton<D>(cc);
         const uint32_t s0 = B.cell_start[key], e0 = B.cell_end[key];
         const float sx = shift[0], sy = shift[1], sz = shift[2];
-        float4 q_next = s0 < e0 ? rel[s0] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (uint32_t j = s0; j < e0; j++) {
-            const float4 q = q_next;
-            if (j + 1 < e0) q_next = rel[j + 1];
-            float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
-            if constexpr (WRAP) {
-                if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
-                if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
-                if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
-            }
-            float r2 = dx * dx;
-            r2 = __builtin_fmaf(dy, dy, r2);
-            if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
-            const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[j]] : list_r2;
-            const bool hit = r2 < lr2 && j != i;
-            const bool inb = ((i ^ j) >> bshift) == 0u;
-            const bool take_a = hit && inb && owns(i, j);
-            const bool take_b = hit && !inb;
-            if (take_a) {
-                put4(fa, na, j);
-                na++;
-                if ((na & 3u) == 0u && na / 4u <= cap4) row[(size_t)(na / 4u - 1u) * n] = fa;
-            }
-            if (take_b) {
-                put4(fb, nb, j);
-                nb++;
-                if ((nb & 3u) == 0u && nb / 4u <= cap4) row[(size_t)(cap4 - nb / 4u) * n] = fb;
+        // Candidates in groups of up to 32: a distance pass sets one bit per candidate within
+        // the list radius, bit arithmetic drops in-block partners this end does not own (and
+        // itself), and only the surviving bits are visited to store entries.
+        for (uint32_t base = s0; base < e0; base += 32u) {
+            const uint32_t cnt = min(32u, e0 - base);
+            uint32_t mask = 0u;
+            auto near = [&](uint32_t u) -> uint32_t {
+                const float4 q = rel[base + u];
+                float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
+                if constexpr (WRAP) {
+                    if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
+                    if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
+                    if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
+                }
+                float r2 = dx * dx;
+                r2 = __builtin_fmaf(dy, dy, r2);
+                if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
+                const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[base + u]] : list_r2;
+                return r2 < lr2 ? 1u : 0u;
+            };
+            uint32_t u = 0;
+            for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
+            if (u < cnt) mask |= near(u) << u;
+            // j = base + u: lt: j < i, gt: j > i; same: j has the parity of i; inb: same block
+            const int t0 = (int)i - (int)base;
+            const uint32_t lt = bits_below(t0), gt = ~bits_below(t0 + 1);
+            const uint32_t same = ((i ^ base) & 1u) ? 0xAAAAAAAAu : 0x55555555u;
+            const uint32_t blo = (i >> bshift) << bshift;
+            const uint32_t inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
+            const uint32_t owned = (gt & ~same) | (lt & same);  // owns(i, j), which excludes j = i
+            mask &= (inb & owned) | ~inb;
+            while (mask) {
+                const uint32_t b = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const uint32_t j = base + b;
+                if ((inb >> b) & 1u) {
+                    put4(fa, na, j);
+                    na++;
+                    if ((na & 3u) == 0u && na / 4u <= cap4) row[(size_t)(na / 4u - 1u) * n] = fa;
+                } else {
+                    put4(fb, nb, j);
+                    nb++;
+                    if ((nb & 3u) == 0u && nb / 4u <= cap4) row[(size_t)(cap4 - nb / 4u) * n] = fb;
+                }
             }
         }
     }
@@ -387,7 +570,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     B.nbr_count[i] = na | (nb << 16);
 }
 
-template <int D, bool MULTI, int BS>
+template <int D, bool MULTI, int BS, bool BOOK>
 __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     const uint32_t* ctl = B.ctl;
     if (!step_runs(ctl)) return;
@@ -498,21 +681,16 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                 }
             }
         }
-        if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr);
     }
     __syncthreads();
     if (!live) return;
+    float wf[2 * D] = {};
+    if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
 #pragma unroll
     for (int a = 0; a < D; a++) k[a] += acc[a][t];
-    const bool pin = (P.pinned >> si) & 1u;
     int32_t v[D];
     int64_t x[D];
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + k[a] + (pin ? 0u : (uint32_t)P.gravity[a]));
-        x[a] = xi[a] + (int64_t)v[a];
-    }
-    bound<D>(P, x, v);
+    advance<D, BOOK>(P, B, i, pm, si, k, xi, ctl[CTL_CLOCK], wf, x, v);
     float r2 = 0.0f;
     bool far = false;
 #pragma unroll
@@ -574,21 +752,29 @@ __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, con
 }
 
 // Leapfrog in place from the stored kicks (cell-list mode).
+// `clock_add`: steps already taken in this step() call (the clock word advances at its end).
 template <int D>
-__global__ __launch_bounds__(TPB) void k_integrate(Params<D> P, Bufs<D> B) {
+__global__ __launch_bounds__(TPB) void k_integrate(Params<D> P, Bufs<D> B, uint32_t clock_add) {
     uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
     uint32_t si = B.species[pm][i];
-    const bool pin = (P.pinned >> si) & 1u;
-    int32_t v[D];
-    int64_t x[D];
+    int64_t xi[D];
+    uint32_t k[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + (uint32_t)B.kicks[a][i] + (pin ? 0u : (uint32_t)P.gravity[a]));
-        x[a] = B.pos[pp][a][i] + (int64_t)v[a];
+        xi[a] = B.pos[pp][a][i];
+        k[a] = (uint32_t)B.kicks[a][i];
     }
-    bound<D>(P, x, v);
+    float wf[2 * D] = {};
+    if (P.ledger && P.soft_walls) {
+        // the soft-wall forces of this step's kicks, for the ledger (the kicks already hold them)
+        uint32_t dummy[D] = {};
+        soft_walls<D>(P, xi, B.pt->kick_c[si], dummy, nullptr, wf);
+    }
+    int32_t v[D];
+    int64_t x[D];
+    advance<D, true>(P, B, i, pm, si, k, xi, B.ctl[CTL_CLOCK] + clock_add, wf, x, v);
 #pragma unroll
     for (int a = 0; a < D; a++) {
         B.pos[pp][a][i] = x[a];
@@ -723,11 +909,28 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         if (!sys.walled[a]) P.periodic |= 1u << a;
         else
             for (int side = 0; side < 2; side++) {
-                RH_REQUIRE(sys.wall_kind[a][side] != RH_WALL_THERMAL, "heat-bath walls are not supported by this engine yet");
-                if (sys.wall_kind[a][side] == RH_WALL_SOFT) P.soft_walls |= 1u << (2 * a + side);
+                const int w = 2 * a + side;
+                if (sys.wall_kind[a][side] == RH_WALL_SOFT) P.soft_walls |= 1u << w;
+                if (sys.wall_kind[a][side] == RH_WALL_THERMAL) {
+                    P.thermal_walls |= 1u << w;
+                    for (uint32_t sp = 0; sp < sys.nspecies; sp++) P.wall_sd[w][sp] = std::sqrt(sys.wall_t[a][side] / sys.masses[sp]);
+                }
             }
         P.gravity[a] = core::gravity_kick(sys.dt, sys.gravity[a]);
     }
+    // Stochastic boundaries: Threefry keyed by the system seed (three.rs:232-265); every walled
+    // axis and the bath book their exchanges in the ledger.
+    P.seed_lo = (uint32_t)sys.seed;
+    P.seed_hi = (uint32_t)(sys.seed >> 32);
+    P.dt = sys.dt;
+    for (uint32_t sp = 0; sp < 4; sp++) P.mass[sp] = sp < sys.nspecies ? sys.masses[sp] : 1.0;
+    if (sys.has_bath) {
+        P.bath = 1u;
+        P.bath_species = sys.bath_species;
+        P.bath_p = sys.bath_rate * sys.dt;
+        for (uint32_t sp = 0; sp < sys.nspecies; sp++) P.bath_sd[sp] = std::sqrt(sys.bath_t / sys.masses[sp]);
+    }
+    P.ledger = P.bath || (P.periodic != (1u << D) - 1u) ? 1u : 0u;
     P.mbits = 32 - __builtin_clz(std::max(ncmax - 1, 1u));
     RH_REQUIRE((D == 3 && P.mbits <= 10) || (D == 2 && P.mbits <= 15), "too many cells for Morton keys");
     key_bits_ = D * P.mbits;
@@ -850,6 +1053,8 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     B_.pe = (double*)dalloc(n * sizeof(double));
     B_.ke = (double*)dalloc(n * sizeof(double));
     B_.ctl = (uint32_t*)dalloc(CTL_WORDS * sizeof(uint32_t));
+    B_.ledger = (unsigned long long*)dalloc(LEDGER_WORDS * sizeof(unsigned long long));
+    RH_HIP(hipMemset(B_.ledger, 0, LEDGER_WORDS * sizeof(unsigned long long)));
     reduce_temp_ = dalloc(reduce_temp_bytes(n));
     d_scalar_ = (double*)dalloc(64);
     if (mode.kind == 1) alloc_list(mode.cap);
@@ -1078,14 +1283,24 @@ void Engine<D>::launch_step() {
     const bool multi = P_.nspecies > 1;
     if (variant_ == 0) {
         uint32_t nb = blocks_for(n_, TPB);
-        if (multi) k_step_list<D, true><<<nb, TPB, 0, stream_>>>(P_, B_);
-        else k_step_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_);
+        if (P_.ledger) {
+            if (multi) k_step_list<D, true, true><<<nb, TPB, 0, stream_>>>(P_, B_);
+            else k_step_list<D, false, true><<<nb, TPB, 0, stream_>>>(P_, B_);
+        } else {
+            if (multi) k_step_list<D, true, false><<<nb, TPB, 0, stream_>>>(P_, B_);
+            else k_step_list<D, false, false><<<nb, TPB, 0, stream_>>>(P_, B_);
+        }
     } else {
         uint32_t bs = P_.block;
         uint32_t nb = blocks_for(n_, bs);
-#define RH_STEP_HALF(BS)                                                              \
-    if (multi) k_step_half<D, true, BS><<<nb, BS, 0, stream_>>>(P_, B_);              \
-    else k_step_half<D, false, BS><<<nb, BS, 0, stream_>>>(P_, B_);
+#define RH_STEP_HALF(BS)                                                                   \
+    if (P_.ledger) {                                                                        \
+        if (multi) k_step_half<D, true, BS, true><<<nb, BS, 0, stream_>>>(P_, B_);          \
+        else k_step_half<D, false, BS, true><<<nb, BS, 0, stream_>>>(P_, B_);               \
+    } else {                                                                                \
+        if (multi) k_step_half<D, true, BS, false><<<nb, BS, 0, stream_>>>(P_, B_);         \
+        else k_step_half<D, false, BS, false><<<nb, BS, 0, stream_>>>(P_, B_);              \
+    }
         if (bs == 256) {
             RH_STEP_HALF(256)
         } else if (bs == 512) {
@@ -1121,7 +1336,7 @@ void Engine<D>::step_cells(uint64_t steps) {
         }
         enqueue_cells_at_current(nullptr);
         enqueue_force_cells(nullptr);
-        k_integrate<D><<<nb, TPB, 0, stream_>>>(P_, B_);
+        k_integrate<D><<<nb, TPB, 0, stream_>>>(P_, B_, (uint32_t)s);
         RH_HIP(hipGetLastError());
         since++;
     }
@@ -1289,6 +1504,21 @@ void Engine<D>::set_species_velocity(uint32_t species, const double v[3]) {
     k_set_species_velocity<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, species, a[0], a[1], a[2]);
     RH_HIP(hipGetLastError());
     RH_HIP(hipStreamSynchronize(stream_));
+}
+
+// Ledger3 (engine3.rs:199-205): energy and impulse exchanged with each wall (2 axis + side)
+// and the bath (6) since the engine was made, and the hits.
+template <int D>
+void Engine<D>::read_ledger(rh_ledger3* out) {
+    unsigned long long L[LEDGER_WORDS];
+    RH_HIP(hipMemcpyAsync(L, B_.ledger, sizeof L, hipMemcpyDeviceToHost, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+    std::memset(out, 0, sizeof *out);
+    for (int w = 0; w < 7; w++) {
+        out->energy[w] = (double)(long long)L[3 * w] / LEDGER_ONE;
+        out->impulse[w] = (double)(long long)L[3 * w + 1] / LEDGER_ONE;
+        out->hits[w] = L[3 * w + 2];
+    }
 }
 
 template <int D>

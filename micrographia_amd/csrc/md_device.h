@@ -138,8 +138,9 @@ __device__ __forceinline__ void pair_kick_pe(const float f[D], const PairTable* 
 }
 
 // Soft (WCA) walls on the centre's distance from each wall (int_engine3.rs:106-126).
+// wf (optional, 2D entries): the force of each soft wall on the particle, for the ledger.
 template <int D, class PT>
-__device__ __forceinline__ void soft_walls(const PT& P, const int64_t x[D], float c, uint32_t k[D], double* wall_pe) {
+__device__ __forceinline__ void soft_walls(const PT& P, const int64_t x[D], float c, uint32_t k[D], double* wall_pe, float* wf = nullptr) {
 #pragma unroll
     for (int a = 0; a < D; a++) {
         for (int side = 0; side < 2; side++) {
@@ -154,6 +155,7 @@ __device__ __forceinline__ void soft_walls(const PT& P, const int64_t x[D], floa
                 int32_t kw = cvt_sat_i32(__builtin_rintf(fw * c));
                 k[a] += side == 0 ? (uint32_t)kw : (uint32_t)0 - (uint32_t)kw;
                 if (wall_pe) *wall_pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
+                if (wf) wf[2 * a + side] = fw;
             }
         }
     }

@@ -44,6 +44,10 @@ def oracle_system(sys: rh.System3) -> O.System3:
     for k in range(16):
         o.pairs[k] = O.Pair(c.pairs[k].eps, c.pairs[k].sigma, c.pairs[k].rc)
     o.seed = c.seed
+    o.has_bath = c.has_bath
+    o.bath_species = c.bath_species
+    o.bath_t = c.bath_t
+    o.bath_rate = c.bath_rate
     return o
 
 
@@ -276,3 +280,79 @@ def test_large_system_properties():
     gpu.step(60)
     gpu.reverse()
     assert gpu.read_state().hash() == s.hash()
+
+
+# ---- stochastic boundaries and the ledger (int_engine3.rs:138-206, three.rs:232-265) --------
+# The draws are f64 on both sides (the device's log/cos/sin against the C library's); a
+# velocity is rounded to an integer displacement, so the two agree unless a one-ulp difference
+# straddles a rounding boundary (about 1e-8 per draw).  The ledger sums fixed-point terms of
+# 2^-32 on the device, so its totals differ from the twin's f64 sums by at most that rounding
+# per term.
+
+def oracle_run_ledger(sys, st, steps, prec=32):
+    pos, vel, sp = st.pos.copy(), st.vel.copy(), st.species.copy()
+    _, led = O.step3(oracle_system(sys), pos, vel, sp, steps, prec)
+    return rh.State3(pos, vel, sp), led
+
+
+def ledger_close(got, want, walls):
+    for w in walls:
+        assert int(got.hits[w]) == int(want.hits[w]), f"hits of wall {w}"
+        n = max(1, int(want.hits[w]))
+        for field in ("energy", "impulse"):
+            g, x = getattr(got, field)[w], getattr(want, field)[w]
+            assert abs(g - x) <= n * 2.0 ** -32 + 1e-9 * abs(x) + 1e-12, f"{field} of wall {w}: {g} vs {x}"
+
+
+@pytest.mark.parametrize("mode", ["cells", "verlet"])
+def test_andersen_bath_matches_the_twin(mode):
+    sys, st = liquid(2000, 0.6, 1.0)
+    sys = sys.with_thermostat(1.6, 2.0)
+    want, led = oracle_run_ledger(sys, st, 150)
+    assert led.hits[6] > 1000
+    gpu = engine(sys, st, MODES[mode](sys, st.n()))
+    gpu.step(150)
+    assert gpu.read_state() == want
+    ledger_close(gpu.read_ledger(), led, [6])
+
+
+@pytest.mark.parametrize("mode", ["cells", "verlet"])
+def test_heat_bath_walls_match_the_twin(mode):
+    sys, st = liquid(2000, 0.3, 1.5, walls=True)
+    sys = sys.with_walls(2, rh.thermal_wall(2.0), rh.thermal_wall(0.5)).with_gravity((0.0, 0.0, -0.2))
+    want, led = oracle_run_ledger(sys, st, 300)
+    assert led.hits[4] + led.hits[5] > 20
+    gpu = engine(sys, st, MODES[mode](sys, st.n()))
+    gpu.step(300)
+    assert gpu.read_state() == want
+    ledger_close(gpu.read_ledger(), led, [4, 5])
+
+
+def squeeze(st, sys, axis):
+    """Move the lattice off the walls of `axis`: positions scaled into [1, L - 1] sigma."""
+    L = int(sys.box_len[axis])
+    st.pos[:, axis] = ONE + (st.pos[:, axis].astype(np.float64) * (L - 2 * ONE) / L).astype(np.int64)
+
+
+def test_hard_and_soft_wall_ledger_matches_the_twin():
+    sys, st = liquid(2000, 0.5, 1.5)
+    sys = sys.with_walls(2, rh.Wall.Soft, rh.Wall.Hard).with_walls(0, rh.Wall.Hard, rh.Wall.Soft)
+    squeeze(st, sys, 0)
+    squeeze(st, sys, 2)
+    want, led = oracle_run_ledger(sys, st, 300)
+    assert min(led.hits[0], led.hits[5]) > 0 and np.isfinite(led.impulse).all()
+    for variant in VARIANTS[:2]:
+        gpu = engine(sys, st, rh.verlet3_with_skin(sys, st.n(), 0.3), *variant)
+        gpu.step(300)
+        assert gpu.read_state() == want
+        ledger_close(gpu.read_ledger(), led, [0, 1, 4, 5])
+
+
+def test_bath_sets_the_temperature():
+    """crates/md/tests/three.rs:114-121: an Andersen bath at 1.6 brings a cold liquid to kT 1.6."""
+    sys, st = liquid(4000, 0.6, 0.5)
+    sys = sys.with_thermostat(1.6, 2.0)
+    gpu = engine(sys, st, rh.verlet3_with_skin(sys, st.n(), 0.3))
+    gpu.step(1500)
+    t = 2.0 * gpu.read_kinetic_total() / (3 * st.n())
+    assert abs(t - 1.6) < 0.1, t
