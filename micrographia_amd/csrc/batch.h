@@ -41,7 +41,7 @@ struct BBufs {
     uint32_t* species[2];
     uint32_t* ids[2];
     int64_t* ref[3];
-    float* rel[3];
+    float4* rel;  // position relative to its cell's corner (sigma, f32), for list builds
     uint32_t* key;
     uint32_t* rank;
     uint32_t* nbr;
@@ -122,6 +122,7 @@ class Batch {
     std::vector<BSys> h_sys_;
     std::vector<int> free_ids_;
     uint32_t grid_wgs_ = 0;  // work-groups covering the slot high-water mark
+    uint32_t kinds_ = 0;     // bit 2 (dim == 3) + (species > 1): kinds of system present
     bool tables_dirty_ = true;
     uint32_t graph_chunk_ = 100;
     hipGraphExec_t graph_ = nullptr;

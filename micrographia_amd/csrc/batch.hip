@@ -23,7 +23,7 @@ namespace rh {
 
 constexpr uint32_t LIST_UNIT = 4096;  // list-arena allocation unit (entries)
 constexpr uint32_t CELL_UNIT = 256;   // cell-arena allocation unit (cells)
-constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 12 + 8 + 4;
+constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 16 + 8 + 4;
 
 // ---------------------------------------------------------------------------------------------
 // Kernels
@@ -129,14 +129,16 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
     uint32_t key = B.key[slot];
     uint32_t ns = S.slot0 + B.cell_start[S.cell0 + key] + B.rank[slot];
     uint32_t c[3] = {key % S.nc[0], (key / S.nc[0]) % S.nc[1], D == 3 ? key / (S.nc[0] * S.nc[1]) : 0u};
+    float r[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < D; a++) {
         int64_t x = B.pos[pp][a][slot];
         B.pos[pp ^ 1][a][ns] = x;
         B.ref[a][ns] = x;
         B.vel[pm ^ 1][a][ns] = B.vel[pm][a][slot];
-        B.rel[a][ns] = (float)((double)x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
+        r[a] = (float)((double)x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
     }
+    B.rel[ns] = make_float4(r[0], r[1], r[2], 0.f);
     B.species[pm ^ 1][ns] = B.species[pm][slot];
     B.ids[pm ^ 1][ns] = B.ids[pm][slot];
 }
@@ -155,33 +157,40 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B) {
 
 // The list of local slot i (in the freshly scattered buffers): f32 cell-relative separations
 // against each species pair's (rc + skin)(1 + 1e-4), in-block pairs (same 256-slot block) owned by
-// one end, pairs across blocks listed at both ends.  variant 0 lists every pair at both ends.
+// one end, pairs across blocks listed at both ends; variant 0 lists every pair at both ends.  As
+// in the single-system engine: candidates in groups of 32 through a distance mask, entries stored
+// in four-entry chunks (uint4 at nbr0 + 4 (c * nslots + i)), in-block chunks from the front and
+// the others from the back.
 template <int D, bool MULTI>
 __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local, uint32_t variant) {
     const uint32_t pm = C.pmeta ^ 1u;
     const uint32_t slot = S.slot0 + local;
     const uint32_t* sp = B.species[pm] + S.slot0;
-    float ri[D];
+    const float4 me = B.rel[slot];
+    const float ri[3] = {me.x, me.y, me.z};
     int64_t xi[D];
-    for (int a = 0; a < D; a++) {
-        ri[a] = B.rel[a][slot];
-        xi[a] = B.pos[C.ppos ^ 1u][a][slot];
-    }
-    uint32_t si = MULTI ? sp[local] : 0u;
+#pragma unroll
+    for (int a = 0; a < D; a++) xi[a] = B.pos[C.ppos ^ 1u][a][slot];
+    const uint32_t si = MULTI ? sp[local] : 0u;
     const float list_r2 = S.pt.list_r2[0];
     uint32_t c[D];
     cell_of<D>(S, xi, c);
-    const uint32_t cap = S.cap, stride = S.nslots, blk = local / BATCH_WG;
-    uint32_t* col = B.nbr + S.nbr0 + local;
+    const uint32_t cap4 = S.cap / 4u, stride = S.nslots;
+    uint4* row = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + local;
+    uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     const uint32_t* cs = B.cell_start + S.cell0;
     const uint32_t* ce = B.cell_end + S.cell0;
-    const float* rel[D];
-    for (int a = 0; a < D; a++) rel[a] = B.rel[a] + S.slot0;
+    const float4* rel = B.rel + S.slot0;
+    float wl[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; a++) wl[a] = (S.nc[a] == 1u && ((S.periodic >> a) & 1u)) ? S.cell_w[a] : 0.f;
+    const bool any_wrap = (wl[0] != 0.f) | (wl[1] != 0.f) | (wl[2] != 0.f);
+    const uint32_t blo = local / BATCH_WG * BATCH_WG;
     uint32_t na = 0, nb = 0;
     for (int o = 0; o < stencil_size(D); o++) {
-        int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
+        const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
         uint32_t cc[D];
-        float shift[D];
+        float shift[3] = {0.f, 0.f, 0.f};
         bool skip = false;
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -195,40 +204,66 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
             shift[a] = (float)off[a] * S.cell_w[a] - ri[a];
         }
         if (skip) continue;
-        uint32_t key = lin_cell<D>(S, cc);
-        uint32_t s0 = cs[key], e0 = ce[key];
-        for (uint32_t j = s0; j < e0; j++) {
-            if (j == local) continue;
-            float r2 = 0.0f;
-#pragma unroll
-            for (int a = 0; a < D; a++) {
-                float d = rel[a][j] + shift[a];
-                if (S.nc[a] == 1u && ((S.periodic >> a) & 1u)) {
-                    if (d > 0.5f * S.cell_w[a]) d -= S.cell_w[a];
-                    else if (d < -0.5f * S.cell_w[a]) d += S.cell_w[a];
+        const uint32_t key = lin_cell<D>(S, cc);
+        const uint32_t s0 = cs[key], e0 = ce[key];
+        const float sx = shift[0], sy = shift[1], sz = shift[2];
+        for (uint32_t base = s0; base < e0; base += 32u) {
+            const uint32_t cnt = min(32u, e0 - base);
+            uint32_t mask = 0u;
+            auto near = [&](uint32_t u) -> uint32_t {
+                const float4 q = rel[base + u];
+                float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
+                if (any_wrap) {
+                    if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
+                    if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
+                    if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
                 }
-                r2 = __builtin_fmaf(d, d, r2);
+                float r2 = dx * dx;
+                r2 = __builtin_fmaf(dy, dy, r2);
+                if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
+                const float lr2 = MULTI ? S.pt.list_r2[si * 4 + sp[base + u]] : list_r2;
+                return r2 < lr2 ? 1u : 0u;
+            };
+            uint32_t u = 0;
+            for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
+            if (u < cnt) mask |= near(u) << u;
+            const int t0 = (int)local - (int)base;
+            const uint32_t self = (t0 >= 0 && t0 < 32) ? (1u << t0) : 0u;
+            uint32_t inb = 0u;
+            if (variant == 1) {
+                const uint32_t lt = bits_below(t0), gt = ~bits_below(t0 + 1);
+                const uint32_t same = ((local ^ base) & 1u) ? 0xAAAAAAAAu : 0x55555555u;
+                inb = bits_below((int)(blo + BATCH_WG) - (int)base) & ~bits_below((int)blo - (int)base);
+                const uint32_t owned = (gt & ~same) | (lt & same);
+                mask &= (inb & owned) | ~inb;
             }
-            float lr2 = MULTI ? S.pt.list_r2[si * 4 + sp[j]] : list_r2;
-            if (r2 < lr2) {
-                if (variant == 1 && j / BATCH_WG == blk) {
-                    if (owns(local, j)) {
-                        if (na + nb < cap) col[(size_t)na * stride] = j;
-                        na++;
-                    }
+            mask &= ~self;
+            while (mask) {
+                const uint32_t b = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const uint32_t j = base + b;
+                if ((inb >> b) & 1u) {
+                    put4(fa, na, j);
+                    na++;
+                    if ((na & 3u) == 0u && na / 4u <= cap4) row[(size_t)(na / 4u - 1u) * stride] = fa;
                 } else {
-                    if (na + nb < cap) col[(size_t)(cap - 1 - nb) * stride] = j;
+                    put4(fb, nb, j);
                     nb++;
+                    if ((nb & 3u) == 0u && nb / 4u <= cap4) row[(size_t)(cap4 - nb / 4u) * stride] = fb;
                 }
             }
         }
     }
-    uint32_t count = na + nb;
-    if (count > cap) {
+    const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
+    const bool over = ca + cb > cap4;
+    if (!over) {
+        if (na & 3u) row[(size_t)(ca - 1u) * stride] = fa;
+        if (nb & 3u) row[(size_t)(cap4 - cb) * stride] = fb;
+    } else {
         atomicOr(&C.overflow, 1u);
         na = nb = 0;
     }
-    atomicMax(&C.maxcount, count);
+    atomicMax(&C.maxcount, 4u * (ca + cb));
     B.nbr_count[slot] = na | (nb << 16);
 }
 
@@ -300,48 +335,79 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         const float rc2_0 = pt.rc2[0], sigma2_0 = pt.sigma2[0], six_eps4_0 = pt.six_eps4[0], r2f_0 = pt.r2_fast[0];
         const uint32_t cnt = B.nbr_count[slot];
         const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
-        const uint32_t* col = B.nbr + S.nbr0 + local;
+        const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr + S.nbr0) + local;
         const size_t stride = S.nslots;
+        const uint32_t cap4 = S.cap / 4u;
         const bool rc = S.range_check != 0;
-        for (uint32_t e = 0; e < na; e++) {
-            const uint32_t j = col[(size_t)e * stride];
-            int64_t xj[D];
+        // in-block pairs, both ends: chunks 0, 1, ... (next chunk in flight, four neighbours'
+        // positions loaded together)
+        {
+            const uint32_t nch = (na + 3u) / 4u;
+            uint4 q_next = nch > 0 ? rowc[0] : make_uint4(local, local, local, local);
+            for (uint32_t ch = 0; ch < nch; ch++) {
+                const uint4 q = q_next;
+                if (ch + 1 < nch) q_next = rowc[(size_t)(ch + 1) * stride];
+                const uint32_t m = na - 4u * ch;
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : local, m > 2u ? q.z : local, m > 3u ? q.w : local};
+                int64_t xs[4][D];
 #pragma unroll
-            for (int a = 0; a < D; a++) xj[a] = px[a][j];
-            float f[D];
-            bool ok = true;
+                for (int e = 0; e < 4; e++)
 #pragma unroll
-            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xj[a], a), f[a]);
-            if (rc && !ok) continue;
-            int32_t kj[D];
-            if constexpr (MULTI) {
-                uint32_t sj = sp[j];
-                uint32_t p = si * 4 + sj;
-                pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
-            } else {
-                pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if ((uint32_t)e >= m) break;
+                    const uint32_t j = js[e];
+                    float f[D];
+                    bool ok = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xs[e][a], a), f[a]);
+                    if (rc && !ok) continue;
+                    int32_t kj[D];
+                    if constexpr (MULTI) {
+                        uint32_t sj = sp[j];
+                        uint32_t p = si * 4 + sj;
+                        pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
+                    } else {
+                        pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                    }
+                    const uint32_t lj = j - base;
+#pragma unroll
+                    for (int a = 0; a < D; a++)
+                        if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+                }
             }
-            const uint32_t lj = j - base;
-#pragma unroll
-            for (int a = 0; a < D; a++)
-                if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
         }
-        const uint32_t* colb = col + (size_t)(S.cap - 1) * stride;
-        for (uint32_t e = 0; e < nb; e++) {
-            const uint32_t j = colb[-(ptrdiff_t)((size_t)e * stride)];
-            int64_t xj[D];
+        // the other pairs, this end only: chunks cap/4 - 1, cap/4 - 2, ...
+        {
+            const uint32_t nch = (nb + 3u) / 4u;
+            const uint4* rowb = rowc + (size_t)(cap4 - 1u) * stride;
+            uint4 q_next = nch > 0 ? rowb[0] : make_uint4(local, local, local, local);
+            for (uint32_t ch = 0; ch < nch; ch++) {
+                const uint4 q = q_next;
+                if (ch + 1 < nch) q_next = rowb[-(ptrdiff_t)((size_t)(ch + 1) * stride)];
+                const uint32_t m = nb - 4u * ch;
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : local, m > 2u ? q.z : local, m > 3u ? q.w : local};
+                int64_t xs[4][D];
 #pragma unroll
-            for (int a = 0; a < D; a++) xj[a] = px[a][j];
-            float f[D];
-            bool ok = true;
+                for (int e = 0; e < 4; e++)
 #pragma unroll
-            for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xj[a], a), f[a]);
-            if (rc && !ok) continue;
-            if constexpr (MULTI) {
-                uint32_t p = si * 4 + sp[j];
-                pair_kick<D>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, k);
-            } else {
-                pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
+                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if ((uint32_t)e >= m) break;
+                    float f[D];
+                    bool ok = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xs[e][a], a), f[a]);
+                    if (rc && !ok) continue;
+                    if constexpr (MULTI) {
+                        uint32_t p = si * 4 + sp[js[e]];
+                        pair_kick<D>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, k);
+                    } else {
+                        pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
+                    }
+                }
             }
         }
         if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
@@ -374,21 +440,19 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&C.stale_next, 1u);
 }
 
+// One launch per kind of system present (dimension, one or more species), each compiled with
+// only its own path so the common single-species kernels keep their registers low; work-groups
+// of other kinds exit at once.
+template <int D, bool MULTI>
 __global__ __launch_bounds__(BATCH_WG) void k_bb_step(BBufs B) {
     const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
     uint32_t s = B.wg_sys[wg];
     if (s == ~0u) return;
     const BSys& S = B.sys[s];
+    if (S.dim != (uint32_t)D || (S.nspecies > 1) != MULTI) return;
     BCtl& C = B.ctl[s];
     if (!bb_runs(S, C)) return;
-    bool multi = S.nspecies > 1;
-    if (S.dim == 3) {
-        if (multi) bb_step<3, true>(B, S, C, wg);
-        else bb_step<3, false>(B, S, C, wg);
-    } else {
-        if (multi) bb_step<2, true>(B, S, C, wg);
-        else bb_step<2, false>(B, S, C, wg);
-    }
+    bb_step<D, MULTI>(B, S, C, wg);
 }
 
 __global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys) {
@@ -501,10 +565,8 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
         B_.species[p] = (uint32_t*)dalloc(slot_cap * 4);
         B_.ids[p] = (uint32_t*)dalloc(slot_cap * 4);
     }
-    for (int a = 0; a < 3; a++) {
-        B_.ref[a] = (int64_t*)dalloc(slot_cap * 8);
-        B_.rel[a] = (float*)dalloc(slot_cap * 4);
-    }
+    for (int a = 0; a < 3; a++) B_.ref[a] = (int64_t*)dalloc(slot_cap * 8);
+    B_.rel = (float4*)dalloc(slot_cap * 16);
     B_.key = (uint32_t*)dalloc(slot_cap * 4);
     B_.rank = (uint32_t*)dalloc(slot_cap * 4);
     B_.nbr_count = (uint32_t*)dalloc(slot_cap * 4);
@@ -613,7 +675,7 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
     double r = rcm + skin;
     double dens = (double)n / vol;
     double c = D == 3 ? std::ceil(4.0 / 3.0 * pi * r * r * r * std::fmax(2.0 * dens, 1.2)) : std::ceil(pi * r * r * std::fmax(2.0 * dens, 1.2));
-    *cap = (uint32_t)std::min(c + 8.0, 65535.0);
+    *cap = ((uint32_t)std::min(c + 8.0, 65532.0) + 3u) & ~3u;  // whole four-entry chunks
 }
 
 uint64_t Batch::bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double skin) const {
@@ -730,8 +792,11 @@ void Batch::upload_tables() {
     }
     RH_HIP(hipMemcpy(d_wg_, wg.data(), wg.size() * 4, hipMemcpyHostToDevice));
     uint32_t new_wgs = (uint32_t)std::max<uint64_t>(hw, 1);
-    if (new_wgs != grid_wgs_) invalidate_graph();
+    uint32_t kinds = 0;
+    for (auto& kv : members_) kinds |= 1u << ((kv.second.sys.dim == 3 ? 2u : 0u) + (kv.second.sys.nspecies > 1 ? 1u : 0u));
+    if (new_wgs != grid_wgs_ || kinds != kinds_) invalidate_graph();
     grid_wgs_ = new_wgs;
+    kinds_ = kinds;
     tables_dirty_ = false;
 }
 
@@ -750,7 +815,10 @@ void Batch::enqueue_iteration() {
     k_bb_scatter<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_build<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_, 1u);
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
-    k_bb_step<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 1u) k_bb_step<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 2u) k_bb_step<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 4u) k_bb_step<3, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 8u) k_bb_step<3, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
 }
 
@@ -788,7 +856,7 @@ void Batch::handle_overflow() {
         if (!c.overflow) continue;
         BatchMember& m = kv.second;
         uint32_t want = std::max<uint32_t>(c.maxcount + c.maxcount / 4 + 8, m.cap + m.cap / 2);
-        want = std::min<uint32_t>(want, 65535u);
+        want = (std::min<uint32_t>(want, 65532u) + 3u) & ~3u;
         uint64_t lunits = ((uint64_t)want * m.nslots + LIST_UNIT - 1) / LIST_UNIT;
         uint64_t l0;
         lists_.free(m.nbr0 / LIST_UNIT, m.nbr_entries / LIST_UNIT);

@@ -439,20 +439,6 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // Half-list variant
 // ---------------------------------------------------------------------------------------------
 
-// Bits u < t of a 32-bit mask (t may be out of [0, 32]).
-__device__ __forceinline__ uint32_t bits_below(int t) {
-    return t <= 0 ? 0u : (t >= 32 ? 0xffffffffu : ((1u << t) - 1u));
-}
-
-// Entry k & 3 of a four-entry chunk, without dynamic register indexing.
-__device__ __forceinline__ void put4(uint4& q, uint32_t k, uint32_t j) {
-    const uint32_t s = k & 3u;
-    q.x = s == 0u ? j : q.x;
-    q.y = s == 1u ? j : q.y;
-    q.z = s == 2u ? j : q.z;
-    q.w = s == 3u ? j : q.w;
-}
-
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do

@@ -195,6 +195,20 @@ __device__ __forceinline__ void stencil_cell(const PT& P, const uint32_t c[D], i
 
 constexpr int stencil_size(int d) { return d == 3 ? 27 : 9; }
 
+// Bits u < t of a 32-bit mask (t may be out of [0, 32]).
+__device__ __forceinline__ uint32_t bits_below(int t) {
+    return t <= 0 ? 0u : (t >= 32 ? 0xffffffffu : ((1u << t) - 1u));
+}
+
+// Entry k & 3 of a four-entry list chunk, without dynamic register indexing.
+__device__ __forceinline__ void put4(uint4& q, uint32_t k, uint32_t j) {
+    const uint32_t s = k & 3u;
+    q.x = s == 0u ? j : q.x;
+    q.y = s == 1u ? j : q.y;
+    q.z = s == 2u ? j : q.z;
+    q.w = s == 3u ? j : q.w;
+}
+
 // Which end of an in-block pair lists it: alternate by the parity of i + j so every particle
 // owns about half of its in-block neighbours whatever its place in the block.
 __device__ __forceinline__ bool owns(uint32_t i, uint32_t j) {
