@@ -594,76 +594,50 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
         const bool rc = P.range_check != 0;
         // The list is read a four-entry chunk (16 B) at a time, the next chunk in flight while
         // this one is evaluated; the positions of a chunk's four neighbours are loaded together
-        // before any of its pairs is evaluated.
+        // before any of its pairs is evaluated.  One loop walks the in-block chunks (front of
+        // the row; both ends' kicks, the partner's into LDS) and then the cross-block chunks
+        // (back of the row; this end only): a wave then runs as many chunks as its longest
+        // list, however each lane's list divides between the two kinds.
         const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr) + i;
         const uint32_t cap4 = P.cap / 4u;
-        // in-block pairs, both ends: chunks 0, 1, ...
-        {
-            const uint32_t nch = (na + 3u) / 4u;
-            uint4 q_next = nch > 0 ? rowc[0] : make_uint4(i, i, i, i);
-            for (uint32_t ch = 0; ch < nch; ch++) {
-                const uint4 q = q_next;
-                if (ch + 1 < nch) q_next = rowc[(size_t)(ch + 1) * stride];
-                const uint32_t m = na - 4u * ch;  // entries in this chunk: min(m, 4)
-                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-                int64_t xs[4][D];
+        const uint32_t ca = (na + 3u) / 4u, nch = ca + (nb + 3u) / 4u;
+        auto chunk_at = [&](uint32_t ch) -> uint4 {
+            return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
+        };
+        uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(i, i, i, i);
+        for (uint32_t ch = 0; ch < nch; ch++) {
+            const uint4 q = q_next;
+            if (ch + 1 < nch) q_next = chunk_at(ch + 1);
+            const bool inblock = ch < ca;
+            const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);  // entries: min(m, 4)
+            const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+            int64_t xs[4][D];
 #pragma unroll
-                for (int e = 0; e < 4; e++)
+            for (int e = 0; e < 4; e++)
 #pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+                for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if ((uint32_t)e >= m) break;
-                    const uint32_t j = js[e];
-                    float f[D];
-                    bool ok = true;
+            for (int e = 0; e < 4; e++) {
+                if ((uint32_t)e >= m) break;
+                const uint32_t j = js[e];
+                float f[D];
+                bool ok = true;
 #pragma unroll
-                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xs[e][a], a), f[a]);
-                    if (rc && !ok) continue;
-                    int32_t kj[D];
-                    if constexpr (MULTI) {
-                        uint32_t sj = sp[j];
-                        uint32_t p = si * 4 + sj;
-                        pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
-                    } else {
-                        pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                    }
+                for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xs[e][a], a), f[a]);
+                if (rc && !ok) continue;
+                int32_t kj[D];
+                if constexpr (MULTI) {
+                    uint32_t sj = sp[j];
+                    uint32_t p = si * 4 + sj;
+                    pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+                } else {
+                    pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                }
+                if (inblock) {
                     const uint32_t lj = j - base;
 #pragma unroll
                     for (int a = 0; a < D; a++)
                         if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
-                }
-            }
-        }
-        // pairs across blocks, this end only: chunks cap/4 - 1, cap/4 - 2, ...
-        {
-            const uint32_t nch = (nb + 3u) / 4u;
-            const uint4* rowb = rowc + (size_t)(cap4 - 1u) * stride;
-            uint4 q_next = nch > 0 ? rowb[0] : make_uint4(i, i, i, i);
-            for (uint32_t ch = 0; ch < nch; ch++) {
-                const uint4 q = q_next;
-                if (ch + 1 < nch) q_next = rowb[-(ptrdiff_t)((size_t)(ch + 1) * stride)];
-                const uint32_t m = nb - 4u * ch;
-                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-                int64_t xs[4][D];
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-#pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if ((uint32_t)e >= m) break;
-                    float f[D];
-                    bool ok = true;
-#pragma unroll
-                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xs[e][a], a), f[a]);
-                    if (rc && !ok) continue;
-                    if constexpr (MULTI) {
-                        uint32_t p = si * 4 + sp[js[e]];
-                        pair_kick<D>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
-                    } else {
-                        pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
-                    }
                 }
             }
         }
