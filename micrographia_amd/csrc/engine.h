@@ -49,6 +49,8 @@ struct Params {
     uint32_t cap;
     int64_t box[3];
     int64_t half[3];
+    double boxf[3];        // box length as f64 (positions are kept as exact f64 integers)
+    double inv_box[3];
     double cell_scale[3];  // cells per position unit
     float kick_c[4];
     int32_t gravity[3];
@@ -73,11 +75,13 @@ struct Params {
 
 template <int D>
 struct Bufs {
-    int64_t* pos[2][D];  // [parity][axis]
+    // Positions in 2^-32 sigma units, held as f64: integers below 2^53, so exact (the C ABI
+    // takes and returns i64).
+    double* pos[2][D];   // [parity][axis]
     int32_t* vel[2][D];
     uint32_t* species[2];
     uint32_t* ids[2];
-    int64_t* ref[D];     // positions at the last list build
+    double* ref[D];      // positions at the last list build
     float4* rel;         // position relative to its cell's corner (sigma, f32; w unused), for list builds
     uint32_t* nbr;       // column-major: entry k of slot i at k*n + i
     uint32_t* nbr_count;

@@ -173,21 +173,21 @@ __device__ void bath_redraw(const Params<D>& P, unsigned long long* L, uint32_t 
 // from the wall's bath (three.rs:232-243), and the exchange is booked in the ledger
 // (int_engine3.rs:186-206).
 template <int D>
-__device__ void bound_booked(const Params<D>& P, unsigned long long* L, int64_t x[D], int32_t v[D], uint32_t id, uint32_t clock, uint32_t s) {
+__device__ void bound_booked(const Params<D>& P, unsigned long long* L, double x[D], int32_t v[D], uint32_t id, uint32_t clock, uint32_t s) {
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const int64_t l = P.box[a];
+        const double l = P.boxf[a];
         if ((P.periodic >> a) & 1u) {
-            if (x[a] < 0) x[a] += l;
+            if (x[a] < 0.0) x[a] += l;
             else if (x[a] >= l) x[a] -= l;
             continue;
         }
         int side = -1;
-        if (x[a] < 0) {
-            x[a] = -1 - x[a];
+        if (x[a] < 0.0) {
+            x[a] = -1.0 - x[a];
             side = 0;
         } else if (x[a] >= l) {
-            x[a] = 2 * l - 1 - x[a];
+            x[a] = 2.0 * l - 1.0 - x[a];
             side = 1;
         }
         if (side < 0) continue;
@@ -237,14 +237,14 @@ __device__ void bound_booked(const Params<D>& P, unsigned long long* L, int64_t 
 // plain periodic / hard-wall leapfrog only (no ledger, no stochastic boundaries), so the hot
 // kernels of systems without them carry none of that code's registers.
 template <int D, bool BOOK>
-__device__ __forceinline__ void advance(const Params<D>& P, const Bufs<D>& B, uint32_t i, uint32_t pm, uint32_t si, const uint32_t k[D], const int64_t xi[D],
-                                        uint32_t clock, const float* wf, int64_t x[D], int32_t v[D]) {
+__device__ __forceinline__ void advance(const Params<D>& P, const Bufs<D>& B, uint32_t i, uint32_t pm, uint32_t si, const uint32_t k[D], const double xi[D],
+                                        uint32_t clock, const float* wf, double x[D], int32_t v[D]) {
     const bool pin = (P.pinned >> si) & 1u;
 #pragma unroll
     for (int a = 0; a < D; a++) v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + k[a] + (pin ? 0u : (uint32_t)P.gravity[a]));
     if (!BOOK || !P.ledger) {
 #pragma unroll
-        for (int a = 0; a < D; a++) x[a] = xi[a] + (int64_t)v[a];
+        for (int a = 0; a < D; a++) x[a] = xi[a] + (double)v[a];
         bound<D>(P, x, v);
         return;
     }
@@ -254,7 +254,7 @@ __device__ __forceinline__ void advance(const Params<D>& P, const Bufs<D>& B, ui
         if (wf[w] != 0.0f) ledger_add(B.ledger, w, 1, (double)wf[w] * P.dt);
     if (P.bath) bath_redraw<D>(P, B.ledger, id, clock, si, v);
 #pragma unroll
-    for (int a = 0; a < D; a++) x[a] = xi[a] + (int64_t)v[a];
+    for (int a = 0; a < D; a++) x[a] = xi[a] + (double)v[a];
     bound_booked<D>(P, B.ledger, x, v, id, clock, si);
 }
 
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(TPB) void k_keys(Params<D> P, Bufs<D> B, uint32_t* 
     uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= P.n) return;
     uint32_t pp = B.ctl[CTL_PPOS];
-    int64_t x[D];
+    double x[D];
     for (int a = 0; a < D; a++) x[a] = B.pos[pp][a][i];
     uint32_t c[D];
     cell_of<D>(P, x, c);
@@ -294,10 +294,10 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
     if (k >= P.n) return;
     uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
     uint32_t i = perm[k];
-    int64_t xs[D];
+    double xs[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        int64_t x = B.pos[pp][a][i];
+        const double x = B.pos[pp][a][i];
         xs[a] = x;
         B.pos[pp ^ 1][a][k] = x;
         if (write_ref) B.ref[a][k] = x;
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
     cell_of<D>(P, xs, c);
     float r[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; a++) r[a] = (float)((double)xs[a] * 0x1p-32 - (double)c[a] * P.cell_w64[a]);
+    for (int a = 0; a < D; a++) r[a] = (float)(xs[a] * 0x1p-32 - (double)c[a] * P.cell_w64[a]);
     B.rel[k] = make_float4(r[0], r[1], r[2], 0.f);
     B.species[pm ^ 1][k] = B.species[pm][i];
     B.ids[pm ^ 1][k] = B.ids[pm][i];
@@ -331,10 +331,10 @@ __global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, cons
     uint32_t i = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= P.n) return;
     uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
-    const int64_t* px[D];
+    const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
     const uint32_t* sp = B.species[pm];
-    int64_t xi[D];
+    double xi[D];
     for (int a = 0; a < D; a++) xi[a] = px[a][i];
     uint32_t si = MULTI ? sp[i] : 0u;
     const float list_r2 = B.pt->list_r2[0];
@@ -381,10 +381,10 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     uint32_t i = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
-    const int64_t* px[D];
+    const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
     const uint32_t* sp = B.species[pm];
-    int64_t xi[D];
+    double xi[D];
     for (int a = 0; a < D; a++) xi[a] = px[a][i];
     uint32_t si = MULTI ? sp[i] : 0u;
     const PairTable* __restrict__ pt = B.pt;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     for (uint32_t e = 0; e < m; e++) {
         const uint32_t j = j_next;
         if (e + 1 < m) j_next = nb[(size_t)(e + 1) * stride];
-        int64_t xj[D];
+        double xj[D];
 #pragma unroll
         for (int a = 0; a < D; a++) xj[a] = px[a][j];
         float f[D];
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
     // leapfrog: v += kick + g, x += v, boundaries
     int32_t v[D];
-    int64_t x[D];
+    double x[D];
     advance<D, BOOK>(P, B, i, pm, si, k, xi, ctl[CTL_CLOCK], wf, x, v);
     float r2 = 0.0f;
     bool far = false;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
     const uint32_t* sp = B.species[pm];
     const float4* __restrict__ rel = B.rel;
-    int64_t xi[D];
+    double xi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) xi[a] = B.pos[pp][a][i];
     const float4 me = rel[i];
@@ -568,12 +568,12 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     for (int a = 0; a < D; a++) acc[a][t] = 0u;
     __syncthreads();
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
-    const int64_t* px[D];
+    const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
     const uint32_t* sp = B.species[pm];
     const PairTable* __restrict__ pt = B.pt;
     const bool live = i < P.n;
-    int64_t xi[D];
+    double xi[D];
     uint32_t k[D];
     uint32_t si = 0;
     float c = P.kick_c[0];
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             const bool inblock = ch < ca;
             const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);  // entries: min(m, 4)
             const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-            int64_t xs[4][D];
+            double xs[4][D];
 #pragma unroll
             for (int e = 0; e < 4; e++)
 #pragma unroll
@@ -621,23 +621,31 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                 if ((uint32_t)e >= m) break;
                 const uint32_t j = js[e];
                 float f[D];
-                bool ok = true;
+                double d[D];
 #pragma unroll
-                for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(P, xi[a], xs[e][a], a), f[a]);
-                if (rc && !ok) continue;
+                for (int a = 0; a < D; a++) {
+                    d[a] = separation<D>(P, xi[a], xs[e][a], a);
+                    f[a] = sep_f_in_range(d[a]);
+                }
+                if (rc) {
+                    bool ok = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) ok &= sep_in_range(d[a]);
+                    if (!ok) continue;
+                }
                 int32_t kj[D];
+                bool hit;
                 if constexpr (MULTI) {
                     uint32_t sj = sp[j];
                     uint32_t p = si * 4 + sj;
-                    pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+                    hit = pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
                 } else {
-                    pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                    hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
                 }
-                if (inblock) {
+                if (inblock && hit) {
                     const uint32_t lj = j - base;
 #pragma unroll
-                    for (int a = 0; a < D; a++)
-                        if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+                    for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
                 }
             }
         }
@@ -649,7 +657,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
 #pragma unroll
     for (int a = 0; a < D; a++) k[a] += acc[a][t];
     int32_t v[D];
-    int64_t x[D];
+    double x[D];
     advance<D, BOOK>(P, B, i, pm, si, k, xi, ctl[CTL_CLOCK], wf, x, v);
     float r2 = 0.0f;
     bool far = false;
@@ -677,10 +685,10 @@ __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, con
     uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
-    const int64_t* px[D];
+    const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
     const uint32_t* sp = B.species[pm];
-    int64_t xi[D];
+    double xi[D];
     for (int a = 0; a < D; a++) xi[a] = px[a][i];
     uint32_t si = MULTI ? sp[i] : 0u;
     float c = B.pt->kick_c[si];
@@ -719,7 +727,7 @@ __global__ __launch_bounds__(TPB) void k_integrate(Params<D> P, Bufs<D> B, uint3
     if (i >= P.n) return;
     const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
     uint32_t si = B.species[pm][i];
-    int64_t xi[D];
+    double xi[D];
     uint32_t k[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
@@ -733,7 +741,7 @@ __global__ __launch_bounds__(TPB) void k_integrate(Params<D> P, Bufs<D> B, uint3
         soft_walls<D>(P, xi, B.pt->kick_c[si], dummy, nullptr, wf);
     }
     int32_t v[D];
-    int64_t x[D];
+    double x[D];
     advance<D, true>(P, B, i, pm, si, k, xi, B.ctl[CTL_CLOCK] + clock_add, wf, x, v);
 #pragma unroll
     for (int a = 0; a < D; a++) {
@@ -785,8 +793,9 @@ __global__ __launch_bounds__(TPB) void k_kick_region(Params<D> P, Bufs<D> B, flo
     int32_t add[3] = {a0, a1, a2};
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        int64_t x = B.pos[pp][a][i];
-        float xf = (float)(uint32_t)((uint64_t)x >> 32) + __builtin_ldexpf((float)(uint32_t)x, -32);
+        const double x = B.pos[pp][a][i];
+        const double whole = __builtin_floor(x * 0x1p-32);  // the i64's two 32-bit words, exactly
+        float xf = (float)(uint32_t)whole + __builtin_ldexpf((float)(uint32_t)(x - whole * 4294967296.0), -32);
         if (xf < lo[a] || xf >= hi[a]) return;
     }
 #pragma unroll
@@ -863,8 +872,11 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
             P.nc[a] = 1;
         }
         ncmax = std::max(ncmax, P.nc[a]);
+        RH_REQUIRE(sys.box_len[a] < (int64_t(1) << 52), "box longer than 2^20 sigma");
         P.box[a] = sys.box_len[a];
         P.half[a] = sys.box_len[a] / 2;
+        P.boxf[a] = (double)sys.box_len[a];
+        P.inv_box[a] = 1.0 / P.boxf[a];
         P.cell_scale[a] = (double)P.nc[a] / (double)sys.box_len[a];
         if (!sys.walled[a]) P.periodic |= 1u << a;
         else
@@ -996,14 +1008,14 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (int p = 0; p < 2; p++) {
         for (int a = 0; a < D; a++) {
-            B_.pos[p][a] = (int64_t*)dalloc(n * sizeof(int64_t));
+            B_.pos[p][a] = (double*)dalloc(n * sizeof(double));
             B_.vel[p][a] = (int32_t*)dalloc(n * sizeof(int32_t));
         }
         B_.species[p] = (uint32_t*)dalloc(n * sizeof(uint32_t));
         B_.ids[p] = (uint32_t*)dalloc(n * sizeof(uint32_t));
     }
     for (int a = 0; a < D; a++) {
-        B_.ref[a] = (int64_t*)dalloc(n * sizeof(int64_t));
+        B_.ref[a] = (double*)dalloc(n * sizeof(double));
         B_.kicks[a] = (int32_t*)dalloc(n * sizeof(int32_t));
     }
     B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
@@ -1061,12 +1073,12 @@ void Engine<D>::upload(const int64_t* pos, const int32_t* vel, const uint32_t* s
         for (int a = 0; a < D; a++)
             RH_REQUIRE(pos[D * i + a] >= 0 && pos[D * i + a] < P_.box[a], "position outside the box");
     }
-    std::vector<int64_t> px(n_);
+    std::vector<double> px(n_);
     std::vector<int32_t> vx(n_);
     std::vector<uint32_t> ids(n_);
     for (int a = 0; a < D; a++) {
-        for (uint64_t i = 0; i < n_; i++) px[i] = pos[D * i + a];
-        RH_HIP(hipMemcpy(B_.pos[0][a], px.data(), n_ * sizeof(int64_t), hipMemcpyHostToDevice));
+        for (uint64_t i = 0; i < n_; i++) px[i] = (double)pos[D * i + a];  // exact: |x| < 2^52
+        RH_HIP(hipMemcpy(B_.pos[0][a], px.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
         for (uint64_t i = 0; i < n_; i++) vx[i] = vel[D * i + a];
         RH_HIP(hipMemcpy(B_.vel[0][a], vx.data(), n_ * sizeof(int32_t), hipMemcpyHostToDevice));
     }
@@ -1361,14 +1373,14 @@ void Engine<D>::read_state(int64_t* pos, int32_t* vel, uint32_t* species) {
     read_ctl(ctl);
     uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
     std::vector<uint32_t> ids(n_), sp(n_);
-    std::vector<int64_t> px(n_);
+    std::vector<double> px(n_);
     std::vector<int32_t> vx(n_);
     RH_HIP(hipMemcpy(ids.data(), B_.ids[pm], n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (int a = 0; a < D; a++) {
-        RH_HIP(hipMemcpy(px.data(), B_.pos[pp][a], n_ * sizeof(int64_t), hipMemcpyDeviceToHost));
+        RH_HIP(hipMemcpy(px.data(), B_.pos[pp][a], n_ * sizeof(double), hipMemcpyDeviceToHost));
         RH_HIP(hipMemcpy(vx.data(), B_.vel[pm][a], n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
         for (uint64_t s = 0; s < n_; s++) {
-            if (pos) pos[D * (uint64_t)ids[s] + a] = px[s];
+            if (pos) pos[D * (uint64_t)ids[s] + a] = (int64_t)px[s];
             if (vel) vel[D * (uint64_t)ids[s] + a] = vx[s];
         }
     }

@@ -69,6 +69,86 @@ __device__ __forceinline__ bool sep_f(int64_t d, float& f) {
     return ok;
 }
 
+// ---- The same operations on positions held in f64 ------------------------------------------
+// The single-system engine keeps positions as f64 values that are integers (2^-32 sigma units)
+// below 2^53 in magnitude, so every position, every difference of two positions and every
+// position plus a step displacement is exact in f64, and the results below equal the i64
+// versions bit for bit.  (The parameter struct adds boxf, halff and inv_box per axis.)
+
+template <int D, class PT>
+__device__ __forceinline__ void cell_of(const PT& P, const double x[D], uint32_t c[D]) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double f = x[a] * P.cell_scale[a];
+        uint32_t ci = f > 0.0 ? (uint32_t)f : 0u;
+        c[a] = ci < P.nc[a] - 1 ? ci : P.nc[a] - 1;
+    }
+}
+
+// Minimum image of b - a on periodic axes.  d - L rint(d / L) is exact (d and L rint(d / L)
+// are integers below 2^53) and equals fixed::min_image's choice for every |d| other than L/2
+// itself (where either image is as far as the box allows, beyond every cutoff the engine
+// accepts, so no kick or energy depends on the choice).
+template <int D, class PT>
+__device__ __forceinline__ double separation(const PT& P, double a, double b, int ax) {
+    double d = b - a;
+    if ((P.periodic >> ax) & 1u) d = __builtin_fma(-P.boxf[ax], __builtin_rint(d * P.inv_box[ax]), d);
+    return d;
+}
+
+// fixed::sep_f32 on an f64 difference: trunc(d / 16) (exact: a power-of-two scale, then the
+// integer part), rounded once to f32, times 2^-28; false if |d| >= 8 sigma.
+__device__ __forceinline__ bool sep_f(double d, float& f) {
+    const bool ok = d >= -34359738368.0 && d < 34359738368.0;
+    f = (float)__builtin_trunc(d * 0.0625) * 0x1p-28f;
+    return ok;
+}
+
+// The conversion alone, for list pairs whose separations are known to be in range.
+__device__ __forceinline__ float sep_f_in_range(double d) { return (float)__builtin_trunc(d * 0.0625) * 0x1p-28f; }
+
+__device__ __forceinline__ bool sep_in_range(double d) { return d >= -34359738368.0 && d < 34359738368.0; }
+
+template <int D, class PT>
+__device__ __forceinline__ void soft_walls(const PT& P, const double x[D], float c, uint32_t k[D], double* wall_pe, float* wf = nullptr) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        for (int side = 0; side < 2; side++) {
+            if (!((P.soft_walls >> (2 * a + side)) & 1u)) continue;
+            double d = side == 0 ? x[a] : P.boxf[a] - x[a];
+            float df;
+            if (!sep_f(d, df)) continue;
+            if (df < SOFT_RC_F) {
+                float s2 = 1.0f / (df * df);
+                float s6 = s2 * s2 * s2;
+                float fw = 24.0f * s6 * (2.0f * s6 - 1.0f) / df;
+                int32_t kw = cvt_sat_i32(__builtin_rintf(fw * c));
+                k[a] += side == 0 ? (uint32_t)kw : (uint32_t)0 - (uint32_t)kw;
+                if (wall_pe) *wall_pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
+                if (wf) wf[2 * a + side] = fw;
+            }
+        }
+    }
+}
+
+template <int D, class PT>
+__device__ __forceinline__ void bound(const PT& P, double x[D], int32_t v[D]) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const double l = P.boxf[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0.0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+        } else if (x[a] < 0.0) {
+            x[a] = -1.0 - x[a];
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        } else if (x[a] >= l) {
+            x[a] = 2.0 * l - 1.0 - x[a];
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        }
+    }
+}
+
 // The reciprocal and the division steps of the hardware's correctly rounded f32 division
 // (v_rcp, one refinement of the reciprocal, then two correction steps), without its operand
 // scaling and special-case fixup.  For operands where the hardware sequence scales nothing
@@ -220,12 +300,13 @@ __device__ __forceinline__ bool owns(uint32_t i, uint32_t j) {
 // rounded separately with its own factor (int_engine.rs:17-22): for equal factors it is the
 // conversion of -rint(a f), the exact negation of this end's rounded value.
 template <int D, bool MULTI>
-__device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float r2_fast, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
+__device__ __forceinline__ bool pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float r2_fast, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
     float r2 = f[0] * f[0] + f[1] * f[1];
     if constexpr (D == 3) r2 = r2 + f[2] * f[2];
 #pragma unroll
     for (int ax = 0; ax < D; ax++) kj[ax] = 0;
-    if (r2 < rc2) {
+    const bool hit = r2 < rc2;
+    if (hit) {
         float fr = pair_fr(r2, sigma2, six_eps4, r2_fast);
         float a = -fr * ci;
         if constexpr (MULTI) {
@@ -244,6 +325,7 @@ __device__ __forceinline__ void pair_kick2(const float f[D], float rc2, float si
             }
         }
     }
+    return hit;
 }
 
 }  // namespace rh
