@@ -130,6 +130,7 @@ SIGNATURES = {
     "rh_engine2_kicks": ([P, P, P], i32),
     "rh_engine2_read_kinetic_total": ([P, P], i32),
     "rh_engine2_list_stats": ([P, P], i32),
+    "rh_engine2_set_arith": ([P, i32], i32),
     "rh_engine2_tune": ([P, i32, u32], i32),
     "rh_engine2_stream": ([P, P], i32),
     "rh_batch_create": ([u64, u32, P], i32),

@@ -271,6 +271,9 @@ int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke) {
 int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out) {
     return guarded([&] { *out = eng(e)->list_stats(); });
 }
+int rh_engine2_set_arith(rh_engine2* e, int arith) {
+    return guarded([&] { eng(e)->set_arith(arith); });
+}
 int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block) {
     return guarded([&] { eng(e)->tune(variant, block); });
 }

@@ -292,6 +292,32 @@ double kick_coeff(double dt, double mass, bool pinned) { return pinned ? 0.0 : d
 int32_t to_step_displacement(double dt, double v) { return sat_i32(std::round(v * dt * ONE_F)); }
 int32_t gravity_kick(double dt, double g) { return sat_i32(std::round(g * dt * dt * ONE_F)); }
 
+IntTables int_tables(const rh_pair* pairs, uint32_t ns, const double* masses, uint32_t pinned, double dt) {
+    IntTables t;
+    t.ns = ns;
+    const double rcm = rc_max(pairs, ns);
+    const double r2_scale = (double)(uint64_t(1) << TAB_R2_BITS), step = (double)(uint64_t(1) << TAB_STEP_BITS);
+    t.len = (sat_u64(rcm * rcm * r2_scale) >> TAB_STEP_BITS) + 2;
+    t.data.assign((size_t)ns * ns * t.len, 0);
+    for (uint32_t a = 0; a < ns; a++) {
+        const double c = kick_coeff(dt, masses[a], (pinned >> a) & 1u);
+        for (uint32_t b = 0; b < ns; b++) {
+            const rh_pair& p = pairs[a * 4 + b];
+            t.rc2[a * ns + b] = sat_u32(p.rc * p.rc * r2_scale);
+            rh_pair continued = p;
+            continued.rc = INFINITY;  // the force continued past the cutoff, which applies separately
+            for (uint64_t k = 0; k < t.len; k++) {
+                const double r2 = std::fmax((double)(k * (uint64_t(1) << TAB_STEP_BITS)) / r2_scale, 0.36);
+                double f = 0.0;
+                if (r2 < p.rc * p.rc + 2.0 * step / r2_scale) f = pair_f_over_r(continued, r2) * c * (double)(1u << TAB_T_BITS);
+                const double r = std::fmin(std::fmax(std::round(f), -2147483648.0), 2147483647.0);
+                t.data[((size_t)a * ns + b) * t.len + k] = (int32_t)r;
+            }
+        }
+    }
+    return t;
+}
+
 }  // namespace rh::core
 
 namespace rh {

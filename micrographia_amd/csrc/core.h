@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "../../include/rahman_hip.h"
 
@@ -44,5 +45,18 @@ double rc_max(const rh_pair* pairs, uint32_t ns);
 double kick_coeff(double dt, double mass, bool pinned);
 int32_t to_step_displacement(double dt, double v);
 int32_t gravity_kick(double dt, double g);
+
+// md_core::table::IntTables (table.rs:17-73): all-integer pair forces for the 2D engine.  Entry
+// k of species pair (a, b) is round(F/r(r2) c_a 2^T_BITS) at r2 = k 2^STEP_BITS / 2^R2_BITS
+// (clamped below r2 = 0.36), continued one step past the cutoff; rc2 in 2^-R2_BITS sigma^2.
+constexpr int TAB_SEP_BITS = 20, TAB_R2_BITS = 24, TAB_T_BITS = 2, TAB_STEP_BITS = 12;
+constexpr int TAB_KICK_SHIFT = TAB_T_BITS + TAB_SEP_BITS;
+struct IntTables {
+    uint32_t ns = 0;
+    uint64_t len = 0;
+    std::vector<int32_t> data;  // [(a * ns + b) * len + k]
+    uint32_t rc2[16] = {};      // [a * ns + b]
+};
+IntTables int_tables(const rh_pair* pairs, uint32_t ns, const double* masses, uint32_t pinned, double dt);
 
 }  // namespace rh::core

@@ -35,6 +35,8 @@ struct PairTable {
     float rc2[16], eps4[16], six_eps4[16], sigma2[16], shift[16], list_r2[16];
     float kick_c[4];
     float r2_fast[16];  // pairs with r2 >= this use the shared-reciprocal division (0: never)
+    // f64 constants for the energy readout of the all-integer forces (Pair::energy in f64)
+    double eps4_64[16], sigma2_64[16], rc2_64[16], shift_64[16];
 };
 
 template <int D>
@@ -70,6 +72,9 @@ struct Params {
     double wall_sd[6][4];    // sqrt(T / m) per heat-bath wall and species
     double dt;
     double mass[4];
+    // All-integer pair forces from tables (md::Arith::Int, 2D only)
+    uint32_t arith;          // 0: f32 pair forces, 1: integer tables
+    uint32_t itab_len;       // entries per species pair
     PairTable pt;
 };
 
@@ -93,6 +98,8 @@ struct Bufs {
     uint32_t* ctl;
     const PairTable* pt;
     unsigned long long* ledger;  // [7][3]: energy, impulse (2^-32 units), hits per wall; 6 = bath
+    const int32_t* itab;         // integer force tables [(a * ns + b) * len + k] (arith 1)
+    const uint32_t* irc2;        // their cutoffs [a * ns + b], 2^-24 sigma^2
 };
 
 // The smallest r2 of a species pair from which the shared-reciprocal division of the pair kick
@@ -138,6 +145,9 @@ class Engine {
     void set_species_velocity(uint32_t species, const double v[3]);
     rh_list_stats list_stats();
     void read_ledger(rh_ledger3* out);
+    // Pair-force arithmetic (md::Arith, engine.rs:92-100): 0 f32 pair forces, 1 all-integer
+    // tables (2D, cell-list mode).
+    void set_arith(int arith);
     hipStream_t stream() const { return stream_; }
     void set_timing(bool on) { timing_ = on; }
     void kernel_ms(double* force_ms, uint64_t* force_launches, double* rebuild_ms, uint64_t* rebuilds) const;
@@ -201,6 +211,7 @@ class Engine {
     uint32_t block_ = 512;
     std::vector<void*> allocs_;
     void* dalloc(size_t bytes);
+    void upload_int_tables();
 };
 
 }  // namespace rh

@@ -678,6 +678,44 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
 // Cell-list forces (ForceMode::CellList, and kicks()/reverse()/energy readouts)
 // ---------------------------------------------------------------------------------------------
 
+// md_core::table::IntTables::kick (table.rs:75-105) on an exact separation d = r_j - r_i (2^-32
+// sigma units, held in f64): magnitudes rounded to 2^-20 sigma, r2 to 2^-24 sigma^2, the table
+// interpolated linearly in integers, kick = -T q >> (T_BITS + SEP_BITS) on magnitudes, then signed.
+template <int D>
+__device__ __forceinline__ bool int_kick2(const Params<D>& P, const Bufs<D>& B, uint32_t sa, uint32_t sb, double dx, double dy, int32_t out[2]) {
+    const double limit = 34359738368.0;  // 8 sigma
+    if (!(dx >= -limit && dx < limit && dy >= -limit && dy < limit)) return false;
+    // (|d| + 2^11) >> 12 = floor(|d| / 2^12 + 1/2): exact in f64 for |d| < 2^35
+    const int64_t mx = (int64_t)__builtin_floor(__builtin_fabs(dx) * 0x1p-12 + 0.5);
+    const int64_t my = (int64_t)__builtin_floor(__builtin_fabs(dy) * 0x1p-12 + 0.5);
+    const int64_t q[2] = {dx < 0.0 ? -mx : mx, dy < 0.0 ? -my : my};
+    const uint32_t r2 = (uint32_t)(((uint64_t)(q[0] * q[0] + q[1] * q[1]) + (uint64_t(1) << 15)) >> 16);
+    const uint32_t pair = sa * P.nspecies + sb;
+    if (r2 >= B.irc2[pair]) return false;
+    const uint32_t idx = r2 >> 12;
+    const int64_t frac = (int64_t)(r2 & 4095u);
+    const int32_t* tab = B.itab + (size_t)pair * P.itab_len;
+    const int64_t t0 = tab[idx], t1 = tab[idx + 1];
+    const int64_t t = t0 + (((t1 - t0) * frac) >> 12);
+    const uint64_t at = t < 0 ? (uint64_t)0 - (uint64_t)t : (uint64_t)t;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        const uint64_t aq = q[a] < 0 ? (uint64_t)0 - (uint64_t)q[a] : (uint64_t)q[a];
+        const int64_t m = (int64_t)((at * aq) >> 22);  // |t| < 2^31, |q| < 2^23: fits 64 bits
+        const bool neg = (t < 0) != (q[a] < 0);
+        out[a] = (int32_t)(uint32_t)(uint64_t)(neg ? m : -m);
+    }
+    return true;
+}
+
+// Pair::energy in f64 (pair.rs:36-43), for the energy readout of the integer forces.
+__device__ __forceinline__ double pair_energy_f64(const PairTable* pt, uint32_t p, double r2) {
+    if (r2 >= pt->rc2_64[p]) return 0.0;
+    const double s2 = pt->sigma2_64[p] / r2;
+    const double s6 = s2 * s2 * s2;
+    return pt->eps4_64[p] * s6 * (s6 - 1.0) - pt->shift_64[p];
+}
+
 // items: slot of each sorted position (null: slots are already sorted, items[q] = q).
 template <int D, bool MULTI>
 __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, const uint32_t* items, const uint32_t* gate) {
@@ -706,6 +744,20 @@ __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, con
         for (uint32_t q = s; q < e; q++) {
             uint32_t j = items ? items[q] : q;
             if (j == i) continue;
+            if constexpr (D == 2) {
+                if (P.arith) {
+                    const uint32_t sj = MULTI ? sp[j] : 0u;
+                    const double dx = separation<D>(P, xi[0], px[0][j], 0), dy = separation<D>(P, xi[1], px[1][j], 1);
+                    int32_t kk[2];
+                    if (int_kick2<D>(P, B, si, sj, dx, dy, kk)) {
+                        k[0] += (uint32_t)kk[0];
+                        k[1] += (uint32_t)kk[1];
+                        const double fx = dx * 0x1p-32, fy = dy * 0x1p-32;
+                        pe += pair_energy_f64(B.pt, si * 4 + sj, fx * fx + fy * fy);
+                    }
+                    continue;
+                }
+            }
             float f[D];
             bool ok = true;
 #pragma unroll
@@ -925,6 +977,10 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
             double r = (p.rc + skin) * (1.0 + 1e-4);
             P.pt.list_r2[k] = (float)(r * r);
             P.pt.r2_fast[k] = fast_division_floor(P.pt.rc2[k], P.pt.sigma2[k], P.pt.six_eps4[k]);
+            P.pt.eps4_64[k] = 4.0 * p.eps;
+            P.pt.sigma2_64[k] = p.sigma * p.sigma;
+            P.pt.rc2_64[k] = p.rc * p.rc;
+            P.pt.shift_64[k] = core::pair_shift(p);
         }
     for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
     P.cap = mode.kind == 1 ? mode.cap : 0;
@@ -1433,8 +1489,10 @@ void Engine<D>::update_system(const SystemDesc& sys) {
         P_.pt.list_r2[k] = std::max(P_.pt.list_r2[k], old.pt.list_r2[k]);
     }
     if (grew) write_ctl(CTL_STALE, 1u);
-    upload_tables();
     sys_ = sys;
+    P_.arith = old.arith;
+    if (P_.arith) upload_int_tables();  // the tables follow the new pairs, masses and dt
+    upload_tables();
 }
 
 template <int D>
@@ -1476,6 +1534,34 @@ void Engine<D>::set_species_velocity(uint32_t species, const double v[3]) {
     k_set_species_velocity<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, species, a[0], a[1], a[2]);
     RH_HIP(hipGetLastError());
     RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+void Engine<D>::upload_int_tables() {
+    const core::IntTables t = core::int_tables(sys_.pairs, sys_.nspecies, sys_.masses, sys_.pinned, sys_.dt);
+    RH_HIP(hipStreamSynchronize(stream_));
+    int32_t* tab = (int32_t*)dalloc(t.data.size() * sizeof(int32_t));
+    uint32_t* rc2 = (uint32_t*)dalloc(sizeof t.rc2);
+    RH_HIP(hipMemcpy(tab, t.data.data(), t.data.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    RH_HIP(hipMemcpy(rc2, t.rc2, sizeof t.rc2, hipMemcpyHostToDevice));
+    B_.itab = tab;
+    B_.irc2 = rc2;
+    P_.itab_len = (uint32_t)t.len;
+}
+
+// md::Arith (engine.rs:92-100, with_options :609-625): Int replaces the f32 pair forces by the
+// all-integer tables, which give bit-identical trajectories on every device.  Pair forces only.
+template <int D>
+void Engine<D>::set_arith(int arith) {
+    RH_REQUIRE(arith == 0 || arith == 1, "arith must be 0 (f32 pair forces) or 1 (integer tables)");
+    if (arith == 1) {
+        RH_REQUIRE(D == 2, "integer force tables are a 2D feature");
+        RH_REQUIRE(mode_.kind == 0, "integer forces run in cell-list mode");
+        RH_REQUIRE(P_.soft_walls == 0, "integer forces cover pair forces only");
+        upload_int_tables();
+    }
+    RH_HIP(hipStreamSynchronize(stream_));
+    P_.arith = (uint32_t)arith;
 }
 
 // Ledger3 (engine3.rs:199-205): energy and impulse exchanged with each wall (2 axis + side)

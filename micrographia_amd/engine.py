@@ -240,6 +240,11 @@ class GpuEngine:
     def tune(self, variant: int, block: int = 512) -> None:
         check(load().rh_engine2_tune(self._h, int(variant), int(block)))
 
+    def set_arith(self, arith: int) -> None:
+        """md::Arith: 0 f32 pair forces, 1 all-integer tables (cell-list mode; bit-identical on
+        every device, md_ref::IntEngine's integer forces)."""
+        check(load().rh_engine2_set_arith(self._h, int(arith)))
+
     def stream(self) -> int:
         v = C.c_void_p()
         check(load().rh_engine2_stream(self._h, C.byref(v)))

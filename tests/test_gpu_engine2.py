@@ -93,3 +93,35 @@ def test_walls_gravity_mixture_match_the_twin():
     gpu.step(400)
     gpu.reverse()
     assert gpu.read_state() == s0
+
+
+# ---- all-integer pair forces (md::Arith::Int; md_core::table::IntTables) -----------------------
+
+def golden_setup():
+    """crates/md/src/checks.rs:156-164 (golden_int_setup), built with the product's md-core."""
+    box = rh.box_for(3000, 0.6, 1.0)
+    sys = rh.System.new(box, 0.005, rh.Pair.lj(2.5), [1.0, 2.0]).with_pair(0, 1, rh.Pair(0.7, 1.05, 2.5))
+    st = rh.fluid(sys, 3000, 1.0, 2026)
+    st.species[::4] = 1
+    st.vel = rh.maxwell2(sys, st.species, 1.0, 2026)
+    return sys, st
+
+
+def test_integer_kicks_match_the_twin():
+    sys, st = golden_setup()
+    gpu = rh.GpuEngine(sys, st, rh.ForceMode.cell_list())
+    gpu.set_arith(1)
+    k, pe = gpu.kicks()
+    want, pe_want = O.kicks2(oracle_system(sys), st.pos, st.species, prec=0)
+    assert np.array_equal(k, want)
+    assert abs(pe - pe_want) <= 1e-9 * abs(pe_want)
+
+
+def test_golden_integer_hash_on_the_gpu():
+    """The reference's one stored trajectory value: GOLDEN_INT_HASH after 1000 all-integer
+    steps of golden_int_setup() (crates/md/src/checks.rs:156-167, crates/md/tests/integer.rs:84-98)."""
+    sys, st = golden_setup()
+    gpu = rh.GpuEngine(sys, st, rh.ForceMode.cell_list())
+    gpu.set_arith(1)
+    gpu.step(1000)
+    assert gpu.read_state().hash() == 0x3AAEFF680A2F0278
