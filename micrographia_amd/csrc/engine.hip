@@ -561,27 +561,29 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     const uint32_t* ctl = B.ctl;
     if (!step_runs(ctl)) return;
     __shared__ uint32_t acc[D][BS];
+    // The block's own positions: in-block neighbours are read from LDS, not gathered from memory.
+    __shared__ double lpos[D][BS];
     const uint32_t t = threadIdx.x;
     const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
     const uint32_t i = base + t;
-#pragma unroll
-    for (int a = 0; a < D; a++) acc[a][t] = 0u;
-    __syncthreads();
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
-    const uint32_t* sp = B.species[pm];
-    const PairTable* __restrict__ pt = B.pt;
     const bool live = i < P.n;
     double xi[D];
     uint32_t k[D];
-    uint32_t si = 0;
-    float c = P.kick_c[0];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        xi[a] = live ? px[a][i] : 0;
+        xi[a] = live ? px[a][i] : 0.0;
         k[a] = 0u;
+        acc[a][t] = 0u;
+        lpos[a][t] = xi[a];
     }
+    __syncthreads();
+    const uint32_t* sp = B.species[pm];
+    const PairTable* __restrict__ pt = B.pt;
+    uint32_t si = 0;
+    float c = P.kick_c[0];
     if (live) {
         if (MULTI) {
             si = sp[i];
@@ -612,10 +614,17 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);  // entries: min(m, 4)
             const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
             double xs[4][D];
+            if (inblock) {
 #pragma unroll
-            for (int e = 0; e < 4; e++)
+                for (int e = 0; e < 4; e++)
 #pragma unroll
-                for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+                    for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+#pragma unroll
+                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+            }
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 if ((uint32_t)e >= m) break;
