@@ -21,6 +21,7 @@ struct BSys {
     uint32_t periodic, nspecies, pinned, soft_walls, range_check, cap;
     uint64_t nbr0;  // list entry k of local slot i at nbr0 + k * nslots + i
     int64_t box[3], half[3];
+    double boxf[3], inv_box[3];  // positions are exact f64 integers (2^-32 sigma units)
     double cell_scale[3], cell_w64[3];
     float cell_w[3];
     float kick_c[4];
@@ -36,11 +37,11 @@ struct BCtl {
 };
 
 struct BBufs {
-    int64_t* pos[2][3];
+    double* pos[2][3];  // exact integers below 2^53 (2^-32 sigma units)
     int32_t* vel[2][3];
     uint32_t* species[2];
     uint32_t* ids[2];
-    int64_t* ref[3];
+    double* ref[3];
     float4* rel;  // position relative to its cell's corner (sigma, f32), for list builds
     uint32_t* key;
     uint32_t* rank;

@@ -46,7 +46,7 @@ __global__ void k_bb_decide(const BSys* S, BCtl* C, uint32_t nsys) {
 
 template <int D>
 __device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C, uint32_t slot) {
-    int64_t x[D];
+    double x[D];
     for (int a = 0; a < D; a++) x[a] = B.pos[C.ppos][a][slot];
     uint32_t c[D];
     cell_of<D>(S, x, c);
@@ -132,11 +132,11 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
     float r[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        int64_t x = B.pos[pp][a][slot];
+        const double x = B.pos[pp][a][slot];
         B.pos[pp ^ 1][a][ns] = x;
         B.ref[a][ns] = x;
         B.vel[pm ^ 1][a][ns] = B.vel[pm][a][slot];
-        r[a] = (float)((double)x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
+        r[a] = (float)(x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
     }
     B.rel[ns] = make_float4(r[0], r[1], r[2], 0.f);
     B.species[pm ^ 1][ns] = B.species[pm][slot];
@@ -168,7 +168,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     const uint32_t* sp = B.species[pm] + S.slot0;
     const float4 me = B.rel[slot];
     const float ri[3] = {me.x, me.y, me.z};
-    int64_t xi[D];
+    double xi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) xi[a] = B.pos[C.ppos ^ 1u][a][slot];
     const uint32_t si = MULTI ? sp[local] : 0u;
@@ -305,28 +305,29 @@ __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S
 template <int D, bool MULTI>
 __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t wg) {
     __shared__ uint32_t acc[D][BATCH_WG];
+    __shared__ double lpos[D][BATCH_WG];  // the block's positions, for in-block neighbours
     const uint32_t t = threadIdx.x;
-#pragma unroll
-    for (int a = 0; a < D; a++) acc[a][t] = 0u;
-    __syncthreads();
     const uint32_t slot = wg * BATCH_WG + t;
     const uint32_t local = slot - S.slot0;
     const uint32_t base = wg * BATCH_WG - S.slot0;  // first local slot of this block
     const bool live = local < S.n;
     const uint32_t pp = C.ppos, pm = C.pmeta;
-    const int64_t* px[D];
+    const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a] + S.slot0;
-    const uint32_t* sp = B.species[pm] + S.slot0;
-    const PairTable& pt = S.pt;
-    int64_t xi[D];
+    double xi[D];
     uint32_t k[D];
-    uint32_t si = 0;
-    float c = S.kick_c[0];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        xi[a] = live ? px[a][local] : 0;
+        xi[a] = live ? px[a][local] : 0.0;
         k[a] = 0u;
+        acc[a][t] = 0u;
+        lpos[a][t] = xi[a];
     }
+    __syncthreads();
+    const uint32_t* sp = B.species[pm] + S.slot0;
+    const PairTable& pt = S.pt;
+    uint32_t si = 0;
+    float c = S.kick_c[0];
     if (live) {
         if (MULTI) {
             si = sp[local];
@@ -339,90 +340,78 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         const size_t stride = S.nslots;
         const uint32_t cap4 = S.cap / 4u;
         const bool rc = S.range_check != 0;
-        // in-block pairs, both ends: chunks 0, 1, ... (next chunk in flight, four neighbours'
-        // positions loaded together)
-        {
-            const uint32_t nch = (na + 3u) / 4u;
-            uint4 q_next = nch > 0 ? rowc[0] : make_uint4(local, local, local, local);
-            for (uint32_t ch = 0; ch < nch; ch++) {
-                const uint4 q = q_next;
-                if (ch + 1 < nch) q_next = rowc[(size_t)(ch + 1) * stride];
-                const uint32_t m = na - 4u * ch;
-                const uint32_t js[4] = {q.x, m > 1u ? q.y : local, m > 2u ? q.z : local, m > 3u ? q.w : local};
-                int64_t xs[4][D];
+        // As the single-system engine's step: one loop over the in-block chunks (both ends'
+        // kicks, the partner's into LDS; positions from the block's LDS copy) and then the other
+        // chunks (this end only), next chunk in flight, four neighbours' positions together.
+        const uint32_t ca = (na + 3u) / 4u, nch = ca + (nb + 3u) / 4u;
+        auto chunk_at = [&](uint32_t ch) -> uint4 {
+            return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
+        };
+        uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(local, local, local, local);
+        for (uint32_t ch = 0; ch < nch; ch++) {
+            const uint4 q = q_next;
+            if (ch + 1 < nch) q_next = chunk_at(ch + 1);
+            const bool inblock = ch < ca;
+            const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);
+            const uint32_t js[4] = {q.x, m > 1u ? q.y : local, m > 2u ? q.z : local, m > 3u ? q.w : local};
+            double xs[4][D];
+            if (inblock) {
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+#pragma unroll
+                    for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
+            } else {
 #pragma unroll
                 for (int e = 0; e < 4; e++)
 #pragma unroll
                     for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+            }
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if ((uint32_t)e >= m) break;
-                    const uint32_t j = js[e];
-                    float f[D];
+            for (int e = 0; e < 4; e++) {
+                if ((uint32_t)e >= m) break;
+                const uint32_t j = js[e];
+                float f[D];
+                double d[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    d[a] = separation<D>(S, xi[a], xs[e][a], a);
+                    f[a] = sep_f_in_range(d[a]);
+                }
+                if (rc) {
                     bool ok = true;
 #pragma unroll
-                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xs[e][a], a), f[a]);
-                    if (rc && !ok) continue;
-                    int32_t kj[D];
-                    if constexpr (MULTI) {
-                        uint32_t sj = sp[j];
-                        uint32_t p = si * 4 + sj;
-                        pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
-                    } else {
-                        pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                    }
+                    for (int a = 0; a < D; a++) ok &= sep_in_range(d[a]);
+                    if (!ok) continue;
+                }
+                int32_t kj[D];
+                bool hit;
+                if constexpr (MULTI) {
+                    uint32_t sj = sp[j];
+                    uint32_t p = si * 4 + sj;
+                    hit = pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
+                } else {
+                    hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                }
+                if (inblock && hit) {
                     const uint32_t lj = j - base;
 #pragma unroll
-                    for (int a = 0; a < D; a++)
-                        if (kj[a]) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+                    for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
                 }
             }
         }
-        // the other pairs, this end only: chunks cap/4 - 1, cap/4 - 2, ...
-        {
-            const uint32_t nch = (nb + 3u) / 4u;
-            const uint4* rowb = rowc + (size_t)(cap4 - 1u) * stride;
-            uint4 q_next = nch > 0 ? rowb[0] : make_uint4(local, local, local, local);
-            for (uint32_t ch = 0; ch < nch; ch++) {
-                const uint4 q = q_next;
-                if (ch + 1 < nch) q_next = rowb[-(ptrdiff_t)((size_t)(ch + 1) * stride)];
-                const uint32_t m = nb - 4u * ch;
-                const uint32_t js[4] = {q.x, m > 1u ? q.y : local, m > 2u ? q.z : local, m > 3u ? q.w : local};
-                int64_t xs[4][D];
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-#pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if ((uint32_t)e >= m) break;
-                    float f[D];
-                    bool ok = true;
-#pragma unroll
-                    for (int a = 0; a < D; a++) ok &= sep_f(separation<D>(S, xi[a], xs[e][a], a), f[a]);
-                    if (rc && !ok) continue;
-                    if constexpr (MULTI) {
-                        uint32_t p = si * 4 + sp[js[e]];
-                        pair_kick<D>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, k);
-                    } else {
-                        pair_kick<D>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, k);
-                    }
-                }
-            }
-        }
-        if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
     }
     __syncthreads();
     if (!live) return;
+    if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
 #pragma unroll
     for (int a = 0; a < D; a++) k[a] += acc[a][t];
     const bool pin = (S.pinned >> si) & 1u;
     int32_t v[D];
-    int64_t x[D];
+    double x[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
         v[a] = (int32_t)((uint32_t)B.vel[pm][a][slot] + k[a] + (pin ? 0u : (uint32_t)S.gravity[a]));
-        x[a] = xi[a] + (int64_t)v[a];
+        x[a] = xi[a] + (double)v[a];
     }
     bound<D>(S, x, v);
     float r2 = 0.0f;
@@ -559,13 +548,13 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     std::memset(&B_, 0, sizeof B_);
     for (int p = 0; p < 2; p++) {
         for (int a = 0; a < 3; a++) {
-            B_.pos[p][a] = (int64_t*)dalloc(slot_cap * 8);
+            B_.pos[p][a] = (double*)dalloc(slot_cap * 8);
             B_.vel[p][a] = (int32_t*)dalloc(slot_cap * 4);
         }
         B_.species[p] = (uint32_t*)dalloc(slot_cap * 4);
         B_.ids[p] = (uint32_t*)dalloc(slot_cap * 4);
     }
-    for (int a = 0; a < 3; a++) B_.ref[a] = (int64_t*)dalloc(slot_cap * 8);
+    for (int a = 0; a < 3; a++) B_.ref[a] = (double*)dalloc(slot_cap * 8);
     B_.rel = (float4*)dalloc(slot_cap * 16);
     B_.key = (uint32_t*)dalloc(slot_cap * 4);
     B_.rank = (uint32_t*)dalloc(slot_cap * 4);
@@ -634,8 +623,11 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
             d.nc[a] = 1;
         }
         ncells *= d.nc[a];
+        RH_REQUIRE(sys.box_len[a] < (int64_t(1) << 52), "box longer than 2^20 sigma");
         d.box[a] = sys.box_len[a];
         d.half[a] = sys.box_len[a] / 2;
+        d.boxf[a] = (double)sys.box_len[a];
+        d.inv_box[a] = 1.0 / d.boxf[a];
         d.cell_scale[a] = (double)d.nc[a] / (double)sys.box_len[a];
         d.cell_w64[a] = b[a] / (double)d.nc[a];
         d.cell_w[a] = (float)d.cell_w64[a];
@@ -736,11 +728,11 @@ int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int3
     h_sys_[id] = d;
     // Upload the state into parity 0 of the new range (nothing in flight touches it).
     RH_HIP(hipStreamSynchronize(stream_));
-    std::vector<int64_t> px(n);
+    std::vector<double> px(n);
     std::vector<int32_t> vx(n);
     std::vector<uint32_t> ids(n);
     for (int a = 0; a < D; a++) {
-        for (uint64_t i = 0; i < n; i++) px[i] = pos[D * i + a];
+        for (uint64_t i = 0; i < n; i++) px[i] = (double)pos[D * i + a];  // exact: |x| < 2^52
         RH_HIP(hipMemcpy(B_.pos[0][a] + d.slot0, px.data(), n * 8, hipMemcpyHostToDevice));
         for (uint64_t i = 0; i < n; i++) vx[i] = vel[D * i + a];
         RH_HIP(hipMemcpy(B_.vel[0][a] + d.slot0, vx.data(), n * 4, hipMemcpyHostToDevice));
@@ -898,14 +890,14 @@ void Batch::read_state(int id, int64_t* pos, int32_t* vel, uint32_t* species) {
     const int D = m.sys.dim;
     const uint64_t n = m.n;
     std::vector<uint32_t> ids(n), sp(n);
-    std::vector<int64_t> px(n);
+    std::vector<double> px(n);
     std::vector<int32_t> vx(n);
     RH_HIP(hipMemcpy(ids.data(), B_.ids[c.pmeta] + m.slot0, n * 4, hipMemcpyDeviceToHost));
     for (int a = 0; a < D; a++) {
         RH_HIP(hipMemcpy(px.data(), B_.pos[c.ppos][a] + m.slot0, n * 8, hipMemcpyDeviceToHost));
         RH_HIP(hipMemcpy(vx.data(), B_.vel[c.pmeta][a] + m.slot0, n * 4, hipMemcpyDeviceToHost));
         for (uint64_t s = 0; s < n; s++) {
-            if (pos) pos[D * (uint64_t)ids[s] + a] = px[s];
+            if (pos) pos[D * (uint64_t)ids[s] + a] = (int64_t)px[s];
             if (vel) vel[D * (uint64_t)ids[s] + a] = vx[s];
         }
     }
