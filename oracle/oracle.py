@@ -57,6 +57,52 @@ class System2(C.Structure):
         ("walled", C.c_uint32 * 2),
         ("gravity", C.c_double * 2),
         ("seed", C.c_uint64),
+        ("wall_kind", (C.c_uint32 * 2) * 2),
+        ("wall_t", (C.c_double * 2) * 2),
+        ("compartments", C.c_uint32 * 2),
+        ("diathermal", C.c_uint32),
+        ("has_bath", C.c_uint32),
+        ("bath_species", C.c_uint32),
+        ("bath_axis", C.c_uint32),
+        ("bath_t", C.c_double),
+        ("bath_rate", C.c_double),
+        ("bath_drift", C.c_double),
+        ("bath_region", C.c_int64 * 2),
+        ("has_membrane", C.c_uint32),
+        ("membrane_axis", C.c_uint32),
+        ("membrane_blocks", C.c_uint32),
+        ("pore_shift", C.c_uint32),
+        ("membrane_position", C.c_int64),
+        ("pore_width", C.c_int64),
+        ("has_piston", C.c_uint32),
+        ("piston_free", C.c_uint32),
+        ("piston_speed", C.c_double),
+        ("piston_mass", C.c_double),
+        ("piston_force", C.c_double),
+        ("piston_start", C.c_double),
+        ("piston_min", C.c_double),
+        ("piston_max", C.c_double),
+        ("nbond_types", C.c_uint32),
+        ("bond_fene", C.c_uint32 * 4),
+        ("bond_k", C.c_double * 4),
+        ("bond_r0", C.c_double * 4),
+        ("nbonds", C.c_uint64),
+        ("bonds", C.c_void_p),
+    ]
+
+
+class State2(C.Structure):
+    """IntEngine's run state (int_engine.rs:23-48): clock, piston, ledgers."""
+
+    _fields_ = [
+        ("clock", C.c_uint32),
+        ("piston_w", C.c_int64),
+        ("piston_v", C.c_int32),
+        ("piston_work", C.c_double),
+        ("ledger", Ledger3),
+        ("comp_energy", (C.c_double * 256) * 7),
+        ("comp_impulse", (C.c_double * 256) * 7),
+        ("comp_hits", (C.c_uint64 * 256) * 7),
     ]
 
 
@@ -128,6 +174,11 @@ def lib():
         L.orc2_kicks.argtypes = [C.c_int, P, u64, P, P, P, P]
         L.orc2_kicks_brute.argtypes = [C.c_int, P, u64, P, P, P]
         L.orc2_step.argtypes = [C.c_int, P, u64, P, P, P, u64]
+        L.orc2_state_init.argtypes = [P, P]
+        L.orc2_state_init.restype = None
+        L.orc2_run.argtypes = [C.c_int, P, P, u64, P, P, P, u64]
+        L.orc2_reverse_run.argtypes = [C.c_int, P, P, u64, P, P, P]
+        L.orc2_forces.argtypes = [C.c_int, P, P, u64, P, P, P, P, P, P]
         L.orc2_reverse.argtypes = [C.c_int, P, u64, P, P, P]
         L.orc2_energy.argtypes = [C.c_int, P, u64, P, P, P, P, P]
         L.orc2_int_table_len.argtypes = [P]
@@ -378,6 +429,35 @@ def energy2(sys, pos, vel, species, prec=64):
     ke, pe = C.c_double(), C.c_double()
     _check(lib().orc2_energy(prec, C.byref(sys), len(sp), ptr(pos), ptr(vel), ptr(sp), C.byref(ke), C.byref(pe)), "energy2")
     return ke.value, pe.value
+
+
+def state2(sys) -> State2:
+    st = State2()
+    lib().orc2_state_init(C.byref(sys), C.byref(st))
+    return st
+
+
+def run2(sys, st: State2, pos, vel, species, steps, prec=32):
+    """IntEngine::step with the full 2D boundaries; pos, vel in place, st advanced."""
+    sp = np.ascontiguousarray(species, np.uint32)
+    _check(lib().orc2_run(prec, C.byref(sys), C.byref(st), len(sp), ptr(pos), ptr(vel), ptr(sp), steps), "run2")
+
+
+def reverse_run2(sys, st: State2, pos, vel, species, prec=32):
+    sp = np.ascontiguousarray(species, np.uint32)
+    _check(lib().orc2_reverse_run(prec, C.byref(sys), C.byref(st), len(sp), ptr(pos), ptr(vel), ptr(sp)), "reverse_run2")
+
+
+def forces2(sys, st: State2, pos, species, prec=32):
+    """(kicks, pe, wall_force (n, 7), piston_force)"""
+    pos = np.ascontiguousarray(pos, np.int64)
+    sp = np.ascontiguousarray(species, np.uint32)
+    n = len(sp)
+    k = np.zeros((n, 2), np.int32)
+    wf = np.zeros((n, 7), np.float64)
+    pe, pf = C.c_double(), C.c_double()
+    _check(lib().orc2_forces(prec, C.byref(sys), C.byref(st), n, ptr(pos), ptr(sp), ptr(k), C.byref(pe), ptr(wf), C.byref(pf)), "forces2")
+    return k, pe.value, wf, pf.value
 
 
 def golden_setup():

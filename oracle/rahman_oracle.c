@@ -1079,11 +1079,24 @@ int orc2_int_kick(const orc_system2 *sys, const int32_t *data, const uint32_t *r
     return 1;
 }
 
-/* ---- 2D forces: the reference loop of IntEngine::forces (int_engine.rs:130-251) over the
- * stable cell list (crates/md-ref/src/cells.rs), periodic or hard-walled axes.  prec 0 selects
- * the integer tables. ---- */
+/* ---- 2D forces and step: IntEngine (crates/md-ref/src/int_engine.rs:130-400) with the full
+ * md_core::System boundaries (system.rs, walls.rs) over the stable cell list
+ * (crates/md-ref/src/cells.rs).  prec 0 selects the integer tables. ---- */
+#define WALLS2 7
+#define MEMBRANE_LOW 4
+#define MEMBRANE_HIGH 5
+#define BATH2 6
+#define MAX_BONDS 4
+#define PISTON_FORCE_SCALE 65536.0
+
 static int64_t separation2(const orc_system2 *s, int64_t a, int64_t b, int axis) {
     return s->walled[axis] ? b - a : orc_min_image(a, b, s->box_len[axis]);
+}
+
+/* system.rs:253-257, 229-231 */
+static int64_t compartment_of2(const orc_system2 *s, int64_t p, int a) { return s->compartments[a] ? p >> s->compartments[a] : 0; }
+static int interact_across2(const orc_system2 *s, const int64_t *a, const int64_t *b) {
+    return s->diathermal || (compartment_of2(s, a[0], 0) == compartment_of2(s, b[0], 0) && compartment_of2(s, a[1], 1) == compartment_of2(s, b[1], 1));
 }
 
 /* cells for the species present (int_engine.rs:108-116) */
@@ -1102,6 +1115,15 @@ static void present_cells2(const orc_system2 *s, uint64_t n, const uint32_t *spe
     nc[1] = c[1] > nc[1] ? c[1] : nc[1];
 }
 
+/* fixed::sep_long_f32 (fixed.rs:65-72) */
+static float sep_long_f32(int64_t d) {
+    float f;
+    if (orc_sep_f32(d, &f)) return f;
+    uint64_t m = d < 0 ? (uint64_t)0 - (uint64_t)d : (uint64_t)d;
+    float mag = (float)(uint32_t)(m >> 32) + (float)(uint32_t)m * (1.0f / 4294967296.0f);
+    return d < 0 ? -mag : mag;
+}
+
 #define DEFINE_FORCES2(NAME, F, CONSTS, MKCONSTS, SEP, LIT)                                                  \
     static void NAME(const orc_system2 *s, uint64_t i, const int64_t *pos, const uint32_t *sp, uint64_t j,    \
                      const CONSTS *consts, F cf, int32_t k[2], F *pe_i) {                                      \
@@ -1116,7 +1138,7 @@ static void present_cells2(const orc_system2 *s, uint64_t n, const uint32_t *spe
         F s2 = pc.sigma2 / r2;                                                                                 \
         F s6 = s2 * s2 * s2;                                                                                   \
         F fr = LIT(6.0) * pc.eps4 * s6 * (LIT(2.0) * s6 - one) / r2;                                           \
-        F u = pc.eps4 * s6 * (s6 - one) - pc.shift;                                                            \
+        F u = pc.eps4 * s6 * (s6 - one) - pc.shift;                                                           \
         F aa = -fr * cf;                                                                                       \
         k[0] = (int32_t)((uint32_t)k[0] + (uint32_t)sat_i32((double)RINT(aa * fx)));                          \
         k[1] = (int32_t)((uint32_t)k[1] + (uint32_t)sat_i32((double)RINT(aa * fy)));                          \
@@ -1126,7 +1148,113 @@ static void present_cells2(const orc_system2 *s, uint64_t n, const uint32_t *spe
 DEFINE_FORCES2(pair2_f32, float, consts_f, consts_f32, orc_sep_f32, LIT_F)
 DEFINE_FORCES2(pair2_f64, double, consts_d, consts_f64, sep_f64, LIT_D)
 
-static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *pos, const uint32_t *sp, int32_t *kicks, double *pe_out, int brute) {
+/* The per-particle terms after the pairs (int_engine.rs:181-248): bonds (BondConsts::eval,
+ * real.rs:167-196, on sep_long separations), soft walls, and the piston as a soft wall at
+ * height piston_w pushing down.  wf: the particle's force on each wall (outward). */
+#define DEFINE_EXTRAS2(NAME, F, SEP, SEPLONG, LIT)                                                           \
+    static void NAME(const orc_system2 *s, const orc_state2 *st, uint64_t i, const int64_t *pos,             \
+                     const uint32_t *partners, F cf, int32_t k[2], double *pe, double *wf, int64_t *piston_fixed) { \
+        const int64_t *pi = &pos[2 * i];                                                                      \
+        const F one = LIT(1.0), half = LIT(0.5);                                                             \
+        F bond_pe = LIT(0.0);                                                                                 \
+        for (int b = 0; b < MAX_BONDS; b++) {                                                                 \
+            uint32_t j = partners[8 * i + 2 * b];                                                             \
+            if (j == UINT32_MAX) break;                                                                       \
+            uint32_t t = partners[8 * i + 2 * b + 1];                                                         \
+            const int64_t *pj = &pos[2 * (uint64_t)j];                                                        \
+            F fx = SEPLONG(separation2(s, pi[0], pj[0], 0)), fy = SEPLONG(separation2(s, pi[1], pj[1], 1));  \
+            F r2 = fx * fx + fy * fy;                                                                         \
+            F bk = (F)s->bond_k[t], br0 = (F)s->bond_r0[t], fr, u;                                            \
+            if (s->bond_fene[t]) {                                                                            \
+                F x = r2 / (br0 * br0);                                                                       \
+                F cap = (F)0.999999;                                                                          \
+                if (x > cap) x = cap;                                                                         \
+                u = -half * bk * br0 * br0 * (F)log((double)(one - x));                                       \
+                fr = -bk / (one - x);                                                                         \
+            } else if (br0 == LIT(0.0)) {                                                                     \
+                fr = -bk;                                                                                     \
+                u = half * bk * r2;                                                                           \
+            } else {                                                                                          \
+                F r = (F)sqrt((double)r2);                                                                    \
+                F dr = r - br0;                                                                               \
+                fr = -bk * dr / r;                                                                            \
+                u = half * bk * dr * dr;                                                                      \
+            }                                                                                                 \
+            F aa = -fr * cf;                                                                                  \
+            k[0] = (int32_t)((uint32_t)k[0] + (uint32_t)sat_i32((double)RINT(aa * fx)));                     \
+            k[1] = (int32_t)((uint32_t)k[1] + (uint32_t)sat_i32((double)RINT(aa * fy)));                     \
+            bond_pe += u;                                                                                     \
+        }                                                                                                     \
+        *pe += 0.5 * (double)bond_pe;                                                                         \
+        for (int a = 0; a < 2; a++) {                                                                         \
+            if (!s->walled[a]) continue;                                                                      \
+            for (int side = 0; side < 2; side++) {                                                            \
+                if (s->wall_kind[a][side] != ORC_WALL_SOFT) continue;                                         \
+                int64_t d = side == 0 ? pi[a] : s->box_len[a] - pi[a];                                        \
+                F df;                                                                                         \
+                if (!SEP(d, &df)) continue;                                                                   \
+                if (df < (F)SOFT_RC) {                                                                        \
+                    F s2 = one / (df * df);                                                                   \
+                    F s6 = s2 * s2 * s2;                                                                      \
+                    F f = LIT(24.0) * s6 * (LIT(2.0) * s6 - one) / df;                                        \
+                    int32_t kw = sat_i32((double)RINT(f * cf));                                               \
+                    k[a] = (int32_t)((uint32_t)k[a] + (side == 0 ? (uint32_t)kw : (uint32_t)0 - (uint32_t)kw)); \
+                    if (wf) wf[WALLS2 * i + 2 * a + side] += (double)f;                                       \
+                    *pe += (double)(LIT(4.0) * s6 * (s6 - one) + one);                                        \
+                }                                                                                             \
+            }                                                                                                 \
+        }                                                                                                     \
+        if (s->has_piston) {                                                                                  \
+            F df;                                                                                             \
+            if (SEP(st->piston_w - pi[1], &df) && df < (F)SOFT_RC) {                                          \
+                F s2 = one / (df * df);                                                                       \
+                F s6 = s2 * s2 * s2;                                                                          \
+                F f = LIT(24.0) * s6 * (LIT(2.0) * s6 - one) / df;                                            \
+                k[1] = (int32_t)((uint32_t)k[1] - (uint32_t)sat_i32((double)RINT(f * cf)));                  \
+                *piston_fixed += (int64_t)round((double)f * PISTON_FORCE_SCALE);                              \
+                *pe += (double)(LIT(4.0) * s6 * (s6 - one) + one);                                            \
+            }                                                                                                 \
+        }                                                                                                     \
+    }
+
+static double sep_long_f64(int64_t d) { return orc_to_f64(d); }
+DEFINE_EXTRAS2(extras2_f32, float, orc_sep_f32, sep_long_f32, LIT_F)
+DEFINE_EXTRAS2(extras2_f64, double, sep_f64, sep_long_f64, LIT_D)
+
+/* System::partner_table (system.rs:312-324): per particle id, up to 4 (partner id, type). */
+static uint32_t *partner_table2(const orc_system2 *s, uint64_t n) {
+    uint32_t *t = malloc((n ? n : 1) * 8 * sizeof(uint32_t));
+    uint8_t *used = calloc(n ? n : 1, 1);
+    if (!t || !used) {
+        free(t);
+        free(used);
+        return NULL;
+    }
+    for (uint64_t k = 0; k < n * 8; k++) t[k] = UINT32_MAX;
+    for (uint64_t b = 0; b < s->nbonds; b++) {
+        const uint32_t *bd = &s->bonds[3 * b];
+        for (int e = 0; e < 2; e++) {
+            uint32_t me = bd[e], other = bd[1 - e];
+            if (me >= n || used[me] >= MAX_BONDS) continue;
+            t[8 * (uint64_t)me + 2 * used[me]] = other;
+            t[8 * (uint64_t)me + 2 * used[me] + 1] = bd[2];
+            used[me]++;
+        }
+    }
+    free(used);
+    return t;
+}
+
+void orc2_state_init(const orc_system2 *s, orc_state2 *st) {
+    memset(st, 0, sizeof *st);
+    if (s->has_piston) {
+        st->piston_w = orc_to_fixed(s->piston_start);
+        st->piston_v = s->piston_free ? 0 : to_step2(s, s->piston_speed);
+    }
+}
+
+static int forces2_full(int prec, const orc_system2 *s, const orc_state2 *st, uint64_t n, const int64_t *pos, const uint32_t *sp, int32_t *kicks, double *pe_out,
+                        double *wall_force, double *piston_force, int brute) {
     if (s->nspecies < 1 || s->nspecies > 4) return -3;
     if (prec != 0 && prec != 32 && prec != 64) return -4;
     consts_f cf32[16];
@@ -1144,6 +1272,7 @@ static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *po
         if (!tab) return -2;
         orc2_int_tables(s, tab, rc2);
     }
+    uint32_t *partners = partner_table2(s, n);
     uint32_t nc[2];
     present_cells2(s, n, sp, nc);
     if (nc[0] < 3 || nc[1] < 3) {
@@ -1156,6 +1285,7 @@ static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *po
     uint64_t *fill = malloc((ncell + 1) * sizeof(uint64_t));
     uint64_t *items = malloc((n ? n : 1) * sizeof(uint64_t));
     uint32_t *cell = malloc((n ? n : 1) * 2 * sizeof(uint32_t));
+    if (!partners || !start || !fill || !items || !cell) return -2;
     double b0 = orc_to_f64(s->box_len[0]), b1 = orc_to_f64(s->box_len[1]);
     for (uint64_t i = 0; i < n; i++) {
         cell[2 * i] = cell_of(orc_to_f64(pos[2 * i]), b0, nc[0]);
@@ -1166,6 +1296,7 @@ static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *po
     memcpy(fill, start, (ncell + 1) * sizeof(uint64_t));
     for (uint64_t i = 0; i < n; i++) items[fill[(uint64_t)cell[2 * i + 1] * nc[0] + cell[2 * i]]++] = i;
     double pe = 0.0;
+    int64_t piston_fixed = 0;
     for (uint64_t i = 0; i < n; i++) {
         int32_t k[2] = {0, 0};
         float pe32 = 0.0f;
@@ -1186,6 +1317,7 @@ static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *po
             for (uint64_t q = q0; q < q1; q++) {
                 uint64_t j = brute ? q : items[q];
                 if (j == i) continue;
+                if (!interact_across2(s, &pos[2 * i], &pos[2 * j])) continue;
                 if (prec == 32) {
                     pair2_f32(s, i, pos, sp, j, cf32, c32, k, &pe32);
                 } else if (prec == 64) {
@@ -1202,65 +1334,236 @@ static int forces2(int prec, const orc_system2 *s, uint64_t n, const int64_t *po
                 }
             }
         }
+        double extra = 0.0;
+        if (prec == 32) extras2_f32(s, st, i, pos, partners, c32, k, &extra, wall_force, &piston_fixed);
+        else extras2_f64(s, st, i, pos, partners, c64, k, &extra, wall_force, &piston_fixed);
         kicks[2 * i] = k[0];
         kicks[2 * i + 1] = k[1];
-        pe += 0.5 * (prec == 32 ? (double)pe32 : pe64);
+        pe += 0.5 * (prec == 32 ? (double)pe32 : pe64) + extra;
     }
     if (pe_out) *pe_out = pe;
+    if (piston_force) *piston_force = (double)piston_fixed / PISTON_FORCE_SCALE;
     free(start);
     free(fill);
     free(items);
     free(cell);
     free(tab);
+    free(partners);
     return 0;
 }
 
+int orc2_forces(int prec, const orc_system2 *sys, const orc_state2 *st, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks, double *pe, double *wall_force, double *piston_force) {
+    return forces2_full(prec, sys, st, n, pos, species, kicks, pe, wall_force, piston_force, 0);
+}
+
 int orc2_kicks(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks, double *pe) {
-    return forces2(prec, sys, n, pos, species, kicks, pe, 0);
-}
-
-int orc2_kicks_brute(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks) {
-    return forces2(prec, sys, n, pos, species, kicks, NULL, 1);
-}
-
-/* IntEngine::step, periodic / hard-wall subset (int_engine.rs:253-344) */
-int orc2_step(int prec, const orc_system2 *s, uint64_t n, int64_t *pos, int32_t *vel, const uint32_t *species, uint64_t steps) {
-    int32_t *kicks = malloc((n ? n : 1) * 2 * sizeof(int32_t));
-    if (!kicks) return -2;
-    int32_t g[2] = {gravity_kick(s->gravity[0], s->dt), gravity_kick(s->gravity[1], s->dt)};
-    int rc = 0;
-    for (uint64_t st = 0; st < steps; st++) {
-        rc = forces2(prec, s, n, pos, species, kicks, NULL, 0);
-        if (rc) break;
-        for (uint64_t i = 0; i < n; i++) {
-            int pin = pinned2(s, species[i]);
-            for (int a = 0; a < 2; a++) {
-                int32_t v = (int32_t)((uint32_t)vel[2 * i + a] + (uint32_t)kicks[2 * i + a] + (uint32_t)(pin ? 0 : g[a]));
-                int64_t x = pos[2 * i + a] + (int64_t)v;
-                int64_t l = s->box_len[a];
-                if (!s->walled[a]) {
-                    x = orc_wrap(x, l);
-                } else if (x < 0) {
-                    x = -1 - x;
-                    v = (int32_t)((uint32_t)0 - (uint32_t)v);
-                } else if (x >= l) {
-                    x = 2 * l - 1 - x;
-                    v = (int32_t)((uint32_t)0 - (uint32_t)v);
-                }
-                vel[2 * i + a] = v;
-                pos[2 * i + a] = x;
-            }
-        }
-    }
-    free(kicks);
+    orc_state2 *st = malloc(sizeof *st);
+    if (!st) return -2;
+    orc2_state_init(sys, st);
+    int rc = forces2_full(prec, sys, st, n, pos, species, kicks, pe, NULL, NULL, 0);
+    free(st);
     return rc;
 }
 
-int orc2_reverse(int prec, const orc_system2 *s, uint64_t n, const int64_t *pos, int32_t *vel, const uint32_t *species) {
+int orc2_kicks_brute(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks) {
+    orc_state2 *st = malloc(sizeof *st);
+    if (!st) return -2;
+    orc2_state_init(sys, st);
+    int rc = forces2_full(prec, sys, st, n, pos, species, kicks, NULL, NULL, NULL, 1);
+    free(st);
+    return rc;
+}
+
+/* walls::thermal_velocity (walls.rs:100-112), 2D */
+static void thermal_velocity2(const orc_system2 *s, uint32_t id, uint32_t clock, double t, double m, double inward, double *vn, double *vt) {
+    uint32_t k0 = (uint32_t)s->seed, k1 = (uint32_t)(s->seed >> 32) ^ STREAM_WALL;
+    uint32_t r[2], r2[2];
+    threefry(id, clock, k0, k1, 20, r);
+    threefry(id, clock, k0, k1 ^ 1u, 20, r2);
+    double u1 = 1.0 - orc_to_unit_f64(r[0], r[1]);
+    double u2 = 1.0 - orc_to_unit_f64(r2[0], 0), u3 = orc_to_unit_f64(r2[1], 0);
+    double sd = sqrt(t / m);
+    *vn = inward * sd * sqrt(-2.0 * log(u1));
+    *vt = sd * sqrt(-2.0 * log(u2)) * cos(TAU * u3);
+}
+
+/* walls::andersen (walls.rs:122-139) with Thermostat::acts_on (system.rs:172-176) */
+static int andersen2(const orc_system2 *s, uint32_t id, uint32_t clock, uint32_t sp, const int64_t *p, double out[2]) {
+    if (!s->has_bath) return 0;
+    if (!((s->bath_species >> sp) & 1u)) return 0;
+    int64_t x = p[s->bath_axis];
+    if (!(x >= s->bath_region[0] && x < s->bath_region[1])) return 0;
+    double m = s->masses[sp];
+    uint32_t k0 = (uint32_t)s->seed, k1 = (uint32_t)(s->seed >> 32) ^ STREAM_THERMO;
+    uint32_t r[2], r2[2];
+    threefry(id, clock, k0, k1, 20, r);
+    if (orc_to_unit_f64(r[0], 0) >= s->bath_rate * s->dt) return 0;
+    threefry(id, clock, k0, k1 ^ 1u, 20, r2);
+    double u1 = 1.0 - orc_to_unit_f64(r[1], 0);
+    double u2 = orc_to_unit_f64(r2[0], 0);
+    double sd = sqrt(s->bath_t / m);
+    double rad = sqrt(-2.0 * log(u1));
+    double a = TAU * u2;
+    out[0] = sd * rad * cos(a);
+    out[1] = sd * rad * sin(a);
+    return 1;
+}
+
+static void book2(orc_state2 *st, int w, uint64_t c, double de, double dp, int hit) {
+    st->ledger.energy[w] += de;
+    st->ledger.impulse[w] += dp;
+    st->ledger.hits[w] += (uint64_t)hit;
+    if (c < ORC_MAX_COMPARTMENTS) {
+        st->comp_energy[w][c] += de;
+        st->comp_impulse[w][c] += dp;
+        st->comp_hits[w][c] += (uint64_t)hit;
+    }
+}
+
+/* walls::piston_kick (walls.rs:143-146) */
+static int32_t piston_kick2(const orc_system2 *s, double net, double mass) { return sat_i32(round(net * s->dt * s->dt / mass * ONE_F)); }
+
+/* IntEngine::step (int_engine.rs:253-344) with move_piston (:346-361) and collide (:364-387) */
+int orc2_run(int prec, const orc_system2 *s, orc_state2 *st, uint64_t n, int64_t *pos, int32_t *vel, const uint32_t *species, uint64_t steps) {
+    int32_t *kicks = malloc((n ? n : 1) * 2 * sizeof(int32_t));
+    double *wf = malloc((n ? n : 1) * WALLS2 * sizeof(double));
+    if (!kicks || !wf) return -2;
+    int32_t g[2] = {gravity_kick(s->gravity[0], s->dt), gravity_kick(s->gravity[1], s->dt)};
+    int rc = 0;
+    for (uint64_t stp = 0; stp < steps; stp++) {
+        memset(wf, 0, n * WALLS2 * sizeof(double));
+        double piston_force = 0.0;
+        rc = forces2_full(prec, s, st, n, pos, species, kicks, NULL, wf, &piston_force, 0);
+        if (rc) break;
+        for (uint64_t i = 0; i < n; i++)
+            for (int w = 0; w < WALLS2; w++)
+                if (wf[WALLS2 * i + w] != 0.0) {
+                    int axis = w / 2;
+                    uint64_t c = (uint64_t)compartment_of2(s, pos[2 * i + (1 - axis)], 1 - axis);
+                    book2(st, w, c, 0.0, wf[WALLS2 * i + w] * s->dt, 0);
+                }
+        for (uint64_t i = 0; i < n; i++) {
+            uint32_t sp = species[i];
+            int pin = pinned2(s, sp);
+            double m = s->masses[sp];
+            int32_t v[2];
+            int64_t x[2] = {pos[2 * i], pos[2 * i + 1]};
+            int64_t home[2] = {compartment_of2(s, x[0], 0), compartment_of2(s, x[1], 1)};
+            for (int a = 0; a < 2; a++) v[a] = (int32_t)((uint32_t)vel[2 * i + a] + (uint32_t)kicks[2 * i + a] + (uint32_t)(pin ? 0 : g[a]));
+            double u[2];
+            if (!pin && andersen2(s, (uint32_t)i, st->clock, sp, x, u)) {
+                double before[2] = {velocity2(s, v[0]), velocity2(s, v[1])};
+                v[0] = to_step2(s, u[0]);
+                v[1] = to_step2(s, u[1]);
+                v[0] = (int32_t)((uint32_t)v[0] + (uint32_t)to_step2(s, s->bath_drift));
+                if (s->has_piston && !s->piston_free) {
+                    double f = orc_to_f64(x[1]) / orc_to_f64(st->piston_w);
+                    f = fmin(fmax(f, 0.0), 1.0);
+                    v[1] = (int32_t)((uint32_t)v[1] + (uint32_t)sat_i32(round((double)st->piston_v * f)));
+                }
+                double after[2] = {velocity2(s, v[0]), velocity2(s, v[1])};
+                double de = 0.5 * m * (after[0] * after[0] + after[1] * after[1] - before[0] * before[0] - before[1] * before[1]);
+                book2(st, BATH2, 0, de, 0.0, 1);
+            }
+            int64_t old[2] = {x[0], x[1]};
+            for (int a = 0; a < 2; a++) x[a] += (int64_t)v[a];
+            if (s->has_membrane) {
+                int a = (int)s->membrane_axis;
+                int64_t along = old[1 - a];
+                int in_pore = s->pore_shift != 0 && (along & (((int64_t)1 << s->pore_shift) - 1)) < s->pore_width;
+                if (((s->membrane_blocks >> sp) & 1u) && !in_pore) {
+                    int64_t w = s->membrane_position;
+                    int lo_old = old[a] < w, lo_new = x[a] < w;
+                    if (lo_old != lo_new) {
+                        int face = lo_old ? MEMBRANE_LOW : MEMBRANE_HIGH;
+                        x[a] = 2 * w - 1 - x[a];
+                        double dp = 2.0 * m * fabs(velocity2(s, v[a]));
+                        v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+                        book2(st, face, (uint64_t)compartment_of2(s, pos[2 * i + (1 - a)], 1 - a), 0.0, dp, 1);
+                    }
+                }
+            }
+            for (int a = 0; a < 2; a++) {
+                int64_t l = s->box_len[a];
+                if (!s->walled[a]) {
+                    x[a] = orc_wrap(x[a], l);
+                    continue;
+                }
+                int side = -1;
+                if (x[a] < 0) {
+                    x[a] = -1 - x[a];
+                    side = 0;
+                } else if (x[a] >= l) {
+                    x[a] = 2 * l - 1 - x[a];
+                    side = 1;
+                }
+                if (side >= 0) {
+                    int w = 2 * a + side;
+                    double before[2] = {velocity2(s, v[0]), velocity2(s, v[1])};
+                    if (s->wall_kind[a][side] == ORC_WALL_THERMAL) {
+                        double vn, vt;
+                        thermal_velocity2(s, (uint32_t)i, st->clock, s->wall_t[a][side], m, side == 0 ? 1.0 : -1.0, &vn, &vt);
+                        v[a] = to_step2(s, vn);
+                        v[1 - a] = to_step2(s, vt);
+                    } else {
+                        v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+                    }
+                    double after[2] = {velocity2(s, v[0]), velocity2(s, v[1])};
+                    double kea = 0.5 * m * (after[0] * after[0] + after[1] * after[1]);
+                    double keb = 0.5 * m * (before[0] * before[0] + before[1] * before[1]);
+                    book2(st, w, (uint64_t)compartment_of2(s, pos[2 * i + (1 - a)], 1 - a), kea - keb, fabs(m * (before[a] - after[a])), 1);
+                } else if (s->compartments[a] != 0) {
+                    uint32_t shift = s->compartments[a];
+                    if ((x[a] >> shift) != home[a]) {
+                        int64_t lo = home[a] << shift, hi = (home[a] + 1) << shift;
+                        x[a] = x[a] < lo ? 2 * lo - 1 - x[a] : (x[a] >= hi ? 2 * hi - 1 - x[a] : x[a]);
+                        v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+                    }
+                }
+            }
+            if (s->has_piston && x[1] >= st->piston_w) {
+                x[1] = 2 * st->piston_w - 1 - x[1];
+                v[1] = (int32_t)((uint32_t)0 - (uint32_t)v[1]);
+            }
+            vel[2 * i] = v[0];
+            vel[2 * i + 1] = v[1];
+            pos[2 * i] = x[0];
+            pos[2 * i + 1] = x[1];
+        }
+        if (s->has_piston) {
+            if (s->piston_free) st->piston_v = (int32_t)((uint32_t)st->piston_v + (uint32_t)piston_kick2(s, piston_force - s->piston_force, s->piston_mass));
+            int64_t old = st->piston_w;
+            st->piston_w += (int64_t)st->piston_v;
+            int64_t lo = orc_to_fixed(s->piston_min), hi = orc_to_fixed(s->piston_max);
+            if (st->piston_w < lo || st->piston_w > hi) {
+                st->piston_w = st->piston_w < lo ? lo : hi;
+                st->piston_v = 0;
+            }
+            st->piston_work += piston_force * orc_to_f64(st->piston_w - old);
+        }
+        st->clock += 1;
+    }
+    free(kicks);
+    free(wf);
+    return rc;
+}
+
+int orc2_step(int prec, const orc_system2 *s, uint64_t n, int64_t *pos, int32_t *vel, const uint32_t *species, uint64_t steps) {
+    orc_state2 *st = malloc(sizeof *st);
+    if (!st) return -2;
+    orc2_state_init(s, st);
+    int rc = orc2_run(prec, s, st, n, pos, vel, species, steps);
+    free(st);
+    return rc;
+}
+
+/* IntEngine::reverse (int_engine.rs:392-409) */
+int orc2_reverse_run(int prec, const orc_system2 *s, orc_state2 *st, uint64_t n, const int64_t *pos, int32_t *vel, const uint32_t *species) {
     int32_t *kicks = malloc((n ? n : 1) * 2 * sizeof(int32_t));
     if (!kicks) return -2;
-    int rc = forces2(prec, s, n, pos, species, kicks, NULL, 0);
-    if (!rc)
+    double piston_force = 0.0;
+    int rc = forces2_full(prec, s, st, n, pos, species, kicks, NULL, NULL, &piston_force, 0);
+    if (!rc) {
         for (uint64_t i = 0; i < n; i++) {
             int pin = pinned2(s, species[i]);
             for (int a = 0; a < 2; a++) {
@@ -1268,16 +1571,32 @@ int orc2_reverse(int prec, const orc_system2 *s, uint64_t n, const int64_t *pos,
                 vel[2 * i + a] = (int32_t)((uint32_t)0 - ((uint32_t)vel[2 * i + a] + (uint32_t)kicks[2 * i + a] + g));
             }
         }
+        if (s->has_piston) {
+            int32_t kick = s->piston_free ? piston_kick2(s, piston_force - s->piston_force, s->piston_mass) : 0;
+            st->piston_v = (int32_t)((uint32_t)0 - ((uint32_t)st->piston_v + (uint32_t)kick));
+        }
+    }
     free(kicks);
+    return rc;
+}
+
+int orc2_reverse(int prec, const orc_system2 *s, uint64_t n, const int64_t *pos, int32_t *vel, const uint32_t *species) {
+    orc_state2 *st = malloc(sizeof *st);
+    if (!st) return -2;
+    orc2_state_init(s, st);
+    int rc = orc2_reverse_run(prec, s, st, n, pos, vel, species);
+    free(st);
     return rc;
 }
 
 /* IntEngine::energy (int_engine.rs:417-446) */
 int orc2_energy(int prec, const orc_system2 *s, uint64_t n, const int64_t *pos, const int32_t *vel, const uint32_t *species, double *kinetic, double *potential) {
     int32_t *kicks = malloc((n ? n : 1) * 2 * sizeof(int32_t));
-    if (!kicks) return -2;
+    orc_state2 *st = malloc(sizeof *st);
+    if (!kicks || !st) return -2;
+    orc2_state_init(s, st);
     double pe = 0.0, ke = 0.0;
-    int rc = forces2(prec, s, n, pos, species, kicks, &pe, 0);
+    int rc = forces2_full(prec, s, st, n, pos, species, kicks, &pe, NULL, NULL, 0);
     if (!rc)
         for (uint64_t i = 0; i < n; i++) {
             double m = s->masses[species[i]];
@@ -1292,6 +1611,7 @@ int orc2_energy(int prec, const orc_system2 *s, uint64_t n, const int64_t *pos, 
     *kinetic = ke;
     *potential = pe;
     free(kicks);
+    free(st);
     return rc;
 }
 

@@ -119,10 +119,49 @@ typedef struct {
     uint32_t pinned;
     double masses[4];
     orc_pair pairs[16];
-    uint32_t walled[2]; /* hard walls on this axis */
+    uint32_t walled[2]; /* walls on this axis (kinds below; all zero: hard) */
     double gravity[2];
     uint64_t seed;
+    /* md_core::System's boundaries (crates/md-core/src/system.rs:26-61, 71-180) */
+    uint32_t wall_kind[2][2];   /* ORC_WALL_* per axis and side */
+    double wall_t[2][2];
+    uint32_t compartments[2];   /* shift (log2 of the side in position units), 0: none */
+    uint32_t diathermal;        /* CompartmentWalls::Diathermal */
+    uint32_t has_bath, bath_species, bath_axis;
+    double bath_t, bath_rate, bath_drift;
+    int64_t bath_region[2];
+    uint32_t has_membrane, membrane_axis, membrane_blocks, pore_shift;
+    int64_t membrane_position, pore_width;
+    uint32_t has_piston, piston_free;
+    double piston_speed, piston_mass, piston_force, piston_start, piston_min, piston_max;
+    uint32_t nbond_types;
+    uint32_t bond_fene[4];
+    double bond_k[4], bond_r0[4];
+    uint64_t nbonds;
+    const uint32_t *bonds;      /* [id_a, id_b, type] per bond */
 } orc_system2;
+
+/* IntEngine's run state beyond positions and velocities (int_engine.rs:23-48): the clock of the
+ * stochastic boundaries, the piston, and the ledger in total and per compartment along each
+ * wall. */
+#define ORC_MAX_COMPARTMENTS 256
+typedef struct {
+    uint32_t clock;
+    int64_t piston_w;
+    int32_t piston_v;
+    double piston_work;
+    orc_ledger3 ledger;
+    double comp_energy[7][ORC_MAX_COMPARTMENTS], comp_impulse[7][ORC_MAX_COMPARTMENTS];
+    uint64_t comp_hits[7][ORC_MAX_COMPARTMENTS];
+} orc_state2;
+
+void orc2_state_init(const orc_system2 *sys, orc_state2 *st);
+/* IntEngine::forces (int_engine.rs:130-251): kicks, potential, per-particle force on each wall
+ * (wall_force[7 i + w], may be NULL) and the piston force (may be NULL). */
+int orc2_forces(int prec, const orc_system2 *sys, const orc_state2 *st, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks, double *pe, double *wall_force, double *piston_force);
+/* IntEngine::step / reverse with the full 2D boundaries (int_engine.rs:253-400) */
+int orc2_run(int prec, const orc_system2 *sys, orc_state2 *st, uint64_t n, int64_t *pos, int32_t *vel, const uint32_t *species, uint64_t steps);
+int orc2_reverse_run(int prec, const orc_system2 *sys, orc_state2 *st, uint64_t n, const int64_t *pos, int32_t *vel, const uint32_t *species);
 
 void orc2_box_for(uint64_t n, double density, double aspect, int64_t out[2]);
 void orc2_lattice(uint64_t n, const int64_t box_len[2], double jitter, uint64_t seed, int64_t *pos);
