@@ -7,7 +7,7 @@
  *   - a state is passed as three flat arrays, positions i64[D*n], velocities i32[D*n] and
  *     species u32[n], in particle-id order: md_core::three::State3 (three.rs:267-273);
  *   - rh_engine3_* replaces md::GpuEngine3 (crates/md/src/engine3.rs:307-648) and rh_engine2_*
- *     md::GpuEngine's periodic / hard-walled subset (crates/md/src/engine.rs:567-1098).
+ *     md::GpuEngine (crates/md/src/engine.rs:567-1098).
  * Every call returns RH_OK or a negative status; rh_last_error() describes the last failure on
  * the calling thread.  Nothing unwinds or aborts across the boundary.  An engine handle belongs
  * to one host thread; it owns one HIP stream; step() returns once the steps are done on the
@@ -58,7 +58,23 @@ typedef struct rh_system3 {
     double bath_t, bath_rate;
 } rh_system3;
 
-/* md_core::System, periodic / hard-walled subset (crates/md-core/src/system.rs:26-61) */
+/* md_core::System (crates/md-core/src/system.rs:26-61) with its boundaries (system.rs:71-180).
+ * Zero-initialised boundary fields mean "none" (hard walls on walled axes, no compartments,
+ * thermostat, membrane, piston or bonds).
+ *   wall_kind / wall_t   Wall per walled axis and side (RH_WALL_*), temperature of heat-bath walls
+ *   compartments[a]      0, or log2 of the compartment side in position units (>= 32): hard walls
+ *                        every 2^shift along a walled axis (System::with_compartments)
+ *   diathermal           CompartmentWalls::Diathermal (pairs interact across compartment walls)
+ *   has_bath ...         Thermostat {t, rate, species, region [lo, hi) along bath_axis (position
+ *                        units), drift (sigma/tau, along x)}; a caller without a region passes
+ *                        INT64_MIN, INT64_MAX (System::with_thermostat's default)
+ *   has_membrane ...     Membrane {axis, position (position units), blocks (bit per species),
+ *                        pore_shift (0: no pores), pore_width (position units)}
+ *   has_piston ...       Piston {drive: piston_free = 0 Speed(piston_speed), 1 Free {piston_mass,
+ *                        piston_force}; start, min, max (sigma)}
+ *   bond_* / bonds       BondType per type (bond_fene[t] = 1: Fene, 0: Harmonic; k, r0) and
+ *                        nbonds triples [id_a, id_b, type] (the caller keeps them alive during
+ *                        the call that takes the system). */
 typedef struct rh_system2 {
     int64_t box_len[2];
     double dt;
@@ -69,7 +85,26 @@ typedef struct rh_system2 {
     uint32_t walled[2];
     double gravity[2];
     uint64_t seed;
+    uint32_t wall_kind[2][2];
+    double wall_t[2][2];
+    uint32_t compartments[2];
+    uint32_t diathermal;
+    uint32_t has_bath, bath_species, bath_axis;
+    double bath_t, bath_rate, bath_drift;
+    int64_t bath_region[2];
+    uint32_t has_membrane, membrane_axis, membrane_blocks, pore_shift;
+    int64_t membrane_position, pore_width;
+    uint32_t has_piston, piston_free;
+    double piston_speed, piston_mass, piston_force, piston_start, piston_min, piston_max;
+    uint32_t nbond_types;
+    uint32_t bond_fene[4];
+    double bond_k[4], bond_r0[4];
+    uint64_t nbonds;
+    const uint32_t* bonds;
 } rh_system2;
+
+#define RH_MAX_COMPARTMENTS 256 /* md_core::system::MAX_COMPARTMENTS (system.rs:178) */
+#define RH_MAX_BONDS 4          /* md_core::system::MAX_BONDS (system.rs:67) */
 
 /* md::ForceMode (crates/md/src/engine.rs:104-110): kind 0 = CellList, 1 = Verlet{skin, cap}.
  * cap 0 asks for the engine's own capacity rule. */
@@ -84,6 +119,13 @@ typedef struct rh_ledger3 {
     double energy[7], impulse[7];
     uint64_t hits[7];
 } rh_ledger3;
+
+/* md_core::walls::Ledger (crates/md-core/src/walls.rs:10-13, 41-57): entries 0-3 the walls (2 axis +
+ * side), 4 and 5 the membrane's low and high faces, 6 the thermostat. */
+typedef struct rh_ledger2 {
+    double energy[7], impulse[7];
+    uint64_t hits[7];
+} rh_ledger2;
 
 /* Counters of the list machinery (the reference's read_list_stats, engine3.rs:272-279, plus
  * this engine's own). */
@@ -168,6 +210,32 @@ int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out);
 int rh_engine2_set_arith(rh_engine2* e, int arith);
 int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block);
 int rh_engine2_stream(rh_engine2* e, void** stream);
+int rh_engine2_n(const rh_engine2* e, uint64_t* n);                                          /* engine.rs:777-779 */
+int rh_engine2_steps(const rh_engine2* e, int64_t* steps);                                   /* GpuEngine::steps, :595 */
+int rh_engine2_update_system(rh_engine2* e, const rh_system2* sys);                           /* engine.rs:895-934 */
+int rh_engine2_set_sort_interval(rh_engine2* e, uint32_t interval);                           /* sort_interval */
+int rh_engine2_sort_now(rh_engine2* e);                                                       /* engine.rs:943-948 */
+int rh_engine2_read_potential(rh_engine2* e, double* pe);                                     /* kicks().1, :1090-1097 */
+int rh_engine2_read_positions(rh_engine2* e, int64_t* pos);                                   /* :1085-1088 */
+int rh_engine2_kick_region(rh_engine2* e, const double lo[2], const double hi[2], double scale, const double add[2]); /* :1026-1038 */
+int rh_engine2_set_species(rh_engine2* e, const uint32_t* ids, uint64_t count, uint32_t species);                /* :1042-1053 */
+int rh_engine2_set_species_velocity(rh_engine2* e, uint32_t species, const double v[2]);                         /* :1056-1064 */
+/* The ledger totals (:953-955) and per wall and compartment along it (Buffers::
+ * read_compartment_ledger, :455-470): energy / impulse / hits are [7][per_wall] arrays. */
+int rh_engine2_read_ledger(rh_engine2* e, rh_ledger2* out);
+int rh_engine2_read_compartment_ledger(rh_engine2* e, uint32_t per_wall, double* energy, double* impulse, uint64_t* hits);
+/* The piston (:958-961): out = {height sigma, velocity sigma/tau, gas force on it eps/sigma,
+ * work done on it by the gas eps}; set_piston_speed (:937-940) for a driven piston (sigma/tau). */
+int rh_engine2_read_piston(rh_engine2* e, double out[4]);
+int rh_engine2_set_piston_speed(rh_engine2* e, double u);
+/* XOR one bit of a coordinate: of the particle with id `particle` (:964-973), or of every
+ * particle whose species is in the bitmask `species` (:1014-1023).  bit < 52 here (positions are
+ * held as exact f64 integers), and a flip that would put a particle outside the box is refused. */
+int rh_engine2_flip_bit(rh_engine2* e, uint32_t particle, uint32_t axis, uint32_t bit);
+int rh_engine2_flip_species_bit(rh_engine2* e, uint32_t species, uint32_t axis, uint32_t bit);
+/* Bonds at run time (:976-1011): *changed = 1 if the bond was added / removed. */
+int rh_engine2_add_bond(rh_engine2* e, uint32_t a, uint32_t b, uint32_t type, int* changed);
+int rh_engine2_remove_bond(rh_engine2* e, uint32_t a, uint32_t b, int* changed);
 
 /* ---- the batched engine (no reference counterpart: the reference runs one system per engine;
  * its 2D "compartments", crates/md-core/src/system.rs:41-48, are the nearest prior art) ----

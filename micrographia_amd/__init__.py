@@ -10,12 +10,16 @@ from .core import (
     ONE,
     UNIT,
     Bath,
+    BondType,
     ForceMode,
+    Membrane,
     Pair,
+    Piston,
     State,
     State3,
     System,
     System3,
+    Thermostat,
     Wall,
     box_for,
     cube_for,
@@ -34,10 +38,11 @@ from .core import (
     verlet_for,
     verlet_with_skin,
 )
-from .engine import GpuEngine, GpuEngine3, ListStats
+from .engine import CompartmentLedger, GpuEngine, GpuEngine3, Ledger, ListStats
 
 __all__ = [
-    "ONE", "UNIT", "Bath", "ForceMode", "Pair", "State", "State3", "System", "System3", "Wall",
+    "ONE", "UNIT", "Bath", "BondType", "ForceMode", "Membrane", "Pair", "Piston", "State", "State3",
+    "System", "System3", "Thermostat", "Wall", "CompartmentLedger", "Ledger",
     "box_for", "cube_for", "fcc", "fcc_cells", "fluid", "lj_liquid", "maxwell", "maxwell2",
     "set_temperature", "thermal_wall", "to_f64", "to_fixed", "verlet3", "verlet3_with_skin",
     "verlet_for", "verlet_with_skin", "GpuEngine", "GpuEngine3", "ListStats", "RahmanError", "load",

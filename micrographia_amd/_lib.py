@@ -57,6 +57,37 @@ class CSystem2(C.Structure):
         ("walled", C.c_uint32 * 2),
         ("gravity", C.c_double * 2),
         ("seed", C.c_uint64),
+        ("wall_kind", (C.c_uint32 * 2) * 2),
+        ("wall_t", (C.c_double * 2) * 2),
+        ("compartments", C.c_uint32 * 2),
+        ("diathermal", C.c_uint32),
+        ("has_bath", C.c_uint32),
+        ("bath_species", C.c_uint32),
+        ("bath_axis", C.c_uint32),
+        ("bath_t", C.c_double),
+        ("bath_rate", C.c_double),
+        ("bath_drift", C.c_double),
+        ("bath_region", C.c_int64 * 2),
+        ("has_membrane", C.c_uint32),
+        ("membrane_axis", C.c_uint32),
+        ("membrane_blocks", C.c_uint32),
+        ("pore_shift", C.c_uint32),
+        ("membrane_position", C.c_int64),
+        ("pore_width", C.c_int64),
+        ("has_piston", C.c_uint32),
+        ("piston_free", C.c_uint32),
+        ("piston_speed", C.c_double),
+        ("piston_mass", C.c_double),
+        ("piston_force", C.c_double),
+        ("piston_start", C.c_double),
+        ("piston_min", C.c_double),
+        ("piston_max", C.c_double),
+        ("nbond_types", C.c_uint32),
+        ("bond_fene", C.c_uint32 * 4),
+        ("bond_k", C.c_double * 4),
+        ("bond_r0", C.c_double * 4),
+        ("nbonds", C.c_uint64),
+        ("bonds", C.c_void_p),
     ]
 
 
@@ -65,6 +96,10 @@ class CMode(C.Structure):
 
 
 class CLedger3(C.Structure):
+    _fields_ = [("energy", C.c_double * 7), ("impulse", C.c_double * 7), ("hits", C.c_uint64 * 7)]
+
+
+class CLedger2(C.Structure):
     _fields_ = [("energy", C.c_double * 7), ("impulse", C.c_double * 7), ("hits", C.c_uint64 * 7)]
 
 
@@ -133,6 +168,24 @@ SIGNATURES = {
     "rh_engine2_set_arith": ([P, i32], i32),
     "rh_engine2_tune": ([P, i32, u32], i32),
     "rh_engine2_stream": ([P, P], i32),
+    "rh_engine2_n": ([P, P], i32),
+    "rh_engine2_steps": ([P, P], i32),
+    "rh_engine2_update_system": ([P, P], i32),
+    "rh_engine2_set_sort_interval": ([P, u32], i32),
+    "rh_engine2_sort_now": ([P], i32),
+    "rh_engine2_read_potential": ([P, P], i32),
+    "rh_engine2_read_positions": ([P, P], i32),
+    "rh_engine2_kick_region": ([P, P, P, dbl, P], i32),
+    "rh_engine2_set_species": ([P, P, u64, u32], i32),
+    "rh_engine2_set_species_velocity": ([P, u32, P], i32),
+    "rh_engine2_read_ledger": ([P, P], i32),
+    "rh_engine2_read_compartment_ledger": ([P, u32, P, P, P], i32),
+    "rh_engine2_read_piston": ([P, P], i32),
+    "rh_engine2_set_piston_speed": ([P, dbl], i32),
+    "rh_engine2_flip_bit": ([P, u32, u32, u32], i32),
+    "rh_engine2_flip_species_bit": ([P, u32, u32, u32], i32),
+    "rh_engine2_add_bond": ([P, u32, u32, u32, P], i32),
+    "rh_engine2_remove_bond": ([P, u32, u32, P], i32),
     "rh_batch_create": ([u64, u32, P], i32),
     "rh_batch_destroy": ([P], i32),
     "rh_batch_add3": ([P, P, u64, P, P, P, dbl, u64, P], i32),

@@ -276,18 +276,87 @@ def lj_liquid(n: int, density: float, t: float, seed_pos: int = 7, seed_vel: int
 
 # ---- 2D (system.rs, init.rs) ----
 
+@dataclass(frozen=True)
+class Thermostat:
+    """md_core::Thermostat (system.rs:156-176): an Andersen bath at temperature t and collision
+    rate `rate`, acting on the species in the bitmask `species` whose coordinate `axis` lies in
+    [region[0], region[1]) (position units), redrawing around a mean x velocity `drift`."""
+
+    t: float
+    rate: float
+    species: int = 0xFFFFFFFF
+    region: Tuple[int, int] = (-(1 << 63), (1 << 63) - 1)
+    axis: int = 0
+    drift: float = 0.0
+
+
+@dataclass(frozen=True)
+class Membrane:
+    """md_core::Membrane (system.rs:109-121): across `axis` at `position` (position units),
+    blocking the species in the bitmask `blocks`; pores `pore_width` wide every 2^pore_shift."""
+
+    axis: int
+    position: int
+    blocks: int
+    pore_shift: int = 0
+    pore_width: int = 0
+
+
+@dataclass(frozen=True)
+class Piston:
+    """md_core::Piston (system.rs:95-107): a soft wall replacing the top wall.  speed is set for
+    PistonDrive::Speed (sigma/tau); mass and force (pushing down, eps/sigma) for PistonDrive::Free."""
+
+    start: float
+    min: float
+    max: float
+    speed: float = 0.0
+    free: bool = False
+    mass: float = 0.0
+    force: float = 0.0
+
+
+@dataclass(frozen=True)
+class BondType:
+    """md_core::BondType (system.rs:70-93): Harmonic U = k/2 (r - r0)^2, or Fene
+    U = -k R0^2/2 ln(1 - (r/R0)^2)."""
+
+    fene: bool
+    k: float
+    r0: float
+
+    @staticmethod
+    def harmonic(k: float, r0: float) -> "BondType":
+        return BondType(False, k, r0)
+
+    @staticmethod
+    def fene_bond(k: float, r0: float) -> "BondType":
+        return BondType(True, k, r0)
+
+    @staticmethod
+    def kremer_grest() -> "BondType":
+        return BondType(True, 30.0, 1.5)
+
+
 @dataclass
 class System:
-    """md_core::System, the periodic / hard-walled subset this engine runs."""
+    """md_core::System (system.rs:26-61) with its boundaries and builders (system.rs:186-353)."""
 
     box_len: Tuple[int, int]
     dt: float
     masses: List[float]
     pairs: List[Pair]
-    walled: Tuple[bool, bool] = (False, False)
+    walls: List[Optional[Tuple[Wall, Wall]]] = field(default_factory=lambda: [None, None])
     gravity: Tuple[float, float] = (0.0, 0.0)
     seed: int = 1
     pinned: int = 0
+    compartments: Tuple[int, int] = (0, 0)
+    diathermal: bool = False
+    thermostat: Optional[Thermostat] = None
+    membrane: Optional[Membrane] = None
+    piston: Optional[Piston] = None
+    bond_types: List[BondType] = field(default_factory=list)
+    bonds: List[Tuple[int, int, int]] = field(default_factory=list)
 
     @staticmethod
     def new(box_len, dt: float, pair: Pair, masses: Sequence[float]) -> "System":
@@ -306,10 +375,17 @@ class System:
         pairs[b * ns + a] = pair
         return replace(self, pairs=pairs)
 
+    def with_walls(self, axis: int, lo: Wall, hi: Wall) -> "System":
+        walls = list(self.walls)
+        walls[axis] = (lo, hi)
+        return replace(self, walls=walls)
+
     def with_hard_walls(self, axis: int) -> "System":
-        w = list(self.walled)
-        w[axis] = True
-        return replace(self, walled=tuple(w))
+        return self.with_walls(axis, Wall.Hard, Wall.Hard)
+
+    @property
+    def walled(self) -> Tuple[bool, bool]:
+        return (self.walls[0] is not None, self.walls[1] is not None)
 
     def with_gravity(self, g) -> "System":
         return replace(self, gravity=tuple(float(x) for x in g))
@@ -317,19 +393,116 @@ class System:
     def with_pinned(self, s: int) -> "System":
         return replace(self, pinned=self.pinned | (1 << s))
 
+    def with_seed(self, seed: int) -> "System":
+        return replace(self, seed=int(seed))
+
+    def with_compartments(self, axis: int, log2_sigma: int) -> "System":
+        """Compartments of side 2^log2_sigma sigma along a walled axis (system.rs:233-239)."""
+        shift = 32 + log2_sigma
+        if self.walls[axis] is None:
+            raise ValueError("compartments need walls on their axis")
+        if self.box_len[axis] % (1 << shift):
+            raise ValueError("box must be a whole number of compartments")
+        c = list(self.compartments)
+        c[axis] = shift
+        return replace(self, compartments=tuple(c))
+
+    def with_diathermal_compartments(self) -> "System":
+        return replace(self, diathermal=True)
+
+    def compartment_of(self, p) -> Tuple[int, int]:
+        return tuple(0 if self.compartments[a] == 0 else int(p[a]) >> self.compartments[a] for a in range(2))
+
+    def compartment_counts(self) -> Tuple[int, int]:
+        return tuple(1 if self.compartments[a] == 0 else self.box_len[a] >> self.compartments[a] for a in range(2))
+
+    def with_thermostat(self, t: float, rate: float) -> "System":
+        th = self.thermostat
+        if th is None:
+            return replace(self, thermostat=Thermostat(t, rate))
+        return replace(self, thermostat=replace(th, t=t, rate=rate))
+
+    def with_fan(self, species: int, x0: float, x1: float, drift: float) -> "System":
+        """The thermostat as a fan: species in [x0, x1) sigma, mean x velocity drift (system.rs:335-344)."""
+        if self.thermostat is None:
+            raise ValueError("a fan is a thermostat with a region: add the thermostat first")
+        th = replace(self.thermostat, species=species, region=(to_fixed(x0), to_fixed(x1)), axis=0, drift=drift)
+        return replace(self, thermostat=th)
+
+    def with_bath_layer(self, y0: float, y1: float) -> "System":
+        """The thermostat confined to y0 <= y < y1 sigma (system.rs:346-353)."""
+        if self.thermostat is None:
+            raise ValueError("a bath layer is a thermostat with a region: add the thermostat first")
+        return replace(self, thermostat=replace(self.thermostat, region=(to_fixed(y0), to_fixed(y1)), axis=1))
+
+    def with_membrane(self, axis: int, position_sigma: float, blocked: Sequence[int]) -> "System":
+        if self.walls[axis] is None:
+            raise ValueError("a membrane needs walls on its axis")
+        blocks = 0
+        for s in blocked:
+            blocks |= 1 << s
+        return replace(self, membrane=Membrane(axis, to_fixed(position_sigma), blocks))
+
+    def with_pores(self, log2_period: int, width: float) -> "System":
+        if self.membrane is None:
+            raise ValueError("add a membrane first")
+        return replace(self, membrane=replace(self.membrane, pore_shift=32 + log2_period, pore_width=to_fixed(width)))
+
+    def with_piston(self, start: float, min_: float, speed: Optional[float] = None, mass: Optional[float] = None, force: float = 0.0) -> "System":
+        """A piston at height start (sigma), stopping at min_ and the box height: driven at
+        `speed` (sigma/tau), or free with `mass` under the external `force` (system.rs:284-291)."""
+        if self.walls[1] is None:
+            raise ValueError("a piston needs walls on y")
+        top = to_f64(self.box_len[1])
+        if not (start <= top and min_ < start):
+            raise ValueError("piston needs min < start <= box height")
+        if mass is not None:
+            return replace(self, piston=Piston(start, min_, top, free=True, mass=mass, force=force))
+        return replace(self, piston=Piston(start, min_, top, speed=float(speed or 0.0)))
+
+    def with_bonds(self, types: Sequence[BondType], bonds: Sequence[Sequence[int]]) -> "System":
+        if len(types) > 4:
+            raise ValueError("at most 4 bond types")
+        count = {}
+        for b in bonds:
+            if not (0 <= b[2] < len(types)):
+                raise ValueError("bond type out of range")
+            for i in (b[0], b[1]):
+                count[i] = count.get(i, 0) + 1
+                if count[i] > 4:
+                    raise ValueError(f"particle {i} has more than 4 bonds")
+        return replace(self, bond_types=list(types), bonds=[tuple(int(x) for x in b) for b in bonds])
+
     def rc_max(self) -> float:
         return max(p.rc for p in self.pairs)
+
+    def periodic(self, axis: int) -> bool:
+        return self.walls[axis] is None
+
+    def is_pinned(self, s: int) -> bool:
+        return bool(self.pinned >> s & 1)
 
     def box_f64(self):
         return [to_f64(b) for b in self.box_len]
 
+    def to_step_displacement(self, v: float) -> int:
+        x = round(v * self.dt * ONE)
+        return max(-(1 << 31), min((1 << 31) - 1, int(x)))
+
     def to_c(self) -> _lib.CSystem2:
+        """The C image; the bond array it points to is kept on the returned object."""
         c = _lib.CSystem2()
         ns = self.species()
         for a in range(2):
             c.box_len[a] = int(self.box_len[a])
-            c.walled[a] = 1 if self.walled[a] else 0
+            w = self.walls[a]
+            c.walled[a] = 0 if w is None else 1
+            if w is not None:
+                for side in range(2):
+                    c.wall_kind[a][side] = w[side].kind
+                    c.wall_t[a][side] = w[side].t
             c.gravity[a] = self.gravity[a]
+            c.compartments[a] = self.compartments[a]
         c.dt = self.dt
         c.nspecies = ns
         c.pinned = self.pinned
@@ -340,6 +513,45 @@ class System:
                 p = self.pairs[a * ns + b]
                 c.pairs[a * 4 + b] = _lib.Pair(p.eps, p.sigma, p.rc)
         c.seed = self.seed
+        c.diathermal = 1 if self.diathermal else 0
+        th = self.thermostat
+        if th is not None:
+            c.has_bath = 1
+            c.bath_species = th.species & 0xFFFFFFFF
+            c.bath_axis = th.axis
+            c.bath_t = th.t
+            c.bath_rate = th.rate
+            c.bath_drift = th.drift
+            c.bath_region[0] = th.region[0]
+            c.bath_region[1] = th.region[1]
+        m = self.membrane
+        if m is not None:
+            c.has_membrane = 1
+            c.membrane_axis = m.axis
+            c.membrane_blocks = m.blocks
+            c.pore_shift = m.pore_shift
+            c.membrane_position = m.position
+            c.pore_width = m.pore_width
+        pst = self.piston
+        if pst is not None:
+            c.has_piston = 1
+            c.piston_free = 1 if pst.free else 0
+            c.piston_speed = pst.speed
+            c.piston_mass = pst.mass
+            c.piston_force = pst.force
+            c.piston_start = pst.start
+            c.piston_min = pst.min
+            c.piston_max = pst.max
+        c.nbond_types = len(self.bond_types)
+        for t, bt in enumerate(self.bond_types):
+            c.bond_fene[t] = 1 if bt.fene else 0
+            c.bond_k[t] = bt.k
+            c.bond_r0[t] = bt.r0
+        if self.bonds:
+            arr = np.ascontiguousarray(np.array(self.bonds, dtype=np.uint32).reshape(-1, 3))
+            c._bonds_keepalive = arr
+            c.nbonds = arr.shape[0]
+            c.bonds = arr.ctypes.data
         return c
 
 

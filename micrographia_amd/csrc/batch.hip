@@ -596,6 +596,8 @@ void Batch::budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) c
 // Cells, list capacity and constants of one system, as Params::new (engine3.rs:70-162).
 void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const {
     RH_REQUIRE(!sys.has_bath, "the Andersen heat bath is not supported by the batched engine");
+    RH_REQUIRE(!sys.compartments[0] && !sys.compartments[1] && !sys.has_membrane && !sys.has_piston && sys.bonds.empty(),
+               "compartments, membranes, pistons and bonds are not supported by the batched engine");
     const int D = sys.dim;
     RH_REQUIRE(D == 2 || D == 3, "dimension must be 2 or 3");
     RH_REQUIRE(sys.nspecies >= 1 && sys.nspecies <= 4, "between 1 and 4 species");

@@ -52,8 +52,7 @@ int guarded(F&& f) {
 
 rh::SystemDesc desc3(const rh_system3* s) {
     RH_REQUIRE(s != nullptr, "null system");
-    rh::SystemDesc d;
-    std::memset(&d, 0, sizeof d);
+    rh::SystemDesc d{};
     d.dim = 3;
     for (int a = 0; a < 3; a++) {
         RH_REQUIRE(s->box_len[a] > 0, "box lengths must be positive");
@@ -84,25 +83,89 @@ rh::SystemDesc desc3(const rh_system3* s) {
     return d;
 }
 
+// md_core::System with its boundaries, checked as System's builders check them
+// (system.rs:204-330).
 rh::SystemDesc desc2(const rh_system2* s) {
     RH_REQUIRE(s != nullptr, "null system");
-    rh::SystemDesc d;
-    std::memset(&d, 0, sizeof d);
+    rh::SystemDesc d{};
     d.dim = 2;
     for (int a = 0; a < 2; a++) {
         RH_REQUIRE(s->box_len[a] > 0, "box lengths must be positive");
         d.box_len[a] = s->box_len[a];
-        d.walled[a] = s->walled[a];
-        d.wall_kind[a][0] = RH_WALL_HARD;
-        d.wall_kind[a][1] = RH_WALL_HARD;
+        d.walled[a] = s->walled[a] ? 1u : 0u;
+        for (int side = 0; side < 2; side++) {
+            const uint32_t kind = s->walled[a] ? s->wall_kind[a][side] : (uint32_t)RH_WALL_HARD;
+            RH_REQUIRE(kind <= RH_WALL_THERMAL, "unknown wall kind");
+            d.wall_kind[a][side] = kind;
+            d.wall_t[a][side] = s->wall_t[a][side];
+            if (kind == RH_WALL_THERMAL) RH_REQUIRE(s->wall_t[a][side] >= 0.0, "heat-bath wall temperature must be >= 0");
+        }
         d.gravity[a] = s->gravity[a];
+        if (s->compartments[a] != 0) {
+            const uint32_t shift = s->compartments[a];
+            RH_REQUIRE(s->walled[a], "compartments need walls on their axis");
+            RH_REQUIRE(shift >= 32 && shift < 52, "compartment side must be 2^k sigma, 0 <= k < 20");
+            RH_REQUIRE(s->box_len[a] % (int64_t(1) << shift) == 0, "box must be a whole number of compartments");
+            RH_REQUIRE((s->box_len[a] >> shift) <= RH_MAX_COMPARTMENTS, "at most 256 compartments along an axis");
+            d.compartments[a] = shift;
+        }
     }
     d.box_len[2] = 1;
     d.dt = s->dt;
     d.nspecies = s->nspecies;
     d.pinned = s->pinned;
+    d.seed = s->seed;
     for (int k = 0; k < 4; k++) d.masses[k] = k < (int)s->nspecies ? s->masses[k] : 1.0;
     for (int k = 0; k < 16; k++) d.pairs[k] = s->pairs[k];
+    d.diathermal = s->diathermal ? 1u : 0u;
+    if (s->has_bath) {
+        RH_REQUIRE(s->bath_t >= 0.0 && s->bath_rate >= 0.0, "bath temperature and rate must be >= 0");
+        RH_REQUIRE(s->bath_axis < 2, "thermostat region axis must be 0 or 1");
+        d.has_bath = 1u;
+        d.bath_species = s->bath_species;
+        d.bath_t = s->bath_t;
+        d.bath_rate = s->bath_rate;
+        d.bath_axis = s->bath_axis;
+        d.bath_drift = s->bath_drift;
+        d.bath_region[0] = s->bath_region[0];
+        d.bath_region[1] = s->bath_region[1];
+    }
+    if (s->has_membrane) {
+        RH_REQUIRE(s->membrane_axis < 2 && s->walled[s->membrane_axis], "a membrane needs walls on its axis");
+        RH_REQUIRE(s->pore_shift == 0 || (s->pore_shift >= 1 && s->pore_shift < 62), "pore period out of range");
+        d.has_membrane = 1u;
+        d.membrane_axis = s->membrane_axis;
+        d.membrane_blocks = s->membrane_blocks;
+        d.pore_shift = s->pore_shift;
+        d.membrane_position = s->membrane_position;
+        d.pore_width = s->pore_width;
+    }
+    if (s->has_piston) {
+        RH_REQUIRE(s->walled[1], "a piston needs walls on y");
+        RH_REQUIRE(s->piston_start <= s->piston_max && s->piston_min < s->piston_start, "piston needs min < start <= max");
+        if (s->piston_free) RH_REQUIRE(s->piston_mass > 0.0, "a free piston needs a positive mass");
+        d.has_piston = 1u;
+        d.piston_free = s->piston_free ? 1u : 0u;
+        d.piston_speed = s->piston_speed;
+        d.piston_mass = s->piston_mass;
+        d.piston_force = s->piston_force;
+        d.piston_start = s->piston_start;
+        d.piston_min = s->piston_min;
+        d.piston_max = s->piston_max;
+    }
+    RH_REQUIRE(s->nbond_types <= 4, "at most 4 bond types");
+    d.nbond_types = s->nbond_types;
+    for (uint32_t t = 0; t < 4; t++) {
+        d.bond_fene[t] = t < s->nbond_types && s->bond_fene[t] ? 1u : 0u;
+        d.bond_k[t] = t < s->nbond_types ? s->bond_k[t] : 0.0;
+        d.bond_r0[t] = t < s->nbond_types ? s->bond_r0[t] : 0.0;
+    }
+    if (s->nbonds) {
+        RH_REQUIRE(s->bonds != nullptr, "null bond list");
+        RH_REQUIRE(s->nbonds < (uint64_t(1) << 30), "too many bonds");
+        d.bonds.assign(s->bonds, s->bonds + 3 * s->nbonds);
+        for (uint64_t b = 0; b < s->nbonds; b++) RH_REQUIRE(d.bonds[3 * b + 2] < s->nbond_types, "bond type out of range");
+    }
     return d;
 }
 
@@ -279,6 +342,86 @@ int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block) {
 }
 int rh_engine2_stream(rh_engine2* e, void** stream) {
     return guarded([&] { *stream = (void*)eng(e)->stream(); });
+}
+int rh_engine2_n(const rh_engine2* e, uint64_t* n) {
+    return guarded([&] { *n = eng(e)->n(); });
+}
+int rh_engine2_steps(const rh_engine2* e, int64_t* steps) {
+    return guarded([&] { *steps = eng(e)->steps(); });
+}
+int rh_engine2_update_system(rh_engine2* e, const rh_system2* sys) {
+    return guarded([&] { eng(e)->update_system(desc2(sys)); });
+}
+int rh_engine2_set_sort_interval(rh_engine2* e, uint32_t interval) {
+    return guarded([&] { eng(e)->set_sort_interval(interval); });
+}
+int rh_engine2_sort_now(rh_engine2* e) {
+    return guarded([&] { eng(e)->sort_now(); });
+}
+int rh_engine2_read_potential(rh_engine2* e, double* pe) {
+    return guarded([&] { *pe = eng(e)->potential(); });
+}
+int rh_engine2_read_positions(rh_engine2* e, int64_t* pos) {
+    return guarded([&] {
+        RH_REQUIRE(pos != nullptr, "null argument");
+        eng(e)->read_positions(pos);
+    });
+}
+int rh_engine2_kick_region(rh_engine2* e, const double lo[2], const double hi[2], double scale, const double add[2]) {
+    return guarded([&] {
+        RH_REQUIRE(lo && hi && add, "null argument");
+        const double l3[3] = {lo[0], lo[1], 0.0}, h3[3] = {hi[0], hi[1], 0.0}, a3[3] = {add[0], add[1], 0.0};
+        eng(e)->kick_region(l3, h3, scale, a3);
+    });
+}
+int rh_engine2_set_species(rh_engine2* e, const uint32_t* ids, uint64_t count, uint32_t species) {
+    return guarded([&] { eng(e)->set_species(ids, count, species); });
+}
+int rh_engine2_set_species_velocity(rh_engine2* e, uint32_t species, const double v[2]) {
+    return guarded([&] {
+        RH_REQUIRE(v != nullptr, "null argument");
+        const double v3[3] = {v[0], v[1], 0.0};
+        eng(e)->set_species_velocity(species, v3);
+    });
+}
+int rh_engine2_read_ledger(rh_engine2* e, rh_ledger2* out) {
+    return guarded([&] {
+        RH_REQUIRE(out != nullptr, "null ledger");
+        rh_ledger3 l;
+        eng(e)->read_ledger(&l);
+        static_assert(sizeof(rh_ledger2) == sizeof(rh_ledger3), "ledger layouts");
+        std::memcpy(out, &l, sizeof l);
+    });
+}
+int rh_engine2_read_compartment_ledger(rh_engine2* e, uint32_t per_wall, double* energy, double* impulse, uint64_t* hits) {
+    return guarded([&] { eng(e)->read_compartment_ledger(per_wall, energy, impulse, hits); });
+}
+int rh_engine2_read_piston(rh_engine2* e, double out[4]) {
+    return guarded([&] {
+        RH_REQUIRE(out != nullptr, "null argument");
+        eng(e)->read_piston(out);
+    });
+}
+int rh_engine2_set_piston_speed(rh_engine2* e, double u) {
+    return guarded([&] { eng(e)->set_piston_speed(u); });
+}
+int rh_engine2_flip_bit(rh_engine2* e, uint32_t particle, uint32_t axis, uint32_t bit) {
+    return guarded([&] { eng(e)->flip_bit(particle, axis, bit); });
+}
+int rh_engine2_flip_species_bit(rh_engine2* e, uint32_t species, uint32_t axis, uint32_t bit) {
+    return guarded([&] { eng(e)->flip_species_bit(species, axis, bit); });
+}
+int rh_engine2_add_bond(rh_engine2* e, uint32_t a, uint32_t b, uint32_t type, int* changed) {
+    return guarded([&] {
+        const bool c = eng(e)->add_bond(a, b, type);
+        if (changed) *changed = c ? 1 : 0;
+    });
+}
+int rh_engine2_remove_bond(rh_engine2* e, uint32_t a, uint32_t b, int* changed) {
+    return guarded([&] {
+        const bool c = eng(e)->remove_bond(a, b);
+        if (changed) *changed = c ? 1 : 0;
+    });
 }
 
 // ---- batched engine ----

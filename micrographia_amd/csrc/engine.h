@@ -26,6 +26,7 @@ enum : uint32_t {
     CTL_DONE_LO = 10,
     CTL_DONE_HI = 11,
     CTL_SKIPPED = 12,    // gated step launches that found the list stale
+    CTL_FLIP_BAD = 13,   // bit flips that would have left the box (flip_bit's check pass)
     CTL_WORDS = 32
 };
 
@@ -75,7 +76,36 @@ struct Params {
     // All-integer pair forces from tables (md::Arith::Int, 2D only)
     uint32_t arith;          // 0: f32 pair forces, 1: integer tables
     uint32_t itab_len;       // entries per species pair
+    // md_core::System's 2D boundaries (system.rs:26-61).  feat selects the boundary path: pair
+    // kicks, then the per-particle terms (bonds, soft walls, piston), then the leapfrog with
+    // every boundary, each its own launch.
+    uint32_t feat;
+    uint32_t comp_filter;    // insulating compartments: pairs in different ones do not interact
+    uint32_t comp_shift[2];  // log2 of the compartment side (position units); 0: none
+    double comp_scale[2];    // 2^-shift, or 0 on an axis without compartments
+    uint32_t bath_axis;      // the thermostat acts where bath_lo <= x[bath_axis] < bath_hi
+    int32_t bath_drift;      // the fan's drift as a displacement per step (x)
+    double bath_lo, bath_hi;
+    uint32_t membrane, mem_axis, mem_blocks, pore_shift;
+    double mem_pos;          // the membrane's position (position units)
+    int64_t pore_mask, pore_width;
+    uint32_t piston, piston_free;
+    double piston_mass, piston_load;  // Free drive: mass and the external force (pushing down)
+    int64_t piston_lo, piston_hi;     // its stops (position units)
+    uint32_t bonds;          // a partner table is present
+    uint32_t bond_fene;      // bit t: bond type t is FENE (else harmonic)
+    float bond_k[4], bond_r0[4];
     PairTable pt;
+};
+
+// The piston's state on the device (GpuEngine's ctl words PISTON_*, engine.rs:57-66).
+struct PistonDev {
+    long long w;                   // height (position units)
+    int32_t v;                     // displacement per step
+    int32_t pad;
+    unsigned long long force_sum;  // the gas force of the current evaluation, 2^-16 eps/sigma (i64)
+    double force;                  // the gas force of the last step (eps/sigma)
+    double work;                   // work done on it by the gas (eps)
 };
 
 template <int D>
@@ -100,6 +130,11 @@ struct Bufs {
     unsigned long long* ledger;  // [7][3]: energy, impulse (2^-32 units), hits per wall; 6 = bath
     const int32_t* itab;         // integer force tables [(a * ns + b) * len + k] (arith 1)
     const uint32_t* irc2;        // their cutoffs [a * ns + b], 2^-24 sigma^2
+    // 2D boundary path
+    const uint32_t* partners;    // per particle id: 4 x (partner id, bond type), UINT32_MAX padded
+    uint32_t* slot_of;           // particle id -> slot (refreshed at every reorder)
+    unsigned long long* cledger; // [7][256][3]: the ledger per wall and compartment along it
+    PistonDev* piston;
 };
 
 // The smallest r2 of a species pair from which the shared-reciprocal division of the pair kick
@@ -120,6 +155,20 @@ struct SystemDesc {
     uint64_t seed;
     uint32_t has_bath, bath_species;
     double bath_t, bath_rate;
+    // 2D only (md_core::System, system.rs:26-61); see rh_system2
+    uint32_t compartments[2];
+    uint32_t diathermal;
+    uint32_t bath_axis;
+    double bath_drift;
+    int64_t bath_region[2];
+    uint32_t has_membrane, membrane_axis, membrane_blocks, pore_shift;
+    int64_t membrane_position, pore_width;
+    uint32_t has_piston, piston_free;
+    double piston_speed, piston_mass, piston_force, piston_start, piston_min, piston_max;
+    uint32_t nbond_types;
+    uint32_t bond_fene[4];
+    double bond_k[4], bond_r0[4];
+    std::vector<uint32_t> bonds;  // [id_a, id_b, type] per bond
 };
 
 template <int D>
@@ -145,6 +194,16 @@ class Engine {
     void set_species_velocity(uint32_t species, const double v[3]);
     rh_list_stats list_stats();
     void read_ledger(rh_ledger3* out);
+    // 2D boundaries (md::GpuEngine, engine.rs:937-1023)
+    void read_compartment_ledger(uint32_t per_wall, double* energy, double* impulse, uint64_t* hits);
+    void read_piston(double out[4]);
+    void set_piston_speed(double u);
+    void flip_bit(uint32_t particle, uint32_t axis, uint32_t bit);
+    void flip_species_bit(uint32_t species, uint32_t axis, uint32_t bit);
+    bool add_bond(uint32_t a, uint32_t b, uint32_t type);
+    bool remove_bond(uint32_t a, uint32_t b);
+    void sort_now();
+    void read_positions(int64_t* pos);
     // Pair-force arithmetic (md::Arith, engine.rs:92-100): 0 f32 pair forces, 1 all-integer
     // tables (2D, cell-list mode).
     void set_arith(int arith);
@@ -175,6 +234,10 @@ class Engine {
     hipEvent_t timer_event();
     void collect_timers();
     void launch_step();
+    void launch_boundary_step(uint32_t clock_add, bool gated);  // the 2D boundary path's step
+    void enqueue_extras(uint32_t flags);
+    void write_partners();
+    void flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit);
 
     SystemDesc sys_;
     rh_mode mode_;
@@ -211,6 +274,8 @@ class Engine {
     uint32_t block_ = 512;
     std::vector<void*> allocs_;
     void* dalloc(size_t bytes);
+    std::vector<uint32_t> partners_h_;  // the bond partner table (bonds may change at run time)
+    void alloc_boundary_buffers();
     void upload_int_tables();
 };
 

@@ -754,6 +754,10 @@ __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, con
             uint32_t j = items ? items[q] : q;
             if (j == i) continue;
             if constexpr (D == 2) {
+                // System::interact_across (system.rs:229-231): insulating compartment walls
+                if (P.comp_filter && (__builtin_floor(px[0][j] * P.comp_scale[0]) != __builtin_floor(xi[0] * P.comp_scale[0]) ||
+                                      __builtin_floor(px[1][j] * P.comp_scale[1]) != __builtin_floor(xi[1] * P.comp_scale[1])))
+                    continue;
                 if (P.arith) {
                     const uint32_t sj = MULTI ? sp[j] : 0u;
                     const double dx = separation<D>(P, xi[0], px[0][j], 0), dy = separation<D>(P, xi[1], px[1][j], 1);
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(TPB) void k_force_cells(Params<D> P, Bufs<D> B, con
             if (ok) pair_kick_pe<D>(f, B.pt, (int)p, c, k, pe);
         }
     }
-    if (P.soft_walls) soft_walls<D>(P, xi, c, k, &wall_pe);
+    if (P.soft_walls && !P.feat) soft_walls<D>(P, xi, c, k, &wall_pe);  // the boundary path adds them after
     for (int a = 0; a < D; a++) B.kicks[a][i] = (int32_t)k[a];
     B.pe[i] = 0.5 * pe + wall_pe;
 }
@@ -893,6 +897,372 @@ __global__ __launch_bounds__(TPB) void k_set_species_velocity(Params<D> P, Bufs<
 }
 
 // ---------------------------------------------------------------------------------------------
+// The 2D boundary path (md_core::System's compartments, membrane, thermostat regions, heat-bath
+// walls, piston and bonds; IntEngine::forces / step, int_engine.rs:130-361).  One step is four
+// launches: pair kicks (cell list or full Verlet list), the per-particle terms (bonds, soft
+// walls, piston), the leapfrog with every boundary, and the piston's own move.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int LEDGER2_COMPARTMENTS = 256;  // md_core::system::MAX_COMPARTMENTS
+constexpr double PISTON_FORCE_SCALE = 65536.0;  // walls::PISTON_FORCE_SCALE
+
+// Compartment coordinate of a position along an axis (System::compartment_of, system.rs:244-247):
+// x >> shift, computed as floor(x 2^-shift), exact for the f64 integer positions.
+__device__ __forceinline__ double compartment_of2(const Params<2>& P, double x, int a) { return __builtin_floor(x * P.comp_scale[a]); }
+
+// Book a wall exchange in the totals and in the entry of compartment c along the wall (only
+// compartments 0..255 have entries, as in CompartmentLedger; a position outside the box, as a
+// piston below zero can make, books in the totals alone).
+__device__ __forceinline__ void book2(const Bufs<2>& B, int w, double c, int field, double value) {
+    ledger_add(B.ledger, w, field, value);
+    if (c >= 0.0 && c < (double)LEDGER2_COMPARTMENTS) ledger_add(B.cledger + 3 * (w * LEDGER2_COMPARTMENTS + (int)c), 0, field, value);
+}
+
+__device__ __forceinline__ void book2_hit(const Bufs<2>& B, int w, double c) {
+    atomicAdd(&B.ledger[3 * w + 2], 1ull);
+    if (c >= 0.0 && c < (double)LEDGER2_COMPARTMENTS) atomicAdd(&B.cledger[3 * (w * LEDGER2_COMPARTMENTS + (int)c) + 2], 1ull);
+}
+
+// fixed::sep_long_f32 (fixed.rs:65-72): the short-range conversion below 8 sigma, else the high
+// and low words of |d| converted separately and added in f32 (bonds may stretch beyond 8 sigma).
+__device__ __forceinline__ float sep_long_f(double d) {
+    float f;
+    if (sep_f(d, f)) return f;
+    const uint64_t m = (uint64_t)__builtin_fabs(d);
+    const float mag = (float)(uint32_t)(m >> 32) + (float)(uint32_t)m * (1.0f / 4294967296.0f);
+    return d < 0.0 ? -mag : mag;
+}
+
+// Pair kicks over the full Verlet list, without the leapfrog (the boundary path's list mode).
+template <bool MULTI>
+__global__ __launch_bounds__(TPB) void k_force_list2(Params<2> P, Bufs<2> B) {
+    const uint32_t* ctl = B.ctl;
+    if (!step_runs(ctl)) return;
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
+    const double* px = B.pos[pp][0];
+    const double* py = B.pos[pp][1];
+    const uint32_t* sp = B.species[pm];
+    const double xi[2] = {px[i], py[i]};
+    const uint32_t si = MULTI ? sp[i] : 0u;
+    const PairTable* __restrict__ pt = B.pt;
+    const float c = pt->kick_c[si];
+    const double ci0 = compartment_of2(P, xi[0], 0), ci1 = compartment_of2(P, xi[1], 1);
+    uint32_t k[2] = {0u, 0u};
+    const uint32_t m = B.nbr_count[i];
+    const uint32_t* nb = B.nbr + i;
+    for (uint32_t e = 0; e < m; e++) {
+        const uint32_t j = nb[(size_t)e * P.n];
+        const double xj[2] = {px[j], py[j]};
+        // System::interact_across (system.rs:229-231)
+        if (P.comp_filter && (compartment_of2(P, xj[0], 0) != ci0 || compartment_of2(P, xj[1], 1) != ci1)) continue;
+        float f[2];
+        bool ok = sep_f(separation<2>(P, xi[0], xj[0], 0), f[0]);
+        ok &= sep_f(separation<2>(P, xi[1], xj[1], 1), f[1]);
+        const uint32_t p = MULTI ? si * 4 + sp[j] : 0u;
+        if (ok) pair_kick<2>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
+    }
+    B.kicks[0][i] = (int32_t)k[0];
+    B.kicks[1][i] = (int32_t)k[1];
+}
+
+enum : uint32_t { EX_GATED = 1, EX_PE = 2, EX_PISTON = 4, EX_BOOK = 8 };
+
+// The per-particle terms after the pairs (int_engine.rs:181-248): bonds (BondType::eval in f32
+// on sep_long separations, rounded per bond), soft walls (their force booked as impulse x dt in
+// the ledger of the compartment along the wall), and the piston, a soft wall at its height
+// pushing down (its gas force summed in 2^-16 units with i64 atomics, so in any order).
+__global__ __launch_bounds__(TPB) void k_extras2(Params<2> P, Bufs<2> B, uint32_t flags) {
+    const uint32_t* ctl = B.ctl;
+    if ((flags & EX_GATED) && !step_runs(ctl)) return;
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
+    const double xi[2] = {B.pos[pp][0][i], B.pos[pp][1][i]};
+    const uint32_t si = B.species[pm][i];
+    const float c = B.pt->kick_c[si];
+    uint32_t k[2] = {(uint32_t)B.kicks[0][i], (uint32_t)B.kicks[1][i]};
+    double pe = 0.0;
+    if (P.bonds) {
+        const uint32_t* row = B.partners + 8 * (size_t)B.ids[pm][i];
+        float bond_pe = 0.0f;
+        for (int b = 0; b < 4; b++) {
+            const uint32_t jid = row[2 * b];
+            if (jid == UINT32_MAX) break;
+            const uint32_t t = row[2 * b + 1];
+            const uint32_t j = B.slot_of[jid];
+            const float fx = sep_long_f(separation<2>(P, xi[0], B.pos[pp][0][j], 0));
+            const float fy = sep_long_f(separation<2>(P, xi[1], B.pos[pp][1][j], 1));
+            const float r2 = fx * fx + fy * fy;
+            const float bk = P.bond_k[t], br0 = P.bond_r0[t];
+            float fr, u;
+            if ((P.bond_fene >> t) & 1u) {
+                float x = r2 / (br0 * br0);
+                if (x > 0.999999f) x = 0.999999f;
+                u = -0.5f * bk * br0 * br0 * (float)log((double)(1.0f - x));
+                fr = -bk / (1.0f - x);
+            } else if (br0 == 0.0f) {
+                fr = -bk;
+                u = 0.5f * bk * r2;
+            } else {
+                const float r = __builtin_sqrtf(r2);
+                const float dr = r - br0;
+                fr = -bk * dr / r;
+                u = 0.5f * bk * dr * dr;
+            }
+            const float a = -fr * c;
+            k[0] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * fx));
+            k[1] += (uint32_t)cvt_sat_i32(__builtin_rintf(a * fy));
+            bond_pe += u;
+        }
+        pe += 0.5 * (double)bond_pe;
+    }
+    if (P.soft_walls) {
+        float wf[4] = {0.f, 0.f, 0.f, 0.f};
+        soft_walls<2>(P, xi, c, k, &pe, wf);
+        if (flags & EX_BOOK) {
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (wf[w] != 0.0f) {
+                    const int a = w / 2;
+                    book2(B, w, compartment_of2(P, xi[1 - a], 1 - a), 1, (double)wf[w] * P.dt);
+                }
+        }
+    }
+    if (P.piston) {
+        float df;
+        if (sep_f((double)B.piston->w - xi[1], df) && df < SOFT_RC_F) {
+            const float s2 = 1.0f / (df * df);
+            const float s6 = s2 * s2 * s2;
+            const float f = 24.0f * s6 * (2.0f * s6 - 1.0f) / df;
+            k[1] -= (uint32_t)cvt_sat_i32(__builtin_rintf(f * c));
+            if (flags & EX_PISTON) atomicAdd(&B.piston->force_sum, (unsigned long long)(long long)round((double)f * PISTON_FORCE_SCALE));
+            pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
+        }
+    }
+    B.kicks[0][i] = (int32_t)k[0];
+    B.kicks[1][i] = (int32_t)k[1];
+    if (flags & EX_PE) B.pe[i] += pe;
+}
+
+// walls::piston_kick (walls.rs:143-146): a displacement-per-step change from a net force.
+__device__ __forceinline__ int32_t piston_kick2(const Params<2>& P, double net) {
+    const double x = round(net * P.dt * P.dt / P.piston_mass * 4294967296.0);
+    if (x != x) return 0;
+    return x >= 2147483647.0 ? INT32_MAX : (x <= -2147483648.0 ? INT32_MIN : (int32_t)x);
+}
+
+// The leapfrog with md_core::System's boundaries (int_engine.rs:266-337): v += kick + g; the
+// thermostat in its region (walls::andersen, walls.rs:120-139, with the fan's drift and the
+// affine flow under a driven piston); x += v; the membrane (walls.rs:17-24, pores
+// system.rs:116-119); walls (hard, soft, heat bath, walls.rs:27-35, 91-104) or the compartment
+// walls (walls.rs:76-86); the piston's mirror.  Exchanges go to the ledger under the compartment
+// along the wall, taken at the start of the step.  LIST: gated on the list, writes the other
+// position buffer and checks the skin; else in place.
+template <bool LIST>
+__global__ __launch_bounds__(TPB) void k_advance2(Params<2> P, Bufs<2> B, uint32_t clock_add) {
+    const uint32_t* ctl = B.ctl;
+    if (LIST && !step_runs(ctl)) return;
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
+    const uint32_t si = B.species[pm][i];
+    const uint32_t id = B.ids[pm][i];
+    const uint32_t clock = ctl[CTL_CLOCK] + clock_add;
+    const bool pin = (P.pinned >> si) & 1u;
+    const double m = P.mass[si];
+    const double xi[2] = {B.pos[pp][0][i], B.pos[pp][1][i]};
+    int32_t v[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) v[a] = (int32_t)((uint32_t)B.vel[pm][a][i] + (uint32_t)B.kicks[a][i] + (pin ? 0u : (uint32_t)P.gravity[a]));
+    const double home[2] = {compartment_of2(P, xi[0], 0), compartment_of2(P, xi[1], 1)};
+    double pw = 0.0;
+    int32_t pv = 0;
+    if (P.piston) {
+        pw = (double)B.piston->w;
+        pv = B.piston->v;
+    }
+    const double xb = P.bath_axis ? xi[1] : xi[0];
+    if (P.bath && !pin && ((P.bath_species >> si) & 1u) && xb >= P.bath_lo && xb < P.bath_hi) {
+        const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_THERMO;
+        uint32_t r[2];
+        threefry2x32(id, clock, k0, k1, r);
+        if (unit_f64(r[0], 0u) < P.bath_p) {
+            uint32_t r2[2];
+            threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+            const double u1 = 1.0 - unit_f64(r[1], 0u);
+            const double u2 = unit_f64(r2[0], 0u);
+            const double sd = P.bath_sd[si];
+            const double rad = sqrt(-2.0 * log(u1));
+            const double ang = TAU_D * u2;
+            const double b0 = velocity(P, v[0]), b1 = velocity(P, v[1]);
+            v[0] = to_step(P, sd * rad * cos(ang));
+            v[1] = to_step(P, sd * rad * sin(ang));
+            v[0] = (int32_t)((uint32_t)v[0] + (uint32_t)P.bath_drift);
+            if (P.piston && !P.piston_free) {
+                const double f = fmin(fmax((xi[1] * 0x1p-32) / (pw * 0x1p-32), 0.0), 1.0);
+                const double q = round((double)pv * f);
+                const int32_t add = q != q ? 0 : (q >= 2147483647.0 ? INT32_MAX : (q <= -2147483648.0 ? INT32_MIN : (int32_t)q));
+                v[1] = (int32_t)((uint32_t)v[1] + (uint32_t)add);
+            }
+            const double a0 = velocity(P, v[0]), a1 = velocity(P, v[1]);
+            book2(B, 6, 0.0, 0, 0.5 * m * (a0 * a0 + a1 * a1 - b0 * b0 - b1 * b1));
+            book2_hit(B, 6, 0.0);
+        }
+    }
+    double x[2] = {xi[0] + (double)v[0], xi[1] + (double)v[1]};
+    if (P.membrane && ((P.mem_blocks >> si) & 1u)) {
+        // across axis a (selects instead of indexing, so the arrays stay in registers)
+        const bool ay = P.mem_axis != 0u;
+        const double old_a = ay ? xi[1] : xi[0], new_a = ay ? x[1] : x[0], along = ay ? xi[0] : xi[1];
+        const bool in_pore = P.pore_shift != 0u && ((long long)along & P.pore_mask) < P.pore_width;
+        const bool lo_old = old_a < P.mem_pos, lo_new = new_a < P.mem_pos;
+        if (!in_pore && lo_old != lo_new) {
+            const int face = lo_old ? 4 : 5;
+            const double mirrored = 2.0 * P.mem_pos - 1.0 - new_a;
+            const int32_t va = ay ? v[1] : v[0];
+            const double dp = 2.0 * m * __builtin_fabs(velocity(P, va));
+            const int32_t flipped = (int32_t)((uint32_t)0 - (uint32_t)va);
+            if (ay) {
+                x[1] = mirrored;
+                v[1] = flipped;
+            } else {
+                x[0] = mirrored;
+                v[0] = flipped;
+            }
+            const double cc = compartment_of2(P, along, ay ? 0 : 1);
+            book2(B, face, cc, 1, dp);
+            book2_hit(B, face, cc);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        const double l = P.boxf[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0.0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+            continue;
+        }
+        int side = -1;
+        if (x[a] < 0.0) {
+            x[a] = -1.0 - x[a];
+            side = 0;
+        } else if (x[a] >= l) {
+            x[a] = 2.0 * l - 1.0 - x[a];
+            side = 1;
+        }
+        if (side >= 0) {
+            const int w = 2 * a + side;
+            const double b0 = velocity(P, v[0]), b1 = velocity(P, v[1]);
+            if ((P.thermal_walls >> w) & 1u) {
+                // walls::thermal_velocity (walls.rs:91-104)
+                const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_WALL;
+                uint32_t r[2], r2[2];
+                threefry2x32(id, clock, k0, k1, r);
+                threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+                const double u1 = 1.0 - unit_f64(r[0], r[1]);
+                const double u2 = 1.0 - unit_f64(r2[0], 0u), u3 = unit_f64(r2[1], 0u);
+                const double sd = P.wall_sd[w][si];
+                const double inward = side == 0 ? 1.0 : -1.0;
+                v[a] = to_step(P, inward * sd * sqrt(-2.0 * log(u1)));
+                v[1 - a] = to_step(P, sd * sqrt(-2.0 * log(u2)) * cos(TAU_D * u3));
+            } else {
+                v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+            }
+            const double a0 = velocity(P, v[0]), a1 = velocity(P, v[1]);
+            const double ka = 0.5 * m * (a0 * a0 + a1 * a1), kb = 0.5 * m * (b0 * b0 + b1 * b1);
+            const double before = a == 0 ? b0 : b1, after = a == 0 ? a0 : a1;
+            const double cc = compartment_of2(P, xi[1 - a], 1 - a);
+            book2(B, w, cc, 0, ka - kb);
+            book2(B, w, cc, 1, __builtin_fabs(m * (before - after)));
+            book2_hit(B, w, cc);
+        } else if (P.comp_shift[a] != 0u && compartment_of2(P, x[a], a) != home[a]) {
+            // walls::mirror_compartment: reflect off the wall of the home compartment
+            const double side_len = 1.0 / P.comp_scale[a];
+            const double lo = home[a] * side_len, hi = (home[a] + 1.0) * side_len;
+            x[a] = x[a] < lo ? 2.0 * lo - 1.0 - x[a] : (x[a] >= hi ? 2.0 * hi - 1.0 - x[a] : x[a]);
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        }
+    }
+    if (P.piston && x[1] >= pw) {
+        x[1] = 2.0 * pw - 1.0 - x[1];
+        v[1] = (int32_t)((uint32_t)0 - (uint32_t)v[1]);
+    }
+    const uint32_t out = LIST ? pp ^ 1u : pp;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        B.pos[out][a][i] = x[a];
+        B.vel[pm][a][i] = v[a];
+    }
+    if (LIST) {
+        float r2 = 0.0f;
+        bool far = false;
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            float d;
+            far |= !sep_f(separation<2>(P, B.ref[a][i], x[a], a), d);
+            r2 += d * d;
+        }
+        const bool stale = far || r2 >= P.half_skin2;
+        const uint64_t bal = ballot64(stale);
+        if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
+    }
+}
+
+// IntEngine::move_piston (int_engine.rs:346-361): a free piston's kick from the net force, the
+// drift, the stops, and the work the gas did on it.  One lane.
+__global__ void k_piston2(Params<2> P, Bufs<2> B, int gated) {
+    if (gated && !step_runs(B.ctl)) return;
+    PistonDev& p = *B.piston;
+    const double force = (double)(long long)p.force_sum / PISTON_FORCE_SCALE;
+    p.force_sum = 0ull;
+    if (P.piston_free) p.v = (int32_t)((uint32_t)p.v + (uint32_t)piston_kick2(P, force - P.piston_load));
+    const long long old = p.w;
+    p.w += (long long)p.v;
+    if (p.w < P.piston_lo || p.w > P.piston_hi) {
+        p.w = p.w < P.piston_lo ? P.piston_lo : P.piston_hi;
+        p.v = 0;
+    }
+    p.work += force * ((double)(p.w - old) * 0x1p-32);
+    p.force = force;
+}
+
+// IntEngine::reverse's piston (int_engine.rs:405-411): v <- -(v + kick).  One lane.
+__global__ void k_piston_reverse2(Params<2> P, Bufs<2> B) {
+    PistonDev& p = *B.piston;
+    const double force = (double)(long long)p.force_sum / PISTON_FORCE_SCALE;
+    p.force_sum = 0ull;
+    const int32_t kick = P.piston_free ? piston_kick2(P, force - P.piston_load) : 0;
+    p.v = (int32_t)((uint32_t)0 - ((uint32_t)p.v + (uint32_t)kick));
+}
+
+// The id -> slot map of the freshly gathered order (before the rebuild commits it).
+__global__ __launch_bounds__(TPB) void k_slot_of(Params<2> P, Bufs<2> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= P.n) return;
+    B.slot_of[B.ids[B.ctl[CTL_PMETA] ^ 1u][k]] = k;
+}
+
+// GpuEngine::flip_bit / flip_species_bit (engine.rs:964-973, 1014-1023): XOR bit `bit` of
+// coordinate `axis` of the particle with id `sel` (by_species 0) or of every particle whose
+// species is in the bitmask `sel`.  check = 1 only counts flips that would leave the box.
+__global__ __launch_bounds__(TPB) void k_flip2(Params<2> P, Bufs<2> B, uint32_t sel, int by_species, uint32_t axis, uint32_t bit, int check) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS], pm = B.ctl[CTL_PMETA];
+    const bool hit = by_species ? ((sel >> B.species[pm][i]) & 1u) != 0u : B.ids[pm][i] == sel;
+    if (!hit) return;
+    const long long x = (long long)B.pos[pp][axis][i] ^ (1ll << bit);
+    if (check) {
+        if (x < 0 || x >= P.box[axis]) atomicAdd(&B.ctl[CTL_FLIP_BAD], 1u);
+        return;
+    }
+    B.pos[pp][axis][i] = (double)x;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------------------
 
@@ -999,6 +1369,46 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         P.cell_w[a] = (float)P.cell_w64[a];
     }
     P.block = block_;
+    if constexpr (D == 2) {
+        // md_core::System's boundaries (system.rs:26-61, 204-353)
+        const bool has_bonds = !sys.bonds.empty() || sys.nbond_types > 0;
+        P.feat = (sys.compartments[0] || sys.compartments[1] || sys.has_membrane || sys.has_piston || has_bonds || sys.has_bath || P.thermal_walls) ? 1u : 0u;
+        for (int a = 0; a < 2; a++)
+            if (sys.compartments[a]) {
+                P.comp_shift[a] = sys.compartments[a];
+                P.comp_scale[a] = std::ldexp(1.0, -(int)sys.compartments[a]);
+            }
+        P.comp_filter = (sys.compartments[0] || sys.compartments[1]) && !sys.diathermal ? 1u : 0u;
+        if (sys.has_bath) {
+            P.bath_axis = sys.bath_axis;
+            P.bath_drift = core::to_step_displacement(sys.dt, sys.bath_drift);
+            P.bath_lo = (double)sys.bath_region[0];
+            P.bath_hi = (double)sys.bath_region[1];
+        }
+        if (sys.has_membrane) {
+            P.membrane = 1u;
+            P.mem_axis = sys.membrane_axis;
+            P.mem_blocks = sys.membrane_blocks;
+            P.pore_shift = sys.pore_shift;
+            P.mem_pos = (double)sys.membrane_position;
+            P.pore_mask = sys.pore_shift ? (int64_t(1) << sys.pore_shift) - 1 : 0;
+            P.pore_width = sys.pore_width;
+        }
+        if (sys.has_piston) {
+            P.piston = 1u;
+            P.piston_free = sys.piston_free;
+            P.piston_mass = sys.piston_mass;
+            P.piston_load = sys.piston_force;
+            P.piston_lo = core::to_fixed(sys.piston_min);
+            P.piston_hi = core::to_fixed(sys.piston_max);
+        }
+        P.bonds = has_bonds ? 1u : 0u;
+        for (uint32_t t = 0; t < 4; t++) {
+            if (sys.bond_fene[t]) P.bond_fene |= 1u << t;
+            P.bond_k[t] = (float)sys.bond_k[t];
+            P.bond_r0[t] = (float)sys.bond_r0[t];
+        }
+    }
     // Listed pairs stay within (list radius + skin) of each other while the list is valid; below
     // 8 sigma the reference's range test cannot fail and is skipped.
     P.range_check = ((rcm + skin) * (1.0 + 1e-4) + skin >= 7.99) ? 1u : 0u;
@@ -1092,6 +1502,23 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     B_.ctl = (uint32_t*)dalloc(CTL_WORDS * sizeof(uint32_t));
     B_.ledger = (unsigned long long*)dalloc(LEDGER_WORDS * sizeof(unsigned long long));
     RH_HIP(hipMemset(B_.ledger, 0, LEDGER_WORDS * sizeof(unsigned long long)));
+    if constexpr (D == 2) {
+        if (P_.bonds) {
+            for (uint64_t b = 0; b < sys.bonds.size() / 3; b++)
+                RH_REQUIRE(sys.bonds[3 * b] < n && sys.bonds[3 * b + 1] < n && sys.bonds[3 * b] != sys.bonds[3 * b + 1], "bond ids must be distinct particles");
+            B_.slot_of = (uint32_t*)dalloc(n * sizeof(uint32_t));
+            B_.partners = (const uint32_t*)dalloc(n * 8 * sizeof(uint32_t));
+            write_partners();
+        }
+        if (P_.piston) {
+            B_.piston = (PistonDev*)dalloc(sizeof(PistonDev));
+            PistonDev p{};
+            p.w = core::to_fixed(sys.piston_start);
+            p.v = sys.piston_free ? 0 : core::to_step_displacement(sys.dt, sys.piston_speed);
+            RH_HIP(hipMemcpy(B_.piston, &p, sizeof p, hipMemcpyHostToDevice));
+        }
+        if (P_.feat) alloc_boundary_buffers();
+    }
     reduce_temp_ = dalloc(reduce_temp_bytes(n));
     d_scalar_ = (double*)dalloc(64);
     if (mode.kind == 1) alloc_list(mode.cap);
@@ -1150,6 +1577,7 @@ void Engine<D>::upload(const int64_t* pos, const int32_t* vel, const uint32_t* s
     RH_HIP(hipMemcpy(B_.species[0], species, n_ * sizeof(uint32_t), hipMemcpyHostToDevice));
     for (uint64_t i = 0; i < n_; i++) ids[i] = (uint32_t)i;
     RH_HIP(hipMemcpy(B_.ids[0], ids.data(), n_ * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (B_.slot_of) RH_HIP(hipMemcpy(B_.slot_of, ids.data(), n_ * sizeof(uint32_t), hipMemcpyHostToDevice));
     std::vector<uint32_t> ctl(CTL_WORDS, 0u);
     ctl[CTL_STALE] = 1u;
     RH_HIP(hipMemcpy(B_.ctl, ctl.data(), CTL_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1202,7 +1630,7 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     RH_HIP(hipGetLastError());
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
-        if (variant_ == 1) {
+        if (variant_ == 1 && !P_.feat) {
             bool wrap = false;
             for (int a = 0; a < D; a++) wrap |= P_.nc[a] == 1u && ((P_.periodic >> a) & 1u);
             if (wrap) {
@@ -1217,6 +1645,12 @@ void Engine<D>::enqueue_rebuild(bool gated) {
             else k_build_list<D, false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
         }
         RH_HIP(hipGetLastError());
+    }
+    if constexpr (D == 2) {
+        if (B_.slot_of) {
+            k_slot_of<<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            RH_HIP(hipGetLastError());
+        }
     }
     if (timing_) record_timer(true, t0);
     k_rebuild_commit<<<1, 1, 0, stream_>>>(B_.ctl);
@@ -1317,6 +1751,12 @@ void Engine<D>::step_list(uint64_t steps) {
 
 template <int D>
 void Engine<D>::launch_step() {
+    if constexpr (D == 2) {
+        if (P_.feat) {
+            launch_boundary_step(0u, true);
+            return;
+        }
+    }
     const bool multi = P_.nspecies > 1;
     if (variant_ == 0) {
         uint32_t nb = blocks_for(n_, TPB);
@@ -1373,8 +1813,12 @@ void Engine<D>::step_cells(uint64_t steps) {
         }
         enqueue_cells_at_current(nullptr);
         enqueue_force_cells(nullptr);
-        k_integrate<D><<<nb, TPB, 0, stream_>>>(P_, B_, (uint32_t)s);
-        RH_HIP(hipGetLastError());
+        if (P_.feat) {
+            launch_boundary_step((uint32_t)s, false);
+        } else {
+            k_integrate<D><<<nb, TPB, 0, stream_>>>(P_, B_, (uint32_t)s);
+            RH_HIP(hipGetLastError());
+        }
         since++;
     }
     k_cells_commit<<<1, 1, 0, stream_>>>(B_.ctl, (uint32_t)steps);
@@ -1395,6 +1839,7 @@ template <int D>
 void Engine<D>::kicks(int32_t* kicks, double* potential) {
     enqueue_cells_at_current(nullptr);
     enqueue_force_cells(nullptr);
+    if (P_.feat) enqueue_extras(EX_PE);
     uint32_t ctl[CTL_WORDS];
     read_ctl(ctl);
     uint32_t pm = ctl[CTL_PMETA];
@@ -1426,8 +1871,15 @@ template <int D>
 void Engine<D>::reverse() {
     enqueue_cells_at_current(nullptr);
     enqueue_force_cells(nullptr);
+    if (P_.feat) enqueue_extras(EX_PISTON);
     k_reverse<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_);
     RH_HIP(hipGetLastError());
+    if constexpr (D == 2) {
+        if (P_.piston) {
+            k_piston_reverse2<<<1, 1, 0, stream_>>>(P_, B_);
+            RH_HIP(hipGetLastError());
+        }
+    }
     RH_HIP(hipStreamSynchronize(stream_));
     steps_ = -steps_;
 }
@@ -1475,6 +1927,14 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     RH_REQUIRE(sys.nspecies == sys_.nspecies, "update_system: the species count must stay the same");
     RH_REQUIRE(core::rc_max(sys.pairs, sys.nspecies) <= core::rc_max(sys_.pairs, sys_.nspecies) + 1e-12, "cutoffs cannot grow at run time");
+    if constexpr (D == 2) {
+        // engine.rs:895-906
+        RH_REQUIRE(sys.compartments[0] == sys_.compartments[0] && sys.compartments[1] == sys_.compartments[1], "update_system: compartments cannot change");
+        RH_REQUIRE(sys.bonds == sys_.bonds && (sys.nbond_types > 0) == (sys_.nbond_types > 0), "bonds cannot change at run time");
+        RH_REQUIRE(sys.has_bath == sys_.has_bath, "the thermostat can be retuned, not added or removed");
+        RH_REQUIRE(sys.has_piston == sys_.has_piston && (!sys.has_piston || sys.piston_free == sys_.piston_free), "update_system: the piston and its drive cannot change");
+        RH_REQUIRE(P_.arith == 0u || std::memcmp(sys.pairs, sys_.pairs, sizeof sys.pairs) == 0, "integer tables are fixed at creation");
+    }
     Params<D> old = P_;
     uint32_t old_bits = key_bits_;
     uint64_t old_keys = ncell_keys_;
@@ -1500,6 +1960,7 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     if (grew) write_ctl(CTL_STALE, 1u);
     sys_ = sys;
     P_.arith = old.arith;
+    if (P_.feat && !old.feat) alloc_boundary_buffers();
     if (P_.arith) upload_int_tables();  // the tables follow the new pairs, masses and dt
     upload_tables();
 }
@@ -1566,7 +2027,7 @@ void Engine<D>::set_arith(int arith) {
     if (arith == 1) {
         RH_REQUIRE(D == 2, "integer force tables are a 2D feature");
         RH_REQUIRE(mode_.kind == 0, "integer forces run in cell-list mode");
-        RH_REQUIRE(P_.soft_walls == 0, "integer forces cover pair forces only");
+        RH_REQUIRE(P_.soft_walls == 0 && !P_.feat, "integer forces cover pair forces only");
         upload_int_tables();
     }
     RH_HIP(hipStreamSynchronize(stream_));
@@ -1609,6 +2070,203 @@ void Engine<D>::kernel_ms(double* force_ms, uint64_t* force_launches, double* re
     if (rebuild_ms) *rebuild_ms = rebuild_ms_;
     if (rebuilds) *rebuilds = rebuild_launches_;
 }
+
+// ---- the 2D boundary path (md::GpuEngine, engine.rs:567-1098) ----
+
+// Entering the boundary path (at creation, or when update_system adds heat-bath walls, a
+// membrane ...): the per-compartment ledger starts from the totals.  Until now only the totals
+// were booked, and a system off the boundary path has no compartments (they cannot change at run
+// time), so every exchange so far belongs to compartment 0 along its wall.
+template <int D>
+void Engine<D>::alloc_boundary_buffers() {
+    const size_t words = 7 * LEDGER2_COMPARTMENTS * 3;
+    if (!B_.cledger) B_.cledger = (unsigned long long*)dalloc(words * sizeof(unsigned long long));
+    std::vector<unsigned long long> init(words, 0ull);
+    RH_HIP(hipStreamSynchronize(stream_));
+    unsigned long long tot[LEDGER_WORDS];
+    RH_HIP(hipMemcpy(tot, B_.ledger, sizeof tot, hipMemcpyDeviceToHost));
+    for (int w = 0; w < 7; w++)
+        for (int f = 0; f < 3; f++) init[3 * (w * LEDGER2_COMPARTMENTS) + f] = tot[3 * w + f];
+    RH_HIP(hipMemcpy(B_.cledger, init.data(), words * sizeof(unsigned long long), hipMemcpyHostToDevice));
+}
+
+// System::partner_table (system.rs:312-324): per particle id, up to 4 (partner id, bond type).
+template <int D>
+void Engine<D>::write_partners() {
+    std::vector<uint32_t> t(n_ * 8, UINT32_MAX);
+    std::vector<uint8_t> used(n_, 0);
+    const auto& bonds = sys_.bonds;
+    for (size_t b = 0; b + 2 < bonds.size(); b += 3) {
+        const uint32_t ends[2][2] = {{bonds[b], bonds[b + 1]}, {bonds[b + 1], bonds[b]}};
+        for (const auto& e : ends) {
+            const uint32_t me = e[0];
+            RH_REQUIRE(used[me] < 4, "a particle has more than 4 bonds");
+            t[8 * (size_t)me + 2 * used[me]] = e[1];
+            t[8 * (size_t)me + 2 * used[me] + 1] = bonds[b + 2];
+            used[me]++;
+        }
+    }
+    RH_HIP(hipStreamSynchronize(stream_));
+    if (partners_h_.empty()) {
+        RH_HIP(hipMemcpy((void*)B_.partners, t.data(), t.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    } else {
+        // only the rows that changed
+        for (uint64_t id = 0; id < n_; id++)
+            if (std::memcmp(&t[8 * id], &partners_h_[8 * id], 8 * sizeof(uint32_t)) != 0)
+                RH_HIP(hipMemcpy((void*)(B_.partners + 8 * id), &t[8 * id], 8 * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    partners_h_.swap(t);
+}
+
+template <int D>
+void Engine<D>::enqueue_extras(uint32_t flags) {
+    if constexpr (D == 2) {
+        k_extras2<<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, flags);
+        RH_HIP(hipGetLastError());
+    }
+}
+
+// One step of the boundary path.  gated: Verlet-list mode (pair kicks over the full list; every
+// launch checks the list on the device); else cell-list mode, whose pair kicks the caller has
+// just enqueued, at clock offset clock_add.
+template <int D>
+void Engine<D>::launch_boundary_step(uint32_t clock_add, bool gated) {
+    if constexpr (D == 2) {
+        const uint32_t nb = blocks_for(n_, TPB);
+        if (gated) {
+            if (P_.nspecies > 1) k_force_list2<true><<<nb, TPB, 0, stream_>>>(P_, B_);
+            else k_force_list2<false><<<nb, TPB, 0, stream_>>>(P_, B_);
+            RH_HIP(hipGetLastError());
+        }
+        enqueue_extras((gated ? EX_GATED : 0u) | EX_PISTON | EX_BOOK);
+        if (gated) k_advance2<true><<<nb, TPB, 0, stream_>>>(P_, B_, 0u);
+        else k_advance2<false><<<nb, TPB, 0, stream_>>>(P_, B_, clock_add);
+        RH_HIP(hipGetLastError());
+        if (P_.piston) {
+            k_piston2<<<1, 1, 0, stream_>>>(P_, B_, gated ? 1 : 0);
+            RH_HIP(hipGetLastError());
+        }
+    }
+}
+
+template <int D>
+void Engine<D>::read_compartment_ledger(uint32_t per_wall, double* energy, double* impulse, uint64_t* hits) {
+    RH_REQUIRE(per_wall >= 1 && per_wall <= (uint32_t)LEDGER2_COMPARTMENTS, "per_wall must be 1..256");
+    std::vector<unsigned long long> L(7 * LEDGER2_COMPARTMENTS * 3, 0ull);
+    if (P_.feat && B_.cledger) {
+        RH_HIP(hipMemcpyAsync(L.data(), B_.cledger, L.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+        RH_HIP(hipStreamSynchronize(stream_));
+    } else {
+        // no compartments: every exchange is compartment 0's
+        unsigned long long tot[LEDGER_WORDS];
+        RH_HIP(hipMemcpyAsync(tot, B_.ledger, sizeof tot, hipMemcpyDeviceToHost, stream_));
+        RH_HIP(hipStreamSynchronize(stream_));
+        for (int w = 0; w < 7; w++)
+            for (int f = 0; f < 3; f++) L[3 * (w * LEDGER2_COMPARTMENTS) + f] = tot[3 * w + f];
+    }
+    for (uint32_t w = 0; w < 7; w++)
+        for (uint32_t c = 0; c < per_wall; c++) {
+            const unsigned long long* e = &L[3 * (w * LEDGER2_COMPARTMENTS + c)];
+            if (energy) energy[w * per_wall + c] = (double)(long long)e[0] / LEDGER_ONE;
+            if (impulse) impulse[w * per_wall + c] = (double)(long long)e[1] / LEDGER_ONE;
+            if (hits) hits[w * per_wall + c] = e[2];
+        }
+}
+
+template <int D>
+void Engine<D>::read_piston(double out[4]) {
+    RH_REQUIRE(B_.piston != nullptr, "the system has no piston");
+    PistonDev p;
+    RH_HIP(hipMemcpyAsync(&p, B_.piston, sizeof p, hipMemcpyDeviceToHost, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+    out[0] = core::to_f64(p.w);
+    out[1] = (double)p.v * core::UNIT / sys_.dt;
+    out[2] = p.force;
+    out[3] = p.work;
+}
+
+template <int D>
+void Engine<D>::set_piston_speed(double u) {
+    RH_REQUIRE(B_.piston != nullptr, "the system has no piston");
+    const int32_t v = core::to_step_displacement(sys_.dt, u);
+    RH_HIP(hipMemcpyAsync(&B_.piston->v, &v, sizeof v, hipMemcpyHostToDevice, stream_));
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+void Engine<D>::flip_bit(uint32_t particle, uint32_t axis, uint32_t bit) {
+    RH_REQUIRE(particle < n_, "particle id out of range");
+    flip(particle, 0, axis, bit);
+}
+
+template <int D>
+void Engine<D>::flip_species_bit(uint32_t species, uint32_t axis, uint32_t bit) { flip(species, 1, axis, bit); }
+
+// Check every flip against the box first, then apply them all (or none), and rebuild the list:
+// a flipped particle may have moved any distance.
+template <int D>
+void Engine<D>::flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit) {
+    if constexpr (D == 2) {
+        RH_REQUIRE(axis < 2, "axis must be 0 or 1");
+        RH_REQUIRE(bit < 52, "bit must be below 52 (positions are held as exact f64 integers)");
+        const uint32_t nb = blocks_for(n_, TPB);
+        write_ctl(CTL_FLIP_BAD, 0u);
+        k_flip2<<<nb, TPB, 0, stream_>>>(P_, B_, sel, by_species, axis, bit, 1);
+        RH_HIP(hipGetLastError());
+        uint32_t ctl[CTL_WORDS];
+        read_ctl(ctl);
+        RH_REQUIRE(ctl[CTL_FLIP_BAD] == 0u, "the flip would put a particle outside the box");
+        k_flip2<<<nb, TPB, 0, stream_>>>(P_, B_, sel, by_species, axis, bit, 0);
+        RH_HIP(hipGetLastError());
+        write_ctl(CTL_STALE, 1u);
+        RH_HIP(hipStreamSynchronize(stream_));
+    } else {
+        RH_REQUIRE(false, "bit flips are a 2D engine command");
+    }
+}
+
+// GpuEngine::add_bond / remove_bond (engine.rs:976-1011)
+template <int D>
+bool Engine<D>::add_bond(uint32_t a, uint32_t b, uint32_t type) {
+    RH_REQUIRE(P_.bonds && type < sys_.nbond_types, "add_bond needs a system built with that bond type");
+    RH_REQUIRE(a < n_ && b < n_, "particle id out of range");
+    auto& bonds = sys_.bonds;
+    uint32_t ca = 0, cb = 0;
+    for (size_t k = 0; k + 2 < bonds.size(); k += 3) {
+        if ((bonds[k] == a && bonds[k + 1] == b) || (bonds[k] == b && bonds[k + 1] == a)) return false;
+        ca += (bonds[k] == a) + (bonds[k + 1] == a);
+        cb += (bonds[k] == b) + (bonds[k + 1] == b);
+    }
+    if (a == b || ca >= 4 || cb >= 4) return false;
+    bonds.insert(bonds.end(), {a, b, type});
+    write_partners();
+    return true;
+}
+
+template <int D>
+bool Engine<D>::remove_bond(uint32_t a, uint32_t b) {
+    if (!P_.bonds) return false;
+    auto& bonds = sys_.bonds;
+    std::vector<uint32_t> keep;
+    keep.reserve(bonds.size());
+    for (size_t k = 0; k + 2 < bonds.size(); k += 3)
+        if (!((bonds[k] == a && bonds[k + 1] == b) || (bonds[k] == b && bonds[k + 1] == a))) keep.insert(keep.end(), {bonds[k], bonds[k + 1], bonds[k + 2]});
+    if (keep.size() == bonds.size()) return false;
+    bonds.swap(keep);
+    write_partners();
+    return true;
+}
+
+// GpuEngine::sort_now (engine.rs:943-948): reorder storage (and rebuild the list) now.
+template <int D>
+void Engine<D>::sort_now() {
+    enqueue_rebuild(false);
+    since_sort_ = 0;
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+void Engine<D>::read_positions(int64_t* pos) { read_state(pos, nullptr, nullptr); }
 
 template class Engine<2>;
 template class Engine<3>;

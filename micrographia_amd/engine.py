@@ -25,6 +25,29 @@ class Ledger3:
 
 
 @dataclass
+class Ledger:
+    """md_core::walls::Ledger (walls.rs:41-57): per entry (walls 0-3 = 2 axis + side, membrane
+    faces 4 and 5, thermostat 6) the energy and impulse exchanged and the hits."""
+
+    energy: np.ndarray
+    impulse: np.ndarray
+    hits: np.ndarray
+
+
+@dataclass
+class CompartmentLedger:
+    """md_core::walls::CompartmentLedger: the ledger per entry and compartment along the wall,
+    arrays of shape (7, per_wall)."""
+
+    energy: np.ndarray
+    impulse: np.ndarray
+    hits: np.ndarray
+
+    def totals(self) -> Ledger:
+        return Ledger(self.energy.sum(axis=1), self.impulse.sum(axis=1), self.hits.sum(axis=1))
+
+
+@dataclass
 class ListStats:
     rebuilds: int
     overflowed: bool
@@ -171,7 +194,9 @@ class GpuEngine3:
 
 
 class GpuEngine:
-    """md::GpuEngine (2D): periodic or hard-walled axes, gravity, pinned species, f32 pair forces."""
+    """md::GpuEngine (2D): f32 pair forces (or the all-integer tables) with md_core::System's
+    boundaries: walls (hard, soft, heat bath), compartments, membrane, thermostat / fan / bath
+    layer, piston, bonds, gravity and pinned species."""
 
     def __init__(self, sys: System, state: State, mode: Optional[ForceMode] = None):
         self.sys = sys
@@ -249,3 +274,85 @@ class GpuEngine:
         v = C.c_void_p()
         check(load().rh_engine2_stream(self._h, C.byref(v)))
         return int(v.value or 0)
+
+    @property
+    def steps(self) -> int:
+        v = C.c_int64()
+        check(load().rh_engine2_steps(self._h, C.byref(v)))
+        return int(v.value)
+
+    @property
+    def sort_interval(self) -> int:
+        return getattr(self, "_sort_interval", 100)
+
+    @sort_interval.setter
+    def sort_interval(self, v: int) -> None:
+        check(load().rh_engine2_set_sort_interval(self._h, int(v)))
+        self._sort_interval = int(v)
+
+    def sort_now(self) -> None:
+        check(load().rh_engine2_sort_now(self._h))
+
+    def update_system(self, sys: System) -> None:
+        cs = sys.to_c()
+        check(load().rh_engine2_update_system(self._h, C.byref(cs)))
+        self.sys = sys
+
+    def kick_region(self, lo: Sequence[float], hi: Sequence[float], scale: float, add: Sequence[float]) -> None:
+        lo_a, hi_a, add_a = (np.ascontiguousarray(x, np.float64) for x in (lo, hi, add))
+        check(load().rh_engine2_kick_region(self._h, ptr(lo_a), ptr(hi_a), float(scale), ptr(add_a)))
+
+    def set_species(self, ids: Sequence[int], species: int) -> None:
+        a = np.ascontiguousarray(ids, np.uint32)
+        check(load().rh_engine2_set_species(self._h, ptr(a), len(a), int(species)))
+
+    def set_species_velocity(self, species: int, v: Sequence[float]) -> None:
+        a = np.ascontiguousarray(v, np.float64)
+        check(load().rh_engine2_set_species_velocity(self._h, int(species), ptr(a)))
+
+    def read_positions(self) -> np.ndarray:
+        pos = np.zeros((self._n, 2), np.int64)
+        check(load().rh_engine2_read_positions(self._h, ptr(pos)))
+        return pos
+
+    def read_potential(self) -> float:
+        v = C.c_double()
+        check(load().rh_engine2_read_potential(self._h, C.byref(v)))
+        return float(v.value)
+
+    def read_ledger(self) -> Ledger:
+        l = _lib.CLedger2()
+        check(load().rh_engine2_read_ledger(self._h, C.byref(l)))
+        return Ledger(np.array(l.energy[:]), np.array(l.impulse[:]), np.array(l.hits[:], dtype=np.uint64))
+
+    def read_compartment_ledger(self, per_wall: int) -> CompartmentLedger:
+        e = np.zeros((7, per_wall), np.float64)
+        p = np.zeros((7, per_wall), np.float64)
+        h = np.zeros((7, per_wall), np.uint64)
+        check(load().rh_engine2_read_compartment_ledger(self._h, int(per_wall), ptr(e), ptr(p), ptr(h)))
+        return CompartmentLedger(e, p, h)
+
+    def read_piston(self) -> Tuple[float, float, float, float]:
+        """(height sigma, velocity sigma/tau, gas force on it eps/sigma, work done on it by the gas eps)."""
+        out = np.zeros(4, np.float64)
+        check(load().rh_engine2_read_piston(self._h, ptr(out)))
+        return tuple(float(x) for x in out)
+
+    def set_piston_speed(self, u: float) -> None:
+        check(load().rh_engine2_set_piston_speed(self._h, float(u)))
+
+    def flip_bit(self, particle: int, axis: int, bit: int) -> None:
+        check(load().rh_engine2_flip_bit(self._h, int(particle), int(axis), int(bit)))
+
+    def flip_species_bit(self, species: int, axis: int, bit: int) -> None:
+        check(load().rh_engine2_flip_species_bit(self._h, int(species), int(axis), int(bit)))
+
+    def add_bond(self, a: int, b: int, ty: int) -> bool:
+        c = C.c_int()
+        check(load().rh_engine2_add_bond(self._h, int(a), int(b), int(ty), C.byref(c)))
+        return bool(c.value)
+
+    def remove_bond(self, a: int, b: int) -> bool:
+        c = C.c_int()
+        check(load().rh_engine2_remove_bond(self._h, int(a), int(b), C.byref(c)))
+        return bool(c.value)

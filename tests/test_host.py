@@ -112,3 +112,38 @@ def test_division_floor_bounds():
     assert 0 < f.value < 0.01  # r below 0.1 sigma falls back to IEEE division
     _lib.check(_lib.load().rh_pair_division_floor(1e-35, 1.0, 2.5, ctypes.byref(f)))
     assert f.value == float("inf")  # tiny forces: never the fast path
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every field of the ctypes mirrors sits at the header's offset (gcc's offsetof), and the
+    structs have the header's size, so no call reads past or beside what Python passes."""
+    import shutil
+    import subprocess
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    structs = {"rh_system2": _lib.CSystem2, "rh_system3": _lib.CSystem3, "rh_mode": _lib.CMode,
+               "rh_ledger2": _lib.CLedger2, "rh_ledger3": _lib.CLedger3, "rh_list_stats": _lib.CListStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rahman_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(__file__), "..", "include")
+    subprocess.run([gcc, "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in out:
+        if not line:
+            continue
+        cname, fname, value = line.split()
+        py = structs[cname]
+        got = ctypes.sizeof(py) if fname == "sizeof" else getattr(py, fname).offset
+        assert got == int(value), (cname, fname, got, value)
+        seen += 1
+    assert seen == sum(len(p._fields_) + 1 for p in structs.values())
