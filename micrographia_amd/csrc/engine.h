@@ -60,6 +60,7 @@ struct Params {
     float half_skin2;
     uint32_t block;        // half-list variant: particles per work-group block
     uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
+    double wrap_margin;    // position units: particles this far from every periodic face need no minimum image
     float cell_w[3];       // cell width (sigma), f32
     double cell_w64[3];    // cell width (sigma)
     // Stochastic boundaries and the ledger (int_engine3.rs:138-206, three.rs:232-265)

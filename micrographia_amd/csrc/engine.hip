@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <deque>
+#include <type_traits>
 
 namespace rh {
 
@@ -563,6 +564,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     __shared__ uint32_t acc[D][BS];
     // The block's own positions: in-block neighbours are read from LDS, not gathered from memory.
     __shared__ double lpos[D][BS];
+    __shared__ uint32_t wave_wraps[BS / 64];
     const uint32_t t = threadIdx.x;
     const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
     const uint32_t i = base + t;
@@ -572,14 +574,23 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     const bool live = i < P.n;
     double xi[D];
     uint32_t k[D];
+    bool edge = false;  // within wrap_margin of a periodic face: some partner may be an image
 #pragma unroll
     for (int a = 0; a < D; a++) {
         xi[a] = live ? px[a][i] : 0.0;
         k[a] = 0u;
         acc[a][t] = 0u;
         lpos[a][t] = xi[a];
+        edge |= live && ((P.periodic >> a) & 1u) && !(xi[a] >= P.wrap_margin && xi[a] < P.boxf[a] - P.wrap_margin);
     }
+    const uint64_t edge_lanes = ballot64(edge);
+    if (__lane_id() == 0) wave_wraps[t / 64] = edge_lanes != 0ull ? 1u : 0u;
     __syncthreads();
+    // A block none of whose particles is near a periodic face takes plain differences: every
+    // listed partner then lies within the box without wrapping, so b - a IS the minimum image.
+    bool wraps = false;
+#pragma unroll
+    for (int w = 0; w < BS / 64; w++) wraps |= wave_wraps[w] != 0u;
     const uint32_t* sp = B.species[pm];
     const PairTable* __restrict__ pt = B.pt;
     uint32_t si = 0;
@@ -607,6 +618,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
         };
         uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(i, i, i, i);
+        auto pairs = [&](auto wrap) {
         for (uint32_t ch = 0; ch < nch; ch++) {
             const uint4 q = q_next;
             if (ch + 1 < nch) q_next = chunk_at(ch + 1);
@@ -633,7 +645,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                 double d[D];
 #pragma unroll
                 for (int a = 0; a < D; a++) {
-                    d[a] = separation<D>(P, xi[a], xs[e][a], a);
+                    d[a] = decltype(wrap)::value ? separation<D>(P, xi[a], xs[e][a], a) : xs[e][a] - xi[a];
                     f[a] = sep_f_in_range(d[a]);
                 }
                 if (rc) {
@@ -658,6 +670,9 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                 }
             }
         }
+        };
+        if (wraps) pairs(std::true_type{});
+        else pairs(std::false_type{});
     }
     __syncthreads();
     if (!live) return;
@@ -1369,6 +1384,9 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         P.cell_w[a] = (float)P.cell_w64[a];
     }
     P.block = block_;
+    // A particle further than this from every periodic face has all its listed partners inside
+    // the box without wrapping: the list radius plus a skin of motion at each end, with room.
+    P.wrap_margin = ((rcm + skin) * (1.0 + 1e-4) + 2.0 * skin) * 1.01 * 4294967296.0 + 4294967296.0;
     if constexpr (D == 2) {
         // md_core::System's boundaries (system.rs:26-61, 204-353)
         const bool has_bonds = !sys.bonds.empty() || sys.nbond_types > 0;
@@ -1951,6 +1969,7 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     P_.mbits = old.mbits;
     P_.cap = old.cap;
+    P_.wrap_margin = std::max(P_.wrap_margin, old.wrap_margin);
     P_.half_skin2 = old.half_skin2;
     bool grew = false;
     for (int k = 0; k < 16; k++) {

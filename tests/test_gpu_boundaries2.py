@@ -240,7 +240,7 @@ def test_bonds_match_and_reverse(mode):
 
 def test_bonds_added_and_removed_at_run_time():
     sys, st = walled_fluid(40, 40, 600, t=1.0)
-    bonds = chain_bonds(st.pos, st.n())
+    bonds = [(a, b, 0) for a, b, _ in chain_bonds(st.pos, st.n())]
     sys = sys.with_bonds([rh.BondType.harmonic(40.0, 1.0)], bonds)
     e, twin = engine(sys, st, "verlet"), Twin(sys, st)
     e.step(100)

@@ -241,3 +241,64 @@ def test_compartment_ledger_sums_to_the_totals():
         assert st.ledger.hits[w] == sum(st.comp_hits[w][:])
         assert abs(st.ledger.impulse[w] - sum(st.comp_impulse[w][:])) <= 1e-9 * max(1.0, abs(st.ledger.impulse[w]))
     assert st.ledger.impulse[2] > 0  # the soft floor was pushed
+
+
+def kinetic_per_particle(sys, vel, sp):
+    m = np.array([sys.masses[s] for s in sp])
+    v = vel.astype(np.float64) / ONE / sys.dt
+    return float(np.sum(0.5 * m * (v ** 2).sum(axis=1)) / len(sp))
+
+
+def test_thermostat_brings_a_cold_gas_to_its_temperature():
+    """Andersen redraws at T: in 2D the mean kinetic energy per particle goes to kT (two degrees
+    of freedom), for either mass (a wrong sd = T/m instead of sqrt(T/m) fails this)."""
+    sys, pos, vel, sp = walled_fluid(30, 30, 200, t=0.2, masses=(1.0, 3.0))
+    sp[1::2] = 1
+    sys.has_bath = 1
+    sys.bath_species = 0xFFFFFFFF
+    sys.bath_t = 2.0
+    sys.bath_rate = 10.0
+    sys.bath_region[0] = -(1 << 63)
+    sys.bath_region[1] = (1 << 63) - 1
+    st = O.state2(sys)
+    samples = []
+    for _ in range(20):
+        O.run2(sys, st, pos, vel, sp, 100)
+        samples.append(kinetic_per_particle(sys, vel, sp))
+    assert abs(np.mean(samples[5:]) - 2.0) < 0.15, np.mean(samples[5:])
+
+
+def test_heat_bath_walls_bring_a_dilute_gas_to_their_temperature():
+    """Every wall a heat bath at T: a dilute gas equilibrates to kT = T per particle."""
+    sys, pos, vel, sp = walled_fluid(12, 12, 30, t=0.2)
+    for a in range(2):
+        for side in range(2):
+            sys.wall_kind[a][side] = 2
+            sys.wall_t[a][side] = 1.5
+    st = O.state2(sys)
+    samples = []
+    for _ in range(60):
+        O.run2(sys, st, pos, vel, sp, 200)
+        samples.append(kinetic_per_particle(sys, vel, sp))
+    assert abs(np.mean(samples[20:]) - 1.5) < 0.25, np.mean(samples[20:])
+
+
+def test_free_piston_falls_freely_until_it_meets_the_gas():
+    """With no particle within reach, a free piston under a load F follows the integer leapfrog
+    exactly: v_k = k round(-F dt^2 / M 2^32), w_k = w_0 + v_1 + ... + v_k, and no work is done."""
+    sys, pos, vel, sp = walled_fluid(20, 20, 40, t=0.5)
+    pos[:, 1] = pos[:, 1] // 4  # the gas in the bottom quarter
+    sys.box_len[1] = 60 * ONE
+    sys.has_piston = 1
+    sys.piston_free = 1
+    sys.piston_mass = 7.0
+    sys.piston_force = 3.0
+    sys.piston_start = 59.0
+    sys.piston_min = 1.0
+    sys.piston_max = 60.0
+    st = O.state2(sys)
+    kick = int(round(-3.0 * 0.005 * 0.005 / 7.0 * ONE))
+    O.run2(sys, st, pos, vel, sp, 200)
+    assert st.piston_v == 200 * kick
+    assert st.piston_w == int(round(59.0 * ONE)) + kick * 200 * 201 // 2
+    assert st.piston_work == 0.0
