@@ -965,7 +965,7 @@ __global__ __launch_bounds__(TPB) void k_force_list2(Params<2> P, Bufs<2> B) {
     const float c = pt->kick_c[si];
     const double ci0 = compartment_of2(P, xi[0], 0), ci1 = compartment_of2(P, xi[1], 1);
     uint32_t k[2] = {0u, 0u};
-    const uint32_t m = B.nbr_count[i];
+    const uint32_t m = min(B.nbr_count[i], P.cap);  // a valid full list never exceeds cap
     const uint32_t* nb = B.nbr + i;
     for (uint32_t e = 0; e < m; e++) {
         const uint32_t j = nb[(size_t)e * P.n];
@@ -1980,6 +1980,9 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     sys_ = sys;
     P_.arith = old.arith;
     if (P_.feat && !old.feat) alloc_boundary_buffers();
+    // The boundary path reads a full list, the plain path the half list: a change of path needs
+    // the list rebuilt in the other form before the next step.
+    if (P_.feat != old.feat) write_ctl(CTL_STALE, 1u);
     if (P_.arith) upload_int_tables();  // the tables follow the new pairs, masses and dt
     upload_tables();
 }
