@@ -629,7 +629,7 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         d.box[a] = sys.box_len[a];
         d.half[a] = sys.box_len[a] / 2;
         d.boxf[a] = (double)sys.box_len[a];
-        d.inv_box[a] = 1.0 / d.boxf[a];
+        d.inv_box[a] = sys.walled[a] ? 0.0 : 1.0 / d.boxf[a];  // 0 on walled axes: no image
         d.cell_scale[a] = (double)d.nc[a] / (double)sys.box_len[a];
         d.cell_w64[a] = b[a] / (double)d.nc[a];
         d.cell_w[a] = (float)d.cell_w64[a];

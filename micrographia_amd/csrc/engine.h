@@ -53,14 +53,14 @@ struct Params {
     int64_t box[3];
     int64_t half[3];
     double boxf[3];        // box length as f64 (positions are kept as exact f64 integers)
-    double inv_box[3];
+    double inv_box[3];     // 1 / box on periodic axes, 0 on walled ones
     double cell_scale[3];  // cells per position unit
     float kick_c[4];
     int32_t gravity[3];
     float half_skin2;
+    float list_r2_max;     // the largest list radius squared over the species pairs
     uint32_t block;        // half-list variant: particles per work-group block
     uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
-    double wrap_margin;    // position units: particles this far from every periodic face need no minimum image
     float cell_w[3];       // cell width (sigma), f32
     double cell_w64[3];    // cell width (sigma)
     // Stochastic boundaries and the ledger (int_engine3.rs:138-206, three.rs:232-265)

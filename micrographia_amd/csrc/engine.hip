@@ -18,7 +18,6 @@
 #include <algorithm>
 #include <cmath>
 #include <deque>
-#include <type_traits>
 
 namespace rh {
 
@@ -480,6 +479,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         uint32_t cc[D];
         float shift[3] = {0.f, 0.f, 0.f};
         bool skip = false;
+        float gap2 = 0.f;  // squared distance from the particle to the neighbour cell's box
 #pragma unroll
         for (int a = 0; a < D; a++) {
             int v = (int)c[a] + off[a];
@@ -490,8 +490,13 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             }
             cc[a] = (uint32_t)v;
             shift[a] = (float)off[a] * P.cell_w[a] - ri[a];
+            // the nearest face of a side cell, less 1e-3 sigma of slack for rounding
+            const float g = off[a] < 0 ? ri[a] : (off[a] > 0 ? P.cell_w[a] - ri[a] : 0.f);
+            const float gs = fmaxf(g - 1e-3f, 0.f);
+            gap2 = __builtin_fmaf(gs, gs, gap2);
         }
-        if (skip) continue;
+        // A side cell whose box lies beyond the largest list radius holds no partner.
+        if (skip || gap2 > P.list_r2_max) continue;
         const uint32_t key = morton<D>(cc);
         const uint32_t s0 = B.cell_start[key], e0 = B.cell_end[key];
         const float sx = shift[0], sy = shift[1], sz = shift[2];
@@ -564,7 +569,6 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     __shared__ uint32_t acc[D][BS];
     // The block's own positions: in-block neighbours are read from LDS, not gathered from memory.
     __shared__ double lpos[D][BS];
-    __shared__ uint32_t wave_wraps[BS / 64];
     const uint32_t t = threadIdx.x;
     const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
     const uint32_t i = base + t;
@@ -574,23 +578,14 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     const bool live = i < P.n;
     double xi[D];
     uint32_t k[D];
-    bool edge = false;  // within wrap_margin of a periodic face: some partner may be an image
 #pragma unroll
     for (int a = 0; a < D; a++) {
         xi[a] = live ? px[a][i] : 0.0;
         k[a] = 0u;
         acc[a][t] = 0u;
         lpos[a][t] = xi[a];
-        edge |= live && ((P.periodic >> a) & 1u) && !(xi[a] >= P.wrap_margin && xi[a] < P.boxf[a] - P.wrap_margin);
     }
-    const uint64_t edge_lanes = ballot64(edge);
-    if (__lane_id() == 0) wave_wraps[t / 64] = edge_lanes != 0ull ? 1u : 0u;
     __syncthreads();
-    // A block none of whose particles is near a periodic face takes plain differences: every
-    // listed partner then lies within the box without wrapping, so b - a IS the minimum image.
-    bool wraps = false;
-#pragma unroll
-    for (int w = 0; w < BS / 64; w++) wraps |= wave_wraps[w] != 0u;
     const uint32_t* sp = B.species[pm];
     const PairTable* __restrict__ pt = B.pt;
     uint32_t si = 0;
@@ -618,7 +613,6 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
             return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
         };
         uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(i, i, i, i);
-        auto pairs = [&](auto wrap) {
         for (uint32_t ch = 0; ch < nch; ch++) {
             const uint4 q = q_next;
             if (ch + 1 < nch) q_next = chunk_at(ch + 1);
@@ -645,7 +639,7 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                 double d[D];
 #pragma unroll
                 for (int a = 0; a < D; a++) {
-                    d[a] = decltype(wrap)::value ? separation<D>(P, xi[a], xs[e][a], a) : xs[e][a] - xi[a];
+                    d[a] = separation<D>(P, xi[a], xs[e][a], a);
                     f[a] = sep_f_in_range(d[a]);
                 }
                 if (rc) {
@@ -670,9 +664,6 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                 }
             }
         }
-        };
-        if (wraps) pairs(std::true_type{});
-        else pairs(std::false_type{});
     }
     __syncthreads();
     if (!live) return;
@@ -1322,7 +1313,7 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         P.box[a] = sys.box_len[a];
         P.half[a] = sys.box_len[a] / 2;
         P.boxf[a] = (double)sys.box_len[a];
-        P.inv_box[a] = 1.0 / P.boxf[a];
+        P.inv_box[a] = sys.walled[a] ? 0.0 : 1.0 / P.boxf[a];  // 0 on walled axes: no image
         P.cell_scale[a] = (double)P.nc[a] / (double)sys.box_len[a];
         if (!sys.walled[a]) P.periodic |= 1u << a;
         else
@@ -1377,6 +1368,9 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
             P.pt.shift_64[k] = core::pair_shift(p);
         }
     for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
+    P.list_r2_max = 0.f;
+    for (uint32_t a = 0; a < sys.nspecies; a++)
+        for (uint32_t bb = 0; bb < sys.nspecies; bb++) P.list_r2_max = std::max(P.list_r2_max, P.pt.list_r2[a * 4 + bb]);
     P.cap = mode.kind == 1 ? mode.cap : 0;
     RH_REQUIRE(P.cap < 65536, "list capacity must be below 65536");
     for (int a = 0; a < D; a++) {
@@ -1384,9 +1378,6 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         P.cell_w[a] = (float)P.cell_w64[a];
     }
     P.block = block_;
-    // A particle further than this from every periodic face has all its listed partners inside
-    // the box without wrapping: the list radius plus a skin of motion at each end, with room.
-    P.wrap_margin = ((rcm + skin) * (1.0 + 1e-4) + 2.0 * skin) * 1.01 * 4294967296.0 + 4294967296.0;
     if constexpr (D == 2) {
         // md_core::System's boundaries (system.rs:26-61, 204-353)
         const bool has_bonds = !sys.bonds.empty() || sys.nbond_types > 0;
@@ -1969,13 +1960,13 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     P_.mbits = old.mbits;
     P_.cap = old.cap;
-    P_.wrap_margin = std::max(P_.wrap_margin, old.wrap_margin);
     P_.half_skin2 = old.half_skin2;
     bool grew = false;
     for (int k = 0; k < 16; k++) {
         grew |= P_.pt.list_r2[k] > old.pt.list_r2[k];
         P_.pt.list_r2[k] = std::max(P_.pt.list_r2[k], old.pt.list_r2[k]);
     }
+    P_.list_r2_max = std::max(P_.list_r2_max, old.list_r2_max);
     if (grew) write_ctl(CTL_STALE, 1u);
     sys_ = sys;
     P_.arith = old.arith;

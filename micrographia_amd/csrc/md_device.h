@@ -88,12 +88,12 @@ __device__ __forceinline__ void cell_of(const PT& P, const double x[D], uint32_t
 // Minimum image of b - a on periodic axes.  d - L rint(d / L) is exact (d and L rint(d / L)
 // are integers below 2^53) and equals fixed::min_image's choice for every |d| other than L/2
 // itself (where either image is as far as the box allows, beyond every cutoff the engine
-// accepts, so no kick or energy depends on the choice).
+// accepts, so no kick or energy depends on the choice).  Walled axes carry inv_box = 0, which
+// makes the correction exactly zero (rint(0) = 0, d - L 0 = d) without a per-axis select.
 template <int D, class PT>
 __device__ __forceinline__ double separation(const PT& P, double a, double b, int ax) {
-    double d = b - a;
-    if ((P.periodic >> ax) & 1u) d = __builtin_fma(-P.boxf[ax], __builtin_rint(d * P.inv_box[ax]), d);
-    return d;
+    const double d = b - a;
+    return __builtin_fma(-P.boxf[ax], __builtin_rint(d * P.inv_box[ax]), d);
 }
 
 // fixed::sep_f32 on an f64 difference: trunc(d / 16) (exact: a power-of-two scale, then the
