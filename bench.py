@@ -94,6 +94,58 @@ def cpu_baseline(n: int, density: float, t: float, seed: int, budget_s: float = 
     }
 
 
+def c1_fluid(seed: int):
+    """The 2D LJ fluid of the reference's CPU configuration: N = 4096 on a 64 x 64 lattice in a
+    periodic box at density 0.7, Maxwell T = 1.0, rc 2.5 shifted, dt 0.005 (init.rs:105-121)."""
+    import micrographia_amd as rh
+
+    sys_ = rh.System.new(rh.box_for(4096, 0.7, 1.0), 0.005, rh.Pair.lj(2.5), [1.0])
+    return sys_, rh.fluid(sys_, 4096, 1.0, seed)
+
+
+def c1_gpu_leg(seeds, steps: int = 1000):
+    """The same 2D fluid on the 2D engine (Verlet lists, skin 0.4), 1000 steps per seed."""
+    import torch
+
+    import micrographia_amd as rh
+
+    out = {}
+    for seed in seeds:
+        sys_, st = c1_fluid(seed)
+        e = rh.GpuEngine(sys_, st)
+        e.step(10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.step(steps)
+        torch.cuda.synchronize()
+        out[str(seed)] = 4096 * steps / (time.perf_counter() - t0)
+        e.close()
+    return {"workload": f"2D LJ fluid, N=4096, rho 0.7, T 1.0, rc 2.5, dt 0.005, {steps} steps, Verlet skin 0.4", "atom_steps_per_s_by_seed": out}
+
+
+def cpu_baseline_c1(seed: int, steps: int = 1000):
+    """The oracle's IntEngine<f64> (md-ref's 2D engine with f64 forces), one thread, on the same
+    2D fluid: the reference's CPU configuration as specified."""
+    from oracle import oracle as O
+
+    if not os.path.exists(O.LIB):
+        O.build()
+    sys_, st = c1_fluid(seed)
+    osys = O.system2(sys_.box_len)
+    pos, vel, sp = st.pos.copy(), st.vel.copy(), st.species.copy()
+    t0 = time.perf_counter()
+    O.step2(osys, pos, vel, sp, steps, 64)
+    el = time.perf_counter() - t0
+    return {
+        "value": 4096 * steps / el,
+        "unit": "atom-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} steps of the 2D fluid (N=4096, rho 0.7, T 1.0, seed {seed}) with the oracle's IntEngine<f64>, {el:.1f} s",
+        "ns_per_particle_step": el / (4096 * steps) * 1e9,
+    }
+
+
 def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool):
     """The batched ensemble: `count` independent LJ systems (2D/3D 50/50, N log-uniform in
     [256, 65536], density U[0.3, 0.9], T U[0.5, 2.0], own seeds) advanced together by the batched
@@ -216,6 +268,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-plain", action="store_true", help="skip the one-system-at-a-time comparisons")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--seeds", default="2,3", help="further seeds of the headline liquid to time (comma list; empty: none)")
+    ap.add_argument("--no-c1", dest="c1", action="store_false", help="skip the 2D N=4096 fluid (GPU and CPU)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -316,6 +370,26 @@ def main() -> int:
     }
     eng.close()
     del eng
+    extra_seeds = [int(x) for x in args.seeds.split(",") if x.strip()] if args.seeds else []
+    if extra_seeds:
+        by_seed = {str(args.seed): value}
+        for sd in extra_seeds:
+            log(f"[bench] seed {sd}")
+            s2, st2 = make_liquid(args.n, args.density, args.temperature, sd)
+            e2 = rh.GpuEngine3(s2, st2, rh.verlet3_with_skin(s2, args.n, args.skin))
+            e2.tune(args.variant, args.block)
+            e2.step(args.warmup)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e2.step(args.steps)
+            torch.cuda.synchronize()
+            by_seed[str(sd)] = args.n * args.steps / (time.perf_counter() - t0)
+            e2.close()
+            del e2
+        out["value_by_seed"] = by_seed
+    if args.c1:
+        log("[bench] 2D fluid, N=4096")
+        out["c1"] = c1_gpu_leg([1, 2, 3])
     if args.calibrate_copy:
         x = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
         y = torch.empty_like(x)
@@ -330,6 +404,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, one thread)")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.density, args.temperature, args.seed, args.cpu_budget)
+        if args.c1:
+            out["cpu_baseline"]["c1"] = cpu_baseline_c1(1)
     print(json.dumps(out), flush=True)
     return 0
 

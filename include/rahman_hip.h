@@ -206,7 +206,7 @@ int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke);
 int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out);
 /* md::Arith (crates/md/src/engine.rs:92-100; GpuEngine::with_options :609-625): 0 = f32 pair
  * forces, 1 = all-integer pair forces from md_core::table::IntTables (table.rs:17-105), which are
- * bit-identical on every device (cell-list mode, pair forces only). */
+ * bit-identical on every device (pair forces only; either force mode). */
 int rh_engine2_set_arith(rh_engine2* e, int arith);
 int rh_engine2_tune(rh_engine2* e, int variant, uint32_t block);
 int rh_engine2_stream(rh_engine2* e, void** stream);

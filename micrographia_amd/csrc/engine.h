@@ -236,6 +236,9 @@ class Engine {
     void collect_timers();
     void launch_step();
     void launch_boundary_step(uint32_t clock_add, bool gated);  // the 2D boundary path's step
+    // The split path (pair kicks, per-particle terms, leapfrog as separate launches over a full
+    // list): 2D systems with boundaries, and integer forces in Verlet mode.
+    bool split_path() const { return P_.feat != 0u || P_.arith != 0u; }
     void enqueue_extras(uint32_t flags);
     void write_partners();
     void flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit);

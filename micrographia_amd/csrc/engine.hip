@@ -963,9 +963,19 @@ __global__ __launch_bounds__(TPB) void k_force_list2(Params<2> P, Bufs<2> B) {
         const double xj[2] = {px[j], py[j]};
         // System::interact_across (system.rs:229-231)
         if (P.comp_filter && (compartment_of2(P, xj[0], 0) != ci0 || compartment_of2(P, xj[1], 1) != ci1)) continue;
+        const double dx = separation<2>(P, xi[0], xj[0], 0), dy = separation<2>(P, xi[1], xj[1], 1);
+        if (P.arith) {
+            // md::Arith::Int: the all-integer table kick
+            int32_t kk[2];
+            if (int_kick2<2>(P, B, si, MULTI ? sp[j] : 0u, dx, dy, kk)) {
+                k[0] += (uint32_t)kk[0];
+                k[1] += (uint32_t)kk[1];
+            }
+            continue;
+        }
         float f[2];
-        bool ok = sep_f(separation<2>(P, xi[0], xj[0], 0), f[0]);
-        ok &= sep_f(separation<2>(P, xi[1], xj[1], 1), f[1]);
+        bool ok = sep_f(dx, f[0]);
+        ok &= sep_f(dy, f[1]);
         const uint32_t p = MULTI ? si * 4 + sp[j] : 0u;
         if (ok) pair_kick<2>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, k);
     }
@@ -1272,6 +1282,23 @@ __global__ __launch_bounds__(TPB) void k_flip2(Params<2> P, Bufs<2> B, uint32_t 
 // Host side
 // ---------------------------------------------------------------------------------------------
 
+// A 2D system with any of md_core::System's boundaries beyond hard and soft walls: it runs on
+// the split path.
+static bool boundary_features2(const SystemDesc& sys) {
+    bool thermal = false;
+    for (int a = 0; a < 2; a++)
+        for (int side = 0; side < 2; side++) thermal |= sys.walled[a] && sys.wall_kind[a][side] == RH_WALL_THERMAL;
+    return sys.compartments[0] || sys.compartments[1] || sys.has_membrane || sys.has_piston || !sys.bonds.empty() || sys.nbond_types > 0 ||
+           sys.has_bath || thermal;
+}
+
+static bool has_soft_walls(const SystemDesc& sys) {
+    bool soft = false;
+    for (int a = 0; a < sys.dim; a++)
+        for (int side = 0; side < 2; side++) soft |= sys.walled[a] && sys.wall_kind[a][side] == RH_WALL_SOFT;
+    return soft;
+}
+
 template <int D>
 void* Engine<D>::dalloc(size_t bytes) {
     void* p = nullptr;
@@ -1381,7 +1408,7 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
     if constexpr (D == 2) {
         // md_core::System's boundaries (system.rs:26-61, 204-353)
         const bool has_bonds = !sys.bonds.empty() || sys.nbond_types > 0;
-        P.feat = (sys.compartments[0] || sys.compartments[1] || sys.has_membrane || sys.has_piston || has_bonds || sys.has_bath || P.thermal_walls) ? 1u : 0u;
+        P.feat = boundary_features2(sys) ? 1u : 0u;
         for (int a = 0; a < 2; a++)
             if (sys.compartments[a]) {
                 P.comp_shift[a] = sys.compartments[a];
@@ -1639,7 +1666,7 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     RH_HIP(hipGetLastError());
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
-        if (variant_ == 1 && !P_.feat) {
+        if (variant_ == 1 && !split_path()) {
             bool wrap = false;
             for (int a = 0; a < D; a++) wrap |= P_.nc[a] == 1u && ((P_.periodic >> a) & 1u);
             if (wrap) {
@@ -1761,7 +1788,7 @@ void Engine<D>::step_list(uint64_t steps) {
 template <int D>
 void Engine<D>::launch_step() {
     if constexpr (D == 2) {
-        if (P_.feat) {
+        if (split_path()) {
             launch_boundary_step(0u, true);
             return;
         }
@@ -1943,6 +1970,7 @@ void Engine<D>::update_system(const SystemDesc& sys) {
         RH_REQUIRE(sys.has_bath == sys_.has_bath, "the thermostat can be retuned, not added or removed");
         RH_REQUIRE(sys.has_piston == sys_.has_piston && (!sys.has_piston || sys.piston_free == sys_.piston_free), "update_system: the piston and its drive cannot change");
         RH_REQUIRE(P_.arith == 0u || std::memcmp(sys.pairs, sys_.pairs, sizeof sys.pairs) == 0, "integer tables are fixed at creation");
+        RH_REQUIRE(P_.arith == 0u || (!boundary_features2(sys) && !has_soft_walls(sys)), "integer forces cover pair forces only");
     }
     Params<D> old = P_;
     uint32_t old_bits = key_bits_;
@@ -1968,8 +1996,8 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     P_.list_r2_max = std::max(P_.list_r2_max, old.list_r2_max);
     if (grew) write_ctl(CTL_STALE, 1u);
-    sys_ = sys;
     P_.arith = old.arith;
+    sys_ = sys;
     if (P_.feat && !old.feat) alloc_boundary_buffers();
     // The boundary path reads a full list, the plain path the half list: a change of path needs
     // the list rebuilt in the other form before the next step.
@@ -2039,12 +2067,15 @@ void Engine<D>::set_arith(int arith) {
     RH_REQUIRE(arith == 0 || arith == 1, "arith must be 0 (f32 pair forces) or 1 (integer tables)");
     if (arith == 1) {
         RH_REQUIRE(D == 2, "integer force tables are a 2D feature");
-        RH_REQUIRE(mode_.kind == 0, "integer forces run in cell-list mode");
         RH_REQUIRE(P_.soft_walls == 0 && !P_.feat, "integer forces cover pair forces only");
         upload_int_tables();
     }
     RH_HIP(hipStreamSynchronize(stream_));
+    const bool was_split = split_path();
     P_.arith = (uint32_t)arith;
+    if (split_path() && !was_split) alloc_boundary_buffers();
+    // Verlet mode: integer forces run over the full list of the split path
+    if (mode_.kind == 1 && split_path() != was_split) write_ctl(CTL_STALE, 1u);
 }
 
 // Ledger3 (engine3.rs:199-205): energy and impulse exchanged with each wall (2 axis + side)

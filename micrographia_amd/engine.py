@@ -266,7 +266,7 @@ class GpuEngine:
         check(load().rh_engine2_tune(self._h, int(variant), int(block)))
 
     def set_arith(self, arith: int) -> None:
-        """md::Arith: 0 f32 pair forces, 1 all-integer tables (cell-list mode; bit-identical on
+        """md::Arith: 0 f32 pair forces, 1 all-integer tables (pair forces only; bit-identical on
         every device, md_ref::IntEngine's integer forces)."""
         check(load().rh_engine2_set_arith(self._h, int(arith)))
 

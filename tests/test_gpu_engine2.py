@@ -117,11 +117,19 @@ def test_integer_kicks_match_the_twin():
     assert abs(pe - pe_want) <= 1e-9 * abs(pe_want)
 
 
-def test_golden_integer_hash_on_the_gpu():
+@pytest.mark.parametrize("verlet", [False, True])
+def test_golden_integer_hash_on_the_gpu(verlet):
     """The reference's one stored trajectory value: GOLDEN_INT_HASH after 1000 all-integer
-    steps of golden_int_setup() (crates/md/src/checks.rs:156-167, crates/md/tests/integer.rs:84-98)."""
+    steps of golden_int_setup() (crates/md/src/checks.rs:156-167, crates/md/tests/integer.rs:84-98),
+    in cell-list mode and over Verlet lists (which must give the same trajectory)."""
     sys, st = golden_setup()
-    gpu = rh.GpuEngine(sys, st, rh.ForceMode.cell_list())
+    mode = rh.verlet_with_skin(sys, st.n(), 0.3) if verlet else rh.ForceMode.cell_list()
+    gpu = rh.GpuEngine(sys, st, mode)
+    gpu.step(7)  # f32 steps first, then switch: the list is rebuilt in the integer path's form
+    gpu.upload(st)
     gpu.set_arith(1)
-    gpu.step(1000)
+    gpu.step(400)
+    gpu.step(600)
     assert gpu.read_state().hash() == 0x3AAEFF680A2F0278
+    if verlet:
+        assert gpu.read_list_stats().rebuilds > 10
