@@ -114,6 +114,10 @@ struct Bufs {
     // Positions in 2^-32 sigma units, held as f64: integers below 2^53, so exact (the C ABI
     // takes and returns i64).
     double* pos[2][D];   // [parity][axis]
+    // The same positions packed per particle (x, y, z, 0 / x, y), written beside the SoA copy by
+    // the reorder and the half-list step: a neighbour gather reads one 32-byte record (one cache
+    // line) instead of one 8-byte word from each axis array.
+    double* aos[2];
     int32_t* vel[2][D];
     uint32_t* species[2];
     uint32_t* ids[2];

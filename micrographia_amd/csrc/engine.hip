@@ -303,6 +303,8 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
         if (write_ref) B.ref[a][k] = x;
         B.vel[pm ^ 1][a][k] = B.vel[pm][a][i];
     }
+    if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[k] = make_double4(xs[0], xs[1], xs[2], 0.0);
+    else reinterpret_cast<double2*>(B.aos[pp ^ 1])[k] = make_double2(xs[0], xs[1]);
     uint32_t c[D];
     cell_of<D>(P, xs, c);
     float r[3] = {0.f, 0.f, 0.f};
@@ -627,9 +629,18 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
                     for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
             } else {
 #pragma unroll
-                for (int e = 0; e < 4; e++)
-#pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+                for (int e = 0; e < 4; e++) {
+                    if constexpr (D == 3) {
+                        const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e]];
+                        xs[e][0] = q4.x;
+                        xs[e][1] = q4.y;
+                        xs[e][2] = q4.z;
+                    } else {
+                        const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e]];
+                        xs[e][0] = q2.x;
+                        xs[e][1] = q2.y;
+                    }
+                }
             }
 #pragma unroll
             for (int e = 0; e < 4; e++) {
@@ -684,6 +695,8 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
         far |= !sep_f(separation<D>(P, B.ref[a][i], x[a], a), d);
         r2 += d * d;
     }
+    if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[i] = make_double4(x[0], x[1], x[2], 0.0);
+    else reinterpret_cast<double2*>(B.aos[pp ^ 1])[i] = make_double2(x[0], x[1]);
     bool stale = far || r2 >= P.half_skin2;
     uint64_t bal = ballot64(stale);
     if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
@@ -1523,6 +1536,7 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
             B_.vel[p][a] = (int32_t*)dalloc(n * sizeof(int32_t));
         }
         B_.species[p] = (uint32_t*)dalloc(n * sizeof(uint32_t));
+        B_.aos[p] = (double*)dalloc(n * (D == 3 ? 4 : 2) * sizeof(double));
         B_.ids[p] = (uint32_t*)dalloc(n * sizeof(uint32_t));
     }
     for (int a = 0; a < D; a++) {
