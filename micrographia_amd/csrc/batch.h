@@ -42,6 +42,7 @@ struct BBufs {
     uint32_t* species[2];
     uint32_t* ids[2];
     double* ref[3];
+    double4* aos[2];  // positions packed per slot (x, y, z or 0, 0), for the step's neighbour gathers
     float4* rel;  // position relative to its cell's corner (sigma, f32), for list builds
     uint32_t* key;
     uint32_t* rank;

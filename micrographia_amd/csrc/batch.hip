@@ -23,7 +23,7 @@ namespace rh {
 
 constexpr uint32_t LIST_UNIT = 4096;  // list-arena allocation unit (entries)
 constexpr uint32_t CELL_UNIT = 256;   // cell-arena allocation unit (cells)
-constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 16 + 8 + 4;
+constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 4;
 
 // ---------------------------------------------------------------------------------------------
 // Kernels
@@ -130,14 +130,17 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
     uint32_t ns = S.slot0 + B.cell_start[S.cell0 + key] + B.rank[slot];
     uint32_t c[3] = {key % S.nc[0], (key / S.nc[0]) % S.nc[1], D == 3 ? key / (S.nc[0] * S.nc[1]) : 0u};
     float r[3] = {0.f, 0.f, 0.f};
+    double xs[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const double x = B.pos[pp][a][slot];
+        xs[a] = x;
         B.pos[pp ^ 1][a][ns] = x;
         B.ref[a][ns] = x;
         B.vel[pm ^ 1][a][ns] = B.vel[pm][a][slot];
         r[a] = (float)(x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
     }
+    B.aos[pp ^ 1][ns] = make_double4(xs[0], xs[1], xs[2], 0.0);
     B.rel[ns] = make_float4(r[0], r[1], r[2], 0.f);
     B.species[pm ^ 1][ns] = B.species[pm][slot];
     B.ids[pm ^ 1][ns] = B.ids[pm][slot];
@@ -361,10 +364,21 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
 #pragma unroll
                     for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
             } else {
+                // one 32-byte record per neighbour (the packed copy) instead of one word per axis
+                const double4* pq = B.aos[pp] + S.slot0;
 #pragma unroll
-                for (int e = 0; e < 4; e++)
-#pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = px[a][js[e]];
+                for (int e = 0; e < 4; e++) {
+                    if constexpr (D == 3) {
+                        const double4 q4 = pq[js[e]];
+                        xs[e][0] = q4.x;
+                        xs[e][1] = q4.y;
+                        xs[e][2] = q4.z;
+                    } else {
+                        const double2 q2 = reinterpret_cast<const double2*>(pq + js[e])[0];
+                        xs[e][0] = q2.x;
+                        xs[e][1] = q2.y;
+                    }
+                }
             }
 #pragma unroll
             for (int e = 0; e < 4; e++) {
@@ -424,6 +438,7 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         far |= !sep_f(separation<D>(S, B.ref[a][slot], x[a], a), d);
         r2 += d * d;
     }
+    B.aos[pp ^ 1][slot] = make_double4(x[0], x[1], D == 3 ? x[D - 1] : 0.0, 0.0);
     bool stale = far || r2 >= S.half_skin2;
     uint64_t bal = ballot64(stale);
     if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&C.stale_next, 1u);
@@ -553,6 +568,7 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
         }
         B_.species[p] = (uint32_t*)dalloc(slot_cap * 4);
         B_.ids[p] = (uint32_t*)dalloc(slot_cap * 4);
+        B_.aos[p] = (double4*)dalloc(slot_cap * 32);
     }
     for (int a = 0; a < 3; a++) B_.ref[a] = (double*)dalloc(slot_cap * 8);
     B_.rel = (float4*)dalloc(slot_cap * 16);
