@@ -266,7 +266,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
         atomicOr(&C.overflow, 1u);
         na = nb = 0;
     }
-    atomicMax(&C.maxcount, 4u * (ca + cb));
+    if (4u * (ca + cb) > *(volatile const uint32_t*)&C.maxcount) atomicMax(&C.maxcount, 4u * (ca + cb));
     B.nbr_count[slot] = na | (nb << 16);
 }
 

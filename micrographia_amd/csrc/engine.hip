@@ -368,7 +368,7 @@ __global__ __launch_bounds__(TPB) void k_build_list(Params<D> P, Bufs<D> B, cons
     if (count > cap) {
         atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
     }
-    atomicMax(&B.ctl[CTL_MAXCOUNT], count);
+    if (count > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], count);
     B.nbr_count[i] = count < cap ? count : cap;
 }
 
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         na = nb = 0;
     }
-    atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
+    if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
     B.nbr_count[i] = na | (nb << 16);
 }
 
