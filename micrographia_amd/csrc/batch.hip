@@ -441,7 +441,8 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     B.aos[pp ^ 1][slot] = make_double4(x[0], x[1], D == 3 ? x[D - 1] : 0.0, 0.0);
     bool stale = far || r2 >= S.half_skin2;
     uint64_t bal = ballot64(stale);
-    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&C.stale_next, 1u);
+    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&C.stale_next == 0u)
+        atomicOr(&C.stale_next, 1u);
 }
 
 // One launch per kind of system present (dimension, one or more species), each compiled with

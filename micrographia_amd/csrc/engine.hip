@@ -433,7 +433,8 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
     }
     bool stale = far || r2 >= P.half_skin2;
     uint64_t bal = ballot64(stale);
-    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
+    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&B.ctl[CTL_STALE_NEXT] == 0u)
+        atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
 }
 
 
@@ -699,7 +700,8 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     else reinterpret_cast<double2*>(B.aos[pp ^ 1])[i] = make_double2(x[0], x[1]);
     bool stale = far || r2 >= P.half_skin2;
     uint64_t bal = ballot64(stale);
-    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
+    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&B.ctl[CTL_STALE_NEXT] == 0u)
+        atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1235,7 +1237,8 @@ __global__ __launch_bounds__(TPB) void k_advance2(Params<2> P, Bufs<2> B, uint32
         }
         const bool stale = far || r2 >= P.half_skin2;
         const uint64_t bal = ballot64(stale);
-        if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1)) atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
+        if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&B.ctl[CTL_STALE_NEXT] == 0u)
+        atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
     }
 }
 
