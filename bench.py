@@ -157,23 +157,39 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool)
     from micrographia_amd.batch import BatchEngine
     from micrographia_amd.workloads import ensemble
 
+    from micrographia_amd.autotune import BatchTuner
+
     specs = ensemble(count, meta_seed=meta_seed)
     t0 = time.perf_counter()
     built = [s.build() for s in specs]
     atoms = sum(st.n() for _, st in built)
     log(f"[bench] batched: {count} systems, {atoms} atoms, host init {time.perf_counter() - t0:.1f} s")
-    b = BatchEngine(budget_bytes=0, max_systems=max(count, 1), graph_chunk=chunk)
-    need = sum(b.bytes_needed(sys_, st.n()) for sys_, st in built)
-    ids = [b.add(sys_, st, steps + chunk) for sys_, st in built]
-    assert all(i >= 0 for i in ids), "the ensemble does not fit the arena"
-    b.step_all(chunk)  # warm-up chunk: first lists, graph capture
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b.step_all(steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    left = b.poll()
-    assert all(v == 0 for v in left.values())
+    # the autotuner: per (dimension, size class), list variant x cell occupancy, timed on up to 8
+    # of the ensemble's own systems per bucket
+    tuner = BatchTuner(per_bucket=8)
+    tuner.tune(built)
+    log(f"[bench] batched: tuned {len(tuner.table)} buckets in {tuner.seconds:.1f} s")
+
+    def run_ensemble(tuned: bool):
+        b = BatchEngine(budget_bytes=0, max_systems=max(count, 1), graph_chunk=chunk)
+        if tuned:
+            tuner.install(b)
+        need = sum(b.bytes_needed(sys_, st.n()) for sys_, st in built)
+        ids = [b.add(sys_, st, steps + chunk) for sys_, st in built]
+        assert all(i >= 0 for i in ids), "the ensemble does not fit the arena"
+        b.step_all(chunk)  # warm-up chunk: first lists, graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step_all(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        left = b.poll()
+        assert all(v == 0 for v in left.values())
+        b.close()
+        return el, need
+
+    el_default, _ = run_ensemble(False)
+    el, need = run_ensemble(True)
     out = {
         "workload": f"{count} independent LJ systems (2D/3D 50/50, N log-uniform [256, 65536], rho U[0.3,0.9], T U[0.5,2.0], rc 2.5, skin 0.3, dt 0.005), meta-seed {meta_seed}, hipGraph chunks of {chunk} steps",
         "systems": count,
@@ -185,8 +201,12 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool)
         "systems_per_s": count * steps / chunk / el,
         "systems_per_s_unit": f"systems advanced by one {chunk}-step chunk per second",
         "arena_bytes_needed": need,
+        "autotuned": {
+            "table": {k: {"variant": v[0], "cell_factor": v[1]} for k, v in sorted(tuner.table.items())},
+            "tuning_seconds": tuner.seconds,
+            "atom_steps_per_s_default_tiles": atoms * steps / el_default,
+        },
     }
-    b.close()
     if plain:
         # the same systems, one engine at a time: stepping timed, engine creation not
         tot = 0.0

@@ -19,6 +19,7 @@ struct BSys {
     uint32_t nc[3];
     uint32_t ncells, cell0;
     uint32_t periodic, nspecies, pinned, soft_walls, range_check, cap;
+    uint32_t variant;  // 1: half list within work-groups (default), 0: every pair at both ends
     uint64_t nbr0;  // list entry k of local slot i at nbr0 + k * nslots + i
     int64_t box[3], half[3];
     double boxf[3], inv_box[3];  // positions are exact f64 integers (2^-32 sigma units)
@@ -105,8 +106,16 @@ class Batch {
     uint64_t free_slots() const { return slots_.capacity() - slots_.used(); }
     uint32_t active() const { return (uint32_t)members_.size(); }
     void budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) const;
+    // Tile settings for systems added from now on, per dimension and size class (floor(log2 n)):
+    // the list variant and the cell occupancy (cells at least cell_factor x (rc + skin) wide).
+    void set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor);
 
    private:
+    struct Tuning {
+        uint32_t variant = 1;
+        double cell_factor = 1.0;
+    };
+    Tuning tuning_[2][32];
     void plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const;
     void upload_tables();
     void enqueue_iteration();

@@ -165,7 +165,8 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B) {
 // in four-entry chunks (uint4 at nbr0 + 4 (c * nslots + i)), in-block chunks from the front and
 // the others from the back.
 template <int D, bool MULTI>
-__device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local, uint32_t variant) {
+__device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local) {
+    const uint32_t variant = S.variant;
     const uint32_t pm = C.pmeta ^ 1u;
     const uint32_t slot = S.slot0 + local;
     const uint32_t* sp = B.species[pm] + S.slot0;
@@ -270,7 +271,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     B.nbr_count[slot] = na | (nb << 16);
 }
 
-__global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t variant) {
+__global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B) {
     const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
     uint32_t s = B.wg_sys[wg];
     if (s == ~0u) return;
@@ -281,11 +282,11 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t variant
     if (local >= S.n) return;
     bool multi = S.nspecies > 1;
     if (S.dim == 3) {
-        if (multi) bb_build<3, true>(B, S, C, local, variant);
-        else bb_build<3, false>(B, S, C, local, variant);
+        if (multi) bb_build<3, true>(B, S, C, local);
+        else bb_build<3, false>(B, S, C, local);
     } else {
-        if (multi) bb_build<2, true>(B, S, C, local, variant);
-        else bb_build<2, false>(B, S, C, local, variant);
+        if (multi) bb_build<2, true>(B, S, C, local);
+        else bb_build<2, false>(B, S, C, local);
     }
 }
 
@@ -634,9 +635,13 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         vol *= b[a];
     }
     double margin = 1e-3 + 1e-6 * bmax;
+    uint32_t size_class = 0;
+    while (size_class < 31 && (uint64_t(2) << size_class) <= n) size_class++;
+    const Tuning& tune = tuning_[D - 2][size_class];
+    d.variant = tune.variant;
     uint64_t ncells = 1;
     for (int a = 0; a < D; a++) {
-        d.nc[a] = core::sat_u32(std::floor(b[a] / (rcm + skin + margin)));
+        d.nc[a] = core::sat_u32(std::floor(b[a] / ((rcm + skin) * tune.cell_factor + margin)));
         if (d.nc[a] < 3) {
             RH_REQUIRE(b[a] > 2.0 * (rcm + skin) * (1.0 + 1e-4) + margin, "box too small: shorter than twice the cutoff plus skin");
             d.nc[a] = 1;
@@ -687,6 +692,14 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
     double dens = (double)n / vol;
     double c = D == 3 ? std::ceil(4.0 / 3.0 * pi * r * r * r * std::fmax(2.0 * dens, 1.2)) : std::ceil(pi * r * r * std::fmax(2.0 * dens, 1.2));
     *cap = ((uint32_t)std::min(c + 8.0, 65532.0) + 3u) & ~3u;  // whole four-entry chunks
+}
+
+void Batch::set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor) {
+    RH_REQUIRE(dim == 2 || dim == 3, "dimension must be 2 or 3");
+    RH_REQUIRE(size_class < 32, "size class is floor(log2 n), below 32");
+    RH_REQUIRE(variant <= 1, "list variant must be 0 (full) or 1 (half)");
+    RH_REQUIRE(cell_factor >= 1.0 && cell_factor <= 4.0, "cell factor must be in [1, 4]");
+    tuning_[dim - 2][size_class] = Tuning{variant, cell_factor};
 }
 
 uint64_t Batch::bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double skin) const {
@@ -824,7 +837,7 @@ void Batch::enqueue_iteration() {
     k_bb_count<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
     k_bb_scatter<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    k_bb_build<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_, 1u);
+    k_bb_build<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
     if (kinds_ & 1u) k_bb_step<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     if (kinds_ & 2u) k_bb_step<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);

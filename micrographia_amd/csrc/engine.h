@@ -58,7 +58,6 @@ struct Params {
     float kick_c[4];
     int32_t gravity[3];
     float half_skin2;
-    float list_r2_max;     // the largest list radius squared over the species pairs
     uint32_t block;        // half-list variant: particles per work-group block
     uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
     float cell_w[3];       // cell width (sigma), f32

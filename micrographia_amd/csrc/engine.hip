@@ -482,7 +482,6 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         uint32_t cc[D];
         float shift[3] = {0.f, 0.f, 0.f};
         bool skip = false;
-        float gap2 = 0.f;  // squared distance from the particle to the neighbour cell's box
 #pragma unroll
         for (int a = 0; a < D; a++) {
             int v = (int)c[a] + off[a];
@@ -493,13 +492,8 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             }
             cc[a] = (uint32_t)v;
             shift[a] = (float)off[a] * P.cell_w[a] - ri[a];
-            // the nearest face of a side cell, less 1e-3 sigma of slack for rounding
-            const float g = off[a] < 0 ? ri[a] : (off[a] > 0 ? P.cell_w[a] - ri[a] : 0.f);
-            const float gs = fmaxf(g - 1e-3f, 0.f);
-            gap2 = __builtin_fmaf(gs, gs, gap2);
         }
-        // A side cell whose box lies beyond the largest list radius holds no partner.
-        if (skip || gap2 > P.list_r2_max) continue;
+        if (skip) continue;
         const uint32_t key = morton<D>(cc);
         const uint32_t s0 = B.cell_start[key], e0 = B.cell_end[key];
         const float sx = shift[0], sy = shift[1], sz = shift[2];
@@ -1411,9 +1405,6 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
             P.pt.shift_64[k] = core::pair_shift(p);
         }
     for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
-    P.list_r2_max = 0.f;
-    for (uint32_t a = 0; a < sys.nspecies; a++)
-        for (uint32_t bb = 0; bb < sys.nspecies; bb++) P.list_r2_max = std::max(P.list_r2_max, P.pt.list_r2[a * 4 + bb]);
     P.cap = mode.kind == 1 ? mode.cap : 0;
     RH_REQUIRE(P.cap < 65536, "list capacity must be below 65536");
     for (int a = 0; a < D; a++) {
@@ -2011,7 +2002,6 @@ void Engine<D>::update_system(const SystemDesc& sys) {
         grew |= P_.pt.list_r2[k] > old.pt.list_r2[k];
         P_.pt.list_r2[k] = std::max(P_.pt.list_r2[k], old.pt.list_r2[k]);
     }
-    P_.list_r2_max = std::max(P_.list_r2_max, old.list_r2_max);
     if (grew) write_ctl(CTL_STALE, 1u);
     P_.arith = old.arith;
     sys_ = sys;

@@ -93,8 +93,11 @@ class Server:
     chunk of batch iterations (the admission and retirement granularity)."""
 
     def __init__(self, budget_bytes: int, max_systems: int = 4096, chunk_s: float = 0.02, skin: float = 0.3,
-                 workers: int = 8, keep_states: bool = False, big_n: int = 131072, active_atoms: int = 6_000_000):
+                 workers: int = 8, keep_states: bool = False, big_n: int = 131072, active_atoms: int = 6_000_000,
+                 tuning=None):
         self.batch = BatchEngine(budget_bytes=budget_bytes, max_systems=max_systems, graph_chunk=0)
+        if tuning is not None:
+            tuning.install(self.batch)  # an autotune.BatchTuner's per-bucket tile settings
         self.chunk_s = chunk_s
         self.skin = skin
         self.pool = cf.ThreadPoolExecutor(max_workers=workers)

@@ -65,10 +65,16 @@ def mixed_systems():
     return out
 
 
-@pytest.mark.parametrize("graph", [0, 50])
-def test_each_system_matches_the_oracle(graph):
+@pytest.mark.parametrize("graph,tiles", [(0, None), (50, None), (50, (0, 1.5)), (0, (1, 1.25))])
+def test_each_system_matches_the_oracle(graph, tiles):
+    """Every system's trajectory equals the oracle's, whatever the tile settings (list variant and
+    cell occupancy, as the autotuner installs them per size bucket)."""
     systems = mixed_systems()
     b = BatchEngine(budget_bytes=2 * GB, max_systems=64, graph_chunk=graph)
+    if tiles is not None:
+        for dim in (2, 3):
+            for cls in range(8, 16):
+                b.set_tuning(dim, cls, *tiles)
     ids = [b.add(sys, st, 150, skin=0.3) for sys, st in systems]
     assert all(i >= 0 for i in ids)
     b.step_all(150)
