@@ -28,7 +28,7 @@ BYTES_PER_ATOM_STEP = 72  # read+write 3 x i64 positions and 3 x i32 velocities
 # WRITE_SIZE, each calibrated against a plain 1 GiB copy (profiles/ keeps the summary and the
 # commands).  Keyed by (variant, block); a counter pass cannot run inside this process.
 MEASURED_TRAFFIC = {
-    (1, 1024): (2.553e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
+    (1, 2048): (2.676e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
 }
 
 
@@ -146,11 +146,12 @@ def cpu_baseline_c1(seed: int, steps: int = 1000):
     }
 
 
-def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool):
+def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool, table: str = None, default_tiles: bool = True):
     """The batched ensemble: `count` independent LJ systems (2D/3D 50/50, N log-uniform in
     [256, 65536], density U[0.3, 0.9], T U[0.5, 2.0], own seeds) advanced together by the batched
     engine in hipGraph chunks of `chunk` steps, against the same systems stepped one at a time by
-    the single-system engines (one system per launch)."""
+    the single-system engines (one system per launch).  `table`: a tuning file to read (buckets
+    found there are not timed again) and write; `default_tiles`: also time the untuned tiles."""
     import torch
 
     import micrographia_amd as rh
@@ -166,7 +167,7 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool)
     log(f"[bench] batched: {count} systems, {atoms} atoms, host init {time.perf_counter() - t0:.1f} s")
     # the autotuner: per (dimension, size class), list variant x cell occupancy, timed on up to 8
     # of the ensemble's own systems per bucket
-    tuner = BatchTuner(per_bucket=8)
+    tuner = BatchTuner(path=table, per_bucket=8)
     tuner.tune(built)
     log(f"[bench] batched: tuned {len(tuner.table)} buckets in {tuner.seconds:.1f} s")
 
@@ -188,7 +189,7 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool)
         b.close()
         return el, need
 
-    el_default, _ = run_ensemble(False)
+    el_default = run_ensemble(False)[0] if default_tiles else None
     el, need = run_ensemble(True)
     out = {
         "workload": f"{count} independent LJ systems (2D/3D 50/50, N log-uniform [256, 65536], rho U[0.3,0.9], T U[0.5,2.0], rc 2.5, skin 0.3, dt 0.005), meta-seed {meta_seed}, hipGraph chunks of {chunk} steps",
@@ -204,7 +205,7 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool)
         "autotuned": {
             "table": {k: {"variant": v[0], "cell_factor": v[1]} for k, v in sorted(tuner.table.items())},
             "tuning_seconds": tuner.seconds,
-            "atom_steps_per_s_default_tiles": atoms * steps / el_default,
+            "atom_steps_per_s_default_tiles": atoms * steps / el_default if el_default else None,
         },
     }
     if plain:
@@ -281,9 +282,11 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-n", type=int, default=262144, help="atoms in the CPU-baseline sample")
     ap.add_argument("--variant", type=int, default=1, help="0 full Verlet list, 1 half list in blocks")
-    ap.add_argument("--block", type=int, default=1024, help="half-list block size (256, 512, 1024)")
+    ap.add_argument("--block", type=int, default=2048, help="half-list block size (256, 512, 1024, 2048)")
     ap.add_argument("--batch-systems", type=int, default=512, help="systems in the batched leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--batch-table", default=None, help="batched tuning table to read and write (JSON)")
+    ap.add_argument("--no-batch-default", dest="batch_default", action="store_false", help="skip the batched run with untuned tiles")
     ap.add_argument("--serve-duration", type=float, default=0.2, help="seconds of Poisson arrivals in the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-plain", action="store_true", help="skip the one-system-at-a-time comparisons")
@@ -418,7 +421,7 @@ def main() -> int:
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = args.calibrate_copy << 20
     if args.batch_systems:
-        out["batched"] = batched_leg(args.batch_systems, args.batch_steps, 100, 2026, not args.no_plain)
+        out["batched"] = batched_leg(args.batch_systems, args.batch_steps, 100, 2026, not args.no_plain, args.batch_table, args.batch_default)
     if args.serve_duration > 0:
         out["serving"] = serving_leg(args.serve_rate, args.serve_duration, 7, not args.no_plain)
     if not args.no_cpu_baseline:

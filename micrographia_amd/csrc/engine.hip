@@ -559,8 +559,13 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     B.nbr_count[i] = na | (nb << 16);
 }
 
+// A block of BS particles runs on min(BS, 1024) threads: with BS = 2048 each thread takes
+// particles t and t + 1024 in turn (fewer pairs cross blocks, so fewer are evaluated twice), its
+// own kicks then kept in LDS beside its partners' instead of in registers.
 template <int D, bool MULTI, int BS, bool BOOK>
-__global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
+__global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) void k_step_half(Params<D> P, Bufs<D> B) {
+    constexpr uint32_t NT = BS > 1024 ? 1024u : (uint32_t)BS;
+    constexpr uint32_t PER = (uint32_t)BS / NT;
     const uint32_t* ctl = B.ctl;
     if (!step_runs(ctl)) return;
     __shared__ uint32_t acc[D][BS];
@@ -568,134 +573,166 @@ __global__ __launch_bounds__(BS) void k_step_half(Params<D> P, Bufs<D> B) {
     __shared__ double lpos[D][BS];
     const uint32_t t = threadIdx.x;
     const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
-    const uint32_t i = base + t;
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
-    const bool live = i < P.n;
-    double xi[D];
     uint32_t k[D];
+    double xi[D];
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-        xi[a] = live ? px[a][i] : 0.0;
-        k[a] = 0u;
-        acc[a][t] = 0u;
-        lpos[a][t] = xi[a];
+    for (uint32_t r = 0; r < PER; r++) {
+        const uint32_t l = t + r * NT;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            const double x = base + l < P.n ? px[a][base + l] : 0.0;
+            if (r == 0) xi[a] = x;
+            lpos[a][l] = x;
+            acc[a][l] = 0u;
+        }
     }
     __syncthreads();
     const uint32_t* sp = B.species[pm];
     const PairTable* __restrict__ pt = B.pt;
-    uint32_t si = 0;
-    float c = P.kick_c[0];
-    if (live) {
-        if (MULTI) {
-            si = sp[i];
-            c = pt->kick_c[si];
+    for (uint32_t r = 0; r < PER; r++) {
+        const uint32_t i = base + t + r * NT;
+        const bool live = i < P.n;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            if (r > 0) xi[a] = lpos[a][t + r * NT];
+            k[a] = 0u;
         }
-        const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
-        const uint32_t cnt = B.nbr_count[i];
-        const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
-        const size_t stride = P.n;
-        const bool rc = P.range_check != 0;
-        // The list is read a four-entry chunk (16 B) at a time, the next chunk in flight while
-        // this one is evaluated; the positions of a chunk's four neighbours are loaded together
-        // before any of its pairs is evaluated.  One loop walks the in-block chunks (front of
-        // the row; both ends' kicks, the partner's into LDS) and then the cross-block chunks
-        // (back of the row; this end only): a wave then runs as many chunks as its longest
-        // list, however each lane's list divides between the two kinds.
-        const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr) + i;
-        const uint32_t cap4 = P.cap / 4u;
-        const uint32_t ca = (na + 3u) / 4u, nch = ca + (nb + 3u) / 4u;
-        auto chunk_at = [&](uint32_t ch) -> uint4 {
-            return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
-        };
-        uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(i, i, i, i);
-        for (uint32_t ch = 0; ch < nch; ch++) {
-            const uint4 q = q_next;
-            if (ch + 1 < nch) q_next = chunk_at(ch + 1);
-            const bool inblock = ch < ca;
-            const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);  // entries: min(m, 4)
-            const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-            double xs[4][D];
-            if (inblock) {
+        uint32_t si = 0;
+        float c = P.kick_c[0];
+        if (live) {
+            if (MULTI) {
+                si = sp[i];
+                c = pt->kick_c[si];
+            }
+            const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
+            const uint32_t cnt = B.nbr_count[i];
+            const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
+            const size_t stride = P.n;
+            const bool rc = P.range_check != 0;
+            // The list is read a four-entry chunk (16 B) at a time, the next chunk in flight while
+            // this one is evaluated; the positions of a chunk's four neighbours are loaded together
+            // before any of its pairs is evaluated.  One loop walks the in-block chunks (front of
+            // the row; both ends' kicks, the partner's into LDS) and then the cross-block chunks
+            // (back of the row; this end only): a wave then runs as many chunks as its longest
+            // list, however each lane's list divides between the two kinds.
+            const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr) + i;
+            const uint32_t cap4 = P.cap / 4u;
+            const uint32_t ca = (na + 3u) / 4u, nch = ca + (nb + 3u) / 4u;
+            auto chunk_at = [&](uint32_t ch) -> uint4 {
+                return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
+            };
+            uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(i, i, i, i);
+            for (uint32_t ch = 0; ch < nch; ch++) {
+                const uint4 q = q_next;
+                if (ch + 1 < nch) q_next = chunk_at(ch + 1);
+                const bool inblock = ch < ca;
+                const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);  // entries: min(m, 4)
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+                double xs[4][D];
+                if (inblock) {
 #pragma unroll
-                for (int e = 0; e < 4; e++)
+                    for (int e = 0; e < 4; e++)
 #pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
-            } else {
+                        for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if constexpr (D == 3) {
+                            const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e]];
+                            xs[e][0] = q4.x;
+                            xs[e][1] = q4.y;
+                            xs[e][2] = q4.z;
+                        } else {
+                            const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e]];
+                            xs[e][0] = q2.x;
+                            xs[e][1] = q2.y;
+                        }
+                    }
+                }
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    if constexpr (D == 3) {
-                        const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e]];
-                        xs[e][0] = q4.x;
-                        xs[e][1] = q4.y;
-                        xs[e][2] = q4.z;
+                    if ((uint32_t)e >= m) break;
+                    const uint32_t j = js[e];
+                    float f[D];
+                    double d[D];
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        d[a] = separation<D>(P, xi[a], xs[e][a], a);
+                        f[a] = sep_f_in_range(d[a]);
+                    }
+                    if (rc) {
+                        bool ok = true;
+#pragma unroll
+                        for (int a = 0; a < D; a++) ok &= sep_in_range(d[a]);
+                        if (!ok) continue;
+                    }
+                    int32_t kj[D];
+                    bool hit;
+                    if constexpr (MULTI) {
+                        uint32_t sj = sp[j];
+                        uint32_t p = si * 4 + sj;
+                        hit = pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
                     } else {
-                        const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e]];
-                        xs[e][0] = q2.x;
-                        xs[e][1] = q2.y;
+                        hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                    }
+                    if (inblock && hit) {
+                        const uint32_t lj = j - base;
+#pragma unroll
+                        for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
                     }
                 }
             }
+            if constexpr (PER > 1) {
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if ((uint32_t)e >= m) break;
-                const uint32_t j = js[e];
-                float f[D];
-                double d[D];
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    d[a] = separation<D>(P, xi[a], xs[e][a], a);
-                    f[a] = sep_f_in_range(d[a]);
-                }
-                if (rc) {
-                    bool ok = true;
-#pragma unroll
-                    for (int a = 0; a < D; a++) ok &= sep_in_range(d[a]);
-                    if (!ok) continue;
-                }
-                int32_t kj[D];
-                bool hit;
-                if constexpr (MULTI) {
-                    uint32_t sj = sp[j];
-                    uint32_t p = si * 4 + sj;
-                    hit = pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
-                } else {
-                    hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                }
-                if (inblock && hit) {
-                    const uint32_t lj = j - base;
-#pragma unroll
-                    for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
-                }
+                for (int a = 0; a < D; a++) atomicAdd(&acc[a][t + r * NT], k[a]);
             }
         }
     }
     __syncthreads();
-    if (!live) return;
-    float wf[2 * D] = {};
-    if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
+    for (uint32_t r = 0; r < PER; r++) {
+        const uint32_t l = t + r * NT;
+        const uint32_t i = base + l;
+        if (i >= P.n) break;
+        uint32_t si = 0;
+        float c = P.kick_c[0];
+        if (MULTI) {
+            si = sp[i];
+            c = pt->kick_c[si];
+        }
+        if constexpr (PER > 1) {
 #pragma unroll
-    for (int a = 0; a < D; a++) k[a] += acc[a][t];
-    int32_t v[D];
-    double x[D];
-    advance<D, BOOK>(P, B, i, pm, si, k, xi, ctl[CTL_CLOCK], wf, x, v);
-    float r2 = 0.0f;
-    bool far = false;
+            for (int a = 0; a < D; a++) {
+                xi[a] = lpos[a][l];
+                k[a] = 0u;
+            }
+        }
+        float wf[2 * D] = {};
+        if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-        B.pos[pp ^ 1][a][i] = x[a];
-        B.vel[pm][a][i] = v[a];
-        float d;
-        far |= !sep_f(separation<D>(P, B.ref[a][i], x[a], a), d);
-        r2 += d * d;
+        for (int a = 0; a < D; a++) k[a] += acc[a][l];
+        int32_t v[D];
+        double x[D];
+        advance<D, BOOK>(P, B, i, pm, si, k, xi, ctl[CTL_CLOCK], wf, x, v);
+        float r2 = 0.0f;
+        bool far = false;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            B.pos[pp ^ 1][a][i] = x[a];
+            B.vel[pm][a][i] = v[a];
+            float d;
+            far |= !sep_f(separation<D>(P, B.ref[a][i], x[a], a), d);
+            r2 += d * d;
+        }
+        if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[i] = make_double4(x[0], x[1], x[2], 0.0);
+        else reinterpret_cast<double2*>(B.aos[pp ^ 1])[i] = make_double2(x[0], x[1]);
+        bool stale = far || r2 >= P.half_skin2;
+        uint64_t bal = ballot64(stale);
+        if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&B.ctl[CTL_STALE_NEXT] == 0u)
+            atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
     }
-    if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[i] = make_double4(x[0], x[1], x[2], 0.0);
-    else reinterpret_cast<double2*>(B.aos[pp ^ 1])[i] = make_double2(x[0], x[1]);
-    bool stale = far || r2 >= P.half_skin2;
-    uint64_t bal = ballot64(stale);
-    if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&B.ctl[CTL_STALE_NEXT] == 0u)
-        atomicOr(&B.ctl[CTL_STALE_NEXT], 1u);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1522,6 +1559,9 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     if (mode.kind == 1) RH_REQUIRE(mode.skin >= 0.0 && mode.cap >= 1, "Verlet mode needs skin >= 0 and cap >= 1");
     for (uint64_t i = 0; i < n; i++) RH_REQUIRE(species[i] < sys.nspecies, "species out of range");
     std::memset(&B_, 0, sizeof B_);
+    // Large systems default to 2048-particle blocks (fewer pairs cross blocks); small ones keep
+    // enough blocks to fill the chip.
+    block_ = n >= (uint64_t(1) << 20) ? 2048u : 512u;
     build_params(sys, mode);
     RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (int p = 0; p < 2; p++) {
@@ -1816,18 +1856,21 @@ void Engine<D>::launch_step() {
         uint32_t nb = blocks_for(n_, bs);
 #define RH_STEP_HALF(BS)                                                                   \
     if (P_.ledger) {                                                                        \
-        if (multi) k_step_half<D, true, BS, true><<<nb, BS, 0, stream_>>>(P_, B_);          \
-        else k_step_half<D, false, BS, true><<<nb, BS, 0, stream_>>>(P_, B_);               \
+        if (multi) k_step_half<D, true, BS, true><<<nb, nt, 0, stream_>>>(P_, B_);          \
+        else k_step_half<D, false, BS, true><<<nb, nt, 0, stream_>>>(P_, B_);               \
     } else {                                                                                \
-        if (multi) k_step_half<D, true, BS, false><<<nb, BS, 0, stream_>>>(P_, B_);         \
-        else k_step_half<D, false, BS, false><<<nb, BS, 0, stream_>>>(P_, B_);              \
+        if (multi) k_step_half<D, true, BS, false><<<nb, nt, 0, stream_>>>(P_, B_);         \
+        else k_step_half<D, false, BS, false><<<nb, nt, 0, stream_>>>(P_, B_);              \
     }
+        const uint32_t nt = bs > 1024u ? 1024u : bs;
         if (bs == 256) {
             RH_STEP_HALF(256)
         } else if (bs == 512) {
             RH_STEP_HALF(512)
-        } else {
+        } else if (bs == 1024) {
             RH_STEP_HALF(1024)
+        } else {
+            RH_STEP_HALF(2048)
         }
 #undef RH_STEP_HALF
     }
@@ -1837,7 +1880,7 @@ void Engine<D>::launch_step() {
 template <int D>
 void Engine<D>::tune(int variant, uint32_t block) {
     RH_REQUIRE(variant == 0 || variant == 1, "kernel variant must be 0 (full list) or 1 (half list)");
-    RH_REQUIRE(block == 256 || block == 512 || block == 1024, "block must be 256, 512 or 1024");
+    RH_REQUIRE(block == 256 || block == 512 || block == 1024 || block == 2048, "block must be 256, 512, 1024 or 2048");
     RH_HIP(hipStreamSynchronize(stream_));
     variant_ = variant;
     block_ = block;

@@ -31,7 +31,7 @@ class FakeEngine:
 @pytest.fixture
 def fake():
     FakeEngine.made = 0
-    FakeEngine.speed = {(1, 1024): 2e-4, (1, 512): 1e-4, (1, 256): 3e-4, (0, 256): 5e-4}
+    FakeEngine.speed = {(1, 2048): 4e-4, (1, 1024): 2e-4, (1, 512): 1e-4, (1, 256): 3e-4, (0, 256): 5e-4}
     return FakeEngine
 
 
@@ -48,7 +48,7 @@ def test_engine_tuner_picks_the_fastest_and_caches_per_bucket(fake, tmp_path):
     saved = json.load(open(path))["engine"]
     assert saved["3d/12/one"] == [1, 512] and saved["3d/13/one"] == [0, 256]
     again = autotune.EngineTuner(path=path)
-    assert again.choose(3, 5000, 1, fake) == (1, 512) and fake.made == made + 8
+    assert again.choose(3, 5000, 1, fake) == (1, 512) and fake.made == made + 2 * len(autotune.ENGINE_CANDIDATES)
 
 
 def test_size_classes():

@@ -54,12 +54,15 @@ def test_kicks_match_the_twin(pair):
     assert abs((peg - pe32) / pe32) < 1e-5
 
 
-@pytest.mark.parametrize("cells", [True, False])
-def test_trajectory_matches_the_twin(cells):
-    """Config 1's engine (2D LJ, rho 0.7, 64x64 lattice) for 300 steps, bit for bit."""
+@pytest.mark.parametrize("cells,block", [(True, None), (False, None), (False, 256), (False, 1024), (False, 2048)])
+def test_trajectory_matches_the_twin(cells, block):
+    """Config 1's engine (2D LJ, rho 0.7, 64x64 lattice) for 300 steps, bit for bit, with the
+    half list in work-groups of each size."""
     sys, s0 = fluid(4096, 0.7, rh.Pair.lj(2.5), 2)
     mode = rh.ForceMode.cell_list() if cells else rh.verlet_for(sys, s0.n())
     gpu = rh.GpuEngine(sys, s0, mode)
+    if block is not None:
+        gpu.tune(1, block)
     gpu.step(300)
     assert gpu.read_state() == oracle_run(sys, s0, 300)
 

@@ -63,7 +63,7 @@ MODES = {
 }
 
 # (kernel variant, block): the full list, and the half list at each block size
-VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024)]
+VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024), (1, 2048)]
 
 
 def engine(sys, st, mode, variant=1, block=512):
@@ -209,7 +209,8 @@ def test_mixture_pinned_gravity_soft_walls_match_the_twin():
     s.vel = rh.maxwell(sys, s.species, 1.0, 5)
     want = oracle_run(sys, s, 150)
     for mode, variant in ((rh.ForceMode.cell_list(), (0, 256)), (rh.verlet3_with_skin(sys, n, 0.3), (0, 256)),
-                          (rh.verlet3_with_skin(sys, n, 0.3), (1, 256)), (rh.verlet3_with_skin(sys, n, 0.3), (1, 1024))):
+                          (rh.verlet3_with_skin(sys, n, 0.3), (1, 256)), (rh.verlet3_with_skin(sys, n, 0.3), (1, 1024)),
+                          (rh.verlet3_with_skin(sys, n, 0.3), (1, 2048))):
         gpu = engine(sys, s, mode, *variant)
         gpu.step(150)
         assert gpu.read_state() == want
@@ -341,7 +342,7 @@ def test_hard_and_soft_wall_ledger_matches_the_twin():
     squeeze(st, sys, 2)
     want, led = oracle_run_ledger(sys, st, 300)
     assert min(led.hits[0], led.hits[5]) > 0 and np.isfinite(led.impulse).all()
-    for variant in VARIANTS[:2]:
+    for variant in VARIANTS[:2] + VARIANTS[-1:]:
         gpu = engine(sys, st, rh.verlet3_with_skin(sys, st.n(), 0.3), *variant)
         gpu.step(300)
         assert gpu.read_state() == want
