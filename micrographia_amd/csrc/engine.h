@@ -54,6 +54,8 @@ struct Params {
     int64_t half[3];
     double boxf[3];        // box length as f64 (positions are kept as exact f64 integers)
     double inv_box[3];     // 1 / box on periodic axes, 0 on walled ones
+    double box16[3];       // box / 16 and 16 / box: the same image on positions held in sixteenths
+    double inv_box16[3];
     double cell_scale[3];  // cells per position unit
     float kick_c[4];
     int32_t gravity[3];
@@ -242,6 +244,9 @@ class Engine {
     // The split path (pair kicks, per-particle terms, leapfrog as separate launches over a full
     // list): 2D systems with boundaries, and integer forces in Verlet mode.
     bool split_path() const { return P_.feat != 0u || P_.arith != 0u; }
+    // The half-list kernels run unless the split path does, or list separations may reach the
+    // 8 sigma range limit (then the full-list kernel keeps the reference's range test).
+    bool half_path() const { return variant_ == 1 && !split_path() && P_.range_check == 0u; }
     void enqueue_extras(uint32_t flags);
     void write_partners();
     void flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit);

@@ -303,8 +303,9 @@ __global__ __launch_bounds__(TPB) void k_gather(Params<D> P, Bufs<D> B, const ui
         if (write_ref) B.ref[a][k] = x;
         B.vel[pm ^ 1][a][k] = B.vel[pm][a][i];
     }
-    if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[k] = make_double4(xs[0], xs[1], xs[2], 0.0);
-    else reinterpret_cast<double2*>(B.aos[pp ^ 1])[k] = make_double2(xs[0], xs[1]);
+    // the packed copy holds sixteenths (the step kernel's gathers take them as they are)
+    if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[k] = make_double4(xs[0] * 0.0625, xs[1] * 0.0625, xs[2] * 0.0625, 0.0);
+    else reinterpret_cast<double2*>(B.aos[pp ^ 1])[k] = make_double2(xs[0] * 0.0625, xs[1] * 0.0625);
     uint32_t c[D];
     cell_of<D>(P, xs, c);
     float r[3] = {0.f, 0.f, 0.f};
@@ -569,7 +570,8 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
     const uint32_t* ctl = B.ctl;
     if (!step_runs(ctl)) return;
     __shared__ uint32_t acc[D][BS];
-    // The block's own positions: in-block neighbours are read from LDS, not gathered from memory.
+    // The block's own positions, in sixteenths (x / 16, exact): in-block neighbours are read from
+    // LDS, not gathered from memory.  The force loop works on sixteenths throughout.
     __shared__ double lpos[D][BS];
     const uint32_t t = threadIdx.x;
     const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
@@ -583,7 +585,7 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
         const uint32_t l = t + r * NT;
 #pragma unroll
         for (int a = 0; a < D; a++) {
-            const double x = base + l < P.n ? px[a][base + l] : 0.0;
+            const double x = base + l < P.n ? px[a][base + l] * 0.0625 : 0.0;
             if (r == 0) xi[a] = x;
             lpos[a][l] = x;
             acc[a][l] = 0u;
@@ -611,7 +613,6 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
             const uint32_t cnt = B.nbr_count[i];
             const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
             const size_t stride = P.n;
-            const bool rc = P.range_check != 0;
             // The list is read a four-entry chunk (16 B) at a time, the next chunk in flight while
             // this one is evaluated; the positions of a chunk's four neighbours are loaded together
             // before any of its pairs is evaluated.  One loop walks the in-block chunks (front of
@@ -656,19 +657,11 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
                 for (int e = 0; e < 4; e++) {
                     if ((uint32_t)e >= m) break;
                     const uint32_t j = js[e];
+                    // list separations stay inside the 8 sigma range (a system whose separations
+                    // may reach it runs the full-list kernel, which tests the range)
                     float f[D];
-                    double d[D];
 #pragma unroll
-                    for (int a = 0; a < D; a++) {
-                        d[a] = separation<D>(P, xi[a], xs[e][a], a);
-                        f[a] = sep_f_in_range(d[a]);
-                    }
-                    if (rc) {
-                        bool ok = true;
-#pragma unroll
-                        for (int a = 0; a < D; a++) ok &= sep_in_range(d[a]);
-                        if (!ok) continue;
-                    }
+                    for (int a = 0; a < D; a++) f[a] = sep_f16(separation16<D>(P, xi[a], xs[e][a], a));
                     int32_t kj[D];
                     bool hit;
                     if constexpr (MULTI) {
@@ -704,11 +697,10 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
         }
         if constexpr (PER > 1) {
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-                xi[a] = lpos[a][l];
-                k[a] = 0u;
-            }
+            for (int a = 0; a < D; a++) k[a] = 0u;
         }
+#pragma unroll
+        for (int a = 0; a < D; a++) xi[a] = (PER > 1 ? lpos[a][l] : xi[a]) * 16.0;
         float wf[2 * D] = {};
         if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
 #pragma unroll
@@ -726,8 +718,8 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
             far |= !sep_f(separation<D>(P, B.ref[a][i], x[a], a), d);
             r2 += d * d;
         }
-        if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[i] = make_double4(x[0], x[1], x[2], 0.0);
-        else reinterpret_cast<double2*>(B.aos[pp ^ 1])[i] = make_double2(x[0], x[1]);
+        if constexpr (D == 3) reinterpret_cast<double4*>(B.aos[pp ^ 1])[i] = make_double4(x[0] * 0.0625, x[1] * 0.0625, x[2] * 0.0625, 0.0);
+        else reinterpret_cast<double2*>(B.aos[pp ^ 1])[i] = make_double2(x[0] * 0.0625, x[1] * 0.0625);
         bool stale = far || r2 >= P.half_skin2;
         uint64_t bal = ballot64(stale);
         if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&B.ctl[CTL_STALE_NEXT] == 0u)
@@ -1388,6 +1380,8 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         P.half[a] = sys.box_len[a] / 2;
         P.boxf[a] = (double)sys.box_len[a];
         P.inv_box[a] = sys.walled[a] ? 0.0 : 1.0 / P.boxf[a];  // 0 on walled axes: no image
+        P.box16[a] = P.boxf[a] * 0.0625;
+        P.inv_box16[a] = P.inv_box[a] * 16.0;
         P.cell_scale[a] = (double)P.nc[a] / (double)sys.box_len[a];
         if (!sys.walled[a]) P.periodic |= 1u << a;
         else
@@ -1714,7 +1708,7 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     RH_HIP(hipGetLastError());
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
-        if (variant_ == 1 && !split_path()) {
+        if (half_path()) {
             bool wrap = false;
             for (int a = 0; a < D; a++) wrap |= P_.nc[a] == 1u && ((P_.periodic >> a) & 1u);
             if (wrap) {
@@ -1842,7 +1836,7 @@ void Engine<D>::launch_step() {
         }
     }
     const bool multi = P_.nspecies > 1;
-    if (variant_ == 0) {
+    if (!half_path()) {
         uint32_t nb = blocks_for(n_, TPB);
         if (P_.ledger) {
             if (multi) k_step_list<D, true, true><<<nb, TPB, 0, stream_>>>(P_, B_);
@@ -2040,6 +2034,9 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     P_.mbits = old.mbits;
     P_.cap = old.cap;
     P_.half_skin2 = old.half_skin2;
+    // Cutoffs only shrink, so a range test that was needed stays safe; keeping it keeps the list
+    // in the form it was built in (full lists for the range-tested kernel).
+    P_.range_check = old.range_check;
     bool grew = false;
     for (int k = 0; k < 16; k++) {
         grew |= P_.pt.list_r2[k] > old.pt.list_r2[k];

@@ -107,6 +107,19 @@ __device__ __forceinline__ bool sep_f(double d, float& f) {
 // The conversion alone, for list pairs whose separations are known to be in range.
 __device__ __forceinline__ float sep_f_in_range(double d) { return (float)__builtin_trunc(d * 0.0625) * 0x1p-28f; }
 
+// The same two steps on positions held in sixteenths (x / 16, exact for the integer positions).
+// Every f64 quantity is its unscaled value times 2^-4 exactly, and so is every rounding: d * inv
+// is the same product (2^-4 d times 2^4 inv), and the fma's single rounding scales with its
+// result.  So separation16(a / 16, b / 16) = separation(a, b) / 16 bit for bit, and
+// sep_f16(d / 16) = sep_f_in_range(d), one f64 multiply fewer per axis.
+template <int D, class PT>
+__device__ __forceinline__ double separation16(const PT& P, double a16, double b16, int ax) {
+    const double d = b16 - a16;
+    return __builtin_fma(-P.box16[ax], __builtin_rint(d * P.inv_box16[ax]), d);
+}
+
+__device__ __forceinline__ float sep_f16(double d16) { return (float)__builtin_trunc(d16) * 0x1p-28f; }
+
 __device__ __forceinline__ bool sep_in_range(double d) { return d >= -34359738368.0 && d < 34359738368.0; }
 
 template <int D, class PT>
@@ -321,7 +334,7 @@ __device__ __forceinline__ bool pair_kick2(const float f[D], float rc2, float si
             for (int ax = 0; ax < D; ax++) {
                 float r = __builtin_rintf(a * f[ax]);
                 ki[ax] += (uint32_t)cvt_sat_i32(r);
-                kj[ax] = cvt_sat_i32(-r);
+                kj[ax] = cvt_sat_i32_neg(r);
             }
         }
     }

@@ -44,6 +44,13 @@ __device__ __forceinline__ int32_t cvt_sat_i32(float x) {
     return r;
 }
 
+// cvt_sat_i32(-x), the negation taken by the instruction's source modifier.
+__device__ __forceinline__ int32_t cvt_sat_i32_neg(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32_e64 %0, -%1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));

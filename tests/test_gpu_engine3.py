@@ -106,6 +106,17 @@ def test_trajectory_is_bit_exact_with_the_f32_twin(mode, variant):
     assert got.hash() == want.hash()
 
 
+def test_long_cutoff_keeps_the_range_test(block=512):
+    """A cutoff of 7.5 sigma: list separations may reach the 8 sigma range of the f32
+    separation, so the engine keeps the reference's range test (the full-list kernel, whatever
+    variant is asked for); the trajectory matches the twin."""
+    sys, s0 = liquid(2000, 0.1, 1.0, walls=True)
+    sys = rh.System3.new(sys.box_len, 0.005, rh.Pair.lj(7.5), [1.0]).with_walls(2, rh.Wall.Hard, rh.Wall.Hard)
+    gpu = engine(sys, s0, rh.verlet3_with_skin(sys, s0.n(), 0.3), 1, block)
+    gpu.step(60)
+    assert gpu.read_state() == oracle_run(sys, s0, 60)
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_periodic_liquid_bit_exact_over_many_rebuilds(variant):
     sys, s0 = liquid(4000, 0.8442, 1.2)
