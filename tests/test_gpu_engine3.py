@@ -117,6 +117,18 @@ def test_long_cutoff_keeps_the_range_test(block=512):
     assert gpu.read_state() == oracle_run(sys, s0, 60)
 
 
+def test_heavy_species_match_the_twin():
+    """A species of mass 1e8 (kick factor about 1e-3: most of its kicks round to zero) beside
+    unit masses, bit for bit with the twin."""
+    sys, s0 = liquid(2000, 0.8, 1.0)
+    sys = rh.System3.new(sys.box_len, 0.005, rh.Pair.lj(2.5), [1.0, 1e8])
+    s0.species[::5] = 1
+    s0.vel = rh.maxwell(sys, s0.species, 1.0, 4)
+    gpu = engine(sys, s0, rh.verlet3_with_skin(sys, s0.n(), 0.3), 1, 512)
+    gpu.step(100)
+    assert gpu.read_state() == oracle_run(sys, s0, 100)
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_periodic_liquid_bit_exact_over_many_rebuilds(variant):
     sys, s0 = liquid(4000, 0.8442, 1.2)
