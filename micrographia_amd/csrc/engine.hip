@@ -446,8 +446,13 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
-// not wrap the stencil.  In-block pairs go to the owner's list from the front; pairs across
-// blocks go to both ends' lists from the back.
+// not wrap the stencil.  In-block pairs go to their owner only; pairs across blocks to both
+// ends.  A lane first writes its particle's row of four-entry chunks (in-block entries from the
+// front, cross-block ones from the back); at the end the wave packs its rows into its region of
+// the chunk stream, the step kernel's form: 64 cap/4 slots per 64-particle wave, chunks
+// back to back in lane order, each with a 16-bit tag (the owner's index in its block of P.block
+// particles in bits 0-10, entries - 1 in bits 11-12, in-block in bit 13), and the wave's chunk
+// count.
 template <int D, bool MULTI, bool WRAP>
 __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
@@ -471,7 +476,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     // c * n + i.  In-block entries fill chunks 0, 1, ... from the front; cross-block entries fill
     // chunks cap/4 - 1, cap/4 - 2, ... from the back.  A lane keeps the chunk it is filling in
     // registers and stores it whole (16 B) when it is full.
-    uint4* row = reinterpret_cast<uint4*>(B.nbr) + i;
+    uint4* row = reinterpret_cast<uint4*>(B.nbr_rows) + i;
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     // single-cell periodic axes: the nearest image of the one cell (cell_w is the box length)
     float wl[3] = {0.f, 0.f, 0.f};
@@ -558,11 +563,38 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     }
     if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
     B.nbr_count[i] = na | (nb << 16);
+    // Pack the wave's rows into its region, chunk k of every lane before chunk k + 1 of any:
+    // neighbouring slots then belong to different particles (the step kernel's neighbouring
+    // lanes add their kicks to different accumulators), and each round's stores are
+    // contiguous.  The lanes past the last particle have left.
+    const uint32_t mine = over ? 0u : ca + cb, lane = __lane_id();
+    const uint32_t wave = i >> 6, own = i & (P.block - 1u);
+    const size_t capw = 64u * cap4;
+    uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
+    uint16_t* metas = B.nbr_meta + (size_t)wave * capw;
+    uint32_t filled = 0u;
+    for (uint32_t k = 0;; k++) {
+        const bool has = k < mine;
+        const uint64_t bal = ballot64(has);
+        if (bal == 0ull) break;
+        if (has) {
+            const uint32_t s = filled + (uint32_t)__popcll(bal & lanemask_lt());
+            const bool in = k < ca;
+            const uint32_t left = in ? na - 4u * k : nb - 4u * (k - ca);
+            chunks[s] = row[(size_t)(in ? k : cap4 - 1u - (k - ca)) * n];
+            metas[s] = (uint16_t)(own | ((min(left, 4u) - 1u) << 11) | (in ? 1u << 13 : 0u));
+        }
+        filled += (uint32_t)__popcll(bal);
+    }
+    if (lane == (uint32_t)__ffsll((long long)ballot64(true)) - 1u) B.wave_chunks[wave] = filled;
 }
 
-// A block of BS particles runs on min(BS, 1024) threads: with BS = 2048 each thread takes
-// particles t and t + 1024 in turn (fewer pairs cross blocks, so fewer are evaluated twice), its
-// own kicks then kept in LDS beside its partners' instead of in registers.
+// A block of BS particles runs on min(BS, 1024) threads (with BS = 2048 each thread advances
+// particles t and t + 1024).  The block's list chunks (its build waves' regions, numbered in
+// order) are dealt out evenly: lane t takes chunks t, t + threads, ... whichever particles own
+// them, so a wave's lanes stay busy however unevenly the pairs divide between particles, and
+// neighbouring lanes read neighbouring chunks.  Every kick goes to the block's LDS
+// accumulators: the owner's after each chunk, an in-block partner's per pair.
 template <int D, bool MULTI, int BS, bool BOOK>
 __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) void k_step_half(Params<D> P, Bufs<D> B) {
     constexpr uint32_t NT = BS > 1024 ? 1024u : (uint32_t)BS;
@@ -574,114 +606,129 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
     // LDS, not gathered from memory.  The force loop works on sixteenths throughout.
     __shared__ double lpos[D][BS];
     const uint32_t t = threadIdx.x;
-    const uint32_t base = xcd_block(blockIdx.x, gridDim.x) * BS;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t base = blk * BS;
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
-    uint32_t k[D];
-    double xi[D];
 #pragma unroll
     for (uint32_t r = 0; r < PER; r++) {
         const uint32_t l = t + r * NT;
 #pragma unroll
         for (int a = 0; a < D; a++) {
-            const double x = base + l < P.n ? px[a][base + l] * 0.0625 : 0.0;
-            if (r == 0) xi[a] = x;
-            lpos[a][l] = x;
+            lpos[a][l] = (base + l < P.n ? px[a][base + l] * 0.0625 : 0.0);
             acc[a][l] = 0u;
         }
+    }
+    // the chunk counts of the block's NW list-build waves, scanned by the first wave
+    constexpr uint32_t NW = (uint32_t)BS / 64u;
+    __shared__ uint32_t wfirst[NW + 1];
+    if (t < 64u) {
+        const uint32_t v = t < NW ? B.wave_chunks[blk * NW + t] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (t >= (uint32_t)d) x += y;
+        }
+        if (t < NW) wfirst[t] = x - v;
+        if (t == NW - 1u) wfirst[NW] = x;
     }
     __syncthreads();
     const uint32_t* sp = B.species[pm];
     const PairTable* __restrict__ pt = B.pt;
-    for (uint32_t r = 0; r < PER; r++) {
-        const uint32_t i = base + t + r * NT;
-        const bool live = i < P.n;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-            if (r > 0) xi[a] = lpos[a][t + r * NT];
-            k[a] = 0u;
+    {
+        const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
+        // The block's chunks, numbered 0 .. nch-1 across its waves' regions in order: chunk g
+        // lies in region w (wfirst[w] <= g < wfirst[w + 1]) at slot g - wfirst[w].
+        const size_t capw = 64u * (P.cap / 4u);
+        const uint4* __restrict__ chunks = reinterpret_cast<const uint4*>(B.nbr) + (size_t)blk * NW * capw;
+        const uint16_t* __restrict__ metas = B.nbr_meta + (size_t)blk * NW * capw;
+        const uint32_t nch = wfirst[NW];
+        uint32_t g = t, w = 0u;
+        auto slot = [&](uint32_t gg) -> size_t {
+            while (gg >= wfirst[w + 1u]) w++;
+            return (size_t)w * capw + (gg - wfirst[w]);
+        };
+        // the next chunk and its tag are in flight while this one is evaluated
+        uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t m_next = 0u;
+        if (g < nch) {
+            const size_t s0 = slot(g);
+            q_next = chunks[s0];
+            m_next = metas[s0];
         }
-        uint32_t si = 0;
-        float c = P.kick_c[0];
-        if (live) {
+        for (; g < nch; g += NT) {
+            const uint4 q = q_next;
+            const uint32_t tag = m_next;
+            if (g + NT < nch) {
+                const size_t s1 = slot(g + NT);
+                q_next = chunks[s1];
+                m_next = metas[s1];
+            }
+            const uint32_t own = tag & 0x7ffu, m = ((tag >> 11) & 3u) + 1u;
+            const bool inblock = (tag >> 13) & 1u;
+            const uint32_t i = base + own;
+            double xi[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
+            uint32_t si = 0;
+            float c = P.kick_c[0];
             if (MULTI) {
                 si = sp[i];
                 c = pt->kick_c[si];
             }
-            const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
-            const uint32_t cnt = B.nbr_count[i];
-            const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
-            const size_t stride = P.n;
-            // The list is read a four-entry chunk (16 B) at a time, the next chunk in flight while
-            // this one is evaluated; the positions of a chunk's four neighbours are loaded together
-            // before any of its pairs is evaluated.  One loop walks the in-block chunks (front of
-            // the row; both ends' kicks, the partner's into LDS) and then the cross-block chunks
-            // (back of the row; this end only): a wave then runs as many chunks as its longest
-            // list, however each lane's list divides between the two kinds.
-            const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr) + i;
-            const uint32_t cap4 = P.cap / 4u;
-            const uint32_t ca = (na + 3u) / 4u, nch = ca + (nb + 3u) / 4u;
-            auto chunk_at = [&](uint32_t ch) -> uint4 {
-                return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
-            };
-            uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(i, i, i, i);
-            for (uint32_t ch = 0; ch < nch; ch++) {
-                const uint4 q = q_next;
-                if (ch + 1 < nch) q_next = chunk_at(ch + 1);
-                const bool inblock = ch < ca;
-                const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);  // entries: min(m, 4)
-                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-                double xs[4][D];
-                if (inblock) {
+            uint32_t k[D];
 #pragma unroll
-                    for (int e = 0; e < 4; e++)
+            for (int a = 0; a < D; a++) k[a] = 0u;
+            const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+            double xs[4][D];
+            if (inblock) {
 #pragma unroll
-                        for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
-                } else {
+                for (int e = 0; e < 4; e++)
 #pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        if constexpr (D == 3) {
-                            const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e]];
-                            xs[e][0] = q4.x;
-                            xs[e][1] = q4.y;
-                            xs[e][2] = q4.z;
-                        } else {
-                            const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e]];
-                            xs[e][0] = q2.x;
-                            xs[e][1] = q2.y;
-                        }
-                    }
-                }
+                    for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
+            } else {
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    if ((uint32_t)e >= m) break;
-                    const uint32_t j = js[e];
-                    // list separations stay inside the 8 sigma range (a system whose separations
-                    // may reach it runs the full-list kernel, which tests the range)
-                    float f[D];
-#pragma unroll
-                    for (int a = 0; a < D; a++) f[a] = sep_f16(separation16<D>(P, xi[a], xs[e][a], a));
-                    int32_t kj[D];
-                    bool hit;
-                    if constexpr (MULTI) {
-                        uint32_t sj = sp[j];
-                        uint32_t p = si * 4 + sj;
-                        hit = pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+                    if constexpr (D == 3) {
+                        const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e]];
+                        xs[e][0] = q4.x;
+                        xs[e][1] = q4.y;
+                        xs[e][2] = q4.z;
                     } else {
-                        hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                    }
-                    if (inblock && hit) {
-                        const uint32_t lj = j - base;
-#pragma unroll
-                        for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+                        const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e]];
+                        xs[e][0] = q2.x;
+                        xs[e][1] = q2.y;
                     }
                 }
             }
-            if constexpr (PER > 1) {
 #pragma unroll
-                for (int a = 0; a < D; a++) atomicAdd(&acc[a][t + r * NT], k[a]);
+            for (int e = 0; e < 4; e++) {
+                if ((uint32_t)e >= m) break;
+                const uint32_t j = js[e];
+                // list separations stay inside the 8 sigma range (a system whose separations
+                // may reach it runs the full-list kernel, which tests the range)
+                float f[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) f[a] = sep_f16(separation16<D>(P, xi[a], xs[e][a], a));
+                int32_t kj[D];
+                bool hit;
+                if constexpr (MULTI) {
+                    uint32_t sj = sp[j];
+                    uint32_t p = si * 4 + sj;
+                    hit = pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+                } else {
+                    hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+                }
+                if (inblock && hit) {
+                    const uint32_t lj = j - base;
+#pragma unroll
+                    for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+                }
             }
+#pragma unroll
+            for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
         }
     }
     __syncthreads();
@@ -695,12 +742,13 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
             si = sp[i];
             c = pt->kick_c[si];
         }
-        if constexpr (PER > 1) {
+        uint32_t k[D];
+        double xi[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) k[a] = 0u;
+        for (int a = 0; a < D; a++) {
+            k[a] = 0u;
+            xi[a] = lpos[a][l] * 16.0;
         }
-#pragma unroll
-        for (int a = 0; a < D; a++) xi[a] = (PER > 1 ? lpos[a][l] : xi[a]) * 16.0;
         float wf[2 * D] = {};
         if (P.soft_walls) soft_walls<D>(P, xi, c, k, nullptr, wf);
 #pragma unroll
@@ -1533,15 +1581,23 @@ void Engine<D>::collect_timers() {
 
 template <int D>
 void Engine<D>::alloc_list(uint32_t cap) {
-    if (B_.nbr) {
-        auto it = std::find(allocs_.begin(), allocs_.end(), (void*)B_.nbr);
+    auto release = [&](void* p) {
+        if (!p) return;
+        auto it = std::find(allocs_.begin(), allocs_.end(), p);
         if (it != allocs_.end()) allocs_.erase(it);
-        RH_HIP(hipFree(B_.nbr));
-        B_.nbr = nullptr;
-    }
+        RH_HIP(hipFree(p));
+    };
+    release(B_.nbr);
+    release(B_.nbr_meta);
+    release(B_.nbr_rows);
     cap = (cap + 3u) & ~3u;  // whole four-entry chunks (the half list stores uint4 chunks)
     P_.cap = cap;
-    B_.nbr = (uint32_t*)dalloc((size_t)cap * n_ * sizeof(uint32_t));
+    // The full lists and the half list's rows take cap entries per particle; the half list's
+    // chunk stream takes cap / 4 chunks per particle slot, for every block size up to 2048.
+    const size_t slots = (n_ + 2047u) & ~size_t(2047);
+    B_.nbr = (uint32_t*)dalloc((size_t)cap * slots * sizeof(uint32_t));
+    B_.nbr_meta = (uint16_t*)dalloc((size_t)(cap / 4u) * slots * sizeof(uint16_t));
+    B_.nbr_rows = (uint32_t*)dalloc((size_t)cap * n_ * sizeof(uint32_t));
 }
 
 template <int D>
@@ -1572,6 +1628,13 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
         B_.kicks[a] = (int32_t*)dalloc(n * sizeof(int32_t));
     }
     B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
+    {
+        // one count per 64-particle wave, for every block size up to 2048; waves past the last
+        // particle stay empty
+        const size_t waves = ((n + 2047u) & ~size_t(2047)) / 64u;
+        B_.wave_chunks = (uint32_t*)dalloc(waves * sizeof(uint32_t));
+        RH_HIP(hipMemset(B_.wave_chunks, 0, waves * sizeof(uint32_t)));
+    }
     B_.rel = (float4*)dalloc(n * sizeof(float4));
     B_.cell_start = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.cell_end = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
@@ -1749,7 +1812,6 @@ template <int D>
 void Engine<D>::step_list(uint64_t steps) {
     constexpr int NSLOTS = 8, DEPTH = 3;
     uint64_t left = steps;
-    uint32_t nb = blocks_for(n_, TPB);
     while (left > 0) {
         uint32_t add = (uint32_t)std::min<uint64_t>(left, 1u << 30);
         left -= add;
