@@ -120,11 +120,10 @@ struct Bufs {
     // line) instead of one 8-byte word from each axis array.
     double* aos[2];
     // Half list: a stream of four-entry chunks in B.nbr, each 64-particle wave of the build
-    // filling a region of 64 cap/4 slots; per chunk a tag (owner in its block | (entries - 1)
-    // << 11 | in-block << 13), and per wave the number of chunks it wrote.
+    // filling a region of 64 cap/4 slots from its start; per chunk a tag (owner in its block |
+    // (entries - 1) << 11 | in-block << 13), and per wave the number of chunks in its region.
     uint16_t* nbr_meta;
     uint32_t* wave_chunks;
-    uint32_t* nbr_rows;     // half list build: each particle's row, chunk c at c n + i, before packing
     int32_t* vel[2][D];
     uint32_t* species[2];
     uint32_t* ids[2];

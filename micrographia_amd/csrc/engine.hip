@@ -447,12 +447,13 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
 // not wrap the stencil.  In-block pairs go to their owner only; pairs across blocks to both
-// ends.  A lane first writes its particle's row of four-entry chunks (in-block entries from the
-// front, cross-block ones from the back); at the end the wave packs its rows into its region of
-// the chunk stream, the step kernel's form: 64 cap/4 slots per 64-particle wave, chunks
-// back to back in lane order, each with a 16-bit tag (the owner's index in its block of P.block
-// particles in bits 0-10, entries - 1 in bits 11-12, in-block in bit 13), and the wave's chunk
-// count.
+// ends.  The list is a stream of four-entry chunks (16 B), each with a 16-bit tag: the owner's
+// index in its block of P.block particles (bits 0-10), entries - 1 (bits 11-12) and whether the
+// entries are in that block (bit 13).  Every 64-particle wave has a region of 64 cap/4 slots:
+// a lane's k-th chunk goes to slot 64 k + lane as it fills, so rows of the region below the
+// wave's shortest list are full, and at the end the wave packs the ragged rows above them down
+// to close the gaps and records its chunk count.  Neighbouring slots then belong to different
+// particles (the step kernel's neighbouring lanes add to different accumulators).
 template <int D, bool MULTI, bool WRAP>
 __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
@@ -470,13 +471,22 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     const float list_r2 = B.pt->list_r2[0];
     uint32_t c[D];
     cell_of<D>(P, xi, c);
-    const uint32_t cap4 = P.cap / 4u, n = P.n;
+    const uint32_t cap4 = P.cap / 4u;
     const uint32_t bshift = __ffs(P.block) - 1;  // the block is a power of two
-    // The list is stored in chunks of four entries: chunk c of particle i is the uint4 at
-    // c * n + i.  In-block entries fill chunks 0, 1, ... from the front; cross-block entries fill
-    // chunks cap/4 - 1, cap/4 - 2, ... from the back.  A lane keeps the chunk it is filling in
-    // registers and stores it whole (16 B) when it is full.
-    uint4* row = reinterpret_cast<uint4*>(B.nbr_rows) + i;
+    // A lane fills one in-block and one cross-block chunk in registers and stores each whole
+    // (16 B) with its tag when it is full.
+    const uint32_t lane = __lane_id(), wave = i >> 6, own = i & (P.block - 1u);
+    const size_t capw = 64u * cap4;
+    uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
+    uint16_t* metas = B.nbr_meta + (size_t)wave * capw;
+    uint32_t made = 0u;  // chunks this lane has stored
+    auto put = [&](const uint4& q, uint32_t tag) {
+        if (made < cap4) {
+            chunks[64u * made + lane] = q;
+            metas[64u * made + lane] = (uint16_t)tag;
+        }
+        made++;
+    };
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     // single-cell periodic axes: the nearest image of the one cell (cell_w is the box length)
     float wl[3] = {0.f, 0.f, 0.f};
@@ -541,48 +551,42 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
                 if ((inb >> b) & 1u) {
                     put4(fa, na, j);
                     na++;
-                    if ((na & 3u) == 0u && na / 4u <= cap4) row[(size_t)(na / 4u - 1u) * n] = fa;
+                    if ((na & 3u) == 0u) put(fa, own | (3u << 11) | (1u << 13));
                 } else {
                     put4(fb, nb, j);
                     nb++;
-                    if ((nb & 3u) == 0u && nb / 4u <= cap4) row[(size_t)(cap4 - nb / 4u) * n] = fb;
+                    if ((nb & 3u) == 0u) put(fb, own | (3u << 11));
                 }
             }
         }
     }
     // the partial last chunks; a list whose chunks do not fit is flagged and regrown
+    if (na & 3u) put(fa, own | (((na & 3u) - 1u) << 11) | (1u << 13));
+    if (nb & 3u) put(fb, own | (((nb & 3u) - 1u) << 11));
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
     const bool over = ca + cb > cap4;
-    if (!over) {
-        if (na & 3u) row[(size_t)(ca - 1u) * n] = fa;
-        if (nb & 3u) row[(size_t)(cap4 - cb) * n] = fb;
-    }
     if (over) {
         atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         na = nb = 0;
     }
     if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
     B.nbr_count[i] = na | (nb << 16);
-    // Pack the wave's rows into its region, chunk k of every lane before chunk k + 1 of any:
-    // neighbouring slots then belong to different particles (the step kernel's neighbouring
-    // lanes add their kicks to different accumulators), and each round's stores are
-    // contiguous.  The lanes past the last particle have left.
-    const uint32_t mine = over ? 0u : ca + cb, lane = __lane_id();
-    const uint32_t wave = i >> 6, own = i & (P.block - 1u);
-    const size_t capw = 64u * cap4;
-    uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
-    uint16_t* metas = B.nbr_meta + (size_t)wave * capw;
+    // Close the gaps: row k's chunks move down to slots filled .. filled + (lanes with a k-th
+    // chunk) - 1.  A chunk never moves up (filled <= 64 k), and row k's writes stay below row
+    // k + 1, so every chunk is read before its slot is reused.  Full rows do not move.  The
+    // lanes past the last particle have left.
+    const uint32_t mine = over ? 0u : ca + cb;
     uint32_t filled = 0u;
     for (uint32_t k = 0;; k++) {
         const bool has = k < mine;
         const uint64_t bal = ballot64(has);
         if (bal == 0ull) break;
-        if (has) {
-            const uint32_t s = filled + (uint32_t)__popcll(bal & lanemask_lt());
-            const bool in = k < ca;
-            const uint32_t left = in ? na - 4u * k : nb - 4u * (k - ca);
-            chunks[s] = row[(size_t)(in ? k : cap4 - 1u - (k - ca)) * n];
-            metas[s] = (uint16_t)(own | ((min(left, 4u) - 1u) << 11) | (in ? 1u << 13 : 0u));
+        const uint32_t s = filled + (uint32_t)__popcll(bal & lanemask_lt());
+        if (has && s != 64u * k + lane) {
+            const uint4 q = chunks[64u * k + lane];
+            const uint16_t tag = metas[64u * k + lane];
+            chunks[s] = q;
+            metas[s] = tag;
         }
         filled += (uint32_t)__popcll(bal);
     }
@@ -1589,15 +1593,13 @@ void Engine<D>::alloc_list(uint32_t cap) {
     };
     release(B_.nbr);
     release(B_.nbr_meta);
-    release(B_.nbr_rows);
     cap = (cap + 3u) & ~3u;  // whole four-entry chunks (the half list stores uint4 chunks)
     P_.cap = cap;
-    // The full lists and the half list's rows take cap entries per particle; the half list's
-    // chunk stream takes cap / 4 chunks per particle slot, for every block size up to 2048.
+    // The full lists take cap entries per particle; the half list's chunk stream takes cap / 4
+    // chunks per particle slot, for every block size up to 2048.
     const size_t slots = (n_ + 2047u) & ~size_t(2047);
     B_.nbr = (uint32_t*)dalloc((size_t)cap * slots * sizeof(uint32_t));
     B_.nbr_meta = (uint16_t*)dalloc((size_t)(cap / 4u) * slots * sizeof(uint16_t));
-    B_.nbr_rows = (uint32_t*)dalloc((size_t)cap * n_ * sizeof(uint32_t));
 }
 
 template <int D>
