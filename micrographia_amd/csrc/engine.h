@@ -119,10 +119,9 @@ struct Bufs {
     // the reorder and the half-list step: a neighbour gather reads one 32-byte record (one cache
     // line) instead of one 8-byte word from each axis array.
     double* aos[2];
-    // Half list: a stream of four-entry chunks in B.nbr, each 64-particle wave of the build
-    // filling a region of 64 cap/4 slots from its start; per chunk a tag (owner in its block |
-    // (entries - 1) << 11 | in-block << 13), and per wave the number of chunks in its region.
-    uint16_t* nbr_meta;
+    // Half list: a stream of tagged four-entry chunks in B.nbr, each 64-particle wave of the build
+    // filling a region of 64 cap/4 slots from its start, and per wave the number of chunks in
+    // its region.
     uint32_t* wave_chunks;
     int32_t* vel[2][D];
     uint32_t* species[2];
@@ -251,7 +250,8 @@ class Engine {
     bool split_path() const { return P_.feat != 0u || P_.arith != 0u; }
     // The half-list kernels run unless the split path does, or list separations may reach the
     // 8 sigma range limit (then the full-list kernel keeps the reference's range test).
-    bool half_path() const { return variant_ == 1 && !split_path() && P_.range_check == 0u; }
+    // (Half-list chunks carry their tag in the top four bits of each index: below 2^28 particles.)
+    bool half_path() const { return variant_ == 1 && !split_path() && P_.range_check == 0u && n_ < (uint64_t(1) << 28); }
     void enqueue_extras(uint32_t flags);
     void write_partners();
     void flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit);

@@ -443,17 +443,26 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // Half-list variant
 // ---------------------------------------------------------------------------------------------
 
+// A half-list chunk's tag in the top four bits of its words: owner bits 0-3 in x, 4-7 in y, 8-10
+// in z (bit 31 of z: in-block), entries - 1 in w.
+__device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entries, bool in) {
+    const uint32_t lo = 0x0fffffffu;
+    return make_uint4((q.x & lo) | (own << 28), (q.y & lo) | ((own >> 4) << 28), (q.z & lo) | ((own >> 8) << 28) | (in ? 1u << 31 : 0u),
+                      (q.w & lo) | ((entries - 1u) << 28));
+}
+
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
 // not wrap the stencil.  In-block pairs go to their owner only; pairs across blocks to both
-// ends.  The list is a stream of four-entry chunks (16 B), each with a 16-bit tag: the owner's
-// index in its block of P.block particles (bits 0-10), entries - 1 (bits 11-12) and whether the
-// entries are in that block (bit 13).  Every 64-particle wave has a region of 64 cap/4 slots:
-// a lane's k-th chunk goes to slot 64 k + lane as it fills, so rows of the region below the
-// wave's shortest list are full, and at the end the wave packs the ragged rows above them down
-// to close the gaps and records its chunk count.  Neighbouring slots then belong to different
-// particles (the step kernel's neighbouring lanes add to different accumulators).
+// ends.  The list is a stream of four-entry chunks (16 B).  Entries are particle indices below
+// 2^28; the top four bits of each word carry the chunk's tag (tag_chunk): the owner's index in
+// its block of P.block particles, the entry count and whether the entries are in that block.
+// Every 64-particle wave has a region of 64 cap/4 slots: a lane's k-th chunk goes to slot
+// 64 k + lane as it fills, so rows of the region below the wave's shortest list are full, and
+// at the end the wave packs the ragged rows above them down to close the gaps and records its
+// chunk count.  Neighbouring slots then belong to different particles (the step kernel's
+// neighbouring lanes add to different accumulators).
 template <int D, bool MULTI, bool WRAP>
 __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
@@ -478,13 +487,9 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     const uint32_t lane = __lane_id(), wave = i >> 6, own = i & (P.block - 1u);
     const size_t capw = 64u * cap4;
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
-    uint16_t* metas = B.nbr_meta + (size_t)wave * capw;
     uint32_t made = 0u;  // chunks this lane has stored
-    auto put = [&](const uint4& q, uint32_t tag) {
-        if (made < cap4) {
-            chunks[64u * made + lane] = q;
-            metas[64u * made + lane] = (uint16_t)tag;
-        }
+    auto put = [&](const uint4& q, uint32_t entries, bool in) {
+        if (made < cap4) chunks[64u * made + lane] = tag_chunk(q, own, entries, in);
         made++;
     };
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
@@ -551,18 +556,18 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
                 if ((inb >> b) & 1u) {
                     put4(fa, na, j);
                     na++;
-                    if ((na & 3u) == 0u) put(fa, own | (3u << 11) | (1u << 13));
+                    if ((na & 3u) == 0u) put(fa, 4u, true);
                 } else {
                     put4(fb, nb, j);
                     nb++;
-                    if ((nb & 3u) == 0u) put(fb, own | (3u << 11));
+                    if ((nb & 3u) == 0u) put(fb, 4u, false);
                 }
             }
         }
     }
     // the partial last chunks; a list whose chunks do not fit is flagged and regrown
-    if (na & 3u) put(fa, own | (((na & 3u) - 1u) << 11) | (1u << 13));
-    if (nb & 3u) put(fb, own | (((nb & 3u) - 1u) << 11));
+    if (na & 3u) put(fa, na & 3u, true);
+    if (nb & 3u) put(fb, nb & 3u, false);
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
     const bool over = ca + cb > cap4;
     if (over) {
@@ -582,12 +587,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         const uint64_t bal = ballot64(has);
         if (bal == 0ull) break;
         const uint32_t s = filled + (uint32_t)__popcll(bal & lanemask_lt());
-        if (has && s != 64u * k + lane) {
-            const uint4 q = chunks[64u * k + lane];
-            const uint16_t tag = metas[64u * k + lane];
-            chunks[s] = q;
-            metas[s] = tag;
-        }
+        if (has && s != 64u * k + lane) chunks[s] = chunks[64u * k + lane];
         filled += (uint32_t)__popcll(bal);
     }
     if (lane == (uint32_t)__ffsll((long long)ballot64(true)) - 1u) B.wave_chunks[wave] = filled;
@@ -647,31 +647,22 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
         // lies in region w (wfirst[w] <= g < wfirst[w + 1]) at slot g - wfirst[w].
         const size_t capw = 64u * (P.cap / 4u);
         const uint4* __restrict__ chunks = reinterpret_cast<const uint4*>(B.nbr) + (size_t)blk * NW * capw;
-        const uint16_t* __restrict__ metas = B.nbr_meta + (size_t)blk * NW * capw;
         const uint32_t nch = wfirst[NW];
         uint32_t g = t, w = 0u;
         auto slot = [&](uint32_t gg) -> size_t {
             while (gg >= wfirst[w + 1u]) w++;
             return (size_t)w * capw + (gg - wfirst[w]);
         };
-        // the next chunk and its tag are in flight while this one is evaluated
+        // the next chunk is in flight while this one is evaluated
         uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
-        uint32_t m_next = 0u;
-        if (g < nch) {
-            const size_t s0 = slot(g);
-            q_next = chunks[s0];
-            m_next = metas[s0];
-        }
+        if (g < nch) q_next = chunks[slot(g)];
         for (; g < nch; g += NT) {
-            const uint4 q = q_next;
-            const uint32_t tag = m_next;
-            if (g + NT < nch) {
-                const size_t s1 = slot(g + NT);
-                q_next = chunks[s1];
-                m_next = metas[s1];
-            }
-            const uint32_t own = tag & 0x7ffu, m = ((tag >> 11) & 3u) + 1u;
-            const bool inblock = (tag >> 13) & 1u;
+            const uint4 qt = q_next;
+            if (g + NT < nch) q_next = chunks[slot(g + NT)];
+            // the tag (tag_chunk) off the top bits, the entries below them
+            const uint32_t own = (qt.x >> 28) | ((qt.y >> 28) << 4) | (((qt.z >> 28) & 7u) << 8), m = (qt.w >> 28) + 1u;
+            const bool inblock = (qt.z >> 31) != 0u;
+            const uint4 q = make_uint4(qt.x & 0x0fffffffu, qt.y & 0x0fffffffu, qt.z & 0x0fffffffu, qt.w & 0x0fffffffu);
             const uint32_t i = base + own;
             double xi[D];
 #pragma unroll
@@ -1592,14 +1583,12 @@ void Engine<D>::alloc_list(uint32_t cap) {
         RH_HIP(hipFree(p));
     };
     release(B_.nbr);
-    release(B_.nbr_meta);
     cap = (cap + 3u) & ~3u;  // whole four-entry chunks (the half list stores uint4 chunks)
     P_.cap = cap;
     // The full lists take cap entries per particle; the half list's chunk stream takes cap / 4
     // chunks per particle slot, for every block size up to 2048.
     const size_t slots = (n_ + 2047u) & ~size_t(2047);
     B_.nbr = (uint32_t*)dalloc((size_t)cap * slots * sizeof(uint32_t));
-    B_.nbr_meta = (uint16_t*)dalloc((size_t)(cap / 4u) * slots * sizeof(uint16_t));
 }
 
 template <int D>
