@@ -454,8 +454,9 @@ __device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entri
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
-// not wrap the stencil.  In-block pairs go to their owner only; pairs across blocks to both
-// ends.  The list is a stream of four-entry chunks (16 B).  Entries are particle indices below
+// not wrap the stencil.  In-block pairs go to their lower-index end only (the step kernel deals
+// chunks out evenly, so how pairs divide between particles costs it nothing); pairs across
+// blocks go to both ends.  The list is a stream of four-entry chunks (16 B).  Entries are particle indices below
 // 2^28; the top four bits of each word carry the chunk's tag (tag_chunk): the owner's index in
 // its block of P.block particles, the entry count and whether the entries are in that block.
 // Every 64-particle wave has a region of 64 cap/4 slots: a lane's k-th chunk goes to slot
@@ -481,7 +482,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     uint32_t c[D];
     cell_of<D>(P, xi, c);
     const uint32_t cap4 = P.cap / 4u;
-    const uint32_t bshift = __ffs(P.block) - 1;  // the block is a power of two
+    const uint32_t blo = i & ~(P.block - 1u);  // the block is a power of two
     // A lane fills one in-block and one cross-block chunk in registers and stores each whole
     // (16 B) with its tag when it is full.
     const uint32_t lane = __lane_id(), wave = i >> 6, own = i & (P.block - 1u);
@@ -518,10 +519,12 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         const uint32_t key = morton<D>(cc);
         const uint32_t s0 = B.cell_start[key], e0 = B.cell_end[key];
         const float sx = shift[0], sy = shift[1], sz = shift[2];
+        // An in-block pair belongs to its lower-index end, so in-block candidates up to i are
+        // not this end's: a cell that begins inside the block is scanned from past i.
         // Candidates in groups of up to 32: a distance pass sets one bit per candidate within
-        // the list radius, bit arithmetic drops in-block partners this end does not own (and
-        // itself), and only the surviving bits are visited to store entries.
-        for (uint32_t base = s0; base < e0; base += 32u) {
+        // the list radius, bit arithmetic drops the in-block ones up to i (i itself included),
+        // and only the surviving bits are visited to store entries.
+        for (uint32_t base = s0 >= blo ? max(s0, i + 1u) : s0; base < e0; base += 32u) {
             const uint32_t cnt = min(32u, e0 - base);
             uint32_t mask = 0u;
             auto near = [&](uint32_t u) -> uint32_t {
@@ -541,14 +544,10 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             uint32_t u = 0;
             for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
             if (u < cnt) mask |= near(u) << u;
-            // j = base + u: lt: j < i, gt: j > i; same: j has the parity of i; inb: same block
-            const int t0 = (int)i - (int)base;
-            const uint32_t lt = bits_below(t0), gt = ~bits_below(t0 + 1);
-            const uint32_t same = ((i ^ base) & 1u) ? 0xAAAAAAAAu : 0x55555555u;
-            const uint32_t blo = (i >> bshift) << bshift;
+            // j = base + u: gt: j > i; inb: j in i's block
+            const uint32_t gt = ~bits_below((int)i + 1 - (int)base);
             const uint32_t inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
-            const uint32_t owned = (gt & ~same) | (lt & same);  // owns(i, j), which excludes j = i
-            mask &= (inb & owned) | ~inb;
+            mask &= (inb & gt) | ~inb;
             while (mask) {
                 const uint32_t b = __builtin_ctz(mask);
                 mask &= mask - 1u;
