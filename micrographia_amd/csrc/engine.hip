@@ -456,14 +456,13 @@ __device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entri
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
 // not wrap the stencil.  In-block pairs go to their lower-index end only (the step kernel deals
 // chunks out evenly, so how pairs divide between particles costs it nothing); pairs across
-// blocks go to both ends.  The list is a stream of four-entry chunks (16 B).  Entries are particle indices below
-// 2^28; the top four bits of each word carry the chunk's tag (tag_chunk): the owner's index in
-// its block of P.block particles, the entry count and whether the entries are in that block.
-// Every 64-particle wave has a region of 64 cap/4 slots: a lane's k-th chunk goes to slot
-// 64 k + lane as it fills, so rows of the region below the wave's shortest list are full, and
-// at the end the wave packs the ragged rows above them down to close the gaps and records its
-// chunk count.  Neighbouring slots then belong to different particles (the step kernel's
-// neighbouring lanes add to different accumulators).
+// blocks go to both ends.  The list is a stream of four-entry chunks (16 B).  Entries are
+// particle indices below 2^28; the top four bits of each word carry the chunk's tag
+// (tag_chunk): the owner's index in its block of P.block particles, the entry count and whether
+// the entries are in that block.  Every 64-particle wave fills a region of 64 cap/4 slots from
+// its start, chunks in the order they fill, and records its chunk count.  The candidate and hit
+// loops run in step across the wave (a lane with nothing left idles), so the stores happen
+// where every lane is present and a wave-uniform count hands out the slots.
 template <int D, bool MULTI, bool WRAP>
 __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;
@@ -483,15 +482,22 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     cell_of<D>(P, xi, c);
     const uint32_t cap4 = P.cap / 4u;
     const uint32_t blo = i & ~(P.block - 1u);  // the block is a power of two
-    // A lane fills one in-block and one cross-block chunk in registers and stores each whole
-    // (16 B) with its tag when it is full.
-    const uint32_t lane = __lane_id(), wave = i >> 6, own = i & (P.block - 1u);
-    const size_t capw = 64u * cap4;
+    const uint32_t wave = i >> 6, own = i & (P.block - 1u);
+    const uint32_t capw = 64u * cap4;
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
-    uint32_t made = 0u;  // chunks this lane has stored
-    auto put = [&](const uint4& q, uint32_t entries, bool in) {
-        if (made < cap4) chunks[64u * made + lane] = tag_chunk(q, own, entries, in);
-        made++;
+    const uint64_t lt = lanemask_lt();
+    // A lane fills one in-block and one cross-block chunk in registers; full chunks are stored
+    // whole (16 B) at the wave's fill point.  `fill` changes only where every lane is present.
+    uint32_t fill = 0u;
+    auto store = [&](bool want, const uint4& q, uint32_t entries, bool in) {
+        const uint64_t bal = ballot64(want);
+        if (bal == 0ull) return;
+        if (want) {
+            const uint32_t slot = fill + (uint32_t)__popcll(bal & lt);
+            if (slot < capw) chunks[slot] = tag_chunk(q, own, entries, in);
+            else atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
+        }
+        fill += (uint32_t)__popcll(bal);
     };
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     // single-cell periodic axes: the nearest image of the one cell (cell_w is the box length)
@@ -515,58 +521,73 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             cc[a] = (uint32_t)v;
             shift[a] = (float)off[a] * P.cell_w[a] - ri[a];
         }
-        if (skip) continue;
-        const uint32_t key = morton<D>(cc);
-        const uint32_t s0 = B.cell_start[key], e0 = B.cell_end[key];
+        uint32_t base = 0u, e0 = 0u;
+        if (!skip) {
+            const uint32_t key = morton<D>(cc);
+            const uint32_t s0 = B.cell_start[key];
+            e0 = B.cell_end[key];
+            // An in-block pair belongs to its lower-index end, so in-block candidates up to i
+            // are not this end's: a cell that begins inside the block is scanned from past i.
+            base = s0 >= blo ? max(s0, i + 1u) : s0;
+        }
         const float sx = shift[0], sy = shift[1], sz = shift[2];
-        // An in-block pair belongs to its lower-index end, so in-block candidates up to i are
-        // not this end's: a cell that begins inside the block is scanned from past i.
         // Candidates in groups of up to 32: a distance pass sets one bit per candidate within
         // the list radius, bit arithmetic drops the in-block ones up to i (i itself included),
-        // and only the surviving bits are visited to store entries.
-        for (uint32_t base = s0 >= blo ? max(s0, i + 1u) : s0; base < e0; base += 32u) {
-            const uint32_t cnt = min(32u, e0 - base);
-            uint32_t mask = 0u;
-            auto near = [&](uint32_t u) -> uint32_t {
-                const float4 q = rel[base + u];
-                float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
-                if constexpr (WRAP) {
-                    if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
-                    if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
-                    if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
+        // and the surviving bits are visited to store entries.
+        for (;; base += 32u) {
+            const bool act = base < e0;
+            if (ballot64(act) == 0ull) break;
+            uint32_t mask = 0u, inb = 0u;
+            if (act) {
+                const uint32_t cnt = min(32u, e0 - base);
+                auto near = [&](uint32_t u) -> uint32_t {
+                    const float4 q = rel[base + u];
+                    float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
+                    if constexpr (WRAP) {
+                        if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
+                        if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
+                        if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
+                    }
+                    float r2 = dx * dx;
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
+                    const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[base + u]] : list_r2;
+                    return r2 < lr2 ? 1u : 0u;
+                };
+                uint32_t u = 0;
+                for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
+                if (u < cnt) mask |= near(u) << u;
+                // j = base + u: gt: j > i; inb: j in i's block
+                const uint32_t gt = ~bits_below((int)i + 1 - (int)base);
+                inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
+                mask &= (inb & gt) | ~inb;
+            }
+            for (;;) {
+                const bool hit = mask != 0u;
+                if (ballot64(hit) == 0ull) break;
+                bool full = false, in = false;
+                if (hit) {
+                    const uint32_t b = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const uint32_t j = base + b;
+                    in = (inb >> b) & 1u;
+                    if (in) {
+                        put4(fa, na, j);
+                        na++;
+                        full = (na & 3u) == 0u;
+                    } else {
+                        put4(fb, nb, j);
+                        nb++;
+                        full = (nb & 3u) == 0u;
+                    }
                 }
-                float r2 = dx * dx;
-                r2 = __builtin_fmaf(dy, dy, r2);
-                if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
-                const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[base + u]] : list_r2;
-                return r2 < lr2 ? 1u : 0u;
-            };
-            uint32_t u = 0;
-            for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
-            if (u < cnt) mask |= near(u) << u;
-            // j = base + u: gt: j > i; inb: j in i's block
-            const uint32_t gt = ~bits_below((int)i + 1 - (int)base);
-            const uint32_t inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
-            mask &= (inb & gt) | ~inb;
-            while (mask) {
-                const uint32_t b = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                const uint32_t j = base + b;
-                if ((inb >> b) & 1u) {
-                    put4(fa, na, j);
-                    na++;
-                    if ((na & 3u) == 0u) put(fa, 4u, true);
-                } else {
-                    put4(fb, nb, j);
-                    nb++;
-                    if ((nb & 3u) == 0u) put(fb, 4u, false);
-                }
+                store(full, in ? fa : fb, 4u, in);
             }
         }
     }
-    // the partial last chunks; a list whose chunks do not fit is flagged and regrown
-    if (na & 3u) put(fa, na & 3u, true);
-    if (nb & 3u) put(fb, nb & 3u, false);
+    // the partial last chunks; a list longer than the capacity is flagged and regrown
+    store((na & 3u) != 0u, fa, na & 3u, true);
+    store((nb & 3u) != 0u, fb, nb & 3u, false);
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
     const bool over = ca + cb > cap4;
     if (over) {
@@ -575,21 +596,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     }
     if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
     B.nbr_count[i] = na | (nb << 16);
-    // Close the gaps: row k's chunks move down to slots filled .. filled + (lanes with a k-th
-    // chunk) - 1.  A chunk never moves up (filled <= 64 k), and row k's writes stay below row
-    // k + 1, so every chunk is read before its slot is reused.  Full rows do not move.  The
-    // lanes past the last particle have left.
-    const uint32_t mine = over ? 0u : ca + cb;
-    uint32_t filled = 0u;
-    for (uint32_t k = 0;; k++) {
-        const bool has = k < mine;
-        const uint64_t bal = ballot64(has);
-        if (bal == 0ull) break;
-        const uint32_t s = filled + (uint32_t)__popcll(bal & lanemask_lt());
-        if (has && s != 64u * k + lane) chunks[s] = chunks[64u * k + lane];
-        filled += (uint32_t)__popcll(bal);
-    }
-    if (lane == (uint32_t)__ffsll((long long)ballot64(true)) - 1u) B.wave_chunks[wave] = filled;
+    if (__lane_id() == 0u) B.wave_chunks[wave] = min(fill, capw);
 }
 
 // A block of BS particles runs on min(BS, 1024) threads (with BS = 2048 each thread advances
