@@ -540,8 +540,12 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             uint32_t mask = 0u, inb = 0u;
             if (act) {
                 const uint32_t cnt = min(32u, e0 - base);
-                auto near = [&](uint32_t u) -> uint32_t {
-                    const float4 q = rel[base + u];
+                // Candidates four at a time, unrolled (addresses by immediate offset, the bit by
+                // a select); a lane past its cell's end tests the rel entries that follow, which
+                // exist (the array is padded by 32) and whose bits are masked off after.
+                const float4* __restrict__ rq = rel + base;
+                auto near = [&](uint32_t u) -> bool {
+                    const float4 q = rq[u];
                     float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
                     if constexpr (WRAP) {
                         if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
@@ -551,12 +555,16 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
                     float r2 = dx * dx;
                     r2 = __builtin_fmaf(dy, dy, r2);
                     if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
-                    const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[base + u]] : list_r2;
-                    return r2 < lr2 ? 1u : 0u;
+                    const float lr2 = MULTI ? B.pt->list_r2[si * 4 + sp[min(base + u, e0 - 1u)]] : list_r2;
+                    return r2 < lr2;
                 };
-                uint32_t u = 0;
-                for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
-                if (u < cnt) mask |= near(u) << u;
+#pragma unroll 1
+                for (uint32_t u0 = 0; u0 < 32u; u0 += 4u) {
+                    if (u0 >= cnt) break;
+#pragma unroll
+                    for (uint32_t k = 0; k < 4u; k++) mask |= near(u0 + k) ? 1u << (u0 + k) : 0u;
+                }
+                mask &= bits_below((int)cnt);
                 // j = base + u: gt: j > i; inb: j in i's block
                 const uint32_t gt = ~bits_below((int)i + 1 - (int)base);
                 inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
@@ -1632,7 +1640,7 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
         B_.wave_chunks = (uint32_t*)dalloc(waves * sizeof(uint32_t));
         RH_HIP(hipMemset(B_.wave_chunks, 0, waves * sizeof(uint32_t)));
     }
-    B_.rel = (float4*)dalloc(n * sizeof(float4));
+    B_.rel = (float4*)dalloc((n + 32u) * sizeof(float4));  // padded: the list build reads up to 31 past a cell
     B_.cell_start = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.cell_end = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.pe = (double*)dalloc(n * sizeof(double));
