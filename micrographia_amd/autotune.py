@@ -21,7 +21,7 @@ import time
 from dataclasses import dataclass, field
 from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
 
-ENGINE_CANDIDATES: Tuple[Tuple[int, int], ...] = ((1, 2048), (1, 1024), (1, 512), (1, 256), (0, 256))
+ENGINE_CANDIDATES: Tuple[Tuple[int, int], ...] = ((1, 4096), (1, 2048), (1, 1024), (1, 512), (1, 256), (0, 256))
 BATCH_CANDIDATES: Tuple[Tuple[int, float], ...] = ((1, 1.0), (1, 1.25), (1, 1.5), (0, 1.0), (0, 1.25))
 
 

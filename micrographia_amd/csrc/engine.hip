@@ -444,11 +444,11 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // ---------------------------------------------------------------------------------------------
 
 // A half-list chunk's tag in the top four bits of its words: owner bits 0-3 in x, 4-7 in y, 8-10
-// in z (bit 31 of z: in-block), entries - 1 in w.
+// in z (bit 31 of z: in-block), entries - 1 in bits 28-29 of w and owner bit 11 in bit 30.
 __device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entries, bool in) {
     const uint32_t lo = 0x0fffffffu;
-    return make_uint4((q.x & lo) | (own << 28), (q.y & lo) | ((own >> 4) << 28), (q.z & lo) | ((own >> 8) << 28) | (in ? 1u << 31 : 0u),
-                      (q.w & lo) | ((entries - 1u) << 28));
+    return make_uint4((q.x & lo) | (own << 28), (q.y & lo) | ((own >> 4) << 28), (q.z & lo) | (((own >> 8) & 7u) << 28) | (in ? 1u << 31 : 0u),
+                      (q.w & lo) | ((entries - 1u) << 28) | (((own >> 11) & 1u) << 30));
 }
 
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
 // neighbouring lanes read neighbouring chunks.  Every kick goes to the block's LDS
 // accumulators: the owner's after each chunk, an in-block partner's per pair.
 template <int D, bool MULTI, int BS, bool BOOK>
-__global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) void k_step_half(Params<D> P, Bufs<D> B) {
+__global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1) void k_step_half(Params<D> P, Bufs<D> B) {
     constexpr uint32_t NT = BS > 1024 ? 1024u : (uint32_t)BS;
     constexpr uint32_t PER = (uint32_t)BS / NT;
     const uint32_t* ctl = B.ctl;
@@ -674,7 +674,8 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS > 1024 && !BOOK ? 8 : 1) 
             const uint4 qt = q_next;
             if (g + NT < nch) q_next = chunks[slot(g + NT)];
             // the tag (tag_chunk) off the top bits, the entries below them
-            const uint32_t own = (qt.x >> 28) | ((qt.y >> 28) << 4) | (((qt.z >> 28) & 7u) << 8), m = (qt.w >> 28) + 1u;
+            const uint32_t own = (qt.x >> 28) | ((qt.y >> 28) << 4) | (((qt.z >> 28) & 7u) << 8) | (((qt.w >> 30) & 1u) << 11);
+            const uint32_t m = ((qt.w >> 28) & 3u) + 1u;
             const bool inblock = (qt.z >> 31) != 0u;
             const uint4 q = make_uint4(qt.x & 0x0fffffffu, qt.y & 0x0fffffffu, qt.z & 0x0fffffffu, qt.w & 0x0fffffffu);
             const uint32_t i = base + own;
@@ -1600,8 +1601,8 @@ void Engine<D>::alloc_list(uint32_t cap) {
     cap = (cap + 3u) & ~3u;  // whole four-entry chunks (the half list stores uint4 chunks)
     P_.cap = cap;
     // The full lists take cap entries per particle; the half list's chunk stream takes cap / 4
-    // chunks per particle slot, for every block size up to 2048.
-    const size_t slots = (n_ + 2047u) & ~size_t(2047);
+    // chunks per particle slot, for every block size up to 4096.
+    const size_t slots = (n_ + 4095u) & ~size_t(4095);
     B_.nbr = (uint32_t*)dalloc((size_t)cap * slots * sizeof(uint32_t));
 }
 
@@ -1634,9 +1635,9 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     }
     B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
     {
-        // one count per 64-particle wave, for every block size up to 2048; waves past the last
+        // one count per 64-particle wave, for every block size up to 4096; waves past the last
         // particle stay empty
-        const size_t waves = ((n + 2047u) & ~size_t(2047)) / 64u;
+        const size_t waves = ((n + 4095u) & ~size_t(4095)) / 64u;
         B_.wave_chunks = (uint32_t*)dalloc(waves * sizeof(uint32_t));
         RH_HIP(hipMemset(B_.wave_chunks, 0, waves * sizeof(uint32_t)));
     }
@@ -1930,8 +1931,10 @@ void Engine<D>::launch_step() {
             RH_STEP_HALF(512)
         } else if (bs == 1024) {
             RH_STEP_HALF(1024)
-        } else {
+        } else if (bs == 2048) {
             RH_STEP_HALF(2048)
+        } else {
+            RH_STEP_HALF(4096)
         }
 #undef RH_STEP_HALF
     }
@@ -1941,7 +1944,7 @@ void Engine<D>::launch_step() {
 template <int D>
 void Engine<D>::tune(int variant, uint32_t block) {
     RH_REQUIRE(variant == 0 || variant == 1, "kernel variant must be 0 (full list) or 1 (half list)");
-    RH_REQUIRE(block == 256 || block == 512 || block == 1024 || block == 2048, "block must be 256, 512, 1024 or 2048");
+    RH_REQUIRE(block == 256 || block == 512 || block == 1024 || block == 2048 || block == 4096, "block must be 256, 512, 1024, 2048 or 4096");
     RH_HIP(hipStreamSynchronize(stream_));
     variant_ = variant;
     block_ = block;

@@ -316,9 +316,7 @@ template <int D, bool MULTI>
 __device__ __forceinline__ bool pair_kick2(const float f[D], float rc2, float sigma2, float six_eps4, float r2_fast, float ci, float cj, uint32_t ki[D], int32_t kj[D]) {
     float r2 = f[0] * f[0] + f[1] * f[1];
     if constexpr (D == 3) r2 = r2 + f[2] * f[2];
-#pragma unroll
-    for (int ax = 0; ax < D; ax++) kj[ax] = 0;
-    const bool hit = r2 < rc2;
+    const bool hit = r2 < rc2;  // kj is set only on a hit
     if (hit) {
         float fr = pair_fr(r2, sigma2, six_eps4, r2_fast);
         float a = -fr * ci;

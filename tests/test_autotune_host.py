@@ -31,7 +31,7 @@ class FakeEngine:
 @pytest.fixture
 def fake():
     FakeEngine.made = 0
-    FakeEngine.speed = {(1, 2048): 4e-4, (1, 1024): 2e-4, (1, 512): 1e-4, (1, 256): 3e-4, (0, 256): 5e-4}
+    FakeEngine.speed = {(1, 4096): 4e-4, (1, 2048): 4e-4, (1, 1024): 2e-4, (1, 512): 1e-4, (1, 256): 3e-4, (0, 256): 5e-4}
     return FakeEngine
 
 

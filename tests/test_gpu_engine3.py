@@ -63,7 +63,7 @@ MODES = {
 }
 
 # (kernel variant, block): the full list, and the half list at each block size
-VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024), (1, 2048)]
+VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024), (1, 2048), (1, 4096)]
 
 
 def engine(sys, st, mode, variant=1, block=512):
