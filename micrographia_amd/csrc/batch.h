@@ -23,6 +23,7 @@ struct BSys {
     uint64_t nbr0;  // list entry k of local slot i at nbr0 + k * nslots + i
     int64_t box[3], half[3];
     double boxf[3], inv_box[3];  // positions are exact f64 integers (2^-32 sigma units)
+    double box16[3], inv_box16[3];  // box / 16 and 16 / box: the image on positions in sixteenths
     double cell_scale[3], cell_w64[3];
     float cell_w[3];
     float kick_c[4];
@@ -43,7 +44,8 @@ struct BBufs {
     uint32_t* species[2];
     uint32_t* ids[2];
     double* ref[3];
-    double4* aos[2];  // positions packed per slot (x, y, z or 0, 0), for the step's neighbour gathers
+    double4* aos[2];  // positions in sixteenths packed per slot (x, y, z or 0, 0), for the step's neighbour gathers
+    uint32_t* wave_chunks;  // list chunks in each 64-slot wave's region
     float4* rel;  // position relative to its cell's corner (sigma, f32), for list builds
     uint32_t* key;
     uint32_t* rank;

@@ -140,7 +140,7 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
         B.vel[pm ^ 1][a][ns] = B.vel[pm][a][slot];
         r[a] = (float)(x * 0x1p-32 - (double)c[a] * S.cell_w64[a]);
     }
-    B.aos[pp ^ 1][ns] = make_double4(xs[0], xs[1], xs[2], 0.0);
+    B.aos[pp ^ 1][ns] = make_double4(xs[0] * 0.0625, xs[1] * 0.0625, xs[2] * 0.0625, 0.0);  // sixteenths
     B.rel[ns] = make_float4(r[0], r[1], r[2], 0.f);
     B.species[pm ^ 1][ns] = B.species[pm][slot];
     B.ids[pm ^ 1][ns] = B.ids[pm][slot];
@@ -159,11 +159,13 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B) {
 }
 
 // The list of local slot i (in the freshly scattered buffers): f32 cell-relative separations
-// against each species pair's (rc + skin)(1 + 1e-4), in-block pairs (same 256-slot block) owned by
-// one end, pairs across blocks listed at both ends; variant 0 lists every pair at both ends.  As
-// in the single-system engine: candidates in groups of 32 through a distance mask, entries stored
-// in four-entry chunks (uint4 at nbr0 + 4 (c * nslots + i)), in-block chunks from the front and
-// the others from the back.
+// against each species pair's (rc + skin)(1 + 1e-4); an in-block pair (same 256-slot block)
+// belongs to its lower-index end, pairs across blocks are listed at both ends; variant 0 lists
+// every pair at both ends.  As in the single-system engine: candidates four at a time in groups
+// of 32 through a distance mask, the candidate and hit loops in step across the wave, and full
+// four-entry chunks (tagged with the owner's index in its block, entries - 1 and in-block, in
+// the top bits of their indices) stored densely, in fill order, in the wave's region of
+// 64 cap/4 chunks at nbr0; the wave's chunk count goes to wave_chunks.
 template <int D, bool MULTI>
 __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local) {
     const uint32_t variant = S.variant;
@@ -179,8 +181,21 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     const float list_r2 = S.pt.list_r2[0];
     uint32_t c[D];
     cell_of<D>(S, xi, c);
-    const uint32_t cap4 = S.cap / 4u, stride = S.nslots;
-    uint4* row = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + local;
+    const uint32_t cap4 = S.cap / 4u, capw = 64u * cap4;
+    const uint32_t own = local & (BATCH_WG - 1u);
+    uint4* chunks = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + (size_t)(local >> 6) * capw;
+    const uint64_t lt = lanemask_lt();
+    uint32_t fill = 0u;  // changes only where every lane of the wave is present
+    auto store = [&](bool want, const uint4& q, uint32_t entries, bool in) {
+        const uint64_t bal = ballot64(want);
+        if (bal == 0ull) return;
+        if (want) {
+            const uint32_t at = fill + (uint32_t)__popcll(bal & lt);
+            if (at < capw) chunks[at] = tag_chunk(q, own, entries, in);
+            else atomicOr(&C.overflow, 1u);
+        }
+        fill += (uint32_t)__popcll(bal);
+    };
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     const uint32_t* cs = B.cell_start + S.cell0;
     const uint32_t* ce = B.cell_end + S.cell0;
@@ -189,7 +204,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
 #pragma unroll
     for (int a = 0; a < D; a++) wl[a] = (S.nc[a] == 1u && ((S.periodic >> a) & 1u)) ? S.cell_w[a] : 0.f;
     const bool any_wrap = (wl[0] != 0.f) | (wl[1] != 0.f) | (wl[2] != 0.f);
-    const uint32_t blo = local / BATCH_WG * BATCH_WG;
+    const uint32_t blo = local & ~(BATCH_WG - 1u);
     uint32_t na = 0, nb = 0;
     for (int o = 0; o < stencil_size(D); o++) {
         const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
@@ -207,68 +222,87 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
             cc[a] = (uint32_t)v;
             shift[a] = (float)off[a] * S.cell_w[a] - ri[a];
         }
-        if (skip) continue;
-        const uint32_t key = lin_cell<D>(S, cc);
-        const uint32_t s0 = cs[key], e0 = ce[key];
+        uint32_t base = 0u, e0 = 0u;
+        if (!skip) {
+            const uint32_t key = lin_cell<D>(S, cc);
+            const uint32_t s0 = cs[key];
+            e0 = ce[key];
+            // in-block candidates up to local are not this end's: scan such a cell from past it
+            base = (variant == 1 && s0 >= blo) ? max(s0, local + 1u) : s0;
+        }
         const float sx = shift[0], sy = shift[1], sz = shift[2];
-        for (uint32_t base = s0; base < e0; base += 32u) {
-            const uint32_t cnt = min(32u, e0 - base);
-            uint32_t mask = 0u;
-            auto near = [&](uint32_t u) -> uint32_t {
-                const float4 q = rel[base + u];
-                float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
-                if (any_wrap) {
-                    if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
-                    if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
-                    if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
+        for (;; base += 32u) {
+            const bool act = base < e0;
+            if (ballot64(act) == 0ull) break;
+            uint32_t mask = 0u, inb = 0u;
+            if (act) {
+                const uint32_t cnt = min(32u, e0 - base);
+                // four at a time, unrolled; a lane past its cell's end tests the rel entries that
+                // follow (the array is padded by 32) and their bits are masked off after
+                const float4* __restrict__ rq = rel + base;
+                auto near = [&](uint32_t u) -> bool {
+                    const float4 q = rq[u];
+                    float dx = q.x + sx, dy = q.y + sy, dz = D == 3 ? q.z + sz : 0.f;
+                    if (any_wrap) {
+                        if (wl[0] != 0.f) dx = dx > 0.5f * wl[0] ? dx - wl[0] : (dx < -0.5f * wl[0] ? dx + wl[0] : dx);
+                        if (wl[1] != 0.f) dy = dy > 0.5f * wl[1] ? dy - wl[1] : (dy < -0.5f * wl[1] ? dy + wl[1] : dy);
+                        if (D == 3 && wl[2] != 0.f) dz = dz > 0.5f * wl[2] ? dz - wl[2] : (dz < -0.5f * wl[2] ? dz + wl[2] : dz);
+                    }
+                    float r2 = dx * dx;
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
+                    const float lr2 = MULTI ? S.pt.list_r2[si * 4 + sp[min(base + u, e0 - 1u)]] : list_r2;
+                    return r2 < lr2;
+                };
+#pragma unroll 1
+                for (uint32_t u0 = 0; u0 < 32u; u0 += 4u) {
+                    if (u0 >= cnt) break;
+#pragma unroll
+                    for (uint32_t k = 0; k < 4u; k++) mask |= near(u0 + k) ? 1u << (u0 + k) : 0u;
                 }
-                float r2 = dx * dx;
-                r2 = __builtin_fmaf(dy, dy, r2);
-                if (D == 3) r2 = __builtin_fmaf(dz, dz, r2);
-                const float lr2 = MULTI ? S.pt.list_r2[si * 4 + sp[base + u]] : list_r2;
-                return r2 < lr2 ? 1u : 0u;
-            };
-            uint32_t u = 0;
-            for (; u + 1 < cnt; u += 2) mask |= (near(u) << u) | (near(u + 1) << (u + 1));
-            if (u < cnt) mask |= near(u) << u;
-            const int t0 = (int)local - (int)base;
-            const uint32_t self = (t0 >= 0 && t0 < 32) ? (1u << t0) : 0u;
-            uint32_t inb = 0u;
-            if (variant == 1) {
-                const uint32_t lt = bits_below(t0), gt = ~bits_below(t0 + 1);
-                const uint32_t same = ((local ^ base) & 1u) ? 0xAAAAAAAAu : 0x55555555u;
-                inb = bits_below((int)(blo + BATCH_WG) - (int)base) & ~bits_below((int)blo - (int)base);
-                const uint32_t owned = (gt & ~same) | (lt & same);
-                mask &= (inb & owned) | ~inb;
+                mask &= bits_below((int)cnt);
+                // j = base + u: gt: j > local; inb: j in local's block (variant 1)
+                const uint32_t gt = ~bits_below((int)local + 1 - (int)base);
+                if (variant == 1) {
+                    inb = bits_below((int)(blo + BATCH_WG) - (int)base) & ~bits_below((int)blo - (int)base);
+                    mask &= (inb & gt) | ~inb;
+                }
+                const int t0 = (int)local - (int)base;
+                mask &= ~((t0 >= 0 && t0 < 32) ? (1u << t0) : 0u);
             }
-            mask &= ~self;
-            while (mask) {
-                const uint32_t b = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                const uint32_t j = base + b;
-                if ((inb >> b) & 1u) {
-                    put4(fa, na, j);
-                    na++;
-                    if ((na & 3u) == 0u && na / 4u <= cap4) row[(size_t)(na / 4u - 1u) * stride] = fa;
-                } else {
-                    put4(fb, nb, j);
-                    nb++;
-                    if ((nb & 3u) == 0u && nb / 4u <= cap4) row[(size_t)(cap4 - nb / 4u) * stride] = fb;
+            for (;;) {
+                const bool hit = mask != 0u;
+                if (ballot64(hit) == 0ull) break;
+                bool full = false, in = false;
+                if (hit) {
+                    const uint32_t b = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const uint32_t j = base + b;
+                    in = (inb >> b) & 1u;
+                    if (in) {
+                        put4(fa, na, j);
+                        na++;
+                        full = (na & 3u) == 0u;
+                    } else {
+                        put4(fb, nb, j);
+                        nb++;
+                        full = (nb & 3u) == 0u;
+                    }
                 }
+                store(full, in ? fa : fb, 4u, in);
             }
         }
     }
+    store((na & 3u) != 0u, fa, na & 3u, true);
+    store((nb & 3u) != 0u, fb, nb & 3u, false);
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
-    const bool over = ca + cb > cap4;
-    if (!over) {
-        if (na & 3u) row[(size_t)(ca - 1u) * stride] = fa;
-        if (nb & 3u) row[(size_t)(cap4 - cb) * stride] = fb;
-    } else {
+    if (ca + cb > cap4) {
         atomicOr(&C.overflow, 1u);
         na = nb = 0;
     }
     if (4u * (ca + cb) > *(volatile const uint32_t*)&C.maxcount) atomicMax(&C.maxcount, 4u * (ca + cb));
     B.nbr_count[slot] = na | (nb << 16);
+    if (__lane_id() == 0u) B.wave_chunks[slot >> 6] = min(fill, capw);
 }
 
 __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B) {
@@ -279,7 +313,11 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B) {
     BCtl& C = B.ctl[s];
     if (!C.do_rebuild) return;
     uint32_t local = wg * BATCH_WG + threadIdx.x - S.slot0;
-    if (local >= S.n) return;
+    if (local >= S.n) {
+        // a wave past the system's last particle holds no chunks
+        if ((threadIdx.x & 63u) == 0u) B.wave_chunks[(wg * BATCH_WG + threadIdx.x) >> 6] = 0u;
+        return;
+    }
     bool multi = S.nspecies > 1;
     if (S.dim == 3) {
         if (multi) bb_build<3, true>(B, S, C, local);
@@ -306,10 +344,15 @@ __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
 
 __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S.active && (C.stale | C.overflow) == 0u && C.remaining != 0u; }
 
+// One 256-slot work-group: as the single-system step kernel, the block's chunks (its four
+// waves' regions, numbered in order) are dealt out evenly over the lanes, every kick goes to the
+// block's LDS accumulators, and positions are staged and gathered as sixteenths.
 template <int D, bool MULTI>
 __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t wg) {
+    constexpr uint32_t NW = BATCH_WG / 64u;
     __shared__ uint32_t acc[D][BATCH_WG];
-    __shared__ double lpos[D][BATCH_WG];  // the block's positions, for in-block neighbours
+    __shared__ double lpos[D][BATCH_WG];  // the block's positions in sixteenths, for in-block neighbours
+    __shared__ uint32_t wfirst[NW + 1];
     const uint32_t t = threadIdx.x;
     const uint32_t slot = wg * BATCH_WG + t;
     const uint32_t local = slot - S.slot0;
@@ -318,46 +361,59 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     const uint32_t pp = C.ppos, pm = C.pmeta;
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a] + S.slot0;
-    double xi[D];
-    uint32_t k[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        xi[a] = live ? px[a][local] : 0.0;
-        k[a] = 0u;
         acc[a][t] = 0u;
-        lpos[a][t] = xi[a];
+        lpos[a][t] = live ? px[a][local] * 0.0625 : 0.0;
+    }
+    if (t < 64u) {
+        const uint32_t v = t < NW ? B.wave_chunks[(wg * BATCH_WG >> 6) + t] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (t >= (uint32_t)d) x += y;
+        }
+        if (t < NW) wfirst[t] = x - v;
+        if (t == NW - 1u) wfirst[NW] = x;
     }
     __syncthreads();
     const uint32_t* sp = B.species[pm] + S.slot0;
     const PairTable& pt = S.pt;
-    uint32_t si = 0;
-    float c = S.kick_c[0];
-    if (live) {
-        if (MULTI) {
-            si = sp[local];
-            c = pt.kick_c[si];
-        }
+    {
         const float rc2_0 = pt.rc2[0], sigma2_0 = pt.sigma2[0], six_eps4_0 = pt.six_eps4[0], r2f_0 = pt.r2_fast[0];
-        const uint32_t cnt = B.nbr_count[slot];
-        const uint32_t na = cnt & 0xffffu, nb = cnt >> 16;
-        const uint4* rowc = reinterpret_cast<const uint4*>(B.nbr + S.nbr0) + local;
-        const size_t stride = S.nslots;
-        const uint32_t cap4 = S.cap / 4u;
+        const size_t capw = 64u * (S.cap / 4u);
+        const uint4* __restrict__ chunks = reinterpret_cast<const uint4*>(B.nbr + S.nbr0) + (size_t)(base >> 6) * capw;
+        const double4* __restrict__ pq = B.aos[pp] + S.slot0;
         const bool rc = S.range_check != 0;
-        // As the single-system engine's step: one loop over the in-block chunks (both ends'
-        // kicks, the partner's into LDS; positions from the block's LDS copy) and then the other
-        // chunks (this end only), next chunk in flight, four neighbours' positions together.
-        const uint32_t ca = (na + 3u) / 4u, nch = ca + (nb + 3u) / 4u;
-        auto chunk_at = [&](uint32_t ch) -> uint4 {
-            return rowc[(size_t)(ch < ca ? ch : cap4 - 1u - (ch - ca)) * stride];
+        const uint32_t nch = wfirst[NW];
+        uint32_t g = t, w = 0u;
+        auto at = [&](uint32_t gg) -> size_t {
+            while (gg >= wfirst[w + 1u]) w++;
+            return (size_t)w * capw + (gg - wfirst[w]);
         };
-        uint4 q_next = nch > 0 ? chunk_at(0) : make_uint4(local, local, local, local);
-        for (uint32_t ch = 0; ch < nch; ch++) {
-            const uint4 q = q_next;
-            if (ch + 1 < nch) q_next = chunk_at(ch + 1);
-            const bool inblock = ch < ca;
-            const uint32_t m = inblock ? na - 4u * ch : nb - 4u * (ch - ca);
-            const uint32_t js[4] = {q.x, m > 1u ? q.y : local, m > 2u ? q.z : local, m > 3u ? q.w : local};
+        uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+        if (g < nch) q_next = chunks[at(g)];
+        for (; g < nch; g += BATCH_WG) {
+            const uint4 qt = q_next;
+            if (g + BATCH_WG < nch) q_next = chunks[at(g + BATCH_WG)];
+            uint32_t own, m;
+            bool inblock;
+            const uint4 q = untag_chunk(qt, own, m, inblock);
+            const uint32_t i = base + own;
+            double xi[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
+            uint32_t si = 0;
+            float c = S.kick_c[0];
+            if (MULTI) {
+                si = sp[i];
+                c = pt.kick_c[si];
+            }
+            uint32_t k[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) k[a] = 0u;
+            const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
             double xs[4][D];
             if (inblock) {
 #pragma unroll
@@ -365,8 +421,6 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
 #pragma unroll
                     for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
             } else {
-                // one 32-byte record per neighbour (the packed copy) instead of one word per axis
-                const double4* pq = B.aos[pp] + S.slot0;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     if constexpr (D == 3) {
@@ -389,13 +443,14 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
                 double d[D];
 #pragma unroll
                 for (int a = 0; a < D; a++) {
-                    d[a] = separation<D>(S, xi[a], xs[e][a], a);
-                    f[a] = sep_f_in_range(d[a]);
+                    d[a] = separation16<D>(S, xi[a], xs[e][a], a);
+                    f[a] = sep_f16(d[a]);
                 }
                 if (rc) {
+                    // the reference's range test on the separation (|d| < 8 sigma, in sixteenths)
                     bool ok = true;
 #pragma unroll
-                    for (int a = 0; a < D; a++) ok &= sep_in_range(d[a]);
+                    for (int a = 0; a < D; a++) ok &= d[a] >= -2147483648.0 && d[a] < 2147483648.0;
                     if (!ok) continue;
                 }
                 int32_t kj[D];
@@ -413,10 +468,25 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
                     for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
                 }
             }
+#pragma unroll
+            for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
         }
     }
     __syncthreads();
     if (!live) return;
+    uint32_t si = 0;
+    float c = S.kick_c[0];
+    if (MULTI) {
+        si = sp[local];
+        c = pt.kick_c[si];
+    }
+    double xi[D];
+    uint32_t k[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        xi[a] = lpos[a][t] * 16.0;
+        k[a] = 0u;
+    }
     if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
 #pragma unroll
     for (int a = 0; a < D; a++) k[a] += acc[a][t];
@@ -439,7 +509,7 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         far |= !sep_f(separation<D>(S, B.ref[a][slot], x[a], a), d);
         r2 += d * d;
     }
-    B.aos[pp ^ 1][slot] = make_double4(x[0], x[1], D == 3 ? x[D - 1] : 0.0, 0.0);
+    B.aos[pp ^ 1][slot] = make_double4(x[0] * 0.0625, x[1] * 0.0625, D == 3 ? x[D - 1] * 0.0625 : 0.0, 0.0);
     bool stale = far || r2 >= S.half_skin2;
     uint64_t bal = ballot64(stale);
     if (bal != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)bal) - 1) && *(volatile const uint32_t*)&C.stale_next == 0u)
@@ -573,7 +643,9 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
         B_.aos[p] = (double4*)dalloc(slot_cap * 32);
     }
     for (int a = 0; a < 3; a++) B_.ref[a] = (double*)dalloc(slot_cap * 8);
-    B_.rel = (float4*)dalloc(slot_cap * 16);
+    B_.rel = (float4*)dalloc((slot_cap + 32) * 16);  // padded: the list build reads up to 31 past a cell
+    B_.wave_chunks = (uint32_t*)dalloc(slot_cap / 64 * 4);
+    RH_HIP(hipMemset(B_.wave_chunks, 0, slot_cap / 64 * 4));
     B_.key = (uint32_t*)dalloc(slot_cap * 4);
     B_.rank = (uint32_t*)dalloc(slot_cap * 4);
     B_.nbr_count = (uint32_t*)dalloc(slot_cap * 4);
@@ -652,6 +724,8 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         d.half[a] = sys.box_len[a] / 2;
         d.boxf[a] = (double)sys.box_len[a];
         d.inv_box[a] = sys.walled[a] ? 0.0 : 1.0 / d.boxf[a];  // 0 on walled axes: no image
+        d.box16[a] = d.boxf[a] * 0.0625;  // the same image on positions held in sixteenths
+        d.inv_box16[a] = d.inv_box[a] * 16.0;
         d.cell_scale[a] = (double)d.nc[a] / (double)sys.box_len[a];
         d.cell_w64[a] = b[a] / (double)d.nc[a];
         d.cell_w[a] = (float)d.cell_w64[a];

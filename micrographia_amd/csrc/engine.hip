@@ -443,14 +443,6 @@ __global__ __launch_bounds__(TPB) void k_step_list(Params<D> P, Bufs<D> B) {
 // Half-list variant
 // ---------------------------------------------------------------------------------------------
 
-// A half-list chunk's tag in the top four bits of its words: owner bits 0-3 in x, 4-7 in y, 8-10
-// in z (bit 31 of z: in-block), entries - 1 in bits 28-29 of w and owner bit 11 in bit 30.
-__device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entries, bool in) {
-    const uint32_t lo = 0x0fffffffu;
-    return make_uint4((q.x & lo) | (own << 28), (q.y & lo) | ((own >> 4) << 28), (q.z & lo) | (((own >> 8) & 7u) << 28) | (in ? 1u << 31 : 0u),
-                      (q.w & lo) | ((entries - 1u) << 28) | (((own >> 11) & 1u) << 30));
-}
-
 // List build from f32 cell-relative coordinates: d = (r_j + offset of j's cell) - r_i, within
 // ~1e-6 sigma of the exact separation, tested against the same (rc + skin)(1 + 1e-4) radius, so
 // every pair the exact test admits is admitted (the 1e-4 margin is ~3e-4 sigma).  Walled axes do
@@ -673,11 +665,9 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
         for (; g < nch; g += NT) {
             const uint4 qt = q_next;
             if (g + NT < nch) q_next = chunks[slot(g + NT)];
-            // the tag (tag_chunk) off the top bits, the entries below them
-            const uint32_t own = (qt.x >> 28) | ((qt.y >> 28) << 4) | (((qt.z >> 28) & 7u) << 8) | (((qt.w >> 30) & 1u) << 11);
-            const uint32_t m = ((qt.w >> 28) & 3u) + 1u;
-            const bool inblock = (qt.z >> 31) != 0u;
-            const uint4 q = make_uint4(qt.x & 0x0fffffffu, qt.y & 0x0fffffffu, qt.z & 0x0fffffffu, qt.w & 0x0fffffffu);
+            uint32_t own, m;
+            bool inblock;
+            const uint4 q = untag_chunk(qt, own, m, inblock);
             const uint32_t i = base + own;
             double xi[D];
 #pragma unroll

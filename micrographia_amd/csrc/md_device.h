@@ -107,6 +107,23 @@ __device__ __forceinline__ bool sep_f(double d, float& f) {
 // The conversion alone, for list pairs whose separations are known to be in range.
 __device__ __forceinline__ float sep_f_in_range(double d) { return (float)__builtin_trunc(d * 0.0625) * 0x1p-28f; }
 
+// A half-list chunk's tag in the top four bits of its words (indices below 2^28): owner bits 0-3
+// in x, 4-7 in y, 8-10 in z (bit 31 of z: in-block), entries - 1 in bits 28-29 of w and owner
+// bit 11 in bit 30.
+__device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entries, bool in) {
+    const uint32_t lo = 0x0fffffffu;
+    return make_uint4((q.x & lo) | (own << 28), (q.y & lo) | ((own >> 4) << 28), (q.z & lo) | (((own >> 8) & 7u) << 28) | (in ? 1u << 31 : 0u),
+                      (q.w & lo) | ((entries - 1u) << 28) | (((own >> 11) & 1u) << 30));
+}
+
+// The indices of a tagged chunk, and its owner, entry count and in-block flag.
+__device__ __forceinline__ uint4 untag_chunk(uint4 qt, uint32_t& own, uint32_t& entries, bool& in) {
+    own = (qt.x >> 28) | ((qt.y >> 28) << 4) | (((qt.z >> 28) & 7u) << 8) | (((qt.w >> 30) & 1u) << 11);
+    entries = ((qt.w >> 28) & 3u) + 1u;
+    in = (qt.z >> 31) != 0u;
+    return make_uint4(qt.x & 0x0fffffffu, qt.y & 0x0fffffffu, qt.z & 0x0fffffffu, qt.w & 0x0fffffffu);
+}
+
 // The same two steps on positions held in sixteenths (x / 16, exact for the integer positions).
 // Every f64 quantity is its unscaled value times 2^-4 exactly, and so is every rounding: d * inv
 // is the same product (2^-4 d times 2^4 inv), and the fma's single rounding scales with its
