@@ -3,9 +3,12 @@
 //   k_bb_count           per slot of a rebuilding system: linear cell, rank in the cell (atomic)
 //   k_bb_scan            per system (one work-group): cell counts -> cell ranges, counts cleared
 //   k_bb_scatter         per slot: move to its cell-sorted slot in the other buffers
-//   k_bb_build           per slot: half list inside 256-slot blocks + full list across blocks
+//   k_bb_build           per slot (one launch per kind of system): in-block pairs at their
+//                        lower-index end, pairs across 256-slot blocks at both ends, as a tagged
+//                        chunk stream per 64-slot wave
 //   k_bb_commit_rebuild  per system
-//   k_bb_step            per slot: forces, leapfrog, half-skin check (every active system)
+//   k_bb_step            per 256-slot work-group (one launch per kind): its chunks dealt
+//                        evenly over the lanes, then per slot the leapfrog and half-skin check
 //   k_bb_commit_step     per system
 // Every system takes exactly one step per iteration (its rebuild, when due, runs first in the
 // same iteration), so the host enqueues iterations without reading anything back and a chunk of
@@ -305,11 +308,14 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     if (__lane_id() == 0u) B.wave_chunks[slot >> 6] = min(fill, capw);
 }
 
+// One launch per kind of system present, as for the step (each keeps only its own registers).
+template <int D, bool MULTI>
 __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B) {
     const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
     uint32_t s = B.wg_sys[wg];
     if (s == ~0u) return;
     const BSys& S = B.sys[s];
+    if (S.dim != (uint32_t)D || (S.nspecies > 1) != MULTI) return;
     BCtl& C = B.ctl[s];
     if (!C.do_rebuild) return;
     uint32_t local = wg * BATCH_WG + threadIdx.x - S.slot0;
@@ -318,14 +324,7 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B) {
         if ((threadIdx.x & 63u) == 0u) B.wave_chunks[(wg * BATCH_WG + threadIdx.x) >> 6] = 0u;
         return;
     }
-    bool multi = S.nspecies > 1;
-    if (S.dim == 3) {
-        if (multi) bb_build<3, true>(B, S, C, local);
-        else bb_build<3, false>(B, S, C, local);
-    } else {
-        if (multi) bb_build<2, true>(B, S, C, local);
-        else bb_build<2, false>(B, S, C, local);
-    }
+    bb_build<D, MULTI>(B, S, C, local);
 }
 
 __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
@@ -911,7 +910,10 @@ void Batch::enqueue_iteration() {
     k_bb_count<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
     k_bb_scatter<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    k_bb_build<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 1u) k_bb_build<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 2u) k_bb_build<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 4u) k_bb_build<3, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 8u) k_bb_build<3, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
     if (kinds_ & 1u) k_bb_step<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     if (kinds_ & 2u) k_bb_step<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
