@@ -184,7 +184,7 @@ int rh_engine3_set_sort_interval(rh_engine3* e, uint32_t interval);             
 /* Kernel choice (this engine's own tuning knob, no reference counterpart): variant 0 = full
  * Verlet list, one lane per particle; 1 = half list inside work-group blocks of `block` particles
  * (256, 512, 1024, 2048 or 4096; the partner's kick accumulated in LDS) and full lists across blocks.
- * The default is variant 1 with blocks of 512, or 2048 from 2^20 particles.  Every setting gives
+ * The default is variant 1 with blocks of 512, 1024 from 2^18 particles, 2048 from 2^20.  Every setting gives
  * the same trajectory.  Forces a list rebuild. */
 int rh_engine3_tune(rh_engine3* e, int variant, uint32_t block);
 /* The engine's HIP stream (hipStream_t), for callers that time or order work around it. */

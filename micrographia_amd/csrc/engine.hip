@@ -1605,9 +1605,9 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     if (mode.kind == 1) RH_REQUIRE(mode.skin >= 0.0 && mode.cap >= 1, "Verlet mode needs skin >= 0 and cap >= 1");
     for (uint64_t i = 0; i < n; i++) RH_REQUIRE(species[i] < sys.nspecies, "species out of range");
     std::memset(&B_, 0, sizeof B_);
-    // Large systems default to 2048-particle blocks (fewer pairs cross blocks); small ones keep
-    // enough blocks to fill the chip.
-    block_ = n >= (uint64_t(1) << 20) ? 2048u : 512u;
+    // Large systems default to large blocks (fewer pairs cross blocks); smaller ones keep enough
+    // blocks to fill the chip (measured best: 512 below 2^18 particles, 1024 to 2^20).
+    block_ = n >= (uint64_t(1) << 20) ? 2048u : (n >= (uint64_t(1) << 18) ? 1024u : 512u);
     build_params(sys, mode);
     RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (int p = 0; p < 2; p++) {
