@@ -210,7 +210,7 @@ def test_storage_permutation_does_not_change_the_trajectory():
     assert np.array_equal(ra.pos[perm], rb.pos) and np.array_equal(ra.vel[perm], rb.vel)
 
 
-@pytest.mark.parametrize("variant", [(0, 256), (1, 512)])
+@pytest.mark.parametrize("variant", [(0, 256), (1, 512), (1, 2048), (1, 4096)])
 def test_a_tiny_list_capacity_regrows_and_stays_exact(variant):
     sys, s = liquid(3000, 0.8442, 1.2)
     gpu = engine(sys, s, rh.ForceMode(1, 0.3, 12), *variant)
@@ -218,6 +218,25 @@ def test_a_tiny_list_capacity_regrows_and_stays_exact(variant):
     stats = gpu.read_list_stats()
     assert stats.cap > 12
     assert gpu.read_state() == oracle_run(sys, s, 150)
+
+
+def test_block_size_switches_mid_run_stay_exact():
+    """The half list is rebuilt for each block size (its chunk tags name owners within a block);
+    switching between sizes and to the full list mid-run leaves the trajectory the twin's."""
+    sys, s0 = liquid(4500, 0.8442, 1.2)
+    gpu = engine(sys, s0, rh.verlet3_with_skin(sys, s0.n(), 0.3), 1, 512)
+    for variant, block in ((1, 512), (1, 2048), (0, 256), (1, 1024), (1, 4096), (1, 256)):
+        gpu.tune(variant, block)
+        gpu.step(30)
+    assert gpu.read_state() == oracle_run(sys, s0, 180)
+
+
+def test_a_few_particles_fill_part_of_one_wave():
+    """100 particles: one full 64-particle wave of the list build and one partly filled."""
+    sys, s0 = liquid(100, 0.1, 1.0)
+    gpu = engine(sys, s0, rh.verlet3_with_skin(sys, s0.n(), 0.3), 1, 256)
+    gpu.step(300)
+    assert gpu.read_state() == oracle_run(sys, s0, 300)
 
 
 def test_mixture_pinned_gravity_soft_walls_match_the_twin():
