@@ -50,7 +50,6 @@ struct BBufs {
     uint32_t* key;
     uint32_t* rank;
     uint32_t* nbr;
-    uint32_t* nbr_count;
     uint32_t* cell_count;
     uint32_t* cell_start;
     uint32_t* cell_end;

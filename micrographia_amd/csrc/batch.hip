@@ -26,7 +26,7 @@ namespace rh {
 
 constexpr uint32_t LIST_UNIT = 4096;  // list-arena allocation unit (entries)
 constexpr uint32_t CELL_UNIT = 256;   // cell-arena allocation unit (cells)
-constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 4;
+constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 1;  // per slot: pos, vel, species+ids, ref, aos, rel, key+rank, wave counts (1/16, rounded up)
 
 // ---------------------------------------------------------------------------------------------
 // Kernels
@@ -299,12 +299,8 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     store((na & 3u) != 0u, fa, na & 3u, true);
     store((nb & 3u) != 0u, fb, nb & 3u, false);
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
-    if (ca + cb > cap4) {
-        atomicOr(&C.overflow, 1u);
-        na = nb = 0;
-    }
+    if (ca + cb > cap4) atomicOr(&C.overflow, 1u);
     if (4u * (ca + cb) > *(volatile const uint32_t*)&C.maxcount) atomicMax(&C.maxcount, 4u * (ca + cb));
-    B.nbr_count[slot] = na | (nb << 16);
     if (__lane_id() == 0u) B.wave_chunks[slot >> 6] = min(fill, capw);
 }
 
@@ -647,7 +643,6 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     RH_HIP(hipMemset(B_.wave_chunks, 0, slot_cap / 64 * 4));
     B_.key = (uint32_t*)dalloc(slot_cap * 4);
     B_.rank = (uint32_t*)dalloc(slot_cap * 4);
-    B_.nbr_count = (uint32_t*)dalloc(slot_cap * 4);
     B_.nbr = (uint32_t*)dalloc(list_cap * 4);
     B_.cell_count = (uint32_t*)dalloc(cell_cap * 4);
     B_.cell_start = (uint32_t*)dalloc(cell_cap * 4);

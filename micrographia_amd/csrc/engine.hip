@@ -589,13 +589,8 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     store((na & 3u) != 0u, fa, na & 3u, true);
     store((nb & 3u) != 0u, fb, nb & 3u, false);
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
-    const bool over = ca + cb > cap4;
-    if (over) {
-        atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
-        na = nb = 0;
-    }
+    if (ca + cb > cap4) atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
     if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
-    B.nbr_count[i] = na | (nb << 16);
     if (__lane_id() == 0u) B.wave_chunks[wave] = min(fill, capw);
 }
 
