@@ -497,6 +497,34 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
 #pragma unroll
     for (int a = 0; a < D; a++) wl[a] = (P.nc[a] == 1u && ((P.periodic >> a) & 1u)) ? P.cell_w[a] : 0.f;
     uint32_t na = 0, nb = 0;
+    // Hit masks wait in a three-entry queue per lane (mask, first candidate) and are visited
+    // one hit per step of a wave-wide loop, run only when some lane's queue is full and at the
+    // end: the loop's length follows the busiest lane's hits over many cells, not the sum over
+    // cells of each cell's busiest lane.
+    uint32_t m0 = 0u, m1 = 0u, m2 = 0u, b0 = 0u, b1 = 0u, b2 = 0u, qn = 0u;
+    auto visit = [&]() {
+        bool full = false, in = false;
+        if (qn != 0u) {
+            const uint32_t b = __builtin_ctz(m0);
+            m0 &= m0 - 1u;
+            const uint32_t j = b0 + b;
+            in = j - blo < P.block;
+            if (in) {
+                put4(fa, na, j);
+                na++;
+                full = (na & 3u) == 0u;
+            } else {
+                put4(fb, nb, j);
+                nb++;
+                full = (nb & 3u) == 0u;
+            }
+            if (m0 == 0u) {
+                m0 = m1; b0 = b1; m1 = m2; b1 = b2; m2 = 0u;
+                qn--;
+            }
+        }
+        store(full, in ? fa : fb, 4u, in);
+    };
     for (int o = 0; o < stencil_size(D); o++) {
         const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
         uint32_t cc[D];
@@ -525,11 +553,11 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
         const float sx = shift[0], sy = shift[1], sz = shift[2];
         // Candidates in groups of up to 32: a distance pass sets one bit per candidate within
         // the list radius, bit arithmetic drops the in-block ones up to i (i itself included),
-        // and the surviving bits are visited to store entries.
+        // and a nonzero mask joins the queue.
         for (;; base += 32u) {
             const bool act = base < e0;
             if (ballot64(act) == 0ull) break;
-            uint32_t mask = 0u, inb = 0u;
+            uint32_t mask = 0u;
             if (act) {
                 const uint32_t cnt = min(32u, e0 - base);
                 // Candidates four at a time, unrolled (addresses by immediate offset, the bit by
@@ -559,32 +587,20 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
                 mask &= bits_below((int)cnt);
                 // j = base + u: gt: j > i; inb: j in i's block
                 const uint32_t gt = ~bits_below((int)i + 1 - (int)base);
-                inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
+                const uint32_t inb = bits_below((int)(blo + P.block) - (int)base) & ~bits_below((int)blo - (int)base);
                 mask &= (inb & gt) | ~inb;
             }
-            for (;;) {
-                const bool hit = mask != 0u;
-                if (ballot64(hit) == 0ull) break;
-                bool full = false, in = false;
-                if (hit) {
-                    const uint32_t b = __builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    const uint32_t j = base + b;
-                    in = (inb >> b) & 1u;
-                    if (in) {
-                        put4(fa, na, j);
-                        na++;
-                        full = (na & 3u) == 0u;
-                    } else {
-                        put4(fb, nb, j);
-                        nb++;
-                        full = (nb & 3u) == 0u;
-                    }
-                }
-                store(full, in ? fa : fb, 4u, in);
+            const bool push = mask != 0u;
+            while (ballot64(push && qn == 3u) != 0ull) visit();
+            if (push) {
+                if (qn == 0u) { m0 = mask; b0 = base; }
+                else if (qn == 1u) { m1 = mask; b1 = base; }
+                else { m2 = mask; b2 = base; }
+                qn++;
             }
         }
     }
+    while (ballot64(qn != 0u) != 0ull) visit();
     // the partial last chunks; a list longer than the capacity is flagged and regrown
     store((na & 3u) != 0u, fa, na & 3u, true);
     store((nb & 3u) != 0u, fb, nb & 3u, false);
