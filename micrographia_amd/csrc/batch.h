@@ -29,6 +29,11 @@ struct BSys {
     float kick_c[4];
     int32_t gravity[3];
     float half_skin2;
+    // stochastic boundaries (3D only): the Andersen bath and heat-bath walls
+    uint32_t stoch, bath, bath_species, thermal_walls, seed_lo, seed_hi;
+    double dt, bath_p;
+    double bath_sd[4];     // sqrt(T / m) of the bath per species (sigma/tau)
+    double wall_sd[6][4];  // sqrt(T / m) per heat-bath wall and species
     PairTable pt;
 };
 

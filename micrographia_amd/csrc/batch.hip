@@ -8,7 +8,8 @@
 //                        chunk stream per 64-slot wave
 //   k_bb_commit_rebuild  per system
 //   k_bb_step            per 256-slot work-group (one launch per kind): its chunks dealt
-//                        evenly over the lanes, then per slot the leapfrog and half-skin check
+//                        evenly over the lanes, then per slot the leapfrog (with the 3D bath
+//                        and heat-bath walls in the stochastic kind) and half-skin check
 //   k_bb_commit_step     per system
 // Every system takes exactly one step per iteration (its rebuild, when due, runs first in the
 // same iteration), so the host enqueues iterations without reading anything back and a chunk of
@@ -342,7 +343,7 @@ __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S
 // One 256-slot work-group: as the single-system step kernel, the block's chunks (its four
 // waves' regions, numbered in order) are dealt out evenly over the lanes, every kick goes to the
 // block's LDS accumulators, and positions are staged and gathered as sixteenths.
-template <int D, bool MULTI>
+template <int D, bool MULTI, bool STOCH>
 __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t wg) {
     constexpr uint32_t NW = BATCH_WG / 64u;
     __shared__ uint32_t acc[D][BATCH_WG];
@@ -489,11 +490,24 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     int32_t v[D];
     double x[D];
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-        v[a] = (int32_t)((uint32_t)B.vel[pm][a][slot] + k[a] + (pin ? 0u : (uint32_t)S.gravity[a]));
-        x[a] = xi[a] + (double)v[a];
+    for (int a = 0; a < D; a++) v[a] = (int32_t)((uint32_t)B.vel[pm][a][slot] + k[a] + (pin ? 0u : (uint32_t)S.gravity[a]));
+    if constexpr (STOCH && D == 3) {
+        // the Andersen bath, then the move and the walls, heat-bath ones drawing the velocity
+        // (the step's clock is the system's before this step is committed)
+        const uint32_t id = B.ids[pm][slot], clock = C.clock;
+        double u[3];
+        if (S.bath && andersen_draw(S, id, clock, si, u)) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) v[a] = to_step(S, u[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) x[a] = xi[a] + (double)v[a];
+        bound_thermal3(S, x, v, id, clock, si);
+    } else {
+#pragma unroll
+        for (int a = 0; a < D; a++) x[a] = xi[a] + (double)v[a];
+        bound<D>(S, x, v);
     }
-    bound<D>(S, x, v);
     float r2 = 0.0f;
     bool far = false;
 #pragma unroll
@@ -511,19 +525,19 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         atomicOr(&C.stale_next, 1u);
 }
 
-// One launch per kind of system present (dimension, one or more species), each compiled with
-// only its own path so the common single-species kernels keep their registers low; work-groups
-// of other kinds exit at once.
-template <int D, bool MULTI>
+// One launch per kind of system present (dimension, one or more species, stochastic boundaries
+// or none), each compiled with only its own path so the common kernels keep their registers
+// low; work-groups of other kinds exit at once.
+template <int D, bool MULTI, bool STOCH>
 __global__ __launch_bounds__(BATCH_WG) void k_bb_step(BBufs B) {
     const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
     uint32_t s = B.wg_sys[wg];
     if (s == ~0u) return;
     const BSys& S = B.sys[s];
-    if (S.dim != (uint32_t)D || (S.nspecies > 1) != MULTI) return;
+    if (S.dim != (uint32_t)D || (S.nspecies > 1) != MULTI || (S.stoch != 0u) != STOCH) return;
     BCtl& C = B.ctl[s];
     if (!bb_runs(S, C)) return;
-    bb_step<D, MULTI>(B, S, C, wg);
+    bb_step<D, MULTI, STOCH>(B, S, C, wg);
 }
 
 __global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys) {
@@ -677,9 +691,18 @@ void Batch::budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) c
     if (cell_cap) *cell_cap = cells_.capacity() * CELL_UNIT;
 }
 
+// A 3D system with the Andersen bath or a heat-bath wall: its steps draw random numbers.
+static bool stochastic(const SystemDesc& sys) {
+    if (sys.dim != 3) return false;
+    bool thermal = false;
+    for (int a = 0; a < 3; a++)
+        for (int side = 0; side < 2; side++) thermal |= sys.walled[a] && sys.wall_kind[a][side] == RH_WALL_THERMAL;
+    return sys.has_bath || thermal;
+}
+
 // Cells, list capacity and constants of one system, as Params::new (engine3.rs:70-162).
 void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const {
-    RH_REQUIRE(!sys.has_bath, "the Andersen heat bath is not supported by the batched engine");
+    RH_REQUIRE(!sys.has_bath || sys.dim == 3, "the 2D thermostat region is not supported by the batched engine");
     RH_REQUIRE(!sys.compartments[0] && !sys.compartments[1] && !sys.has_membrane && !sys.has_piston && sys.bonds.empty(),
                "compartments, membranes, pistons and bonds are not supported by the batched engine");
     const int D = sys.dim;
@@ -726,10 +749,25 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         if (!sys.walled[a]) d.periodic |= 1u << a;
         else
             for (int side = 0; side < 2; side++) {
-                RH_REQUIRE(sys.wall_kind[a][side] != RH_WALL_THERMAL, "heat-bath walls are not supported by this engine yet");
+                RH_REQUIRE(D == 3 || sys.wall_kind[a][side] != RH_WALL_THERMAL, "2D heat-bath walls are not supported by the batched engine");
                 if (sys.wall_kind[a][side] == RH_WALL_SOFT) d.soft_walls |= 1u << (2 * a + side);
+                if (sys.wall_kind[a][side] == RH_WALL_THERMAL) {
+                    d.thermal_walls |= 1u << (2 * a + side);
+                    for (uint32_t sp = 0; sp < sys.nspecies; sp++) d.wall_sd[2 * a + side][sp] = std::sqrt(sys.wall_t[a][side] / sys.masses[sp]);
+                }
             }
         d.gravity[a] = core::gravity_kick(sys.dt, sys.gravity[a]);
+    }
+    // stochastic boundaries (3D): Threefry keyed by the system seed, as the single-system engine
+    d.stoch = stochastic(sys) ? 1u : 0u;
+    d.seed_lo = (uint32_t)sys.seed;
+    d.seed_hi = (uint32_t)(sys.seed >> 32);
+    d.dt = sys.dt;
+    if (sys.has_bath) {
+        d.bath = 1u;
+        d.bath_species = sys.bath_species;
+        d.bath_p = sys.bath_rate * sys.dt;
+        for (uint32_t sp = 0; sp < sys.nspecies; sp++) d.bath_sd[sp] = std::sqrt(sys.bath_t / sys.masses[sp]);
     }
     RH_REQUIRE(ncells < (uint64_t(1) << 31), "too many cells");
     d.ncells = (uint32_t)ncells;
@@ -885,7 +923,11 @@ void Batch::upload_tables() {
     RH_HIP(hipMemcpy(d_wg_, wg.data(), wg.size() * 4, hipMemcpyHostToDevice));
     uint32_t new_wgs = (uint32_t)std::max<uint64_t>(hw, 1);
     uint32_t kinds = 0;
-    for (auto& kv : members_) kinds |= 1u << ((kv.second.sys.dim == 3 ? 2u : 0u) + (kv.second.sys.nspecies > 1 ? 1u : 0u));
+    // bits 0-3: (3D, several species) without stochastic boundaries; bits 4-5: 3D with them
+    for (auto& kv : members_) {
+        const SystemDesc& sy = kv.second.sys;
+        kinds |= 1u << ((stochastic(sy) ? 4u : sy.dim == 3 ? 2u : 0u) + (sy.nspecies > 1 ? 1u : 0u));
+    }
     if (new_wgs != grid_wgs_ || kinds != kinds_) invalidate_graph();
     grid_wgs_ = new_wgs;
     kinds_ = kinds;
@@ -907,13 +949,15 @@ void Batch::enqueue_iteration() {
     k_bb_scatter<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     if (kinds_ & 1u) k_bb_build<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     if (kinds_ & 2u) k_bb_build<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 4u) k_bb_build<3, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 8u) k_bb_build<3, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & (4u | 16u)) k_bb_build<3, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & (8u | 32u)) k_bb_build<3, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
-    if (kinds_ & 1u) k_bb_step<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 2u) k_bb_step<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 4u) k_bb_step<3, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 8u) k_bb_step<3, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 1u) k_bb_step<2, false, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 2u) k_bb_step<2, true, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 4u) k_bb_step<3, false, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 8u) k_bb_step<3, true, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 16u) k_bb_step<3, false, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    if (kinds_ & 32u) k_bb_step<3, true, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
     k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
 }
 

@@ -97,11 +97,8 @@ __global__ void k_flip_pp_pm(uint32_t* ctl) {
 // (IntEngine3::step, int_engine3.rs:138-206; three.rs:232-265)
 // ---------------------------------------------------------------------------------------------
 
-constexpr uint32_t STREAM_WALL = 0x57414c4cu;    // walls::STREAM_WALL
-constexpr uint32_t STREAM_THERMO = 0x5448524du;  // walls::STREAM_THERMO
 constexpr double LEDGER_ONE = 4294967296.0;      // ledger entries in 2^-32 units
 constexpr int LEDGER_WORDS = 21;                 // 7 entries x (energy, impulse, hits)
-constexpr double TAU_D = 6.283185307179586;
 
 // The ledger sums fixed-point values with 64-bit integer atomics, so its totals do not depend on
 // the order of the additions.
@@ -111,45 +108,13 @@ __device__ __forceinline__ void ledger_add(unsigned long long* L, int w, int fie
     atomicAdd(&L[3 * w + field], (unsigned long long)(long long)q);
 }
 
-// threefry::to_unit_f64 (threefry.rs:27-33): 53 random bits in [0, 1)
-__device__ __forceinline__ double unit_f64(uint32_t r0, uint32_t r1) {
-    const uint64_t bits = (((uint64_t)r0 << 21) ^ (uint64_t)(r1 >> 11)) & ((uint64_t(1) << 53) - 1u);
-    return (double)bits * (1.0 / 9007199254740992.0);
-}
-
-// Rust `as i32` of round(v dt 2^32): a velocity (sigma/tau) as a displacement per step.
-template <int D>
-__device__ __forceinline__ int32_t to_step(const Params<D>& P, double v) {
-    const double x = round(v * P.dt * 4294967296.0);
-    if (x != x) return 0;
-    if (x >= 2147483647.0) return INT32_MAX;
-    if (x <= -2147483648.0) return INT32_MIN;
-    return (int32_t)x;
-}
-
-template <int D>
-__device__ __forceinline__ double velocity(const Params<D>& P, int32_t v) {
-    return (double)v * (1.0 / 4294967296.0) / P.dt;
-}
-
 // md_core::three::andersen: with probability rate dt a fresh Maxwell velocity, booked as the
 // bath's (ledger entry 6) energy and hit.
 template <int D>
 __device__ void bath_redraw(const Params<D>& P, unsigned long long* L, uint32_t id, uint32_t clock, uint32_t s, int32_t v[D]) {
     if constexpr (D == 3) {
-        if (!((P.bath_species >> s) & 1u) || ((P.pinned >> s) & 1u)) return;
-        const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_THERMO;
-        uint32_t r[2], r2[2], r3[2];
-        threefry2x32(id, clock, k0, k1, r);
-        if (unit_f64(r[0], 0u) >= P.bath_p) return;
-        threefry2x32(id, clock, k0, k1 ^ 1u, r2);
-        threefry2x32(id, clock, k0, k1 ^ 2u, r3);
-        const double sd = P.bath_sd[s];
-        const double rad = sqrt(-2.0 * log(1.0 - unit_f64(r[1], 0u)));
-        const double a = TAU_D * unit_f64(r2[0], 0u);
-        const double rad3 = sqrt(-2.0 * log(1.0 - unit_f64(r3[0], 0u)));
-        const double a3 = TAU_D * unit_f64(r3[1], 0u);
-        const double u[3] = {sd * rad * cos(a), sd * rad * sin(a), sd * rad3 * cos(a3)};
+        double u[3];
+        if (!andersen_draw(P, id, clock, s, u)) return;
         double before = 0.0, after = 0.0;
 #pragma unroll
         for (int b = 0; b < 3; b++) {
@@ -199,20 +164,11 @@ __device__ void bound_booked(const Params<D>& P, unsigned long long* L, double x
         if constexpr (D == 3) thermal = (P.thermal_walls >> w) & 1u;
         if (thermal) {
             if constexpr (D == 3) {
-                const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_WALL;
-                uint32_t r[2], r2[2];
-                threefry2x32(id, clock, k0, k1, r);
-                threefry2x32(id, clock, k0, k1 ^ 1u, r2);
-                const double u1 = 1.0 - unit_f64(r[0], r[1]);
-                const double u2 = 1.0 - unit_f64(r2[0], 0u), u3 = unit_f64(r2[1], 0u);
-                const double sd = P.wall_sd[w][s];
-                const double inward = side == 0 ? 1.0 : -1.0;
-                const double vn = inward * sd * sqrt(-2.0 * log(u1));
-                const double rad = sd * sqrt(-2.0 * log(u2));
-                const double ang = TAU_D * u3;
+                double vn, t0, t1;
+                thermal_wall_draw(P, id, clock, w, side, s, vn, t0, t1);
                 v[a] = to_step(P, vn);
-                v[(a + 1) % 3] = to_step(P, rad * cos(ang));
-                v[(a + 2) % 3] = to_step(P, rad * sin(ang));
+                v[(a + 1) % 3] = to_step(P, t0);
+                v[(a + 2) % 3] = to_step(P, t1);
             }
         } else {
             v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);

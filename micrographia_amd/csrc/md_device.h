@@ -5,6 +5,7 @@
 // soft_walls).
 #pragma once
 
+#include "prims.h"
 #include "rh_common.h"
 
 namespace rh {
@@ -157,6 +158,111 @@ __device__ __forceinline__ void soft_walls(const PT& P, const double x[D], float
                 if (wall_pe) *wall_pe += (double)(4.0f * s6 * (s6 - 1.0f) + 1.0f);
                 if (wf) wf[2 * a + side] = fw;
             }
+        }
+    }
+}
+
+// Stochastic boundaries (md_core::three, three.rs:232-265): Threefry streams keyed by (particle
+// id, clock) under the system seed.  The draws are shared by the engines that book exchanges in
+// a ledger and those that keep none, so both move particles identically.
+constexpr uint32_t STREAM_WALL = 0x57414c4cu;    // walls::STREAM_WALL
+constexpr uint32_t STREAM_THERMO = 0x5448524du;  // walls::STREAM_THERMO
+constexpr double TAU_D = 6.283185307179586;
+
+// threefry::to_unit_f64 (threefry.rs:27-33): 53 random bits in [0, 1)
+__device__ __forceinline__ double unit_f64(uint32_t r0, uint32_t r1) {
+    const uint64_t bits = (((uint64_t)r0 << 21) ^ (uint64_t)(r1 >> 11)) & ((uint64_t(1) << 53) - 1u);
+    return (double)bits * (1.0 / 9007199254740992.0);
+}
+
+// Rust `as i32` of round(v dt 2^32): a velocity (sigma/tau) as a displacement per step.
+template <class PT>
+__device__ __forceinline__ int32_t to_step(const PT& P, double v) {
+    const double x = round(v * P.dt * 4294967296.0);
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return INT32_MAX;
+    if (x <= -2147483648.0) return INT32_MIN;
+    return (int32_t)x;
+}
+
+template <class PT>
+__device__ __forceinline__ double velocity(const PT& P, int32_t v) {
+    return (double)v * (1.0 / 4294967296.0) / P.dt;
+}
+
+// md_core::three::andersen's draw: with probability rate dt, for a bath species that is not
+// pinned, a fresh Maxwell velocity u (sigma/tau); false otherwise.
+template <class PT>
+__device__ __forceinline__ bool andersen_draw(const PT& P, uint32_t id, uint32_t clock, uint32_t s, double u[3]) {
+    if (!((P.bath_species >> s) & 1u) || ((P.pinned >> s) & 1u)) return false;
+    const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_THERMO;
+    uint32_t r[2], r2[2], r3[2];
+    threefry2x32(id, clock, k0, k1, r);
+    if (unit_f64(r[0], 0u) >= P.bath_p) return false;
+    threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+    threefry2x32(id, clock, k0, k1 ^ 2u, r3);
+    const double sd = P.bath_sd[s];
+    const double rad = sqrt(-2.0 * log(1.0 - unit_f64(r[1], 0u)));
+    const double a = TAU_D * unit_f64(r2[0], 0u);
+    const double rad3 = sqrt(-2.0 * log(1.0 - unit_f64(r3[0], 0u)));
+    const double a3 = TAU_D * unit_f64(r3[1], 0u);
+    u[0] = sd * rad * cos(a);
+    u[1] = sd * rad * sin(a);
+    u[2] = sd * rad3 * cos(a3);
+    return true;
+}
+
+// The velocity a heat-bath wall w hands a particle of species s that crossed it (three.rs:232-243):
+// the normal component pointing back in (side 0 is the low wall), the two tangential ones from
+// one Box-Muller pair.
+template <class PT>
+__device__ __forceinline__ void thermal_wall_draw(const PT& P, uint32_t id, uint32_t clock, int w, int side, uint32_t s, double& vn, double& t0, double& t1) {
+    const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_WALL;
+    uint32_t r[2], r2[2];
+    threefry2x32(id, clock, k0, k1, r);
+    threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+    const double u1 = 1.0 - unit_f64(r[0], r[1]);
+    const double u2 = 1.0 - unit_f64(r2[0], 0u), u3 = unit_f64(r2[1], 0u);
+    const double sd = P.wall_sd[w][s];
+    const double inward = side == 0 ? 1.0 : -1.0;
+    vn = inward * sd * sqrt(-2.0 * log(u1));
+    const double rad = sd * sqrt(-2.0 * log(u2));
+    const double ang = TAU_D * u3;
+    t0 = rad * cos(ang);
+    t1 = rad * sin(ang);
+}
+
+// The 3D wrap / mirror with heat-bath walls and no ledger: a particle that crossed a heat-bath
+// wall leaves it with a drawn velocity, one that crossed a hard or soft wall with its normal
+// velocity reversed.
+template <class PT>
+__device__ __forceinline__ void bound_thermal3(const PT& P, double x[3], int32_t v[3], uint32_t id, uint32_t clock, uint32_t s) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double l = P.boxf[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0.0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+            continue;
+        }
+        int side = -1;
+        if (x[a] < 0.0) {
+            x[a] = -1.0 - x[a];
+            side = 0;
+        } else if (x[a] >= l) {
+            x[a] = 2.0 * l - 1.0 - x[a];
+            side = 1;
+        }
+        if (side < 0) continue;
+        const int w = 2 * a + side;
+        if ((P.thermal_walls >> w) & 1u) {
+            double vn, t0, t1;
+            thermal_wall_draw(P, id, clock, w, side, s, vn, t0, t1);
+            v[a] = to_step(P, vn);
+            v[(a + 1) % 3] = to_step(P, t0);
+            v[(a + 2) % 3] = to_step(P, t1);
+        } else {
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
         }
     }
 }

@@ -1,6 +1,8 @@
 """The batched engine: every system in a mixed batch (2D and 3D, small and large, single-cell
 boxes, mixtures) is bit-identical to the oracle and to its own solo run, with and without
 hipGraph chunks, and when systems join and leave mid-run."""
+from dataclasses import replace
+
 import numpy as np
 import pytest
 
@@ -25,6 +27,12 @@ def oracle_state(sys, st, steps):
             o.gravity[a] = c.gravity[a]
             for s in range(2):
                 o.wall_kind[a][s] = c.wall_kind[a][s]
+                o.wall_t[a][s] = c.wall_t[a][s]
+        o.seed = c.seed
+        o.has_bath = c.has_bath
+        o.bath_species = c.bath_species
+        o.bath_t = c.bath_t
+        o.bath_rate = c.bath_rate
     else:
         o = O.System2()
         for a in range(2):
@@ -85,6 +93,48 @@ def test_each_system_matches_the_oracle(graph, tiles):
         want = oracle_state(sys, st, 150)
         assert got == want, f"system {i} ({type(sys).__name__}, n={st.n()}) differs from the oracle"
         assert b.list_stats(i).steps_done == 150
+
+
+def stochastic_systems():
+    """3D systems whose steps draw random numbers: the Andersen bath, heat-bath walls under
+    gravity, and a mixture with the bath on one species only."""
+    sys, st = make_system(3, 2000, 0.6, 1.0, 21)
+    bath = (sys.with_thermostat(1.6, 2.0).with_seed(77), st)
+    sys, st = make_system(3, 2000, 0.3, 1.5, 22)
+    sys = sys.with_walls(2, rh.thermal_wall(2.0), rh.thermal_wall(0.5)).with_gravity((0.0, 0.0, -0.2)).with_seed(78)
+    st.pos[:, 2] = np.clip(st.pos[:, 2], (1 << 32) // 2, sys.box_len[2] - (1 << 32) // 2)
+    walls = (sys, st)
+    sys, st = make_system(3, 3000, 0.7, 1.0, 23)
+    sys = rh.System3.new(sys.box_len, 0.005, rh.Pair.lj(2.5), [1.0, 3.0]).with_thermostat(0.8, 4.0).with_seed(79)
+    sys = replace(sys, thermostat=replace(sys.thermostat, species=0b10))
+    st.species[::3] = 1
+    st.vel = rh.maxwell(sys, st.species, 1.0, 23)
+    mix = (sys, st)
+    return [bath, walls, mix]
+
+
+@pytest.mark.parametrize("graph", [0, 50])
+def test_bath_and_heat_bath_walls_match_the_oracle(graph):
+    """The Andersen bath and heat-bath walls draw the same Threefry numbers as md-core
+    (three.rs:232-265), keyed by particle id and the system's own clock, so each system's
+    trajectory equals the oracle's bit for bit beside systems without them."""
+    stoch = stochastic_systems()
+    plain = [make_system(3, 1500, 0.8442, 1.2, 24), make_system(2, 3000, 0.7, 1.0, 25)]
+    b = BatchEngine(budget_bytes=2 * GB, max_systems=16, graph_chunk=graph)
+    steps = [150, 300, 200, 150, 150]
+    systems = stoch + plain
+    ids = [b.add(sys, st, k, skin=0.3) for (sys, st), k in zip(systems, steps)]
+    assert all(i >= 0 for i in ids)
+    b.step_all(max(steps))
+    for i, (sys, st), k in zip(ids, systems, steps):
+        want = oracle_state(sys, st, k)
+        assert b.read_state(i) == want, f"system {i} (n={st.n()}) differs from the oracle"
+    # the draws took effect: without the bath, or with hard walls in place of heat-bath ones,
+    # the trajectories differ
+    (s0, st0), (s1, st1) = stoch[0], stoch[1]
+    assert b.read_state(ids[0]) != oracle_state(replace(s0, thermostat=None), st0, steps[0])
+    hard = s1.with_walls(2, rh.Wall.Hard, rh.Wall.Hard)
+    assert b.read_state(ids[1]) != oracle_state(hard, st1, steps[1])
 
 
 def test_alone_and_in_the_mix_agree():
