@@ -6,7 +6,8 @@ admitted into the batched engine's device arena, shortest job first, as soon as 
 and cell ranges fit in the free memory (the admission rule is the device-memory budget); they run
 inside the shared launches with every other admitted request and leave when their steps are
 done, freeing their ranges for waiting requests.  Requests large enough to fill the GPU alone run
-in the single-system engine on a second lane.  State stays on the device from admission to
+in the single-system engine on a second lane, and so do requests whose systems the batched engine
+does not run (the 2D thermostat region, compartments, membrane, piston, bonds).  State stays on the device from admission to
 completion; only the final state is read back.  Each request's result is bit-identical to running
 it alone.
 """
@@ -20,6 +21,7 @@ from collections import deque
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
+from ._lib import RahmanError
 from .batch import BatchEngine
 from .workloads import Request
 
@@ -86,7 +88,9 @@ class Server:
       the GPU; more would only slow every active request down), and advance together.
     - Requests with `big_n` particles or more fill the GPU on their own: a worker thread runs
       them one at a time, shortest job first, in the single-system engine, concurrently with
-      the batch lane (each lane has its own HIP stream).
+      the batch lane (each lane has its own HIP stream).  Requests the batched engine refuses
+      (systems it does not run) take this lane too; one the single-system engine also refuses
+      is reported as rejected.
 
     Every result is bit-identical to running the request alone, whatever lane it took.
     budget_bytes: device memory for the batch arena; chunk_s: target duration of one enqueued
@@ -148,7 +152,12 @@ class Server:
                     big_queue.remove((r, f))
                 sys, st = f.result()
                 admitted = now()
-                e = _solo_engine(sys, st, self.skin)
+                try:
+                    e = _solo_engine(sys, st, self.skin)
+                except RahmanError:
+                    with lock:
+                        rejected.append(r)  # the engine refuses the system itself
+                    continue
                 try:
                     e.step(r.steps)
                     fin = e.read_state()
@@ -182,7 +191,15 @@ class Server:
                         still.append((r, fut))
                         continue
                     sys, st = fut.result()
-                    i = b.add(sys, st, r.steps, skin=self.skin)
+                    try:
+                        i = b.add(sys, st, r.steps, skin=self.skin)
+                    except RahmanError:
+                        # a system the batched engine does not run (the 2D thermostat region,
+                        # compartments, membrane, piston, bonds): the single-system lane takes it
+                        with big_cv:
+                            big_queue.append((r, fut))
+                            big_cv.notify()
+                        continue
                     if i < 0:
                         if not active:
                             rejected.append(r)  # does not fit even in an empty arena

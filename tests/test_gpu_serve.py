@@ -1,11 +1,13 @@
 """The serving loop: requests arriving over time are admitted against the device-memory budget,
 batched with whatever else is running, and each result is bit-identical to the request run
 alone in the single-system engine."""
+from dataclasses import dataclass
+
 import pytest
 
 import micrographia_amd as rh
 from micrographia_amd.serve import Server, serve_plain
-from micrographia_amd.workloads import request_stream
+from micrographia_amd.workloads import Request, request_stream
 
 pytestmark = pytest.mark.gpu
 
@@ -59,3 +61,26 @@ def test_both_lanes_together_stay_exact():
     assert any(r.n >= 8000 for r in reqs) and any(r.n < 8000 for r in reqs)
     for c in rep.completed:
         assert c.state_hash == solo_hash(c.request), f"request {c.request.index} (n={c.request.n})"
+
+
+@dataclass
+class ThermostatRequest(Request):
+    """A request whose system has a thermostat: the batched engine runs it in 3D and refuses it
+    in 2D (the thermostat region belongs to the 2D boundary path)."""
+
+    def build(self):
+        sys, st = super().build()
+        return sys.with_thermostat(1.2, 2.0), st
+
+
+def test_requests_the_batch_refuses_take_the_single_system_lane():
+    reqs = request_stream(600.0, 0.03, meta_seed=31, n_lo=500, n_hi=6000, steps_lo=40, steps_hi=150)
+    reqs = [ThermostatRequest(**{f: getattr(r, f) for f in r.__dataclass_fields__}) if k % 2 == 0 else r for k, r in enumerate(reqs)]
+    assert any(isinstance(r, ThermostatRequest) and r.dim == 2 for r in reqs)
+    assert any(isinstance(r, ThermostatRequest) and r.dim == 3 for r in reqs)
+    srv = Server(budget_bytes=1 << 30, max_systems=64, chunk_s=0.005, big_n=1 << 30)
+    rep = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert len(rep.completed) == len(reqs) and not rep.rejected
+    for c in rep.completed:
+        assert c.state_hash == solo_hash(c.request), f"request {c.request.index} (n={c.request.n}, dim={c.request.dim})"
