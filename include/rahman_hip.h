@@ -280,6 +280,17 @@ int rh_batch_bytes_needed2(rh_batch* b, const rh_system2* sys, uint64_t n, doubl
  * with respect to the host (they synchronise the stream before returning). */
 int rh_prims_exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 int rh_prims_radix_sort_u32(uint32_t* d_keys, uint32_t* d_vals, uint64_t n, uint32_t key_bits, void* stream);
+/* No counterpart in the reference: a variant of rh_prims_radix_sort_u32 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc/hipFree: the caller's arrays are one buffer of the
+ * sort's ping-pong pair, and a fixed device array inside the library (about 8.4 MB, held by each
+ * GPU it runs on until the process ends) holds the other buffer, the digit histograms and the
+ * scan's block sums.  n <= RH_SORT_NOALLOC_MAX, else RH_ERR_INVALID and nothing runs.  Same
+ * result as rh_prims_radix_sort_u32 (stable, sorted in place), but enqueued on `stream` without
+ * synchronising: the caller synchronises before reading.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (other streams, other threads) race. */
+#define RH_SORT_NOALLOC_MAX (1u << 20)
+int rh_prims_radix_sort_u32_noalloc(uint32_t* d_keys, uint32_t* d_vals, uint64_t n, uint32_t key_bits, void* stream);
 int rh_prims_reduce_sum_f64(const double* d_in, double* d_out, uint64_t n, void* stream);
 int rh_prims_threefry(uint32_t* d_out, uint64_t n, uint32_t ctr0, uint32_t ctr1, uint32_t key0, uint32_t key1, void* stream);
 int rh_prims_atomic_add_i64(const int64_t* d_vals, uint64_t n, int64_t* d_acc, void* stream);

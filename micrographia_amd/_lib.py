@@ -205,6 +205,7 @@ SIGNATURES = {
     "rh_batch_bytes_needed2": ([P, P, u64, dbl, P], i32),
     "rh_prims_exclusive_scan_u32": ([P, P, u64, P], i32),
     "rh_prims_radix_sort_u32": ([P, P, u64, u32, P], i32),
+    "rh_prims_radix_sort_u32_noalloc": ([P, P, u64, u32, P], i32),
     "rh_prims_reduce_sum_f64": ([P, P, u64, P], i32),
     "rh_prims_threefry": ([P, u64, u32, u32, u32, u32, P], i32),
     "rh_prims_atomic_add_i64": ([P, u64, P, P], i32),

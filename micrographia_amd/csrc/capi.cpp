@@ -536,6 +536,10 @@ int rh_prims_radix_sort_u32(uint32_t* d_keys, uint32_t* d_vals, uint64_t n, uint
     });
 }
 
+int rh_prims_radix_sort_u32_noalloc(uint32_t* d_keys, uint32_t* d_vals, uint64_t n, uint32_t key_bits, void* stream) {
+    return guarded([&] { rh::radix_sort_u32_fixed(d_keys, d_vals, n, key_bits, (hipStream_t)stream); });
+}
+
 int rh_prims_reduce_sum_f64(const double* d_in, double* d_out, uint64_t n, void* stream) {
     return guarded([&] {
         hipStream_t s = (hipStream_t)stream;

@@ -34,6 +34,12 @@ struct RadixSort {
     static int result_index(uint32_t key_bits) { return (int)(((key_bits + 7) / 8) & 1u); }
 };
 
+// The same sort with no allocation: the caller's arrays are one buffer of the ping-pong pair and
+// a fixed device array of the library holds the rest (at most SORT_FIXED_MAX items).  Enqueued
+// on s without synchronising.  Calls that overlap on one GPU share that array and race.
+constexpr uint64_t SORT_FIXED_MAX = 1ull << 20;
+void radix_sort_u32_fixed(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s);
+
 // Sum of n doubles into *out (a fixed-shape tree, so the result is run-to-run identical).
 size_t reduce_temp_bytes(uint64_t n);
 void reduce_sum_f64(const double* in, double* out, uint64_t n, void* temp, hipStream_t s);

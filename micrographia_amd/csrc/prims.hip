@@ -3,6 +3,9 @@
 // histograms in LDS.
 #include "prims.h"
 
+#include <mutex>
+#include <string>
+
 namespace rh {
 
 // ---------------------------------------------------------------------------------------------
@@ -191,8 +194,10 @@ void RadixSort::release() {
     cap = 0;
 }
 
-int RadixSort::sort(uint64_t n, uint32_t key_bits, const uint32_t* gate, hipStream_t s) {
-    RH_REQUIRE(n <= cap, "radix sort: more items than reserved");
+// The passes over a (keys, vals) ping-pong pair; hist holds 256 x blocks_for(n, RS_ITEMS) words
+// and scan_temp scan_temp_bytes(256 x that).  Returns the index of the pair holding the result.
+static int radix_sort_passes(uint32_t* const keys[2], uint32_t* const vals[2], uint32_t* hist, void* scan_temp, uint64_t n, uint32_t key_bits,
+                             const uint32_t* gate, hipStream_t s) {
     uint32_t passes = (key_bits + 7) / 8;
     uint32_t nb = blocks_for(n, RS_ITEMS);
     int cur = 0;
@@ -206,6 +211,76 @@ int RadixSort::sort(uint64_t n, uint32_t key_bits, const uint32_t* gate, hipStre
         cur ^= 1;
     }
     return cur;
+}
+
+int RadixSort::sort(uint64_t n, uint32_t key_bits, const uint32_t* gate, hipStream_t s) {
+    RH_REQUIRE(n <= cap, "radix sort: more items than reserved");
+    return radix_sort_passes(keys, vals, hist, scan_temp, n, key_bits, gate, s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The same sort without allocation.  Its scratch is one file-scope device array: the second
+// key/value buffer of the ping-pong pair (the caller's arrays are the first), the digit
+// histograms and the scan's block sums, sized for SORT_FIXED_MAX items.  Every GPU holds its own
+// copy, at its own address, which hipGetSymbolAddress gives once per device; host code uses that
+// address, never the array's name.  The histograms and block sums are written in full before
+// they are read, so nothing in the array needs clearing between calls.
+// ---------------------------------------------------------------------------------------------
+
+constexpr uint64_t SORT_FIXED_BLOCKS = (SORT_FIXED_MAX + RS_ITEMS - 1) / RS_ITEMS;
+constexpr size_t SORT_FIXED_HIST_WORDS = 256 * SORT_FIXED_BLOCKS;
+constexpr size_t SORT_FIXED_SCAN_BYTES = 4096;
+constexpr size_t SORT_FIXED_BYTES = 2 * SORT_FIXED_MAX * sizeof(uint32_t) + SORT_FIXED_HIST_WORDS * sizeof(uint32_t) + SORT_FIXED_SCAN_BYTES;
+
+static __device__ __align__(256) uint8_t scratch[SORT_FIXED_BYTES];
+
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SCRATCH_DEVICES];
+
+// The device address of `scratch` on the current device, looked up on the first call there.
+static uint8_t* scratch_on_current_device() {
+    int dev = -1;
+    RH_HIP(hipGetDevice(&dev));
+    RH_REQUIRE(dev >= 0 && dev < SCRATCH_DEVICES, "fixed-scratch radix sort: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        RH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RH_REQUIRE(p != nullptr, "fixed-scratch radix sort: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<uint8_t*>(g_scratch_ptr[dev]);
+}
+
+__global__ void k_copy_u32x2(const uint32_t* a, uint32_t* a_out, const uint32_t* b, uint32_t* b_out, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        a_out[i] = a[i];
+        b_out[i] = b[i];
+    }
+}
+
+void radix_sort_u32_fixed(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s) {
+    RH_REQUIRE(key_bits >= 1 && key_bits <= 32, "key_bits must be 1..32");
+    RH_REQUIRE(n <= SORT_FIXED_MAX, "fixed-scratch radix sort: n = " + std::to_string(n) + " exceeds its fixed scratch of " +
+                                        std::to_string(SORT_FIXED_MAX) + " items (use rh_prims_radix_sort_u32)");
+    if (n == 0) return;
+    RH_REQUIRE(keys != nullptr && vals != nullptr, "null key or value array");
+    const uint32_t nb = blocks_for(n, RS_ITEMS);
+    RH_REQUIRE(scan_temp_bytes(256ull * nb) <= SORT_FIXED_SCAN_BYTES, "fixed-scratch radix sort: scan scratch too small");
+    uint8_t* base = scratch_on_current_device();
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(base);
+    uint32_t* v1 = k1 + SORT_FIXED_MAX;
+    uint32_t* hist = v1 + SORT_FIXED_MAX;
+    void* scan_temp = hist + SORT_FIXED_HIST_WORDS;
+    uint32_t* const ks[2] = {keys, k1};
+    uint32_t* const vs[2] = {vals, v1};
+    if (radix_sort_passes(ks, vs, hist, scan_temp, n, key_bits, nullptr, s) == 1) {
+        uint32_t cb = blocks_for(n, 256);
+        if (cb > 1024) cb = 1024;
+        k_copy_u32x2<<<cb, 256, 0, s>>>(k1, keys, v1, vals, n);
+        RH_HIP(hipGetLastError());
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
