@@ -61,6 +61,14 @@ struct Params {
     int32_t gravity[3];
     float half_skin2;
     uint32_t block;        // half-list variant: particles per work-group block
+    // Row-ordered list build (3D half list): rows are the (y, z) cells, each binned into
+    // rows_nxf fine cells along x; rows != 0 when the geometry allows it.
+    uint32_t rows;
+    uint32_t rows_nxf;
+    double rows_xscale;    // fine x cells per position unit
+    float rows_inv_wx;     // fine x cells per sigma
+    float rows_lx;         // box length along x (sigma), the shift of a wrapped range
+    float rows_reach;      // list radius of the widest pair plus a rounding margin (sigma)
     uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
     float cell_w[3];       // cell width (sigma), f32
     double cell_w64[3];    // cell width (sigma)
@@ -128,6 +136,12 @@ struct Bufs {
     uint32_t* ids[2];
     double* ref[D];      // positions at the last list build
     float4* rel;         // position relative to its cell's corner (sigma, f32; w unused), for list builds
+    // Row-ordered copy for the row build: per entry (x sigma, y and z relative to the cell
+    // corner, slot | species << 30), ordered by row key; row_start[key] = first entry of the key
+    // (a dense exclusive scan, row_start[nkeys] = n); row_rank = a slot's place within its key.
+    float4* items;
+    uint32_t* row_start;
+    uint32_t* row_rank;
     uint32_t* nbr;       // column-major: entry k of slot i at k*n + i
     uint32_t* nbr_count;
     uint32_t* cell_start;
@@ -223,7 +237,8 @@ class Engine {
     void set_sort_interval(uint32_t v) { sort_interval_ = v; }
     // Kernel variant: 0 full Verlet list (one lane per particle, both ends of every pair), 1 half
     // list within work-group blocks of `block` particles (pairs in a block once, the partner's kick
-    // accumulated in LDS) plus the full list across blocks.
+    // accumulated in LDS) plus the full list across blocks, built from the row-ordered copy where
+    // the geometry allows it; 2 the same half list built over the cell stencil.
     void tune(int variant, uint32_t block);
     int variant() const { return variant_; }
     uint32_t block() const { return P_.block; }
@@ -251,7 +266,14 @@ class Engine {
     // The half-list kernels run unless the split path does, or list separations may reach the
     // 8 sigma range limit (then the full-list kernel keeps the reference's range test).
     // (Half-list chunks carry their tag in the top four bits of each index: below 2^28 particles.)
-    bool half_path() const { return variant_ == 1 && !split_path() && P_.range_check == 0u && n_ < (uint64_t(1) << 28); }
+    bool half_path() const { return variant_ >= 1 && !split_path() && P_.range_check == 0u && n_ < (uint64_t(1) << 28); }
+    // The half list built from the row-ordered copy (variant 1 where the geometry allows it;
+    // variant 2 keeps the cell-stencil build).
+    bool row_path() const { return D == 3 && variant_ == 1 && half_path() && P_.rows != 0u; }
+    void plan_rows();
+    void alloc_rows();
+    uint64_t row_keys_ = 0;
+    void* row_scan_temp_ = nullptr;
     void enqueue_extras(uint32_t flags);
     void write_partners();
     void flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit);

@@ -566,6 +566,269 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     if (__lane_id() == 0u) B.wave_chunks[wave] = min(fill, capw);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Row-ordered half-list build (3D)
+// ---------------------------------------------------------------------------------------------
+// After the reorder, a second copy of the particles is ordered by row key: the row is the
+// particle's (y, z) cell, and within a row particles are binned into fine cells along x
+// (counting sort: count, exclusive scan, scatter; the order within a fine cell is arbitrary,
+// which no result depends on, since list order only changes the order of integer sums).  The
+// list sphere around a particle meets at most the nine rows around its own, and in each of them
+// the candidates whose x lies within the sphere's chord over that row's (y, z) rectangle are one
+// contiguous range of the copy (two where the chord wraps a periodic x boundary).  That is
+// about 245 candidates per particle at liquid density against about 513 in the 27 cells of
+// the stencil build, and nine ranges instead of 27 cells.  Candidates are tested as in the
+// stencil build (f32 separations within ~1e-5 sigma of the exact one against the (rc + skin)
+// (1 + 1e-4) radius, a margin of ~3e-4 sigma), with x absolute (boxes up to 512 sigma along x)
+// and y, z relative to the cell corner.  The chunk stream written is the stencil build's: the
+// same pairs, tags and wave regions, in another order.
+
+// Row key of a position in cell c: (c_z * nc_y + c_y) * nxf + fine x cell.
+__device__ __forceinline__ uint32_t row_key(const Params<3>& P, const double x[3], const uint32_t c[3]) {
+    const double f = x[0] * P.rows_xscale;
+    uint32_t xf = f > 0.0 ? (uint32_t)f : 0u;
+    xf = xf < P.rows_nxf - 1u ? xf : P.rows_nxf - 1u;
+    return (c[2] * P.nc[1] + c[1]) * P.rows_nxf + xf;
+}
+
+__global__ __launch_bounds__(TPB) void k_row_count(Params<3> P, Bufs<3> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u;
+    double x[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) x[a] = B.pos[pp][a][k];
+    uint32_t c[3];
+    cell_of<3>(P, x, c);
+    B.row_rank[k] = atomicAdd(&B.row_start[row_key(P, x, c)], 1u);
+}
+
+__global__ __launch_bounds__(TPB) void k_row_scatter(Params<3> P, Bufs<3> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
+    double x[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) x[a] = B.pos[pp][a][k];
+    uint32_t c[3];
+    cell_of<3>(P, x, c);
+    const uint32_t q = B.row_start[row_key(P, x, c)] + B.row_rank[k];
+    const float4 r = B.rel[k];
+    B.items[q] = make_float4((float)(x[0] * 0x1p-32), r.y, r.z, __uint_as_float(k | (B.species[pm][k] << 30)));
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u, pm = B.ctl[CTL_PMETA] ^ 1u;
+    uint32_t c[3];
+    float Xi;
+    {
+        double xi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) xi[a] = B.pos[pp][a][i];
+        cell_of<3>(P, xi, c);
+        Xi = (float)(xi[0] * 0x1p-32);
+    }
+    const float4 me = B.rel[i];
+    float lr2[4];
+    if constexpr (MULTI) {
+        const uint32_t si = B.species[pm][i];
+#pragma unroll
+        for (uint32_t s = 0; s < 4; s++) lr2[s] = B.pt->list_r2[si * 4 + s];
+    } else {
+        lr2[0] = B.pt->list_r2[0];
+    }
+    const float reach2 = P.rows_reach * P.rows_reach;
+    const float4* __restrict__ items = B.items;
+    const uint32_t cap4 = P.cap / 4u;
+    const uint32_t blo = i & ~(P.block - 1u);  // the block is a power of two
+    const uint32_t wave = i >> 6, own = i & (P.block - 1u);
+    const uint32_t capw = 64u * cap4;
+    uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
+    const uint64_t lt = lanemask_lt();
+    // A lane fills one in-block and one cross-block chunk in registers; full chunks are stored
+    // whole (16 B) at the wave's fill point.  `fill` changes only where every lane is present.
+    uint32_t fill = 0u;
+    auto store = [&](bool want, const uint4& q, uint32_t entries, bool in) {
+        const uint64_t bal = ballot64(want);
+        if (bal == 0ull) return;
+        if (want) {
+            const uint32_t slot = fill + (uint32_t)__popcll(bal & lt);
+            if (slot < capw) chunks[slot] = tag_chunk(q, own, entries, in);
+            else atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
+        }
+        fill += (uint32_t)__popcll(bal);
+    };
+    // Per lane, in LDS: the open in-block (kind 1) and cross-block (kind 0) chunks, entry s of
+    // kind k at ck[4 k + s][lane], and a ring of up to four hit masks, each with its group's
+    // first copy index and its kind in bit 31.  A visit moves as many hits of the front mask
+    // into that kind's open chunk as it has room for (up to four) and stores the chunk when it
+    // is full.  Visits run when some lane's ring has no room for a group's two masks, and at the
+    // end.  (Held in LDS, the ring and chunks leave registers to the candidate loop.)
+    __shared__ uint32_t ck[8][TPB];
+    __shared__ uint32_t qm[4][TPB], qt[4][TPB];
+    const uint32_t t = threadIdx.x;
+    uint32_t na = 0u, nb = 0u, head = 0u, qn = 0u;
+    auto chunk_of = [&](bool in) { return in ? make_uint4(ck[4][t], ck[5][t], ck[6][t], ck[7][t]) : make_uint4(ck[0][t], ck[1][t], ck[2][t], ck[3][t]); };
+    auto visit = [&]() {
+        bool full = false, in = false;
+        if (qn != 0u) {
+            const uint32_t h = head & 3u;
+            uint32_t m = qm[h][t];
+            const uint32_t tag = qt[h][t];
+            in = (tag >> 31) != 0u;
+            const uint32_t b0 = tag & 0x7fffffffu;
+            const uint32_t n = in ? na : nb;
+            const uint32_t r = n & 3u;
+            uint32_t jv[4];
+            bool take[4];
+#pragma unroll
+            for (uint32_t s = 0; s < 4u; s++) {
+                take[s] = s >= r && m != 0u;
+                const uint32_t at = take[s] ? (uint32_t)__builtin_ctz(m) : 0u;
+                m = take[s] ? m & (m - 1u) : m;
+                jv[s] = __float_as_uint(items[b0 + at].w) & 0x0fffffffu;
+            }
+            const uint32_t row = in ? 4u : 0u;
+#pragma unroll
+            for (uint32_t s = 0; s < 4u; s++)
+                if (take[s]) ck[row + s][t] = jv[s];
+            const uint32_t taken = (uint32_t)take[0] + (uint32_t)take[1] + (uint32_t)take[2] + (uint32_t)take[3];
+            const uint32_t n2 = n + taken;
+            full = (n2 & 3u) == 0u;  // at least one was taken, so the chunk closed
+            if (in) na = n2;
+            else nb = n2;
+            if (m == 0u) {
+                head++;
+                qn--;
+            } else {
+                qm[h][t] = m;
+            }
+        }
+        // every lane takes part in the store's ballot, so `fill` stays the same across the wave
+        uint4 q = make_uint4(0u, 0u, 0u, 0u);
+        if (full) q = chunk_of(in);
+        store(full, q, 4u, in);
+    };
+    auto push = [&](uint32_t m, uint32_t tag) {
+        if (m == 0u) return;
+        const uint32_t h = (head + qn) & 3u;
+        qm[h][t] = m;
+        qt[h][t] = tag;
+        qn++;
+    };
+    const bool wrap_x = (P.periodic & 1u) != 0u;
+    const int nxf = (int)P.rows_nxf;
+    for (int o = 0; o < 9; o++) {
+        const int oy = o % 3 - 1, oz = o / 3 - 1;
+        int yc = (int)c[1] + oy, zc = (int)c[2] + oz;
+        bool skip = false;
+        if (yc < 0 || yc >= (int)P.nc[1]) {
+            if (!((P.periodic >> 1) & 1u)) skip = true;
+            yc = yc < 0 ? yc + (int)P.nc[1] : yc - (int)P.nc[1];
+        }
+        if (zc < 0 || zc >= (int)P.nc[2]) {
+            if (!((P.periodic >> 2) & 1u)) skip = true;
+            zc = zc < 0 ? zc + (int)P.nc[2] : zc - (int)P.nc[2];
+        }
+        // distance from the particle to the row's (y, z) rectangle, and the chord along x
+        const float dyl = oy == 0 ? 0.f : (oy > 0 ? P.cell_w[1] - me.y : me.y);
+        const float dzl = oz == 0 ? 0.f : (oz > 0 ? P.cell_w[2] - me.z : me.z);
+        const float d2 = dyl * dyl + dzl * dzl;
+        skip |= !(d2 < reach2);
+        const float ext = skip ? 0.f : __builtin_sqrtf(reach2 - d2);
+        const float sy = (float)oy * P.cell_w[1] - me.y, sz = (float)oz * P.cell_w[2] - me.z;
+        const int flo = (int)__builtin_floorf((Xi - ext) * P.rows_inv_wx);
+        const int fhi = (int)__builtin_floorf((Xi + ext) * P.rows_inv_wx);
+        const uint32_t rowbase = skip ? 0u : ((uint32_t)zc * P.nc[1] + (uint32_t)yc) * P.rows_nxf;
+        // the part of the chord inside [0, nxf) and, on a periodic x axis, the part that wraps
+        for (int seg = 0; seg < 2; seg++) {
+            int a = 1, b = 0;
+            float sx = -Xi;
+            if (!skip) {
+                if (seg == 0) {
+                    a = flo > 0 ? flo : 0;
+                    b = fhi < nxf - 1 ? fhi : nxf - 1;
+                } else if (wrap_x && flo < 0) {
+                    a = flo + nxf;
+                    b = nxf - 1;
+                    sx = -Xi - P.rows_lx;
+                } else if (wrap_x && fhi >= nxf) {
+                    a = 0;
+                    b = fhi - nxf;
+                    sx = -Xi + P.rows_lx;
+                }
+            }
+            uint32_t s0 = 0u, e0 = 0u;
+            if (a <= b) {
+                s0 = B.row_start[rowbase + (uint32_t)a];
+                e0 = B.row_start[rowbase + (uint32_t)b + 1u];
+            }
+            // Candidates in groups of up to 32.  Three masks are built side by side, none
+            // depending on another (so the test has no branch and each candidate is one 16-byte
+            // load): within the list radius, in i's block, and of higher index than i.  An
+            // in-block pair is kept at its lower-index end only.
+            for (uint32_t base = s0;; base += 32u) {
+                const bool act = base < e0;
+                if (ballot64(act) == 0ull) break;
+                uint32_t m_in = 0u, m_out = 0u;
+                if (act) {
+                    const uint32_t cnt = min(32u, e0 - base);
+                    uint32_t mnear = 0u, minb = 0u, mgt = 0u;
+                    // four at a time; a lane past its range's end tests the copy's entries that
+                    // follow (the copy is padded by 32), whose bits are masked off after
+                    const float4* __restrict__ rq = items + base;
+#pragma unroll 1
+                    for (uint32_t u0 = 0; u0 < 32u; u0 += 4u) {
+                        if (u0 >= cnt) break;
+#pragma unroll
+                        for (uint32_t k = 0; k < 4u; k++) {
+                            const float4 q = rq[u0 + k];
+                            const float dx = q.x + sx, dy = q.y + sy, dz = q.z + sz;
+                            float r2 = dx * dx;
+                            r2 = __builtin_fmaf(dy, dy, r2);
+                            r2 = __builtin_fmaf(dz, dz, r2);
+                            const uint32_t w = __float_as_uint(q.w);
+                            const uint32_t j = w & 0x0fffffffu;
+                            float lr;
+                            if constexpr (MULTI) {
+                                const uint32_t sj = w >> 30;
+                                lr = sj == 0u ? lr2[0] : (sj == 1u ? lr2[1] : (sj == 2u ? lr2[2] : lr2[3]));
+                            } else {
+                                lr = lr2[0];
+                            }
+                            const uint32_t bit = 1u << (u0 + k);
+                            mnear |= r2 < lr ? bit : 0u;
+                            minb |= j - blo < P.block ? bit : 0u;
+                            mgt |= j > i ? bit : 0u;
+                        }
+                    }
+                    mnear &= bits_below((int)cnt);
+                    m_in = mnear & minb & mgt;
+                    m_out = mnear & ~minb;
+                }
+                const uint32_t need = (m_in != 0u ? 1u : 0u) + (m_out != 0u ? 1u : 0u);
+                while (ballot64(qn + need > 4u) != 0ull) visit();
+                push(m_in, base | 0x80000000u);
+                push(m_out, base);
+            }
+        }
+    }
+    while (ballot64(qn != 0u) != 0ull) visit();
+    // the partial last chunks; a list longer than the capacity is flagged and regrown
+    store((na & 3u) != 0u, chunk_of(true), na & 3u, true);
+    store((nb & 3u) != 0u, chunk_of(false), nb & 3u, false);
+    const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
+    if (ca + cb > cap4) atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
+    if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
+    if (__lane_id() == 0u) B.wave_chunks[wave] = min(fill, capw);
+}
+
 // A block of BS particles runs on min(BS, 1024) threads (with BS = 2048 each thread advances
 // particles t and t + 1024).  The block's list chunks (its build waves' regions, numbered in
 // order) are dealt out evenly: lane t takes chunks t, t + threads, ... whichever particles own
@@ -1458,6 +1721,7 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
         P.cell_w[a] = (float)P.cell_w64[a];
     }
     P.block = block_;
+    plan_rows();
     if constexpr (D == 2) {
         // md_core::System's boundaries (system.rs:26-61, 204-353)
         const bool has_bonds = !sys.bonds.empty() || sys.nbond_types > 0;
@@ -1546,6 +1810,51 @@ void Engine<D>::collect_timers() {
     ev_used_ = 0;
 }
 
+// The row build's geometry (3D Verlet mode): rows are the (y, z) cells, binned along x into 8,
+// 4, 2 or 1 fine cells per cell, whichever keeps the row keys within 2^25.  It needs three cells
+// on every axis (distinct rows, a chord shorter than the box), and x below 512 sigma (x is held
+// absolute in f32 there, within 3e-5 sigma).
+template <int D>
+void Engine<D>::plan_rows() {
+    Params<D>& P = P_;
+    P.rows = 0u;
+    P.rows_nxf = 0u;
+    row_keys_ = 0;
+    if constexpr (D == 3) {
+        if (mode_.kind != 1) return;
+        for (int a = 0; a < 3; a++)
+            if (P.nc[a] < 3u) return;
+        const double lx = core::to_f64(sys_.box_len[0]);
+        if (lx > 512.0) return;
+        float lr2max = 0.f;
+        for (int k = 0; k < 16; k++) lr2max = std::max(lr2max, P.pt.list_r2[k]);
+        const double reach = std::sqrt((double)lr2max) + 1e-3;
+        uint32_t sub = 8;
+        while (sub > 1 && (uint64_t)P.nc[0] * sub * P.nc[1] * P.nc[2] > (uint64_t(1) << 25)) sub /= 2;
+        const uint64_t keys = (uint64_t)P.nc[0] * sub * P.nc[1] * P.nc[2];
+        if (keys > (uint64_t(1) << 25)) return;
+        const uint32_t nxf = P.nc[0] * sub;
+        if (!(lx > 2.0 * reach + 2.0 * lx / nxf)) return;
+        P.rows = 1u;
+        P.rows_nxf = nxf;
+        P.rows_xscale = (double)nxf / (double)sys_.box_len[0];
+        P.rows_inv_wx = (float)((double)nxf / lx);
+        P.rows_lx = (float)lx;
+        P.rows_reach = (float)reach;
+        row_keys_ = keys;
+    }
+}
+
+template <int D>
+void Engine<D>::alloc_rows() {
+    if (P_.rows == 0u || B_.items != nullptr) return;
+    B_.items = (float4*)dalloc((n_ + 32u) * sizeof(float4));  // padded: the build reads up to 31 past a range
+    RH_HIP(hipMemset(B_.items, 0, (n_ + 32u) * sizeof(float4)));
+    B_.row_start = (uint32_t*)dalloc((row_keys_ + 1u) * sizeof(uint32_t));
+    B_.row_rank = (uint32_t*)dalloc(n_ * sizeof(uint32_t));
+    row_scan_temp_ = dalloc(scan_temp_bytes(row_keys_ + 1u));
+}
+
 template <int D>
 void Engine<D>::alloc_list(uint32_t cap) {
     auto release = [&](void* p) {
@@ -1599,6 +1908,7 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
         RH_HIP(hipMemset(B_.wave_chunks, 0, waves * sizeof(uint32_t)));
     }
     B_.rel = (float4*)dalloc((n + 32u) * sizeof(float4));  // padded: the list build reads up to 31 past a cell
+    alloc_rows();
     B_.cell_start = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.cell_end = (uint32_t*)dalloc(ncell_keys_ * sizeof(uint32_t));
     B_.pe = (double*)dalloc(n * sizeof(double));
@@ -1734,7 +2044,21 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     RH_HIP(hipGetLastError());
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
-        if (half_path()) {
+        if constexpr (D == 3) {
+            if (row_path()) {
+                fill_u32(B_.row_start, 0u, row_keys_ + 1u, gate, stream_);
+                k_row_count<<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+                RH_HIP(hipGetLastError());
+                exclusive_scan_u32(B_.row_start, B_.row_start, row_keys_ + 1u, row_scan_temp_, gate, stream_);
+                k_row_scatter<<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+                RH_HIP(hipGetLastError());
+                if (multi) k_build_rows<true><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+                else k_build_rows<false><<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+            }
+        }
+        if (row_path()) {
+            // built above
+        } else if (half_path()) {
             bool wrap = false;
             for (int a = 0; a < D; a++) wrap |= P_.nc[a] == 1u && ((P_.periodic >> a) & 1u);
             if (wrap) {
@@ -1900,7 +2224,7 @@ void Engine<D>::launch_step() {
 
 template <int D>
 void Engine<D>::tune(int variant, uint32_t block) {
-    RH_REQUIRE(variant == 0 || variant == 1, "kernel variant must be 0 (full list) or 1 (half list)");
+    RH_REQUIRE(variant >= 0 && variant <= 2, "kernel variant must be 0 (full list), 1 (half list) or 2 (half list, cell-stencil build)");
     RH_REQUIRE(block == 256 || block == 512 || block == 1024 || block == 2048 || block == 4096, "block must be 256, 512, 1024, 2048 or 4096");
     RH_HIP(hipStreamSynchronize(stream_));
     variant_ = variant;
@@ -2035,21 +2359,22 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     RH_REQUIRE(sys.nspecies == sys_.nspecies, "update_system: the species count must stay the same");
     RH_REQUIRE(core::rc_max(sys.pairs, sys.nspecies) <= core::rc_max(sys_.pairs, sys_.nspecies) + 1e-12, "cutoffs cannot grow at run time");
+    RH_REQUIRE(sys.has_bath == sys_.has_bath, "the bath can be retuned, not added or removed");
     if constexpr (D == 2) {
         // engine.rs:895-906
         RH_REQUIRE(sys.compartments[0] == sys_.compartments[0] && sys.compartments[1] == sys_.compartments[1], "update_system: compartments cannot change");
         RH_REQUIRE(sys.bonds == sys_.bonds && (sys.nbond_types > 0) == (sys_.nbond_types > 0), "bonds cannot change at run time");
-        RH_REQUIRE(sys.has_bath == sys_.has_bath, "the thermostat can be retuned, not added or removed");
         RH_REQUIRE(sys.has_piston == sys_.has_piston && (!sys.has_piston || sys.piston_free == sys_.piston_free), "update_system: the piston and its drive cannot change");
         RH_REQUIRE(P_.arith == 0u || std::memcmp(sys.pairs, sys_.pairs, sizeof sys.pairs) == 0, "integer tables are fixed at creation");
         RH_REQUIRE(P_.arith == 0u || (!boundary_features2(sys) && !has_soft_walls(sys)), "integer forces cover pair forces only");
     }
     Params<D> old = P_;
     uint32_t old_bits = key_bits_;
-    uint64_t old_keys = ncell_keys_;
+    uint64_t old_keys = ncell_keys_, old_row_keys = row_keys_;
     build_params(sys, mode_);
     key_bits_ = old_bits;
     ncell_keys_ = old_keys;
+    row_keys_ = old_row_keys;
     // Keep the cell geometry the buffers were sized for; a pair whose list radius grew is listed
     // further out from the next rebuild on.
     for (int a = 0; a < 3; a++) {
@@ -2060,6 +2385,13 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     P_.mbits = old.mbits;
     P_.cap = old.cap;
+    // the row build's geometry and reach follow the kept cells and list radii
+    P_.rows = old.rows;
+    P_.rows_nxf = old.rows_nxf;
+    P_.rows_xscale = old.rows_xscale;
+    P_.rows_inv_wx = old.rows_inv_wx;
+    P_.rows_lx = old.rows_lx;
+    P_.rows_reach = old.rows_reach;
     P_.half_skin2 = old.half_skin2;
     // Cutoffs only shrink, so a range test that was needed stays safe; keeping it keeps the list
     // in the form it was built in (full lists for the range-tested kernel).

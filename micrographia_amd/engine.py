@@ -175,8 +175,9 @@ class GpuEngine3:
         return _stats(s)
 
     def tune(self, variant: int, block: int = 512) -> None:
-        """Kernel variant (0 full list, 1 half list in blocks of `block`: 256, 512, 1024, 2048 or 4096);
-        trajectories are identical."""
+        """Kernel variant (0 full list; 1 half list in blocks of `block`: 256, 512, 1024, 2048 or
+        4096, built from the row-ordered copy where the geometry allows; 2 the same half list
+        built over the cell stencil); trajectories are identical."""
         check(load().rh_engine3_tune(self._h, int(variant), int(block)))
 
     def stream(self) -> int:

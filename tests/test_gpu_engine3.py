@@ -62,8 +62,9 @@ MODES = {
     "verlet": lambda sys, n: rh.verlet3_with_skin(sys, n, 0.3),
 }
 
-# (kernel variant, block): the full list, and the half list at each block size
-VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024), (1, 2048), (1, 4096)]
+# (kernel variant, block): the full list, the half list at each block size (built from the
+# row-ordered copy), and the half list built over the cell stencil
+VARIANTS = [(0, 256), (1, 256), (1, 512), (1, 1024), (1, 2048), (1, 4096), (2, 512), (2, 2048)]
 
 
 def engine(sys, st, mode, variant=1, block=512):
