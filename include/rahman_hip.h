@@ -165,6 +165,11 @@ typedef struct rh_engine3 rh_engine3;
 /* GpuEngine3::with_mode (engine3.rs:342-457); mode NULL = GpuEngine3::new (Verlet, skin 0.4). */
 int rh_engine3_create(const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode* mode, rh_engine3** out);
 int rh_engine3_destroy(rh_engine3* e);
+/* Device memory (bytes) an engine for this system would allocate, computed on the host without
+ * allocating (no reference counterpart: the serving loop's admission budget), and the memory an
+ * engine holds now (the same figure, plus any list regrowth). */
+int rh_engine3_bytes_needed(const rh_system3* sys, uint64_t n, const rh_mode* mode, uint64_t* bytes);
+int rh_engine3_device_bytes(const rh_engine3* e, uint64_t* bytes);
 int rh_engine3_n(const rh_engine3* e, uint64_t* n);
 int rh_engine3_steps(const rh_engine3* e, int64_t* steps);
 int rh_engine3_step(rh_engine3* e, uint64_t steps);                     /* engine3.rs:491-522 */
@@ -176,6 +181,10 @@ int rh_engine3_set_species_velocity(rh_engine3* e, uint32_t species, const doubl
 int rh_engine3_upload(rh_engine3* e, const int64_t* pos, const int32_t* vel, const uint32_t* species);          /* :591-602 */
 int rh_engine3_read_state(rh_engine3* e, int64_t* pos, int32_t* vel, uint32_t* species);                        /* :604-606 */
 int rh_engine3_kicks(rh_engine3* e, int32_t* kicks, double* potential);                                         /* :608-616 */
+int rh_engine3_measure_kinetic(rh_engine3* e);                                                                  /* :618-624 */
+/* Kinetic energies (f32; computed in f64 from the half-step velocities, so within one f32
+ * rounding of the reference's f32 values) and species, both in particle-id order. */
+int rh_engine3_read_kinetic(rh_engine3* e, float* ke, uint32_t* species);                                       /* :626-631 */
 int rh_engine3_read_kinetic_total(rh_engine3* e, double* ke);                                                   /* :633-637 */
 int rh_engine3_read_potential(rh_engine3* e, double* pe);                                                       /* :640-643 */
 int rh_engine3_read_ledger(rh_engine3* e, rh_ledger3* out);                                                     /* :645-647 */
@@ -198,11 +207,14 @@ int rh_engine3_kernel_ms(rh_engine3* e, double* force_ms, uint64_t* force_launch
 typedef struct rh_engine2 rh_engine2;
 int rh_engine2_create(const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode* mode, rh_engine2** out);
 int rh_engine2_destroy(rh_engine2* e);
+int rh_engine2_bytes_needed(const rh_system2* sys, uint64_t n, const rh_mode* mode, uint64_t* bytes);
+int rh_engine2_device_bytes(const rh_engine2* e, uint64_t* bytes);
 int rh_engine2_step(rh_engine2* e, uint64_t steps);
 int rh_engine2_reverse(rh_engine2* e);
 int rh_engine2_upload(rh_engine2* e, const int64_t* pos, const int32_t* vel, const uint32_t* species);
 int rh_engine2_read_state(rh_engine2* e, int64_t* pos, int32_t* vel, uint32_t* species);
 int rh_engine2_kicks(rh_engine2* e, int32_t* kicks, double* potential);
+int rh_engine2_compute_forces(rh_engine2* e);                                                /* engine.rs:875-878 */
 int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke);
 int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out);
 /* md::Arith (crates/md/src/engine.rs:92-100; GpuEngine::with_options :609-625): 0 = f32 pair
@@ -230,8 +242,9 @@ int rh_engine2_read_compartment_ledger(rh_engine2* e, uint32_t per_wall, double*
 int rh_engine2_read_piston(rh_engine2* e, double out[4]);
 int rh_engine2_set_piston_speed(rh_engine2* e, double u);
 /* XOR one bit of a coordinate: of the particle with id `particle` (:964-973), or of every
- * particle whose species is in the bitmask `species` (:1014-1023).  bit < 52 here (positions are
- * held as exact f64 integers), and a flip that would put a particle outside the box is refused. */
+ * particle whose species is in the bitmask `species` (:1014-1023).  bit < 64; a flip that would
+ * put a particle outside the box (every bit from 52 up, for any box the engine accepts) is
+ * refused with RH_ERR_INVALID and changes nothing. */
 int rh_engine2_flip_bit(rh_engine2* e, uint32_t particle, uint32_t axis, uint32_t bit);
 int rh_engine2_flip_species_bit(rh_engine2* e, uint32_t species, uint32_t axis, uint32_t bit);
 /* Bonds at run time (:976-1011): *changed = 1 if the bond was added / removed. */

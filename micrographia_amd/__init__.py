@@ -26,6 +26,7 @@ from .core import (
     fcc,
     fcc_cells,
     fluid,
+    force_mode_auto,
     lj_liquid,
     maxwell,
     maxwell2,
@@ -37,13 +38,14 @@ from .core import (
     verlet3_with_skin,
     verlet_for,
     verlet_with_skin,
+    VERLET_MIN_N,
 )
-from .engine import CompartmentLedger, GpuEngine, GpuEngine3, Ledger, ListStats
+from .engine import CompartmentLedger, GpuEngine, GpuEngine3, Ledger, ListStats, bytes_needed
 
 __all__ = [
     "ONE", "UNIT", "Bath", "BondType", "ForceMode", "Membrane", "Pair", "Piston", "State", "State3",
     "System", "System3", "Thermostat", "Wall", "CompartmentLedger", "Ledger",
-    "box_for", "cube_for", "fcc", "fcc_cells", "fluid", "lj_liquid", "maxwell", "maxwell2",
+    "box_for", "cube_for", "fcc", "fcc_cells", "fluid", "force_mode_auto", "lj_liquid", "maxwell", "maxwell2",
     "set_temperature", "thermal_wall", "to_f64", "to_fixed", "verlet3", "verlet3_with_skin",
-    "verlet_for", "verlet_with_skin", "GpuEngine", "GpuEngine3", "ListStats", "RahmanError", "load",
+    "verlet_for", "verlet_with_skin", "VERLET_MIN_N", "GpuEngine", "GpuEngine3", "ListStats", "RahmanError", "bytes_needed", "load",
 ]

@@ -12,6 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librahman_hip.so")
 
 RH_OK = 0
+RH_ERR_INVALID = -1      # a contract violation (the reference's assert!)
+RH_ERR_HIP = -2          # a HIP runtime failure
+RH_ERR_NOMEM = -3        # allocation failed or over budget
+RH_ERR_UNSUPPORTED = -4  # a system or command this engine does not run
 
 
 class RahmanError(RuntimeError):
@@ -135,6 +139,8 @@ SIGNATURES = {
     "rh_verlet_mode3": ([P, u64, dbl, P], i32),
     "rh_verlet_mode2": ([P, u64, dbl, P], i32),
     "rh_engine3_create": ([P, u64, P, P, P, P, P], i32),
+    "rh_engine3_bytes_needed": ([P, u64, P, P], i32),
+    "rh_engine3_device_bytes": ([P, P], i32),
     "rh_engine3_destroy": ([P], i32),
     "rh_engine3_n": ([P, P], i32),
     "rh_engine3_steps": ([P, P], i32),
@@ -147,6 +153,8 @@ SIGNATURES = {
     "rh_engine3_upload": ([P, P, P, P], i32),
     "rh_engine3_read_state": ([P, P, P, P], i32),
     "rh_engine3_kicks": ([P, P, P], i32),
+    "rh_engine3_measure_kinetic": ([P], i32),
+    "rh_engine3_read_kinetic": ([P, P, P], i32),
     "rh_engine3_read_kinetic_total": ([P, P], i32),
     "rh_engine3_read_potential": ([P, P], i32),
     "rh_engine3_read_ledger": ([P, P], i32),
@@ -157,11 +165,14 @@ SIGNATURES = {
     "rh_engine3_set_timing": ([P, i32], i32),
     "rh_engine3_kernel_ms": ([P, P, P, P, P], i32),
     "rh_engine2_create": ([P, u64, P, P, P, P, P], i32),
+    "rh_engine2_bytes_needed": ([P, u64, P, P], i32),
+    "rh_engine2_device_bytes": ([P, P], i32),
     "rh_engine2_destroy": ([P], i32),
     "rh_engine2_step": ([P, u64], i32),
     "rh_engine2_reverse": ([P], i32),
     "rh_engine2_upload": ([P, P, P, P], i32),
     "rh_engine2_read_state": ([P, P, P, P], i32),
+    "rh_engine2_compute_forces": ([P], i32),
     "rh_engine2_kicks": ([P, P, P], i32),
     "rh_engine2_read_kinetic_total": ([P, P], i32),
     "rh_engine2_list_stats": ([P, P], i32),

@@ -14,7 +14,7 @@ from typing import Dict, List, Optional, Union
 
 import numpy as np
 
-from ._lib import CListStats, check, load, ptr
+from ._lib import RH_ERR_NOMEM, CListStats, RahmanError, check, load, ptr
 from .core import State, State3, System, System3
 from .engine import ListStats, _stats
 
@@ -80,9 +80,19 @@ class BatchEngine:
         return {i: int(rem[i]) for i in self._members}
 
     def step_all(self, steps: int) -> None:
-        """Run until every member has finished its steps (and wait)."""
+        """Run until every member has finished its steps (and wait).  A system whose list
+        overflowed stops (its steps are gated) until poll() has regrown the list; the loop then
+        runs the steps still left.  Raises if a round of runs makes no progress: an overflowed
+        list that cannot grow in the arena as it stands."""
         self.run(steps)
-        self.sync()
+        left = self.poll()
+        while any(left.values()):
+            before = left
+            self.run(max(left.values()))
+            left = self.poll()
+            if left == before:
+                stuck = sorted(i for i, v in left.items() if v)
+                raise RahmanError(RH_ERR_NOMEM, f"systems {stuck} cannot continue: their lists overflowed and the arena has no room to grow them")
 
     def read_state(self, id_: int) -> Union[State, State3]:
         sys = self._members[id_]

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librahman_hip.so")
-SOURCES = ["prims.hip", "engine.hip", "batch.hip", "core.cpp", "capi.cpp", "server.cpp"]
+SOURCES = ["prims.hip", "engine.hip", "batch.hip", "core.cpp", "capi.cpp", "range_alloc.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RH_OFFLOAD_ARCH", "gfx950")

@@ -650,3 +650,16 @@ def verlet_with_skin(sys: System, n: int, skin: float) -> ForceMode:
 
 def verlet_for(sys: System, n: int) -> ForceMode:
     return verlet_with_skin(sys, n, 0.4)
+
+
+VERLET_MIN_N = 24_000  # ForceMode::VERLET_MIN_N (crates/md/src/engine.rs:116)
+
+
+def force_mode_auto(sys: System, n: int) -> ForceMode:
+    """ForceMode::auto (engine.rs:119-121): a cell list every step below VERLET_MIN_N
+    particles, Verlet lists with a 0.4 sigma skin from there."""
+    return ForceMode.cell_list() if n < VERLET_MIN_N else verlet_for(sys, n)
+
+
+ForceMode.VERLET_MIN_N = VERLET_MIN_N  # type: ignore[attr-defined]
+ForceMode.auto = staticmethod(force_mode_auto)  # type: ignore[attr-defined]

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "range_alloc.h"
 
 namespace rh {
 
@@ -63,29 +64,13 @@ struct BBufs {
     BCtl* ctl;
 };
 
-// First-fit allocator over [0, capacity) in units; ranges coalesce on free.
-class RangeAlloc {
-   public:
-    explicit RangeAlloc(uint64_t capacity = 0) { reset(capacity); }
-    void reset(uint64_t capacity);
-    bool alloc(uint64_t size, uint64_t* start);
-    void free(uint64_t start, uint64_t size);
-    uint64_t capacity() const { return cap_; }
-    uint64_t used() const { return used_; }
-    uint64_t high_water() const;  // end of the highest allocated range
-
-   private:
-    std::map<uint64_t, uint64_t> free_;  // start -> size
-    std::map<uint64_t, uint64_t> taken_;
-    uint64_t cap_ = 0, used_ = 0;
-};
-
 struct BatchMember {
     SystemDesc sys;
     uint64_t n;
     uint32_t slot0, nslots, cell0, ncells, cap;
     uint64_t nbr0, nbr_entries;
     double skin;
+    bool overflowed = false;  // its list ever overflowed (and was regrown, or is waiting to be)
 };
 
 class Batch {

@@ -562,57 +562,6 @@ __global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys) {
 // RangeAlloc
 // ---------------------------------------------------------------------------------------------
 
-void RangeAlloc::reset(uint64_t capacity) {
-    free_.clear();
-    taken_.clear();
-    cap_ = capacity;
-    used_ = 0;
-    if (capacity) free_[0] = capacity;
-}
-
-bool RangeAlloc::alloc(uint64_t size, uint64_t* start) {
-    if (size == 0) size = 1;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-        if (it->second >= size) {
-            uint64_t s = it->first, rest = it->second - size;
-            free_.erase(it);
-            if (rest) free_[s + size] = rest;
-            taken_[s] = size;
-            used_ += size;
-            *start = s;
-            return true;
-        }
-    }
-    return false;
-}
-
-void RangeAlloc::free(uint64_t start, uint64_t size) {
-    if (size == 0) size = 1;
-    auto t = taken_.find(start);
-    if (t == taken_.end()) return;
-    taken_.erase(t);
-    used_ -= size;
-    auto it = free_.emplace(start, size).first;
-    auto next = std::next(it);
-    if (next != free_.end() && it->first + it->second == next->first) {
-        it->second += next->second;
-        free_.erase(next);
-    }
-    if (it != free_.begin()) {
-        auto prev = std::prev(it);
-        if (prev->first + prev->second == it->first) {
-            prev->second += it->second;
-            free_.erase(it);
-        }
-    }
-}
-
-uint64_t RangeAlloc::high_water() const {
-    if (taken_.empty()) return 0;
-    auto last = std::prev(taken_.end());
-    return last->first + last->second;
-}
-
 // ---------------------------------------------------------------------------------------------
 // Batch host side
 // ---------------------------------------------------------------------------------------------
@@ -708,7 +657,8 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
     const int D = sys.dim;
     RH_REQUIRE(D == 2 || D == 3, "dimension must be 2 or 3");
     RH_REQUIRE(sys.nspecies >= 1 && sys.nspecies <= 4, "between 1 and 4 species");
-    RH_REQUIRE(n >= 1 && n < (1u << 30), "particle count out of range");
+    // list chunks carry their tag in the top four bits of each index (tag_chunk)
+    RH_REQUIRE(n >= 1 && n < (1u << 28), "particle count out of range (below 2^28 per system)");
     RH_REQUIRE(skin >= 0.0, "skin must be >= 0");
     std::memset(&d, 0, sizeof d);
     d.dim = D;
@@ -862,27 +812,36 @@ int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int3
     m.nbr0 = d.nbr0;
     m.nbr_entries = lunits * LIST_UNIT;
     m.skin = skin;
+    // Upload the state into parity 0 of the new range (nothing in flight touches it).  The
+    // system joins only once its upload succeeded; a failure gives its ranges and id back.
+    try {
+        RH_HIP(hipStreamSynchronize(stream_));
+        std::vector<double> px(n);
+        std::vector<int32_t> vx(n);
+        std::vector<uint32_t> ids(n);
+        for (int a = 0; a < D; a++) {
+            for (uint64_t i = 0; i < n; i++) px[i] = (double)pos[D * i + a];  // exact: |x| < 2^52
+            RH_HIP(hipMemcpy(B_.pos[0][a] + d.slot0, px.data(), n * 8, hipMemcpyHostToDevice));
+            for (uint64_t i = 0; i < n; i++) vx[i] = vel[D * i + a];
+            RH_HIP(hipMemcpy(B_.vel[0][a] + d.slot0, vx.data(), n * 4, hipMemcpyHostToDevice));
+        }
+        RH_HIP(hipMemcpy(B_.species[0] + d.slot0, species, n * 4, hipMemcpyHostToDevice));
+        for (uint64_t i = 0; i < n; i++) ids[i] = (uint32_t)i;
+        RH_HIP(hipMemcpy(B_.ids[0] + d.slot0, ids.data(), n * 4, hipMemcpyHostToDevice));
+        BCtl c;
+        std::memset(&c, 0, sizeof c);
+        c.stale = 1u;
+        c.remaining = (uint32_t)steps;
+        RH_HIP(hipMemcpy(d_ctl_ + id, &c, sizeof c, hipMemcpyHostToDevice));
+    } catch (...) {
+        slots_.free(s0, nslots / BATCH_WG);
+        lists_.free(l0, lunits);
+        cells_.free(c0, cunits);
+        free_ids_.push_back(id);
+        throw;
+    }
     members_[id] = m;
     h_sys_[id] = d;
-    // Upload the state into parity 0 of the new range (nothing in flight touches it).
-    RH_HIP(hipStreamSynchronize(stream_));
-    std::vector<double> px(n);
-    std::vector<int32_t> vx(n);
-    std::vector<uint32_t> ids(n);
-    for (int a = 0; a < D; a++) {
-        for (uint64_t i = 0; i < n; i++) px[i] = (double)pos[D * i + a];  // exact: |x| < 2^52
-        RH_HIP(hipMemcpy(B_.pos[0][a] + d.slot0, px.data(), n * 8, hipMemcpyHostToDevice));
-        for (uint64_t i = 0; i < n; i++) vx[i] = vel[D * i + a];
-        RH_HIP(hipMemcpy(B_.vel[0][a] + d.slot0, vx.data(), n * 4, hipMemcpyHostToDevice));
-    }
-    RH_HIP(hipMemcpy(B_.species[0] + d.slot0, species, n * 4, hipMemcpyHostToDevice));
-    for (uint64_t i = 0; i < n; i++) ids[i] = (uint32_t)i;
-    RH_HIP(hipMemcpy(B_.ids[0] + d.slot0, ids.data(), n * 4, hipMemcpyHostToDevice));
-    BCtl c;
-    std::memset(&c, 0, sizeof c);
-    c.stale = 1u;
-    c.remaining = (uint32_t)steps;
-    RH_HIP(hipMemcpy(d_ctl_ + id, &c, sizeof c, hipMemcpyHostToDevice));
     tables_dirty_ = true;
     return id;
 }
@@ -994,6 +953,7 @@ void Batch::handle_overflow() {
         BCtl& c = h_ctl_[id];
         if (!c.overflow) continue;
         BatchMember& m = kv.second;
+        m.overflowed = true;
         uint32_t want = std::max<uint32_t>(c.maxcount + c.maxcount / 4 + 8, m.cap + m.cap / 2);
         want = (std::min<uint32_t>(want, 65532u) + 3u) & ~3u;
         uint64_t lunits = ((uint64_t)want * m.nslots + LIST_UNIT - 1) / LIST_UNIT;
@@ -1062,7 +1022,7 @@ rh_list_stats Batch::stats(int id) {
     RH_HIP(hipMemcpy(&c, d_ctl_ + id, sizeof c, hipMemcpyDeviceToHost));
     rh_list_stats s;
     s.rebuilds = c.rebuilds;
-    s.overflowed = c.overflow;
+    s.overflowed = c.overflow || it->second.overflowed ? 1u : 0u;
     s.cap = it->second.cap;
     s.max_count = c.maxcount;
     s.steps_done = ((uint64_t)c.done_hi << 32) | c.done_lo;

@@ -219,6 +219,20 @@ int rh_engine3_create(const rh_system3* sys, uint64_t n, const int64_t* pos, con
     });
 }
 
+int rh_engine3_bytes_needed(const rh_system3* sys, uint64_t n, const rh_mode* mode, uint64_t* bytes) {
+    return guarded([&] {
+        RH_REQUIRE(bytes != nullptr, "null argument");
+        rh_mode m = mode ? *mode : default_mode3(sys, n);
+        *bytes = rh::Engine<3>(desc3(sys), n, m).planned_bytes();
+    });
+}
+int rh_engine3_device_bytes(const rh_engine3* e, uint64_t* bytes) {
+    return guarded([&] {
+        RH_REQUIRE(e && bytes, "null argument");
+        *bytes = e->e->alloc_bytes();
+    });
+}
+
 int rh_engine3_destroy(rh_engine3* e) {
     return guarded([&] {
         if (!e) return;
@@ -263,6 +277,12 @@ int rh_engine3_kicks(rh_engine3* e, int32_t* kicks, double* potential) {
 int rh_engine3_read_kinetic_total(rh_engine3* e, double* ke) {
     return guarded([&] { *ke = eng(e)->kinetic_total(); });
 }
+int rh_engine3_measure_kinetic(rh_engine3* e) {
+    return guarded([&] { eng(e)->measure_kinetic(); });
+}
+int rh_engine3_read_kinetic(rh_engine3* e, float* ke, uint32_t* species) {
+    return guarded([&] { eng(e)->read_kinetic(ke, species); });
+}
 int rh_engine3_read_potential(rh_engine3* e, double* pe) {
     return guarded([&] { *pe = eng(e)->potential(); });
 }
@@ -306,6 +326,19 @@ int rh_engine2_create(const rh_system2* sys, uint64_t n, const int64_t* pos, con
         *out = h;
     });
 }
+int rh_engine2_bytes_needed(const rh_system2* sys, uint64_t n, const rh_mode* mode, uint64_t* bytes) {
+    return guarded([&] {
+        RH_REQUIRE(bytes != nullptr, "null argument");
+        rh_mode m = mode ? *mode : default_mode2(sys, n);
+        *bytes = rh::Engine<2>(desc2(sys), n, m).planned_bytes();
+    });
+}
+int rh_engine2_device_bytes(const rh_engine2* e, uint64_t* bytes) {
+    return guarded([&] {
+        RH_REQUIRE(e && bytes, "null argument");
+        *bytes = e->e->alloc_bytes();
+    });
+}
 int rh_engine2_destroy(rh_engine2* e) {
     return guarded([&] {
         if (!e) return;
@@ -330,6 +363,9 @@ int rh_engine2_kicks(rh_engine2* e, int32_t* kicks, double* potential) {
 }
 int rh_engine2_read_kinetic_total(rh_engine2* e, double* ke) {
     return guarded([&] { *ke = eng(e)->kinetic_total(); });
+}
+int rh_engine2_compute_forces(rh_engine2* e) {
+    return guarded([&] { eng(e)->compute_forces(); });
 }
 int rh_engine2_list_stats(rh_engine2* e, rh_list_stats* out) {
     return guarded([&] { *out = eng(e)->list_stats(); });

@@ -199,6 +199,10 @@ template <int D>
 class Engine {
    public:
     Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, const rh_mode& mode);
+    // A plan only: the constructor's checks and parameters, no device work (planned_bytes()).
+    Engine(const SystemDesc& sys, uint64_t n, const rh_mode& mode);
+    uint64_t planned_bytes() const;
+    uint64_t alloc_bytes() const { return alloc_bytes_ + sort_bytes_; }
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -212,6 +216,9 @@ class Engine {
     void read_state(int64_t* pos, int32_t* vel, uint32_t* species);
     void kicks(int32_t* kicks, double* potential);
     double kinetic_total();
+    void measure_kinetic();
+    void read_kinetic(float* ke, uint32_t* species);
+    void compute_forces();
     double potential();
     void kick_region(const double lo[3], const double hi[3], double scale, const double add[3]);
     void set_species(const uint32_t* ids, uint64_t count, uint32_t species);
@@ -312,6 +319,7 @@ class Engine {
     int variant_ = 1;
     uint32_t block_ = 512;
     std::vector<void*> allocs_;
+    uint64_t alloc_bytes_ = 0, sort_bytes_ = 0, list_bytes_ = 0;
     void* dalloc(size_t bytes);
     std::vector<uint32_t> partners_h_;  // the bond partner table (bonds may change at run time)
     void alloc_boundary_buffers();

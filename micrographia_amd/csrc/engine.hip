@@ -1622,7 +1622,52 @@ void* Engine<D>::dalloc(size_t bytes) {
     hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
     if (e != hipSuccess) throw Error(std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     allocs_.push_back(p);
+    alloc_bytes_ += bytes ? bytes : 16;
     return p;
+}
+
+// Plans an engine without allocating anything: the checks and parameters of the constructor.
+template <int D>
+Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const rh_mode& mode) : sys_(sys), mode_(mode), n_(n) {
+    RH_REQUIRE(n >= 1, "an engine needs at least one particle");
+    RH_REQUIRE(n < (uint64_t(1) << 31), "too many particles for one engine");
+    RH_REQUIRE(mode.kind <= 1, "unknown force mode");
+    if (mode.kind == 1) RH_REQUIRE(mode.skin >= 0.0 && mode.cap >= 1, "Verlet mode needs skin >= 0 and cap >= 1");
+    std::memset(&B_, 0, sizeof B_);
+    block_ = n >= (uint64_t(1) << 20) ? 2048u : (n >= (uint64_t(1) << 18) ? 1024u : 512u);
+    build_params(sys, mode);
+}
+
+// The device memory the constructor allocates for this plan (the sum of its allocations below,
+// with the list at its initial capacity); an engine's own total is alloc_bytes().
+template <int D>
+uint64_t Engine<D>::planned_bytes() const {
+    const uint64_t n = n_;
+    auto sz = [](uint64_t b) { return b ? b : uint64_t(16); };
+    uint64_t t = 0;
+    t += 2 * (D * (sz(n * 8) + sz(n * 4)) + sz(n * 4) + sz(n * (D == 3 ? 4 : 2) * 8) + sz(n * 4));  // pos, vel, species, aos, ids
+    t += D * (sz(n * 8) + sz(n * 4));                                                               // ref, kicks
+    t += sz(n * 4);                                                                                  // nbr_count
+    t += sz(((n + 4095u) & ~uint64_t(4095)) / 64u * 4);                                              // wave_chunks
+    t += sz((n + 32u) * 16);                                                                         // rel
+    if (P_.rows) t += sz((n + 32u) * 16) + sz((row_keys_ + 1u) * 4) + sz(n * 4) + sz(scan_temp_bytes(row_keys_ + 1u));
+    t += 2 * sz(ncell_keys_ * 4);                                                                    // cell_start, cell_end
+    t += 2 * sz(n * 8);                                                                              // pe, ke
+    t += sz(CTL_WORDS * 4) + sz(LEDGER_WORDS * 8);
+    if constexpr (D == 2) {
+        if (P_.bonds) t += sz(n * 4) + sz(n * 32);
+        if (P_.piston) t += sz(sizeof(PistonDev));
+        if (P_.feat) t += sz(7 * LEDGER2_COMPARTMENTS * 3 * 8);
+    }
+    t += sz(reduce_temp_bytes(n)) + sz(64);
+    if (mode_.kind == 1) {
+        const uint64_t cap = (mode_.cap + 3u) & ~3u;
+        t += sz(cap * ((n + 4095u) & ~uint64_t(4095)) * 4);
+    }
+    const uint64_t nb = blocks_for(n, 4096);  // the radix sort: two key/value pairs, histograms, scan
+    t += 4 * sz(n * 4) + sz(256 * nb * 4) + sz(scan_temp_bytes(256 * nb));
+    t += sz(sizeof(PairTable));
+    return t;
 }
 
 template <int D>
@@ -1863,13 +1908,15 @@ void Engine<D>::alloc_list(uint32_t cap) {
         if (it != allocs_.end()) allocs_.erase(it);
         RH_HIP(hipFree(p));
     };
+    if (B_.nbr) alloc_bytes_ -= list_bytes_;
     release(B_.nbr);
     cap = (cap + 3u) & ~3u;  // whole four-entry chunks (the half list stores uint4 chunks)
     P_.cap = cap;
     // The full lists take cap entries per particle; the half list's chunk stream takes cap / 4
     // chunks per particle slot, for every block size up to 4096.
     const size_t slots = (n_ + 4095u) & ~size_t(4095);
-    B_.nbr = (uint32_t*)dalloc((size_t)cap * slots * sizeof(uint32_t));
+    list_bytes_ = (size_t)cap * slots * sizeof(uint32_t);
+    B_.nbr = (uint32_t*)dalloc(list_bytes_);
 }
 
 template <int D>
@@ -1937,6 +1984,7 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     d_scalar_ = (double*)dalloc(64);
     if (mode.kind == 1) alloc_list(mode.cap);
     sort_.reserve(n);
+    sort_bytes_ = sort_.bytes();
     constexpr int NSLOTS = 8;
     RH_HIP(hipHostMalloc((void**)&h_ctl_, NSLOTS * CTL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
     ev_.resize(NSLOTS);
@@ -2339,11 +2387,47 @@ void Engine<D>::read_state(int64_t* pos, int32_t* vel, uint32_t* species) {
     }
 }
 
+// Every slot's kinetic energy from its half-step velocity, and their sum (GpuEngine3::
+// measure_kinetic, engine3.rs:618-624), enqueued on the engine's stream.
 template <int D>
-double Engine<D>::kinetic_total() {
+void Engine<D>::measure_kinetic() {
     k_kinetic<D><<<blocks_for(n_, TPB), TPB, 0, stream_>>>(P_, B_, sys_.masses[0], sys_.masses[1], sys_.masses[2], sys_.masses[3], sys_.dt);
     RH_HIP(hipGetLastError());
     reduce_sum_f64(B_.ke, d_scalar_, n_, reduce_temp_, stream_);
+}
+
+// Kinetic energies (f32, as the reference returns them; computed in f64 here) and species in
+// particle-id order (engine3.rs:626-631).
+template <int D>
+void Engine<D>::read_kinetic(float* ke, uint32_t* species) {
+    measure_kinetic();
+    uint32_t ctl[CTL_WORDS];
+    read_ctl(ctl);
+    const uint32_t pm = ctl[CTL_PMETA];
+    std::vector<uint32_t> ids(n_), sp(n_);
+    std::vector<double> e(n_);
+    RH_HIP(hipMemcpy(ids.data(), B_.ids[pm], n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    RH_HIP(hipMemcpy(e.data(), B_.ke, n_ * sizeof(double), hipMemcpyDeviceToHost));
+    RH_HIP(hipMemcpy(sp.data(), B_.species[pm], n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint64_t s = 0; s < n_; s++) {
+        if (ke) ke[ids[s]] = (float)e[s];
+        if (species) species[ids[s]] = sp[s];
+    }
+}
+
+// A force evaluation at the current positions without moving (GpuEngine::compute_forces,
+// engine.rs:875-878); kicks() reads its result.
+template <int D>
+void Engine<D>::compute_forces() {
+    enqueue_cells_at_current(nullptr);
+    enqueue_force_cells(nullptr);
+    if (P_.feat) enqueue_extras(EX_PE);
+    RH_HIP(hipStreamSynchronize(stream_));
+}
+
+template <int D>
+double Engine<D>::kinetic_total() {
+    measure_kinetic();
     double out = 0.0;
     RH_HIP(hipMemcpyAsync(&out, d_scalar_, sizeof(double), hipMemcpyDeviceToHost, stream_));
     RH_HIP(hipStreamSynchronize(stream_));
@@ -2658,7 +2742,9 @@ template <int D>
 void Engine<D>::flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit) {
     if constexpr (D == 2) {
         RH_REQUIRE(axis < 2, "axis must be 0 or 1");
-        RH_REQUIRE(bit < 52, "bit must be below 52 (positions are held as exact f64 integers)");
+        // Any of the 64 bits, as the reference flips; a flip that leaves the box (every bit from
+        // 52 up, for any box the engine accepts) is refused by the check pass.
+        RH_REQUIRE(bit < 64, "bit must be below 64");
         const uint32_t nb = blocks_for(n_, TPB);
         write_ctl(CTL_FLIP_BAD, 0u);
         k_flip2<<<nb, TPB, 0, stream_>>>(P_, B_, sel, by_species, axis, bit, 1);

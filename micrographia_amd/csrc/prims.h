@@ -26,6 +26,7 @@ struct RadixSort {
 
     void reserve(uint64_t n);
     void release();
+    uint64_t bytes() const;  // device memory held
     ~RadixSort() { release(); }
     // Sorts keys[0]/vals[0] (n items); returns the index (0 or 1) of the buffers holding the
     // result.  The number of passes is ceil(key_bits / 8), so the result index is known on the

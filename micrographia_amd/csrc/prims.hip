@@ -181,6 +181,12 @@ void RadixSort::reserve(uint64_t n) {
     RH_HIP(hipMalloc(&scan_temp, scan_temp_bytes(hn)));
 }
 
+uint64_t RadixSort::bytes() const {
+    if (!keys[0]) return 0;
+    const uint64_t hn = 256ull * nblocks_cap;
+    return 4 * cap * sizeof(uint32_t) + hn * sizeof(uint32_t) + scan_temp_bytes(hn);
+}
+
 void RadixSort::release() {
     for (int b = 0; b < 2; b++) {
         if (keys[b]) (void)hipFree(keys[b]);

@@ -61,6 +61,17 @@ def _stats(s: _lib.CListStats) -> ListStats:
     return ListStats(int(s.rebuilds), bool(s.overflowed), int(s.cap), int(s.max_count), int(s.steps_done), int(s.launches_skipped))
 
 
+def bytes_needed(sys, n: int, mode: Optional[ForceMode] = None) -> int:
+    """Device memory an engine for `n` particles of `sys` would allocate (a plan on the host; no
+    device work).  The serving loop counts its single-system lane against its budget with it."""
+    cs = sys.to_c()
+    m = mode.to_c() if mode is not None else None
+    out = C.c_uint64()
+    fn = load().rh_engine3_bytes_needed if isinstance(sys, System3) else load().rh_engine2_bytes_needed
+    check(fn(C.byref(cs), int(n), C.byref(m) if m is not None else None, C.byref(out)))
+    return int(out.value)
+
+
 class GpuEngine3:
     """md::GpuEngine3 on one MI355X."""
 
@@ -153,6 +164,25 @@ class GpuEngine3:
         pe = C.c_double()
         check(load().rh_engine3_kicks(self._h, ptr(k), C.byref(pe)))
         return k, float(pe.value)
+
+    def device_bytes(self) -> int:
+        """Device memory the engine holds."""
+        v = C.c_uint64()
+        check(load().rh_engine3_device_bytes(self._h, C.byref(v)))
+        return int(v.value)
+
+    def measure_kinetic(self) -> None:
+        """Every particle's kinetic energy and their sum, from the half-step velocities
+        (engine3.rs:618-624)."""
+        check(load().rh_engine3_measure_kinetic(self._h))
+
+    def read_kinetic(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(kinetic energies as f32, species), both in particle-id order; measures now
+        (engine3.rs:626-631)."""
+        ke = np.zeros(self._n, np.float32)
+        sp = np.zeros(self._n, np.uint32)
+        check(load().rh_engine3_read_kinetic(self._h, ptr(ke), ptr(sp)))
+        return ke, sp
 
     def read_kinetic_total(self) -> float:
         v = C.c_double()
@@ -253,6 +283,16 @@ class GpuEngine:
         pe = C.c_double()
         check(load().rh_engine2_kicks(self._h, ptr(k), C.byref(pe)))
         return k, float(pe.value)
+
+    def device_bytes(self) -> int:
+        """Device memory the engine holds."""
+        v = C.c_uint64()
+        check(load().rh_engine2_device_bytes(self._h, C.byref(v)))
+        return int(v.value)
+
+    def compute_forces(self) -> None:
+        """A force evaluation at the current positions without moving (engine.rs:875-878)."""
+        check(load().rh_engine2_compute_forces(self._h))
 
     def read_kinetic_total(self) -> float:
         v = C.c_double()

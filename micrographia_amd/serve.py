@@ -21,7 +21,7 @@ from collections import deque
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
-from ._lib import RahmanError
+from ._lib import RH_ERR_UNSUPPORTED, RahmanError
 from .batch import BatchEngine
 from .workloads import Request
 
@@ -71,6 +71,18 @@ class ServeReport:
         }
 
 
+def _solo_mode(sys, n, skin):
+    import micrographia_amd as rh
+
+    return rh.verlet3_with_skin(sys, n, skin) if isinstance(sys, rh.System3) else rh.verlet_with_skin(sys, n, skin)
+
+
+def _solo_bytes(sys, n, skin) -> int:
+    from .engine import bytes_needed
+
+    return bytes_needed(sys, n, _solo_mode(sys, n, skin))
+
+
 def _solo_engine(sys, st, skin):
     import micrographia_amd as rh
 
@@ -93,13 +105,20 @@ class Server:
       is reported as rejected.
 
     Every result is bit-identical to running the request alone, whatever lane it took.
-    budget_bytes: device memory for the batch arena; chunk_s: target duration of one enqueued
-    chunk of batch iterations (the admission and retirement granularity)."""
+    budget_bytes: the device memory serving may use, shared by both lanes; big_bytes of it are
+    the single-system lane's (one engine at a time; a request whose engine would need more is
+    rejected), the rest is the batch arena (default: a quarter, at most 4 GiB, for that lane).
+    chunk_s: target duration of one enqueued chunk of batch iterations (the admission and
+    retirement granularity)."""
 
     def __init__(self, budget_bytes: int, max_systems: int = 4096, chunk_s: float = 0.02, skin: float = 0.3,
                  workers: int = 8, keep_states: bool = False, big_n: int = 131072, active_atoms: int = 6_000_000,
-                 tuning=None):
-        self.batch = BatchEngine(budget_bytes=budget_bytes, max_systems=max_systems, graph_chunk=0)
+                 tuning=None, big_bytes: Optional[int] = None):
+        self.big_bytes = int(big_bytes) if big_bytes is not None else min(int(budget_bytes) // 4, 4 << 30)
+        if not 0 < self.big_bytes < budget_bytes:
+            raise ValueError("big_bytes must leave part of the budget to the batch arena")
+        self.budget_bytes = int(budget_bytes)
+        self.batch = BatchEngine(budget_bytes=self.budget_bytes - self.big_bytes, max_systems=max_systems, graph_chunk=0)
         if tuning is not None:
             tuning.install(self.batch)  # an autotune.BatchTuner's per-bucket tile settings
         self.chunk_s = chunk_s
@@ -153,10 +172,12 @@ class Server:
                 sys, st = f.result()
                 admitted = now()
                 try:
+                    if _solo_bytes(sys, st.n(), self.skin) > self.big_bytes:
+                        raise RahmanError(-3, "over the single-system lane's share of the budget")
                     e = _solo_engine(sys, st, self.skin)
                 except RahmanError:
                     with lock:
-                        rejected.append(r)  # the engine refuses the system itself
+                        rejected.append(r)  # over the lane's budget, or the engine refuses it
                     continue
                 try:
                     e.step(r.steps)
@@ -193,7 +214,9 @@ class Server:
                     sys, st = fut.result()
                     try:
                         i = b.add(sys, st, r.steps, skin=self.skin)
-                    except RahmanError:
+                    except RahmanError as err:
+                        if err.code != RH_ERR_UNSUPPORTED:
+                            raise  # a real failure, not a refusal
                         # a system the batched engine does not run (the 2D thermostat region,
                         # compartments, membrane, piston, bonds): the single-system lane takes it
                         with big_cv:

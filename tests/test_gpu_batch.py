@@ -9,48 +9,11 @@ import pytest
 import micrographia_amd as rh
 from micrographia_amd.batch import BatchEngine
 from micrographia_amd.workloads import ensemble, make_system
-from oracle import oracle as O
+from oracle_twin import oracle_state
 
 pytestmark = pytest.mark.gpu
 
 GB = 1 << 30
-
-
-def oracle_state(sys, st, steps):
-    pos, vel, sp = st.pos.copy(), st.vel.copy(), st.species.copy()
-    c = sys.to_c()
-    if isinstance(sys, rh.System3):
-        o = O.System3()
-        for a in range(3):
-            o.box_len[a] = c.box_len[a]
-            o.walled[a] = c.walled[a]
-            o.gravity[a] = c.gravity[a]
-            for s in range(2):
-                o.wall_kind[a][s] = c.wall_kind[a][s]
-                o.wall_t[a][s] = c.wall_t[a][s]
-        o.seed = c.seed
-        o.has_bath = c.has_bath
-        o.bath_species = c.bath_species
-        o.bath_t = c.bath_t
-        o.bath_rate = c.bath_rate
-    else:
-        o = O.System2()
-        for a in range(2):
-            o.box_len[a] = c.box_len[a]
-            o.walled[a] = c.walled[a]
-            o.gravity[a] = c.gravity[a]
-    o.dt = c.dt
-    o.nspecies = c.nspecies
-    o.pinned = c.pinned
-    for k in range(4):
-        o.masses[k] = c.masses[k]
-    for k in range(16):
-        o.pairs[k] = O.Pair(c.pairs[k].eps, c.pairs[k].sigma, c.pairs[k].rc)
-    if isinstance(sys, rh.System3):
-        O.step3(o, pos, vel, sp, steps, 32)
-        return rh.State3(pos, vel, sp)
-    O.step2(o, pos, vel, sp, steps, 32)
-    return rh.State(pos, vel, sp)
 
 
 def mixed_systems():
@@ -167,6 +130,53 @@ def test_join_and_leave_mid_run():
     assert b.read_state(first[2]) == oracle_state(*systems[2], 200)
     for i, (sys, st) in zip(late, systems[3:]):
         assert b.read_state(i) == oracle_state(sys, st, 120)
+
+
+def test_the_full_512_system_ensemble_matches_the_oracle():
+    """The benchmark's own ensemble at full shape: 512 systems, 2D and 3D 50/50, N log-uniform
+    in [256, 65536] (5.9M atoms), density U[0.3, 0.9], T U[0.5, 2.0], meta-seed 2026, advanced
+    through two replays of a captured three-step graph; every system equals the oracle's
+    IntEngine<f32> / IntEngine3<f32> bit for bit."""
+    import concurrent.futures as cf
+
+    specs = ensemble(512, meta_seed=2026)
+    built = [s.build() for s in specs]
+    assert sum(st.n() for _, st in built) > 5_000_000 and max(st.n() for _, st in built) > 60_000
+    b = BatchEngine(budget_bytes=12 * GB, max_systems=1024, graph_chunk=3)
+    ids = [b.add(sys, st, 6) for sys, st in built]
+    assert all(i >= 0 for i in ids)
+    b.step_all(6)
+    assert all(v == 0 for v in b.poll().values())
+    # the oracle releases the interpreter inside its C calls, so threads run it side by side
+    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+        want = list(ex.map(lambda sb: oracle_state(sb[0], sb[1], 6), built))
+    bad = [k for k, (i, w) in enumerate(zip(ids, want)) if b.read_state(i) != w]
+    assert not bad, f"systems {bad[:10]} differ from the oracle"
+
+
+def test_a_list_that_overflows_regrows_and_stays_exact():
+    """A dense cluster (FCC at density 1.6) in a dilute box (mean density 0.15): the capacity
+    rule sizes the list for the box's density, so the first build overflows.  The system stops,
+    poll() regrows its list, step_all() runs the steps still left, and the trajectory equals
+    the oracle's, beside a system that never overflowed."""
+    n = 2000
+    small = rh.cube_for(n, 1.6)
+    big = rh.cube_for(n, 0.15)
+    pos = rh.fcc(n, small, rh.fcc_cells(n, small), 0.02, 5)
+    sys = rh.System3.new(big, 0.002, rh.Pair.lj(2.5), [1.0])
+    sp = np.zeros(n, np.uint32)
+    st = rh.State3(pos, rh.maxwell(sys, sp, 0.5, 5), sp)
+    other = make_system(3, 3000, 0.8, 1.0, 6)
+    b = BatchEngine(budget_bytes=2 * GB, max_systems=8, graph_chunk=0)
+    b.set_tuning(3, 10, 0, 1.0)  # every pair at both ends: ~147 neighbours against a capacity of 120
+    i = b.add(sys, st, 60, skin=0.3)
+    cap0 = b.list_stats(i).cap
+    j = b.add(*other, steps=60)
+    b.step_all(60)
+    assert b.list_stats(i).overflowed and not b.list_stats(j).overflowed
+    assert b.list_stats(i).cap > cap0 and b.list_stats(i).steps_done == 60
+    assert b.read_state(i) == oracle_state(sys, st, 60)
+    assert b.read_state(j) == oracle_state(*other, 60)
 
 
 def test_admission_refuses_what_does_not_fit():

@@ -289,6 +289,12 @@ def test_bit_flips():
     assert np.array_equal(e.read_positions(), want)
     with pytest.raises(rh.RahmanError):
         e.flip_bit(3, 0, 45)  # 2^13 sigma: outside the box
+    # any of the 64 bits is taken (engine.rs:964-973); from bit 52 up every flip leaves the box
+    for bit in (52, 62, 63):
+        with pytest.raises(rh.RahmanError, match="outside the box"):
+            e.flip_bit(3, 1, bit)
+    with pytest.raises(rh.RahmanError):
+        e.flip_bit(3, 0, 64)
     assert np.array_equal(e.read_positions(), want)
     # the run continues exactly like the oracle's from the flipped state
     s = e.read_state()

@@ -295,6 +295,54 @@ def test_kinetic_energy_readout():
     assert abs(gpu.read_kinetic_total() - s.half_step_kinetic(sys)) < 1e-9 * s.half_step_kinetic(sys)
 
 
+def test_planned_memory_equals_what_the_engines_allocate():
+    """bytes_needed() (a host-side plan, the serving loop's admission figure) equals the device
+    memory each engine then holds: 3D in both force modes and both list builds' geometries, 2D
+    plain and with boundaries."""
+    from micrographia_amd.engine import bytes_needed
+
+    for n in (4000, 262144):
+        sys, s = liquid(n, 0.8442, 1.0)
+        for mode in (rh.verlet3_with_skin(sys, n, 0.3), rh.ForceMode.cell_list()):
+            e = rh.GpuEngine3(sys, s, mode)
+            assert e.device_bytes() == bytes_needed(sys, n, mode), (n, mode)
+            e.close()
+    sys2 = rh.System.new((40 << 32, 40 << 32), 0.005, rh.Pair.lj(2.5), [1.0])
+    walled = sys2.with_hard_walls(0).with_hard_walls(1).with_compartments(0, 3)  # five 8-sigma boxes
+    for sy in (sys2, walled):
+        st2 = rh.fluid(sy, 600, 1.0, 3)
+        mode = rh.verlet_with_skin(sy, 600, 0.3)
+        e = rh.GpuEngine(sy, st2, mode)
+        assert e.device_bytes() == bytes_needed(sy, 600, mode)
+        e.close()
+
+
+def test_per_particle_kinetic_energies_in_id_order():
+    """read_kinetic (engine3.rs:626-631): 1/2 m v^2 per particle from the half-step velocity,
+    zero for pinned species, with the species, in particle-id order however storage is sorted."""
+    _, s = liquid(3000, 0.8, 1.2)
+    n = s.n()
+    sp = (np.arange(n) % 3).astype(np.uint32)
+    base, _ = liquid(3000, 0.8, 1.2)
+    sys = rh.System3.new(base.box_len, 0.005, rh.Pair.lj(2.5), [1.0, 2.0, 0.5]).with_pinned(2)
+    s = rh.State3(s.pos, s.vel, sp)
+    gpu = rh.GpuEngine3(sys, s, rh.verlet3_with_skin(sys, n, 0.3))
+    gpu.step(40)  # storage is Morton-sorted by now
+    st = gpu.read_state()
+    ke, species = gpu.read_kinetic()
+    assert ke.dtype == np.float32 and species.dtype == np.uint32
+    assert np.array_equal(species, sp)
+    m = np.array([1.0, 2.0, 0.5])[sp]
+    v = st.vel.astype(np.float64) / 2.0**32 / sys.dt
+    want = 0.5 * m * np.sum(v * v, axis=1)
+    want[sp == 2] = 0.0
+    # one f32 rounding of the f64 value
+    assert np.array_equal(ke, want.astype(np.float32))
+    assert np.all(ke[sp == 2] == 0.0)
+    gpu.measure_kinetic()
+    assert abs(gpu.read_kinetic_total() - want.sum()) < 1e-9 * want.sum()
+
+
 def test_benchmark_size_short_run_bit_exact():
     """262,144 atoms (the single-GPU benchmark's liquid) for a few steps against the oracle."""
     n = 262144

@@ -63,6 +63,41 @@ def test_both_lanes_together_stay_exact():
         assert c.state_hash == solo_hash(c.request), f"request {c.request.index} (n={c.request.n})"
 
 
+def test_serving_at_full_request_sizes_against_the_oracle():
+    """The request sizes of the serving workload's extremes in one stream: two N = 1,000,000
+    requests (3D and 2D; they take the single-system lane), a 2D request of 5,000 steps, and
+    small 2D and 3D requests arriving while those run, at densities and temperatures drawn
+    from its ranges.  The small and the long requests equal the oracle's trajectories bit for
+    bit; the large ones equal their solo runs."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    from oracle_twin import oracle_state
+
+    rng = np.random.default_rng(11)
+    reqs = []
+    for k in range(10):
+        reqs.append(Request(k, 2 + k % 2, int(rng.integers(500, 3001)), float(rng.uniform(0.3, 0.9)), float(rng.uniform(0.5, 2.0)),
+                            1000 + k, int(rng.integers(100, 301)), 0.005 * k))
+    reqs.append(Request(10, 2, 1000, 0.7, 1.0, 2001, 5000, 0.0))
+    reqs.append(Request(11, 3, 1_000_000, 0.8, 1.2, 2002, 100, 0.01))
+    reqs.append(Request(12, 2, 1_000_000, 0.6, 1.0, 2003, 100, 0.02))
+    srv = Server(budget_bytes=16 << 30, max_systems=64, chunk_s=0.01, keep_states=True)
+    rep = srv.serve(reqs, realtime=True)
+    srv.close()
+    assert len(rep.completed) == len(reqs) and not rep.rejected
+    done = {c.request.index: c for c in rep.completed}
+    checked = [r for r in reqs if r.n < 1_000_000]
+    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+        want = list(ex.map(lambda r: oracle_state(*r.build(), r.steps), checked))
+    for r, w in zip(checked, want):
+        assert done[r.index].state == w, f"request {r.index} (n={r.n}, dim={r.dim}, steps={r.steps}) differs from the oracle"
+    for r in reqs:
+        if r.n >= 1_000_000:
+            assert done[r.index].state_hash == solo_hash(r), f"request {r.index} (n={r.n}, dim={r.dim})"
+
+
 @dataclass
 class ThermostatRequest(Request):
     """A request whose system has a thermostat: the batched engine runs it in 3D and refuses it

@@ -85,6 +85,7 @@ def fakes(monkeypatch):
     FakeSolo.ran = []
     monkeypatch.setattr(serve, "BatchEngine", FakeBatch)
     monkeypatch.setattr(serve, "_solo_engine", lambda sys, st, skin: FakeSolo(sys, st))
+    monkeypatch.setattr(serve, "_solo_bytes", lambda sys, n, skin: n)  # a particle a byte
 
 
 def reqs(spec):
@@ -93,7 +94,7 @@ def reqs(spec):
 
 def test_every_request_completes_once_on_its_lane(fakes):
     rs = reqs([(100, 50), (5000, 10), (300, 20), (9000, 5), (200, 80)])
-    srv = serve.Server(budget_bytes=10_000, chunk_s=0.001, big_n=4000)
+    srv = serve.Server(budget_bytes=20_000, big_bytes=10_000, chunk_s=0.001, big_n=4000)
     rep = srv.serve(rs, realtime=False)
     srv.close()
     assert sorted(c.request.index for c in rep.completed) == [0, 1, 2, 3, 4]
@@ -106,7 +107,7 @@ def test_every_request_completes_once_on_its_lane(fakes):
 def test_batch_lane_is_shortest_job_first_under_the_active_cap(fakes):
     # work = n * steps: request 2 (100 x 10) < 0 (200 x 20) < 1 (150 x 100)
     rs = reqs([(200, 20), (150, 100), (100, 10)])
-    srv = serve.Server(budget_bytes=10_000, chunk_s=0.001, big_n=10_000, active_atoms=150)
+    srv = serve.Server(budget_bytes=20_000, big_bytes=10_000, chunk_s=0.001, big_n=10_000, active_atoms=150)
     rep = srv.serve(rs, realtime=False)
     srv.close()
     assert [t for _, t in FakeBatch.log] == [2, 0, 1]
@@ -115,11 +116,50 @@ def test_batch_lane_is_shortest_job_first_under_the_active_cap(fakes):
 
 def test_requests_that_never_fit_are_rejected(fakes):
     rs = reqs([(100, 10), (50_000, 10)])
-    srv = serve.Server(budget_bytes=1_000, chunk_s=0.001, big_n=10**9)
+    srv = serve.Server(budget_bytes=2_000, big_bytes=1_000, chunk_s=0.001, big_n=10**9)
     rep = srv.serve(rs, realtime=False)
     srv.close()
     assert [r.index for r in rep.rejected] == [1]
     assert [c.request.index for c in rep.completed] == [0]
+
+
+def test_the_single_system_lane_counts_against_the_budget(fakes):
+    """Both lanes share budget_bytes: the batch arena gets what the single-system lane's share
+    leaves, and a large request whose engine would need more than that share is rejected."""
+    rs = reqs([(5000, 10), (12_000, 10), (300, 20)])
+    srv = serve.Server(budget_bytes=30_000, big_bytes=10_000, chunk_s=0.001, big_n=4000)
+    assert srv.batch.capacity_atoms == 20_000
+    rep = srv.serve(rs, realtime=False)
+    srv.close()
+    assert [r.index for r in rep.rejected] == [1]
+    assert sorted(c.request.index for c in rep.completed) == [0, 2]
+    with pytest.raises(ValueError):
+        serve.Server(budget_bytes=1_000, big_bytes=1_000)
+
+
+def test_thousands_of_admissions_and_retirements(fakes, monkeypatch):
+    """Churn: 2,000 requests of sizes over three decades through a small arena; every one
+    completes exactly once, on the lane its size selects, with the arena never over-full."""
+    import random
+
+    rng = random.Random(5)
+    spec = [(int(10 ** rng.uniform(1.5, 3.7)), rng.randint(1, 60)) for _ in range(2000)]
+    rs = reqs(spec)
+    peak = [0]
+
+    class Watched(FakeBatch):
+        def add(self, sys, st, steps, skin=0.3):
+            i = super().add(sys, st, steps, skin)
+            peak[0] = max(peak[0], sum(m["n"] for m in self.members.values()))
+            return i
+
+    monkeypatch.setattr(serve, "BatchEngine", Watched)
+    srv = serve.Server(budget_bytes=40_000, big_bytes=8_000, chunk_s=0.0005, big_n=4000)
+    rep = srv.serve(rs, realtime=False)
+    srv.close()
+    assert sorted(c.request.index for c in rep.completed) == list(range(2000)) and not rep.rejected
+    assert sorted(FakeSolo.ran) == sorted(k for k, (n, _) in enumerate(spec) if n >= 4000)
+    assert peak[0] <= 32_000 and rep.peak_active > 10
 
 
 def test_percentiles():
