@@ -63,32 +63,49 @@ def pairs_per_atom(density: float, rc: float = 2.5) -> float:
     return 4.0 / 3.0 * math.pi * rc ** 3 * density
 
 
-def cpu_baseline(n: int, density: float, t: float, seed: int, budget_s: float = 20.0):
-    """The oracle (C restatement of md-ref's IntEngine3<f32>), one thread, on this host, on the
-    same liquid (density, temperature, cutoff) at n atoms: as many whole steps as fit the time
-    budget (at least one).  One step of the 4M-atom system alone takes ~50 s on one core."""
+def _cpu_3d(n: int, density: float, t: float, seed: int, prec: int, budget_s: float):
+    """Whole steps of the oracle's IntEngine3<f32> (prec 32) or <f64> (prec 64) on the liquid,
+    one thread, as many as fit the time budget (at least one)."""
     from oracle import oracle as O
 
-    if not os.path.exists(O.LIB):
-        O.build()
     sys_, state = make_liquid(n, density, t, seed)
     osys = O.system3(sys_.box_len)
     pos, vel, sp = state.pos.copy(), state.vel.copy(), state.species.copy()
     steps, t0 = 0, time.perf_counter()
     while True:
-        O.step3(osys, pos, vel, sp, 1, 32)
+        O.step3(osys, pos, vel, sp, 1, prec)
         steps += 1
         el = time.perf_counter() - t0
         if el > budget_s or el / steps * (steps + 1) > budget_s * 1.5:
             break
-    n = state.n()
+    return steps, el
+
+
+def cpu_baseline(n: int, density: float, t: float, seed: int, budget_s: float = 10.0):
+    """The oracle (C restatement of md-ref's IntEngine3), one thread, on this host, on the same
+    liquid (density, temperature, cutoff) at n atoms, for both of md-ref's 3D precisions:
+    IntEngine3<f32> (the reported value; the GPU engine's arithmetic) and IntEngine3<f64>
+    beside it, each over as many whole steps as fit the time budget.  One step of the 4M-atom
+    system alone takes ~50 s on one core."""
+    from oracle import oracle as O
+
+    if not os.path.exists(O.LIB):
+        O.build()
+    s32, e32 = _cpu_3d(n, density, t, seed, 32, budget_s)
+    s64, e64 = _cpu_3d(n, density, t, seed, 64, budget_s)
     return {
-        "value": n * steps / el,
+        "value": n * s32 / e32,
         "unit": "atom-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} step(s) of the same liquid at N={n} (rho {density}, T {t}, rc 2.5) with the oracle's IntEngine3<f32> (md-ref's cell-list loop, one thread), {el:.1f} s",
-        "ns_per_particle_step": el / (n * steps) * 1e9,
+        "sample": f"{s32} step(s) of the same liquid at N={n} (rho {density}, T {t}, rc 2.5) with the oracle's IntEngine3<f32> (md-ref's cell-list loop, one thread), {e32:.1f} s",
+        "ns_per_particle_step": e32 / (n * s32) * 1e9,
+        "f64": {
+            "value": n * s64 / e64,
+            "unit": "atom-steps/s",
+            "sample": f"{s64} step(s) of the same liquid with the oracle's IntEngine3<f64>, {e64:.1f} s",
+            "ns_per_particle_step": e64 / (n * s64) * 1e9,
+        },
         "host_cpu": _cpu_model(),
         "host_nproc": os.cpu_count(),
     }
@@ -125,24 +142,35 @@ def c1_gpu_leg(seeds, steps: int = 1000):
 
 def cpu_baseline_c1(seed: int, steps: int = 1000):
     """The oracle's IntEngine<f64> (md-ref's 2D engine with f64 forces), one thread, on the same
-    2D fluid: the reference's CPU configuration as specified."""
+    2D fluid: the reference's CPU configuration as specified, on md-ref's default pair path
+    (fast.rs, half shell and branch-free rows; int_engine.rs:89, 136-141), with the reference
+    loop beside it."""
     from oracle import oracle as O
 
     if not os.path.exists(O.LIB):
         O.build()
     sys_, st = c1_fluid(seed)
     osys = O.system2(sys_.box_len)
-    pos, vel, sp = st.pos.copy(), st.vel.copy(), st.species.copy()
-    t0 = time.perf_counter()
-    O.step2(osys, pos, vel, sp, steps, 64)
-    el = time.perf_counter() - t0
+    out = {}
+    for path in ("fast", "loop"):
+        pos, vel, sp = st.pos.copy(), st.vel.copy(), st.species.copy()
+        t0 = time.perf_counter()
+        if path == "fast":
+            with O.fast_path():
+                O.step2(osys, pos, vel, sp, steps, 64)
+        else:
+            O.step2(osys, pos, vel, sp, steps, 64)
+        out[path] = (time.perf_counter() - t0, O.hash2(pos, vel, sp))
+    el, h = out["fast"]
+    assert h == out["loop"][1], "the fast pair path and the reference loop must give the same trajectory"
     return {
         "value": 4096 * steps / el,
         "unit": "atom-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} steps of the 2D fluid (N=4096, rho 0.7, T 1.0, seed {seed}) with the oracle's IntEngine<f64>, {el:.1f} s",
+        "sample": f"{steps} steps of the 2D fluid (N=4096, rho 0.7, T 1.0, seed {seed}) with the oracle's IntEngine<f64> on md-ref's fast pair path, {el:.1f} s",
         "ns_per_particle_step": el / (4096 * steps) * 1e9,
+        "reference_loop": {"value": 4096 * steps / out["loop"][0], "unit": "atom-steps/s", "ns_per_particle_step": out["loop"][0] / (4096 * steps) * 1e9},
     }
 
 
@@ -279,7 +307,7 @@ def main() -> int:
     ap.add_argument("--skin", type=float, default=0.3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--cpu-n", type=int, default=262144, help="atoms in the CPU-baseline sample")
     ap.add_argument("--variant", type=int, default=1, help="0 full Verlet list, 1 half list in blocks")
     ap.add_argument("--block", type=int, default=2048, help="half-list block size (256, 512, 1024, 2048)")

@@ -310,6 +310,13 @@ int rh_prims_atomic_add_i64(const int64_t* d_vals, uint64_t n, int64_t* d_acc, v
 /* The engine's float -> integer kick conversion on device: out[i] = i32(rint(x[i])), saturating,
  * NaN -> 0 (Rust `as i32` after round_ties_even). */
 int rh_prims_kick_convert(const float* d_x, int32_t* d_out, uint64_t n, void* stream);
+/* The stochastic boundaries' draws on the device (md_core::three::andersen and
+ * thermal_velocity, three.rs:232-265, as the engines compute them with the device's f64
+ * log/cos/sin), for keys (id0 + k, clock), k < count: d_out[6k..6k+6) = the Andersen bath's
+ * three velocities and a low heat-bath wall's normal and two tangential ones at temperature t
+ * for mass m, each as the integer displacement per step (round(v dt 2^32), saturating), every
+ * key redrawn.  A check of those draws against the host C library's. */
+int rh_prims_stochastic_draws(uint64_t seed, double t, double mass, double dt, uint32_t id0, uint64_t count, uint32_t clock, int32_t* d_out, void* stream);
 /* The pair kick divides by r2 twice with one shared, refined reciprocal when r2 >= r2_fast
  * (derived per species pair); below it, IEEE division.  rh_pair_division_floor gives r2_fast for
  * a pair; rh_prims_check_pair_division compares the shared-reciprocal result with IEEE division

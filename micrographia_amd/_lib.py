@@ -220,6 +220,7 @@ SIGNATURES = {
     "rh_prims_reduce_sum_f64": ([P, P, u64, P], i32),
     "rh_prims_threefry": ([P, u64, u32, u32, u32, u32, P], i32),
     "rh_prims_atomic_add_i64": ([P, u64, P, P], i32),
+    "rh_prims_stochastic_draws": ([u64, dbl, dbl, dbl, u32, u64, u32, P, P], i32),
     "rh_prims_kick_convert": ([P, P, u64, P], i32),
     "rh_pair_division_floor": ([dbl, dbl, dbl, P], i32),
     "rh_prims_check_pair_division": ([dbl, dbl, dbl, P, P, P, P], i32),

@@ -9,6 +9,7 @@
 
 namespace rh {
 void kick_convert(const float* x, int32_t* out, uint64_t n, hipStream_t s);
+void stochastic_draws(uint64_t seed, double t, double mass, double dt, uint32_t id0, uint64_t count, uint32_t clock, int32_t* out, hipStream_t s);
 void check_division(float r2_lo, float r2_hi, float sigma2, float six_eps4, uint64_t* checked, uint64_t* mismatches, float* first_bad, hipStream_t s);
 }
 
@@ -619,6 +620,14 @@ int rh_prims_check_pair_division(double eps, double sigma, double rc, uint64_t* 
         float lo = rh::fast_division_floor(rc2, sigma2, six_eps4);
         RH_REQUIRE(lo < rc2, "this pair never takes the fast division");
         rh::check_division(lo, rc2, sigma2, six_eps4, checked, mismatches, first_bad, (hipStream_t)stream);
+    });
+}
+
+int rh_prims_stochastic_draws(uint64_t seed, double t, double mass, double dt, uint32_t id0, uint64_t count, uint32_t clock, int32_t* d_out, void* stream) {
+    return guarded([&] {
+        RH_REQUIRE(count < (uint64_t(1) << 32), "count must fit 32 bits");
+        rh::stochastic_draws(seed, t, mass, dt, id0, count, clock, d_out, (hipStream_t)stream);
+        RH_HIP(hipStreamSynchronize((hipStream_t)stream));
     });
 }
 

@@ -1272,7 +1272,12 @@ __global__ __launch_bounds__(TPB) void k_force_list2(Params<2> P, Bufs<2> B) {
     const float c = pt->kick_c[si];
     const double ci0 = compartment_of2(P, xi[0], 0), ci1 = compartment_of2(P, xi[1], 1);
     uint32_t k[2] = {0u, 0u};
-    const uint32_t m = min(B.nbr_count[i], P.cap);  // a valid full list never exceeds cap
+    // A valid full list never exceeds cap; a count beyond it (a list left in another form) is
+    // reported as an overflow, which stops the steps until the list is regrown and rebuilt,
+    // and only the cap entries that exist are read.
+    const uint32_t cnt = B.nbr_count[i];
+    if (cnt > P.cap) atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
+    const uint32_t m = min(cnt, P.cap);
     const uint32_t* nb = B.nbr + i;
     for (uint32_t e = 0; e < m; e++) {
         const uint32_t j = nb[(size_t)e * P.n];
@@ -2857,6 +2862,49 @@ void check_division(float r2_lo, float r2_hi, float sigma2, float six_eps4, uint
     *checked = count;
     *mismatches = m;
     *first_bad = m ? __builtin_bit_cast(float, f) : 0.0f;
+}
+
+// The stochastic boundaries' draws on the device, for keys (id0 + k, clock): per key the
+// Andersen bath's three velocities and a low heat-bath wall's normal and two tangential ones,
+// each as the integer displacement per step the leapfrog stores (the engines' own andersen_draw
+// and thermal_wall_draw, with every key redrawn).
+struct DrawParams {
+    uint32_t seed_lo, seed_hi, bath_species, pinned;
+    double bath_p, dt;
+    double bath_sd[4];
+    double wall_sd[6][4];
+};
+
+__global__ void k_stochastic_draws(DrawParams P, uint32_t id0, uint64_t count, uint32_t clock, int32_t* out) {
+    const uint64_t k = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t id = id0 + (uint32_t)k;
+    double u[3] = {0.0, 0.0, 0.0};
+    andersen_draw(P, id, clock, 0u, u);
+    double vn, t0, t1;
+    thermal_wall_draw(P, id, clock, 0, 0, 0u, vn, t0, t1);
+    int32_t* o = out + 6 * k;
+    o[0] = to_step(P, u[0]);
+    o[1] = to_step(P, u[1]);
+    o[2] = to_step(P, u[2]);
+    o[3] = to_step(P, vn);
+    o[4] = to_step(P, t0);
+    o[5] = to_step(P, t1);
+}
+
+void stochastic_draws(uint64_t seed, double t, double mass, double dt, uint32_t id0, uint64_t count, uint32_t clock, int32_t* out, hipStream_t s) {
+    RH_REQUIRE(t > 0.0 && mass > 0.0 && dt > 0.0, "temperature, mass and dt must be positive");
+    if (count == 0) return;
+    DrawParams P{};
+    P.seed_lo = (uint32_t)seed;
+    P.seed_hi = (uint32_t)(seed >> 32);
+    P.bath_species = 1u;
+    P.bath_p = 2.0;  // above every uniform draw: each key is redrawn
+    P.dt = dt;
+    P.bath_sd[0] = std::sqrt(t / mass);
+    P.wall_sd[0][0] = std::sqrt(t / mass);
+    k_stochastic_draws<<<blocks_for(count, TPB), TPB, 0, s>>>(P, id0, count, clock, out);
+    RH_HIP(hipGetLastError());
 }
 
 void kick_convert(const float* x, int32_t* out, uint64_t n, hipStream_t s) {

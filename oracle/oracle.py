@@ -187,6 +187,10 @@ def lib():
         L.orc2_int_kick.argtypes = [P, P, P, u64, u32, u32, i64, i64, P]
         L.orc2_int_kick.restype = C.c_int
         L.orc2_golden_setup.argtypes = [P, P, P, P]
+        L.orc2_set_fast_path.argtypes = [C.c_int]
+        L.orc2_set_fast_path.restype = None
+        L.orc3_draws.argtypes = [u64, dbl, dbl, dbl, u32, u64, u32, P]
+        L.orc3_draws.restype = None
     return _lib
 
 
@@ -410,6 +414,27 @@ def kicks2_brute(sys, pos, species, prec=32):
     k = np.zeros((len(sp), 2), np.int32)
     _check(lib().orc2_kicks_brute(prec, C.byref(sys), len(sp), ptr(pos), ptr(sp), ptr(k)), "kicks2_brute")
     return k
+
+
+def draws3(seed, t, mass, dt, id0, count, clock):
+    """The stochastic boundaries' draws for keys (id0 + k, clock), as integer displacements:
+    an array (count, 6) (bath x, y, z; heat-bath wall normal and two tangentials)."""
+    out = np.zeros((count, 6), np.int32)
+    lib().orc3_draws(seed, t, mass, dt, id0, count, clock, ptr(out))
+    return out
+
+
+class fast_path:
+    """Within the block the oracle's 2D forces take md-ref's fast pair path (fast.rs), as
+    IntEngine does by default, instead of the reference loop."""
+
+    def __enter__(self):
+        lib().orc2_set_fast_path(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc2_set_fast_path(0)
+        return False
 
 
 def step2(sys, pos, vel, species, steps, prec=32):

@@ -783,6 +783,35 @@ static int andersen3(const orc_system3 *s, uint32_t id, uint32_t clock, uint32_t
     return 1;
 }
 
+/* The same draws for keys (id0 + k, clock) with every key redrawn (bath rate x dt above every
+ * uniform): out[6k..6k+6) = the Andersen bath's three velocities and a low heat-bath wall's
+ * normal and two tangential ones, as integer displacements per step. */
+void orc3_draws(uint64_t seed, double t, double mass, double dt, uint32_t id0, uint64_t count, uint32_t clock, int32_t *out) {
+    orc_system3 s;
+    memset(&s, 0, sizeof s);
+    s.seed = seed;
+    s.dt = dt;
+    s.nspecies = 1;
+    s.masses[0] = mass;
+    s.has_bath = 1;
+    s.bath_species = 1;
+    s.bath_t = t;
+    s.bath_rate = 2.0 / dt;
+    for (uint64_t k = 0; k < count; k++) {
+        uint32_t id = id0 + (uint32_t)k;
+        double u[3] = {0.0, 0.0, 0.0}, vn, vt[2];
+        andersen3(&s, id, clock, 0, u);
+        thermal_velocity3(&s, id, clock, t, mass, 1.0, &vn, vt);
+        int32_t *o = out + 6 * k;
+        o[0] = to_step3(&s, u[0]);
+        o[1] = to_step3(&s, u[1]);
+        o[2] = to_step3(&s, u[2]);
+        o[3] = to_step3(&s, vn);
+        o[4] = to_step3(&s, vt[0]);
+        o[5] = to_step3(&s, vt[1]);
+    }
+}
+
 static int32_t gravity_kick(double g, double dt) { return sat_i32(round(g * dt * dt * ONE_F)); }
 
 /* IntEngine3::step (int_engine3.rs:138-206) */
@@ -1253,6 +1282,176 @@ void orc2_state_init(const orc_system2 *s, orc_state2 *st) {
     }
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* md-ref's fast 2D pair path (crates/md-ref/src/fast.rs), the path IntEngine takes by default   */
+/* (fast: true, int_engine.rs:89; chosen at int_engine.rs:136-141): a counting sort into cells,   */
+/* a half shell of neighbour cells so each pair is visited once, and per cell a branch-free row   */
+/* per particle over the cell's neighbourhood.  Kicks are bit-identical to the reference loop    */
+/* (each end's kick is rounded with its own coefficient from the same force factor, and integer  */
+/* sums do not depend on order); the potential energy is summed in another order.                */
+/* ------------------------------------------------------------------------------------------ */
+
+int orc2_fast_path = 0;
+void orc2_set_fast_path(int on) { orc2_fast_path = on; }
+
+/* fast::eligible (fast.rs:76-80) */
+static int fast_eligible2(const orc_system2 *s, const uint32_t nc[2]) {
+    double side = fmax(orc_to_f64(s->box_len[0]) / nc[0], orc_to_f64(s->box_len[1]) / nc[1]);
+    return (!(s->compartments[0] || s->compartments[1]) || s->diathermal) && 2.0 * side < 7.9;
+}
+
+/* `x as i32` after round_ties_even, without branches (the row loops below stay vectorisable):
+ * NaN -> 0, saturating at the i32 range.  The same values as sat_i32. */
+static inline int32_t sat_i32_rf(float x) {
+    const float c = fmaxf(fminf(x, 2147483520.0f), -2147483648.0f);  /* the largest f32 below 2^31 */
+    int32_t r = (int32_t)c;
+    r = x >= 2147483648.0f ? INT32_MAX : r;
+    return x != x ? 0 : r;
+}
+static inline int32_t sat_i32_rd(double x) {
+    const double c = fmax(fmin(x, 2147483647.0), -2147483648.0);
+    int32_t r = (int32_t)c;
+    return x != x ? 0 : r;
+}
+#define SAT_I32_R(x) _Generic((x), float: sat_i32_rf, double: sat_i32_rd)(x)
+
+/* Real::sep_near (real.rs:59-64 for f32; to_f64 for f64): the separation conversion for pairs
+ * known to lie within 8 sigma. */
+static float sep_near_f32(int64_t d) {
+    float f;
+    orc_sep_f32(d, &f);
+    return f;
+}
+static double sep_near_f64(int64_t d) { return orc_to_f64(d); }
+
+/* fast::pair_kicks (fast.rs:83-237) with row_uniform / row_mixture (:239-294).  kicks (by id)
+ * receive the pair kicks; returns the pairs' potential energy in F. */
+#define DEFINE_FAST2(NAME, F, CONSTS, SEPNEAR, LIT)                                                          \
+    static F NAME(const orc_system2 *s, uint64_t n, const int64_t *pos, const uint32_t *sp, const CONSTS *consts, \
+                  const F *kc, const uint32_t nc[2], int32_t *kicks, int *err) {                               \
+        const uint32_t cnx = nc[0], cny = nc[1], ns = s->nspecies;                                            \
+        const uint64_t ncell = (uint64_t)cnx * cny;                                                           \
+        const double b0 = orc_to_f64(s->box_len[0]), b1 = orc_to_f64(s->box_len[1]);                          \
+        uint32_t *start = calloc(ncell + 1, sizeof(uint32_t)), *fillp = malloc((ncell + 1) * sizeof(uint32_t)); \
+        uint32_t *key = malloc((n ? n : 1) * sizeof(uint32_t)), *ss = malloc((n ? n : 1) * sizeof(uint32_t));  \
+        uint32_t *sid = malloc((n ? n : 1) * sizeof(uint32_t));                                               \
+        int64_t *sx = malloc((n ? n : 1) * sizeof(int64_t)), *sy = malloc((n ? n : 1) * sizeof(int64_t));     \
+        int32_t *kx = calloc(n ? n : 1, sizeof(int32_t)), *ky = calloc(n ? n : 1, sizeof(int32_t));           \
+        const uint64_t cap = n ? n : 1; /* a neighbourhood holds at most every particle */                   \
+        int64_t *nx = malloc(cap * sizeof(int64_t)), *ny = malloc(cap * sizeof(int64_t));                      \
+        uint32_t *ni = malloc(cap * sizeof(uint32_t));                                                        \
+        int32_t *nkx = malloc(cap * sizeof(int32_t)), *nky = malloc(cap * sizeof(int32_t));                  \
+        F *rx = malloc(cap * sizeof(F)), *ry = malloc(cap * sizeof(F));                                       \
+        F pe = LIT(0.0);                                                                                      \
+        *err = 0;                                                                                             \
+        if (!start || !fillp || !key || !ss || !sid || !sx || !sy || !kx || !ky || !nx || !ny || !ni || !nkx || !nky || !rx || !ry) { \
+            *err = -2;                                                                                        \
+            goto done;                                                                                        \
+        }                                                                                                     \
+        /* counting sort into cells (cells.rs:51-55 cell_of, y-major keys) */                                 \
+        for (uint64_t i = 0; i < n; i++) {                                                                    \
+            uint32_t cx = cell_of(orc_to_f64(pos[2 * i]), b0, cnx), cy = cell_of(orc_to_f64(pos[2 * i + 1]), b1, cny); \
+            key[i] = cy * cnx + cx;                                                                           \
+            start[key[i] + 1] += 1;                                                                           \
+        }                                                                                                     \
+        for (uint64_t c = 0; c < ncell; c++) start[c + 1] += start[c];                                        \
+        memcpy(fillp, start, (ncell + 1) * sizeof(uint32_t));                                                 \
+        for (uint64_t i = 0; i < n; i++) {                                                                    \
+            uint32_t q = fillp[key[i]]++;                                                                     \
+            sx[q] = pos[2 * i];                                                                               \
+            sy[q] = pos[2 * i + 1];                                                                           \
+            ss[q] = sp[i];                                                                                    \
+            sid[q] = (uint32_t)i;                                                                             \
+        }                                                                                                     \
+        /* the half shell: the cell itself (later particles) and four of its eight neighbours */              \
+        static const int SHELL[4][2] = {{1, 0}, {-1, 1}, {0, 1}, {1, 1}};                                      \
+        const F zero = LIT(0.0), one = LIT(1.0), two = LIT(2.0), six = LIT(6.0);                              \
+        for (uint32_t cy = 0; cy < cny; cy++)                                                                 \
+            for (uint32_t cx = 0; cx < cnx; cx++) {                                                           \
+                const uint32_t c = cy * cnx + cx, a0 = start[c], a1 = start[c + 1];                           \
+                if (a0 == a1) continue;                                                                       \
+                uint64_t m = 0;                                                                               \
+                /* gather the neighbourhood: own cell first, then the shell, shifted across periodic edges */ \
+                for (int r = -1; r < 4; r++) {                                                                \
+                    uint32_t d = c;                                                                           \
+                    int64_t shx = 0, shy = 0;                                                                 \
+                    if (r >= 0) {                                                                             \
+                        int tx = (int)cx + SHELL[r][0], ty = (int)cy + SHELL[r][1];                           \
+                        if (tx < 0) {                                                                         \
+                            if (s->walled[0]) continue;                                                       \
+                            tx += (int)cnx;                                                                   \
+                            shx = -s->box_len[0];                                                             \
+                        } else if (tx >= (int)cnx) {                                                          \
+                            if (s->walled[0]) continue;                                                       \
+                            tx -= (int)cnx;                                                                   \
+                            shx = s->box_len[0];                                                              \
+                        }                                                                                     \
+                        if (ty >= (int)cny) {                                                                 \
+                            if (s->walled[1]) continue;                                                       \
+                            ty -= (int)cny;                                                                   \
+                            shy = s->box_len[1];                                                              \
+                        }                                                                                     \
+                        d = (uint32_t)ty * cnx + (uint32_t)tx;                                                \
+                    }                                                                                         \
+                    for (uint32_t q = start[d]; q < start[d + 1]; q++) {                                      \
+                        nx[m] = sx[q] + shx;                                                                  \
+                        ny[m] = sy[q] + shy;                                                                  \
+                        ni[m] = q;                                                                            \
+                        m++;                                                                                  \
+                    }                                                                                         \
+                }                                                                                             \
+                memset(nkx, 0, m * sizeof(int32_t));                                                          \
+                memset(nky, 0, m * sizeof(int32_t));                                                          \
+                for (uint32_t p = 0; p < a1 - a0; p++) {                                                      \
+                    const uint32_t sq = a0 + p;                                                               \
+                    const int64_t xs = sx[sq], ys = sy[sq];                                                   \
+                    for (uint64_t k = 0; k < m; k++) {                                                        \
+                        rx[k] = SEPNEAR(nx[k] - xs);                                                          \
+                        ry[k] = SEPNEAR(ny[k] - ys);                                                          \
+                    }                                                                                         \
+                    const uint32_t si = ss[sq];                                                               \
+                    const F ci = kc[si];                                                                      \
+                    int32_t kxi = 0, kyi = 0;                                                                 \
+                    for (uint64_t k = 0; k < m; k++) {                                                        \
+                        const uint32_t sj = ns == 1 ? 0u : ss[ni[k]];                                         \
+                        const CONSTS pc = consts[si * 4 + sj];                                                \
+                        const F x = rx[k], y = ry[k];                                                         \
+                        const F r2 = x * x + y * y;                                                           \
+                        const F s2 = pc.sigma2 / r2;                                                          \
+                        const F s6 = s2 * s2 * s2;                                                            \
+                        const F f = six * pc.eps4 * s6 * (two * s6 - one) / r2;                               \
+                        const F e = pc.eps4 * s6 * (s6 - one) - pc.shift;                                     \
+                        const int on = r2 < pc.rc2 && k >= (uint64_t)p + 1;                                   \
+                        const F fo = on ? f : zero;                                                           \
+                        const F ai = -fo * ci, aj = -fo * kc[sj];                                             \
+                        kxi = (int32_t)((uint32_t)kxi + (uint32_t)SAT_I32_R(RINT(ai * x)));                   \
+                        kyi = (int32_t)((uint32_t)kyi + (uint32_t)SAT_I32_R(RINT(ai * y)));                   \
+                        nkx[k] = (int32_t)((uint32_t)nkx[k] + (uint32_t)SAT_I32_R(RINT(aj * -x)));            \
+                        nky[k] = (int32_t)((uint32_t)nky[k] + (uint32_t)SAT_I32_R(RINT(aj * -y)));            \
+                        pe += on ? e : zero;                                                                  \
+                    }                                                                                         \
+                    kx[sq] = (int32_t)((uint32_t)kx[sq] + (uint32_t)kxi);                                    \
+                    ky[sq] = (int32_t)((uint32_t)ky[sq] + (uint32_t)kyi);                                    \
+                }                                                                                             \
+                /* scatter the neighbourhood's kicks back, once per cell */                                   \
+                for (uint64_t k = 0; k < m; k++) {                                                            \
+                    kx[ni[k]] = (int32_t)((uint32_t)kx[ni[k]] + (uint32_t)nkx[k]);                           \
+                    ky[ni[k]] = (int32_t)((uint32_t)ky[ni[k]] + (uint32_t)nky[k]);                           \
+                }                                                                                             \
+            }                                                                                                 \
+        for (uint64_t q = 0; q < n; q++) {                                                                    \
+            kicks[2 * (uint64_t)sid[q]] = kx[q];                                                              \
+            kicks[2 * (uint64_t)sid[q] + 1] = ky[q];                                                          \
+        }                                                                                                     \
+    done:                                                                                                     \
+        free(start); free(fillp); free(key); free(ss); free(sid); free(sx); free(sy); free(kx); free(ky);    \
+        free(nx); free(ny); free(ni); free(nkx); free(nky); free(rx); free(ry);                               \
+        return pe;                                                                                            \
+    }
+
+DEFINE_FAST2(fast_pairs2_f32, float, consts_f, sep_near_f32, LIT_F)
+DEFINE_FAST2(fast_pairs2_f64, double, consts_d, sep_near_f64, LIT_D)
+
 static int forces2_full(int prec, const orc_system2 *s, const orc_state2 *st, uint64_t n, const int64_t *pos, const uint32_t *sp, int32_t *kicks, double *pe_out,
                         double *wall_force, double *piston_force, int brute) {
     if (s->nspecies < 1 || s->nspecies > 4) return -3;
@@ -1297,13 +1496,33 @@ static int forces2_full(int prec, const orc_system2 *s, const orc_state2 *st, ui
     for (uint64_t i = 0; i < n; i++) items[fill[(uint64_t)cell[2 * i + 1] * nc[0] + cell[2 * i]]++] = i;
     double pe = 0.0;
     int64_t piston_fixed = 0;
+    /* md-ref's fast pair path where IntEngine takes it (int_engine.rs:136-141): the pair kicks
+     * first, then the per-particle terms below */
+    const int fast = orc2_fast_path && !brute && prec != 0 && fast_eligible2(s, nc);
+    if (fast) {
+        int err = 0;
+        if (prec == 32) {
+            float kc[4];
+            for (uint32_t a = 0; a < 4; a++) kc[a] = a < s->nspecies ? (float)kick_coeff2(s, a) : 0.0f;
+            pe += (double)fast_pairs2_f32(s, n, pos, sp, cf32, kc, nc, kicks, &err);
+        } else {
+            double kc[4];
+            for (uint32_t a = 0; a < 4; a++) kc[a] = a < s->nspecies ? kick_coeff2(s, a) : 0.0;
+            pe += fast_pairs2_f64(s, n, pos, sp, cf64, kc, nc, kicks, &err);
+        }
+        if (err) return err;
+    }
     for (uint64_t i = 0; i < n; i++) {
         int32_t k[2] = {0, 0};
+        if (fast) {
+            k[0] = kicks[2 * i];
+            k[1] = kicks[2 * i + 1];
+        }
         float pe32 = 0.0f;
         double pe64 = 0.0;
         float c32 = (float)kick_coeff2(s, sp[i]);
         double c64 = kick_coeff2(s, sp[i]);
-        uint64_t nblock = brute ? 1 : 9;
+        uint64_t nblock = fast ? 0 : (brute ? 1 : 9);
         for (uint64_t blk = 0; blk < nblock; blk++) {
             uint64_t q0 = 0, q1 = n;
             if (!brute) {

@@ -174,6 +174,13 @@ void orc2_cells_for(const orc_system2 *sys, double r, uint32_t out[2]);
 int orc2_kicks(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks, double *pe);
 int orc2_kicks_brute(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, const uint32_t *species, int32_t *kicks);
 int orc2_step(int prec, const orc_system2 *sys, uint64_t n, int64_t *pos, int32_t *vel, const uint32_t *species, uint64_t steps);
+/* md-ref's IntEngine takes its fast pair path (crates/md-ref/src/fast.rs) by default; the oracle
+ * runs the reference loop unless this is set (0 / 1). */
+void orc2_set_fast_path(int on);
+/* md_core::three::andersen and thermal_velocity (three.rs:232-265) with the C library's
+ * log/cos/sin, every key redrawn: 6 integer displacements per key (bath x3, wall normal and
+ * tangentials). */
+void orc3_draws(uint64_t seed, double t, double mass, double dt, uint32_t id0, uint64_t count, uint32_t clock, int32_t *out);
 int orc2_reverse(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, int32_t *vel, const uint32_t *species);
 int orc2_energy(int prec, const orc_system2 *sys, uint64_t n, const int64_t *pos, const int32_t *vel, const uint32_t *species, double *kinetic, double *potential);
 

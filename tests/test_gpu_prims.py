@@ -136,3 +136,38 @@ def test_shared_reciprocal_division_is_ieee_for_every_r2(eps, sigma, rc):
     _lib.check(_lib.load().rh_prims_check_pair_division(eps, sigma, rc, ctypes.byref(checked), ctypes.byref(mism), ctypes.byref(first), None))
     assert checked.value > 10_000_000
     assert mism.value == 0, f"first differing r2 {first.value!r}"
+
+
+def test_stochastic_draws_match_the_host_c_library():
+    """The Andersen bath's and the heat-bath walls' draws (three.rs:232-265) on 100,000,000
+    (id, clock) keys: the device's f64 log / cos / sin against the host C library's (which
+    md-ref's f64 methods call), compared as the integer displacements the leapfrog stores.  A
+    one-ulp difference matters only where it straddles a rounding boundary of that
+    displacement; the test counts those."""
+    import concurrent.futures as cf
+
+    torch = torch_cuda()
+    lib = _lib.load()
+    seed, t, m, dt = 0x5EED_0000_0000_0007, 1.3, 1.0, 0.005
+    chunk = 10_000_000
+    out = torch.empty(chunk * 6, dtype=torch.int32, device="cuda")
+    total = mismatched_keys = mismatched_values = 0
+    for c in range(10):
+        id0, clock = c * chunk, 1000 + 7 * c
+        print(f"draws: chunk {c}, keys {id0}..{id0 + chunk}", flush=True)
+        _lib.check(lib.rh_prims_stochastic_draws(seed, t, m, dt, id0, chunk, clock, dptr(out), None))
+        got = out.cpu().numpy().reshape(chunk, 6)
+        parts = 16
+        step = chunk // parts
+        with cf.ThreadPoolExecutor(max_workers=parts) as ex:
+            want = np.concatenate(list(ex.map(lambda p: O.draws3(seed, t, m, dt, id0 + p * step, step, clock), range(parts))))
+        diff = got != want
+        total += chunk
+        mismatched_keys += int(diff.any(axis=1).sum())
+        mismatched_values += int(diff.sum())
+        if diff.any():
+            k = int(np.argmax(diff.any(axis=1)))
+            print(f"  first mismatch at key {id0 + k}: device {got[k]} host {want[k]}", flush=True)
+    rate = mismatched_keys / total
+    print(f"draws: {total} keys x 6 values, {mismatched_keys} keys ({mismatched_values} values) differ: rate {rate:.3g} per key")
+    assert rate < 1e-6

@@ -286,10 +286,10 @@ def _lj_textbook(r, eps=1.0, sigma=1.0, rc=2.5):
     return u(r) - u(rc)
 
 
-def _two_body_pin():
+def _two_body_pin(prec=64):
     """Two atoms at a separation with three unequal components: each kick equals
     -dU/dr * (d/r) * dt^2/m * 2^32 from a central difference of the textbook law, and the
-    potential equals U(r)."""
+    potential equals U(r) (to f32's rounding at prec 32)."""
     box = [40 * ONE] * 3
     sys = O.system3(box)
     d = np.array([1.03, -0.41, 0.27])
@@ -300,33 +300,122 @@ def _two_body_pin():
     du = (_lj_textbook(r + h) - _lj_textbook(r - h)) / (2 * h)
     c = 0.005 ** 2 * ONE
     want = du / r * c * sep  # kick on atom 0: along +sep when attracted (du > 0)
-    k, pe = O.kicks3(sys, pos, np.zeros(2, np.uint32), prec=64)
+    k, pe = O.kicks3(sys, pos, np.zeros(2, np.uint32), prec=prec)
     kicks_ok = all(abs(k[0, a] - want[a]) <= 1 + 1e-6 * abs(want[a]) for a in range(3)) and np.array_equal(k[0], -k[1])
-    pe_ok = abs(pe - _lj_textbook(r)) <= 1e-9 * max(1.0, abs(_lj_textbook(r)))
+    pe_ok = abs(pe - _lj_textbook(r)) <= (1e-9 if prec == 64 else 1e-6) * max(1.0, abs(_lj_textbook(r)))
     return kicks_ok and pe_ok
 
 
-def _energy_pin():
-    """Total energy of a small liquid is conserved to 1e-3 over 200 steps."""
+def _energy_pin(prec=64):
+    """Total energy of a small liquid is conserved to 1e-3 over 200 steps of the engine at this
+    precision (energies from the f64 evaluation)."""
     sys, pos, vel, sp = _liquid3(500, 0.8, 1.0)
     e0 = sum(O.energy3(sys, pos, vel, sp))
-    O.step3(sys, pos, vel, sp, 200, 64)
+    O.step3(sys, pos, vel, sp, 200, prec)
     e1 = sum(O.energy3(sys, pos, vel, sp))
     return abs(e1 - e0) <= 1e-3 * abs(e0)
 
 
-def test_true_pair_law_passes_the_pins():
+@pytest.mark.parametrize("prec", [32, 64])
+def test_true_pair_law_passes_the_pins(prec):
     O.set_mutation(0)
-    assert _two_body_pin()
-    assert _energy_pin()
+    assert _two_body_pin(prec)
+    assert _energy_pin(prec)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
 @pytest.mark.parametrize("mutation", [1, 2, 3, 4, 5])
-def test_planted_mistake_fails_a_pin(mutation):
-    """1 wrong sign, 2 dropped term, 3 wrong axis index, 4 dropped shift, 5 dropped r2 term."""
+def test_planted_mistake_fails_a_pin(mutation, prec):
+    """1 wrong sign, 2 dropped term, 3 wrong axis index, 4 dropped shift, 5 dropped r2 term:
+    each fails a pin in the f32 engine (the GPU's twin) as in the f64 one."""
     O.set_mutation(mutation)
     try:
-        caught = not _two_body_pin() or not _energy_pin()
+        caught = not _two_body_pin(prec) or not _energy_pin(prec)
     finally:
         O.set_mutation(0)
-    assert caught, f"planted mistake {mutation} passed every pin"
+    assert caught, f"planted mistake {mutation} passed every pin at prec {prec}"
+
+
+# ---- the state hash, pinned by FNV-1a's published vectors ---------------------------------
+
+def _fnv1a64(data: bytes) -> int:
+    """FNV-1a 64 (Fowler, Noll, Vo): offset basis 0xcbf29ce484222325, prime 0x100000001b3."""
+    h = 0xCBF29CE484222325
+    for b in data:
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_fnv1a_matches_its_published_vectors():
+    assert _fnv1a64(b"") == 0xCBF29CE484222325
+    assert _fnv1a64(b"a") == 0xAF63DC4C8601EC8C
+    assert _fnv1a64(b"foobar") == 0x85944171F73967E8
+
+
+def test_state_hash_is_fnv1a_of_the_state_bytes():
+    """State3::hash (three.rs:280-297): FNV-1a over, per particle, each axis's position (i64 LE)
+    and velocity (i32 LE), then the species (u32 LE).  A 40-byte state spelled from a known string
+    hashes to FNV-1a of that string; random states to FNV-1a of their bytes."""
+    text = b"integer positions, integer velocities!\n\0"[:40]
+    assert len(text) == 40
+    pos = np.zeros((1, 3), np.int64)
+    vel = np.zeros((1, 3), np.int32)
+    off = 0
+    for a in range(3):
+        pos[0, a] = np.frombuffer(text[off:off + 8], "<i8")[0]
+        vel[0, a] = np.frombuffer(text[off + 8:off + 12], "<i4")[0]
+        off += 12
+    sp = np.frombuffer(text[36:40], "<u4").copy()
+    assert O.hash3(pos, vel, sp) == _fnv1a64(text)
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 17):
+        pos = rng.integers(-(1 << 62), 1 << 62, (n, 3), dtype=np.int64)
+        vel = rng.integers(-(1 << 31), 1 << 31, (n, 3), dtype=np.int64).astype(np.int32)
+        sp = rng.integers(0, 4, n, dtype=np.uint32)
+        data = b"".join(b"".join(int(pos[i, a]).to_bytes(8, "little", signed=True) + int(vel[i, a]).to_bytes(4, "little", signed=True)
+                                 for a in range(3)) + int(sp[i]).to_bytes(4, "little") for i in range(n))
+        assert O.hash3(pos, vel, sp) == _fnv1a64(data)
+
+
+# ---- the stochastic boundaries' draws (three.rs:232-265), pinned by their distributions ------
+
+def test_heat_bath_wall_and_bath_draws_follow_their_distributions():
+    """A heat-bath wall hands back the Maxwell flux distribution: the normal speed is Rayleigh
+    with mean sqrt(pi T / 2 m) and second moment 2 T / m, the tangential components are
+    Gaussian with variance T / m; the Andersen bath's three components are Gaussian with
+    variance T / m.  200,000 draws, moments within five standard errors."""
+    t, m, dt = 1.7, 2.5, 0.005
+    n = 200_000
+    d = O.draws3(0x1234_5678_9ABC, t, m, dt, 0, n, 41).astype(np.float64) / (dt * 2.0**32)
+    sd2 = t / m
+    bath, vn, tang = d[:, :3], d[:, 3], d[:, 4:]
+    mean_vn = math.sqrt(math.pi * t / (2 * m))
+    se = math.sqrt((2 - math.pi / 2) * sd2 / n)
+    assert abs(vn.mean() - mean_vn) < 5 * se
+    assert np.all(vn > 0)  # into the box from the low wall
+    assert abs((vn ** 2).mean() - 2 * sd2) < 5 * 2 * sd2 / math.sqrt(n) * 1.5
+    for col in (tang[:, 0], tang[:, 1], bath[:, 0], bath[:, 1], bath[:, 2]):
+        assert abs(col.mean()) < 5 * math.sqrt(sd2 / n)
+        assert abs(col.var() - sd2) < 5 * sd2 * math.sqrt(2.0 / n)
+    # the two tangentials (one Box-Muller pair) are uncorrelated, and so are the bath's axes
+    assert abs(np.corrcoef(tang[:, 0], tang[:, 1])[0, 1]) < 5 / math.sqrt(n)
+    assert abs(np.corrcoef(bath[:, 0], bath[:, 2])[0, 1]) < 5 / math.sqrt(n)
+
+
+def test_heat_bath_walls_book_the_energy_they_exchange_3d():
+    """With heat-bath walls top and bottom (no wall forces), the change of a dilute gas's total
+    energy equals the energy the walls booked in the ledger, to the integrator's error."""
+    n = 300
+    sys, pos, vel, sp = O.liquid3(n, 0.1, 0.5)
+    sys.walled[2] = 1
+    for side in (0, 1):
+        sys.wall_kind[2][side] = 2
+        sys.wall_t[2][side] = 2.0
+    pos[:, 2] = np.clip(pos[:, 2], ONE // 2, int(sys.box_len[2]) - ONE // 2)
+    e0 = sum(O.energy3(sys, pos, vel, sp))
+    _, led = O.step3(sys, pos, vel, sp, 3000)
+    e1 = sum(O.energy3(sys, pos, vel, sp))
+    booked = led.energy[4] + led.energy[5]
+    assert led.hits[4] + led.hits[5] > 50 and led.hits[0] == led.hits[1] == 0
+    assert booked > 0  # the hot walls heated the gas
+    assert abs((e1 - e0) - booked) < 0.03 * abs(booked) + 1.0

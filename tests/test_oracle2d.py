@@ -302,3 +302,72 @@ def test_free_piston_falls_freely_until_it_meets_the_gas():
     assert st.piston_v == 200 * kick
     assert st.piston_w == int(round(59.0 * ONE)) + kick * 200 * 201 // 2
     assert st.piston_work == 0.0
+
+
+# ---- md-ref's fast pair path (crates/md-ref/src/fast.rs) against the reference loop ----------
+
+def _fast_cases():
+    """The scenarios of crates/md-ref/tests/fast_path.rs:21-57: an LJ liquid; a WCA mixture of
+    three masses and two sizes with hard walls on x, soft walls on y and gravity; bonded pairs
+    in an LJ solvent."""
+    liq = O.system2(O.box_for(2000, 0.7), masses=(1.0,))
+    pos, vel, sp = O.fluid2(liq, 2000, 0.8, 1)
+    yield "LJ liquid", liq, pos, vel, sp, None
+    wca = lambda s: O.Pair(1.0, s, s * 2.0 ** (1.0 / 6.0))  # noqa: E731
+    mix = O.system2(O.box_for(1500, 0.3, 1.5), masses=(1.0, 4.0, 30.0), pair_=wca(1.0))
+    for a, b, sg in ((0, 2, 2.0), (1, 2, 2.0), (2, 2, 3.0)):
+        mix.pairs[a * 4 + b] = wca(sg)
+        mix.pairs[b * 4 + a] = wca(sg)
+    mix.walled[0] = mix.walled[1] = 1
+    mix.wall_kind[1][0] = mix.wall_kind[1][1] = 1  # soft floor and ceiling
+    mix.gravity[1] = -0.01
+    pos, vel, sp = O.fluid2(mix, 1500, 1.0, 2)
+    sp[:] = [2 if i % 50 == 0 else i % 2 for i in range(1500)]
+    big = pos[sp == 2]
+    keep = [i for i in range(1500) if sp[i] == 2 or all(((b - pos[i]) / ONE) @ ((b - pos[i]) / ONE) > 4.0 for b in big)]
+    pos, sp = np.ascontiguousarray(pos[keep]), np.ascontiguousarray(sp[keep])
+    vel = O.maxwell2(mix, sp, 1.0, 2)
+    yield "WCA mixture with walls and gravity", mix, pos, vel, sp, None
+    chains = O.system2(O.box_for(1200, 0.5), masses=(1.0,))
+    pos, vel, sp = O.fluid2(chains, 1200, 1.0, 3)
+    bonds = np.array([[2 * k, 2 * k + 1, 0] for k in range(200)], np.uint32)
+    chains.nbond_types = 1
+    chains.bond_k[0] = 50.0
+    chains.bond_r0[0] = 1.0
+    chains.nbonds = len(bonds)
+    chains.bonds = bonds.ctypes.data
+    yield "bonded pairs in LJ", chains, pos, vel, sp, bonds
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_fast_pair_path_is_bit_identical_to_the_reference_loop(prec):
+    """Trajectories of 300 steps and the kicks along them are identical bit for bit; the
+    potential (a float sum in another order) agrees to 1e-4 (fast_path.rs:8-18)."""
+    for name, sys, pos, vel, sp, keep_alive in _fast_cases():
+        pa, va = pos.copy(), vel.copy()
+        pb, vb = pos.copy(), vel.copy()
+        with O.fast_path():
+            ka, ea = O.kicks2(sys, pa, sp, prec)
+            O.step2(sys, pa, va, sp, 300, prec)
+            ka2, ea2 = O.kicks2(sys, pa, sp, prec)
+        kb, eb = O.kicks2(sys, pb, sp, prec)
+        O.step2(sys, pb, vb, sp, 300, prec)
+        kb2, eb2 = O.kicks2(sys, pb, sp, prec)
+        assert np.array_equal(ka, kb), name
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb), f"{name}: trajectories differ"
+        assert np.array_equal(ka2, kb2), name
+        for x, y in ((ea, eb), (ea2, eb2)):
+            assert abs(x - y) < 1e-4 * max(abs(y), 1.0), f"{name}: potential {x} against {y}"
+
+
+def test_fast_path_is_exercised_and_not_the_reference_loop():
+    """The switch really changes the code path: the kicks agree bit for bit, while the
+    potential, summed pair by pair in the fast path's cell order instead of particle by
+    particle, differs in its last bits."""
+    sys = O.system2(O.box_for(2000, 0.7), masses=(1.0,))
+    pos, vel, sp = O.fluid2(sys, 2000, 0.8, 1)
+    slow, e_slow = O.kicks2(sys, pos, sp, 32)
+    with O.fast_path():
+        fast, e_fast = O.kicks2(sys, pos, sp, 32)
+    assert np.array_equal(fast, slow)
+    assert e_fast != e_slow and abs(e_fast - e_slow) < 1e-5 * abs(e_slow)
