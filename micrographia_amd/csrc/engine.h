@@ -62,12 +62,14 @@ struct Params {
     float half_skin2;
     uint32_t block;        // half-list variant: particles per work-group block
     // Row-ordered list build (3D half list): rows are the (y, z) cells, each binned into
-    // rows_nxf fine cells along x; rows != 0 when the geometry allows it.
+    // rows_nxf fine cells along x, with rows_g ghost bins at each end on a periodic x axis
+    // (rows_ne bins a row in all); rows != 0 when the geometry allows it.
     uint32_t rows;
     uint32_t rows_nxf;
+    uint32_t rows_g;
+    uint32_t rows_ne;
     double rows_xscale;    // fine x cells per position unit
     float rows_inv_wx;     // fine x cells per sigma
-    float rows_lx;         // box length along x (sigma), the shift of a wrapped range
     float rows_reach;      // list radius of the widest pair plus a rounding margin (sigma)
     uint32_t range_check;  // list separations may reach 8 sigma: keep the reference's range test
     float cell_w[3];       // cell width (sigma), f32
@@ -136,10 +138,14 @@ struct Bufs {
     uint32_t* ids[2];
     double* ref[D];      // positions at the last list build
     float4* rel;         // position relative to its cell's corner (sigma, f32; w unused), for list builds
-    // Row-ordered copy for the row build: per entry (x sigma, y and z relative to the cell
-    // corner, slot | species << 30), ordered by row key; row_start[key] = first entry of the key
-    // (a dense exclusive scan, row_start[nkeys] = n); row_rank = a slot's place within its key.
+    // Row-ordered copy for the row build: per entry (x sigma, y and z relative to the cell's
+    // corner, slot | species << 30 with several species, else the slot), ordered by row and
+    // fine x bin.  row_cnt[r * nxf + b] counts the slots in bin b of row r and row_rank is a
+    // slot's place among them; row_start[r * ne + e] is the first entry of extended bin e of row
+    // r (a dense exclusive scan), whose bins are the row's ghost copies of its last rows_g bins
+    // (x - box), its own bins, then ghost copies of its first rows_g bins (x + box).
     float4* items;
+    uint32_t* row_cnt;
     uint32_t* row_start;
     uint32_t* row_rank;
     uint32_t* nbr;       // column-major: entry k of slot i at k*n + i
@@ -279,7 +285,8 @@ class Engine {
     bool row_path() const { return D == 3 && variant_ == 1 && half_path() && P_.rows != 0u; }
     void plan_rows();
     void alloc_rows();
-    uint64_t row_keys_ = 0;
+    uint64_t row_keys_ = 0;   // extended bins of all rows (row_start has one more entry)
+    uint64_t row_bins_ = 0;   // main bins of all rows (row_cnt)
     void* row_scan_temp_ = nullptr;
     void enqueue_extras(uint32_t flags);
     void write_partners();

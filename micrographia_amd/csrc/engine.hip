@@ -569,26 +569,28 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
 // ---------------------------------------------------------------------------------------------
 // Row-ordered half-list build (3D)
 // ---------------------------------------------------------------------------------------------
-// After the reorder, a second copy of the particles is ordered by row key: the row is the
-// particle's (y, z) cell, and within a row particles are binned into fine cells along x
-// (counting sort: count, exclusive scan, scatter; the order within a fine cell is arbitrary,
-// which no result depends on, since list order only changes the order of integer sums).  The
-// list sphere around a particle meets at most the nine rows around its own, and in each of them
-// the candidates whose x lies within the sphere's chord over that row's (y, z) rectangle are one
-// contiguous range of the copy (two where the chord wraps a periodic x boundary).  That is
-// about 245 candidates per particle at liquid density against about 513 in the 27 cells of
-// the stencil build, and nine ranges instead of 27 cells.  Candidates are tested as in the
-// stencil build (f32 separations within ~1e-5 sigma of the exact one against the (rc + skin)
-// (1 + 1e-4) radius, a margin of ~3e-4 sigma), with x absolute (boxes up to 512 sigma along x)
-// and y, z relative to the cell corner.  The chunk stream written is the stencil build's: the
-// same pairs, tags and wave regions, in another order.
+// After the reorder, a second copy of the particles is ordered by row and fine x bin: a row is
+// a (y, z) cell, and along x each row is binned into fine cells (counting sort: count,
+// exclusive scan, scatter; the order within a fine cell is arbitrary, which no result depends
+// on, since list order only changes the order of integer sums).  On a periodic x axis every row
+// also carries ghost copies of its first and last few bins beyond its ends (shifted by the box),
+// so that the candidates whose x lies within the list sphere's chord over a row's (y, z)
+// rectangle are always one contiguous range of the copy.  The sphere around a particle meets at
+// most the nine rows around its own: about 257 candidates per particle at liquid density against
+// about 513 in the 27 cells of the stencil build.  Candidates are tested as in the stencil build
+// (f32 separations within ~1e-5 sigma of the exact one against the (rc + skin) (1 + 1e-4)
+// radius, a margin of ~3e-4 sigma), with x absolute (boxes up to 512 sigma along x) and y, z
+// relative to the cell's corner.  The chunk stream written is the stencil build's: the same
+// pairs, tags and wave regions, in another order.
 
-// Row key of a position in cell c: (c_z * nc_y + c_y) * nxf + fine x cell.
-__device__ __forceinline__ uint32_t row_key(const Params<3>& P, const double x[3], const uint32_t c[3]) {
+// Main bin of a position: row (z cell * cells along y + y cell), fine x cell.
+__device__ __forceinline__ void row_bin(const Params<3>& P, const double x[3], uint32_t& row, uint32_t& xf) {
+    uint32_t c[3];
+    cell_of<3>(P, x, c);
+    row = c[2] * P.nc[1] + c[1];
     const double f = x[0] * P.rows_xscale;
-    uint32_t xf = f > 0.0 ? (uint32_t)f : 0u;
-    xf = xf < P.rows_nxf - 1u ? xf : P.rows_nxf - 1u;
-    return (c[2] * P.nc[1] + c[1]) * P.rows_nxf + xf;
+    const uint32_t v = f > 0.0 ? (uint32_t)f : 0u;
+    xf = v < P.rows_nxf - 1u ? v : P.rows_nxf - 1u;
 }
 
 __global__ __launch_bounds__(TPB) void k_row_count(Params<3> P, Bufs<3> B, const uint32_t* gate) {
@@ -599,9 +601,26 @@ __global__ __launch_bounds__(TPB) void k_row_count(Params<3> P, Bufs<3> B, const
     double x[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) x[a] = B.pos[pp][a][k];
-    uint32_t c[3];
-    cell_of<3>(P, x, c);
-    B.row_rank[k] = atomicAdd(&B.row_start[row_key(P, x, c)], 1u);
+    uint32_t row, xf;
+    row_bin(P, x, row, xf);
+    B.row_rank[k] = atomicAdd(&B.row_cnt[row * P.rows_nxf + xf], 1u);
+}
+
+// Counts of the extended bins (ghost bins take their source bin's count), for the scan; the
+// entry past the last is zeroed, so the scan leaves the total there.
+__global__ __launch_bounds__(TPB) void k_row_ext(Params<3> P, Bufs<3> B, uint32_t nkeys, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k > nkeys) return;
+    if (k == nkeys) {
+        B.row_start[k] = 0u;
+        return;
+    }
+    const uint32_t row = k / P.rows_ne, e = k - row * P.rows_ne;
+    int b = (int)e - (int)P.rows_g;
+    if (b < 0) b += (int)P.rows_nxf;
+    else if (b >= (int)P.rows_nxf) b -= (int)P.rows_nxf;
+    B.row_start[k] = B.row_cnt[row * P.rows_nxf + (uint32_t)b];
 }
 
 __global__ __launch_bounds__(TPB) void k_row_scatter(Params<3> P, Bufs<3> B, const uint32_t* gate) {
@@ -612,11 +631,26 @@ __global__ __launch_bounds__(TPB) void k_row_scatter(Params<3> P, Bufs<3> B, con
     double x[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) x[a] = B.pos[pp][a][k];
-    uint32_t c[3];
-    cell_of<3>(P, x, c);
-    const uint32_t q = B.row_start[row_key(P, x, c)] + B.row_rank[k];
+    uint32_t row, xf;
+    row_bin(P, x, row, xf);
     const float4 r = B.rel[k];
-    B.items[q] = make_float4((float)(x[0] * 0x1p-32), r.y, r.z, __uint_as_float(k | (B.species[pm][k] << 30)));
+    const float ry = r.y, rz = r.z;
+    const uint32_t rank = B.row_rank[k], base = row * P.rows_ne, g = P.rows_g, nxf = P.rows_nxf;
+    const uint32_t sp = P.nspecies > 1 ? B.species[pm][k] << 30 : 0u;
+    const float w = __uint_as_float(k | sp);
+    B.items[B.row_start[base + g + xf] + rank] = make_float4((float)(x[0] * 0x1p-32), ry, rz, w);
+    if (xf < g)  // a ghost past the row's end
+        B.items[B.row_start[base + g + nxf + xf] + rank] = make_float4((float)((x[0] + P.boxf[0]) * 0x1p-32), ry, rz, w);
+    if (xf + g >= nxf)  // a ghost before the row's start
+        B.items[B.row_start[base + xf + g - nxf] + rank] = make_float4((float)((x[0] - P.boxf[0]) * 0x1p-32), ry, rz, w);
+}
+
+// 2 q + (bit of this lane in the wave mask m): one add with the mask as its carry-in.
+__device__ __forceinline__ uint32_t shift_in(uint32_t q, uint64_t m) {
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(q), "s"(m));
+    return r;
 }
 
 template <bool MULTI>
@@ -634,7 +668,12 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
         cell_of<3>(P, xi, c);
         Xi = (float)(xi[0] * 0x1p-32);
     }
-    const float4 me = B.rel[i];
+    float mey, mez;  // y and z relative to the cell corner
+    {
+        const float4 me = B.rel[i];
+        mey = me.y;
+        mez = me.z;
+    }
     float lr2[4];
     if constexpr (MULTI) {
         const uint32_t si = B.species[pm][i];
@@ -646,11 +685,11 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
     const float reach2 = P.rows_reach * P.rows_reach;
     const float4* __restrict__ items = B.items;
     const uint32_t cap4 = P.cap / 4u;
-    const uint32_t blo = i & ~(P.block - 1u);  // the block is a power of two
-    const uint32_t wave = i >> 6, own = i & (P.block - 1u);
+    // the block (a power of two, at least a wave) and the wave are the same for every lane
+    const uint32_t blo = __builtin_amdgcn_readfirstlane(i & ~(P.block - 1u));
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(i >> 6), own = i & (P.block - 1u);
     const uint32_t capw = 64u * cap4;
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
-    const uint64_t lt = lanemask_lt();
     // A lane fills one in-block and one cross-block chunk in registers; full chunks are stored
     // whole (16 B) at the wave's fill point.  `fill` changes only where every lane is present.
     uint32_t fill = 0u;
@@ -658,17 +697,18 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
         const uint64_t bal = ballot64(want);
         if (bal == 0ull) return;
         if (want) {
-            const uint32_t slot = fill + (uint32_t)__popcll(bal & lt);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            const uint32_t slot = fill + below;
             if (slot < capw) chunks[slot] = tag_chunk(q, own, entries, in);
             else atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         }
         fill += (uint32_t)__popcll(bal);
     };
     // Per lane, in LDS: the open in-block (kind 1) and cross-block (kind 0) chunks, entry s of
-    // kind k at ck[4 k + s][lane], and a ring of up to four hit masks, each with its group's
-    // first copy index and its kind in bit 31.  A visit moves as many hits of the front mask
-    // into that kind's open chunk as it has room for (up to four) and stores the chunk when it
-    // is full.  Visits run when some lane's ring has no room for a group's two masks, and at the
+    // kind k at ck[4 k + s][lane], and a ring of up to four hit masks, each with the copy index
+    // of its bit 0 (bit b stands for that index minus b) and its kind in bit 31.  A visit moves
+    // as many hits of the front mask into that kind's open chunk as it has room for (up to
+    // four) and stores the chunk when it is full.  Visits run when some lane's ring has no room for a group's two masks, and at the
     // end.  (Held in LDS, the ring and chunks leave registers to the candidate loop.)
     __shared__ uint32_t ck[8][TPB];
     __shared__ uint32_t qm[4][TPB], qt[4][TPB];
@@ -682,7 +722,7 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
             uint32_t m = qm[h][t];
             const uint32_t tag = qt[h][t];
             in = (tag >> 31) != 0u;
-            const uint32_t b0 = tag & 0x7fffffffu;
+            const uint32_t b1 = tag & 0x7fffffffu;  // the copy index of bit 0
             const uint32_t n = in ? na : nb;
             const uint32_t r = n & 3u;
             uint32_t jv[4];
@@ -692,7 +732,7 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
                 take[s] = s >= r && m != 0u;
                 const uint32_t at = take[s] ? (uint32_t)__builtin_ctz(m) : 0u;
                 m = take[s] ? m & (m - 1u) : m;
-                jv[s] = __float_as_uint(items[b0 + at].w) & 0x0fffffffu;
+                jv[s] = __float_as_uint(items[b1 - at].w) & 0x0fffffffu;
             }
             const uint32_t row = in ? 4u : 0u;
 #pragma unroll
@@ -722,64 +762,52 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
         qt[h][t] = tag;
         qn++;
     };
-    const bool wrap_x = (P.periodic & 1u) != 0u;
-    const int nxf = (int)P.rows_nxf;
+    // the chord's bins are clamped to the row's extended bins (its ghosts on a periodic x axis)
+    const int glo = -(int)P.rows_g, ghi = (int)(P.rows_nxf + P.rows_g) - 1;
+    const uint32_t ny = P.nc[1], nz = P.nc[2];
+    const float wy = P.cell_w[1], wz = P.cell_w[2], sx = -Xi;
     for (int o = 0; o < 9; o++) {
         const int oy = o % 3 - 1, oz = o / 3 - 1;
-        int yc = (int)c[1] + oy, zc = (int)c[2] + oz;
+        int yy = (int)c[1] + oy, zz = (int)c[2] + oz;
         bool skip = false;
-        if (yc < 0 || yc >= (int)P.nc[1]) {
+        if (yy < 0 || yy >= (int)ny) {
             if (!((P.periodic >> 1) & 1u)) skip = true;
-            yc = yc < 0 ? yc + (int)P.nc[1] : yc - (int)P.nc[1];
+            yy = yy < 0 ? yy + (int)ny : yy - (int)ny;
         }
-        if (zc < 0 || zc >= (int)P.nc[2]) {
+        if (zz < 0 || zz >= (int)nz) {
             if (!((P.periodic >> 2) & 1u)) skip = true;
-            zc = zc < 0 ? zc + (int)P.nc[2] : zc - (int)P.nc[2];
+            zz = zz < 0 ? zz + (int)nz : zz - (int)nz;
         }
         // distance from the particle to the row's (y, z) rectangle, and the chord along x
-        const float dyl = oy == 0 ? 0.f : (oy > 0 ? P.cell_w[1] - me.y : me.y);
-        const float dzl = oz == 0 ? 0.f : (oz > 0 ? P.cell_w[2] - me.z : me.z);
+        const float dyl = oy == 0 ? 0.f : fmaxf(0.f, oy > 0 ? wy - mey : mey);
+        const float dzl = oz == 0 ? 0.f : fmaxf(0.f, oz > 0 ? wz - mez : mez);
         const float d2 = dyl * dyl + dzl * dzl;
         skip |= !(d2 < reach2);
         const float ext = skip ? 0.f : __builtin_sqrtf(reach2 - d2);
-        const float sy = (float)oy * P.cell_w[1] - me.y, sz = (float)oz * P.cell_w[2] - me.z;
-        const int flo = (int)__builtin_floorf((Xi - ext) * P.rows_inv_wx);
-        const int fhi = (int)__builtin_floorf((Xi + ext) * P.rows_inv_wx);
-        const uint32_t rowbase = skip ? 0u : ((uint32_t)zc * P.nc[1] + (uint32_t)yc) * P.rows_nxf;
-        // the part of the chord inside [0, nxf) and, on a periodic x axis, the part that wraps
-        for (int seg = 0; seg < 2; seg++) {
-            int a = 1, b = 0;
-            float sx = -Xi;
-            if (!skip) {
-                if (seg == 0) {
-                    a = flo > 0 ? flo : 0;
-                    b = fhi < nxf - 1 ? fhi : nxf - 1;
-                } else if (wrap_x && flo < 0) {
-                    a = flo + nxf;
-                    b = nxf - 1;
-                    sx = -Xi - P.rows_lx;
-                } else if (wrap_x && fhi >= nxf) {
-                    a = 0;
-                    b = fhi - nxf;
-                    sx = -Xi + P.rows_lx;
-                }
-            }
-            uint32_t s0 = 0u, e0 = 0u;
-            if (a <= b) {
-                s0 = B.row_start[rowbase + (uint32_t)a];
-                e0 = B.row_start[rowbase + (uint32_t)b + 1u];
-            }
-            // Candidates in groups of up to 32.  Three masks are built side by side, none
-            // depending on another (so the test has no branch and each candidate is one 16-byte
-            // load): within the list radius, in i's block, and of higher index than i.  An
-            // in-block pair is kept at its lower-index end only.
+        const float sy = (float)oy * wy - mey, sz = (float)oz * wz - mez;
+        int a = (int)__builtin_floorf((Xi - ext) * P.rows_inv_wx);
+        int b = (int)__builtin_floorf((Xi + ext) * P.rows_inv_wx);
+        a = a > glo ? a : glo;
+        b = b < ghi ? b : ghi;
+        uint32_t s0 = 0u, e0 = 0u;
+        if (!skip && a <= b) {
+            const uint32_t rowbase = ((uint32_t)zz * ny + (uint32_t)yy) * P.rows_ne + P.rows_g;
+            s0 = B.row_start[rowbase + (uint32_t)a];
+            e0 = B.row_start[rowbase + (uint32_t)b + 1u];
+        }
+        {
+            // Candidates in groups of up to 32, tested without a branch (each candidate is one
+            // 16-byte load) into two masks: in-block partners of higher index than i (an
+            // in-block pair is kept at its lower-index end only) and partners outside i's block,
+            // both within the list radius.
             for (uint32_t base = s0;; base += 32u) {
                 const bool act = base < e0;
                 if (ballot64(act) == 0ull) break;
-                uint32_t m_in = 0u, m_out = 0u;
+                // Hits are shifted in as they are tested: candidate base + u ends at bit
+                // n - 1 - u of its mask, n being the number of candidates tested.
+                uint32_t m_in = 0u, m_out = 0u, last = base;
                 if (act) {
                     const uint32_t cnt = min(32u, e0 - base);
-                    uint32_t mnear = 0u, minb = 0u, mgt = 0u;
                     // four at a time; a lane past its range's end tests the copy's entries that
                     // follow (the copy is padded by 32), whose bits are masked off after
                     const float4* __restrict__ rq = items + base;
@@ -794,28 +822,32 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
                             r2 = __builtin_fmaf(dy, dy, r2);
                             r2 = __builtin_fmaf(dz, dz, r2);
                             const uint32_t w = __float_as_uint(q.w);
-                            const uint32_t j = w & 0x0fffffffu;
+                            uint32_t j;
                             float lr;
                             if constexpr (MULTI) {
+                                j = w & 0x0fffffffu;
                                 const uint32_t sj = w >> 30;
                                 lr = sj == 0u ? lr2[0] : (sj == 1u ? lr2[1] : (sj == 2u ? lr2[2] : lr2[3]));
                             } else {
+                                j = w;  // one species: the copy holds the slot alone
                                 lr = lr2[0];
                             }
-                            const uint32_t bit = 1u << (u0 + k);
-                            mnear |= r2 < lr ? bit : 0u;
-                            minb |= j - blo < P.block ? bit : 0u;
-                            mgt |= j > i ? bit : 0u;
+                            // the three tests as wave masks, combined on the scalar unit
+                            const uint64_t near = ballot64(r2 < lr), inb = ballot64(j - blo < P.block), gt = ballot64(j > i);
+                            m_in = shift_in(m_in, near & inb & gt);
+                            m_out = shift_in(m_out, near & ~inb);
                         }
                     }
-                    mnear &= bits_below((int)cnt);
-                    m_in = mnear & minb & mgt;
-                    m_out = mnear & ~minb;
+                    const uint32_t n = (cnt + 3u) & ~3u;
+                    const uint32_t live = ~bits_below((int)(n - cnt));
+                    m_in &= live;
+                    m_out &= live;
+                    last = base + n - 1u;
                 }
                 const uint32_t need = (m_in != 0u ? 1u : 0u) + (m_out != 0u ? 1u : 0u);
                 while (ballot64(qn + need > 4u) != 0ull) visit();
-                push(m_in, base | 0x80000000u);
-                push(m_out, base);
+                push(m_in, last | 0x80000000u);
+                push(m_out, last);
             }
         }
     }
@@ -1655,7 +1687,7 @@ uint64_t Engine<D>::planned_bytes() const {
     t += sz(n * 4);                                                                                  // nbr_count
     t += sz(((n + 4095u) & ~uint64_t(4095)) / 64u * 4);                                              // wave_chunks
     t += sz((n + 32u) * 16);                                                                         // rel
-    if (P_.rows) t += sz((n + 32u) * 16) + sz((row_keys_ + 1u) * 4) + sz(n * 4) + sz(scan_temp_bytes(row_keys_ + 1u));
+    if (P_.rows) t += sz((2 * n + 32u) * 16) + sz(row_bins_ * 4) + sz((row_keys_ + 1u) * 4) + sz(n * 4) + sz(scan_temp_bytes(row_keys_ + 1u));
     t += 2 * sz(ncell_keys_ * 4);                                                                    // cell_start, cell_end
     t += 2 * sz(n * 8);                                                                              // pe, ke
     t += sz(CTL_WORDS * 4) + sz(LEDGER_WORDS * 8);
@@ -1861,15 +1893,17 @@ void Engine<D>::collect_timers() {
 }
 
 // The row build's geometry (3D Verlet mode): rows are the (y, z) cells, binned along x into 8,
-// 4, 2 or 1 fine cells per cell, whichever keeps the row keys within 2^25.  It needs three cells
-// on every axis (distinct rows, a chord shorter than the box), and x below 512 sigma (x is held
-// absolute in f32 there, within 3e-5 sigma).
+// 4, 2 or 1 fine cells per cell, whichever first keeps the extended bins within 2^25.  It needs
+// three cells on every axis (distinct rows, a chord shorter than the box), and x below 512
+// sigma (x is held absolute in f32 there, within 3e-5 sigma).  A periodic x axis adds ghost
+// bins covering the list radius plus one bin at both ends of every row.
 template <int D>
 void Engine<D>::plan_rows() {
     Params<D>& P = P_;
     P.rows = 0u;
     P.rows_nxf = 0u;
     row_keys_ = 0;
+    row_bins_ = 0;
     if constexpr (D == 3) {
         if (mode_.kind != 1) return;
         for (int a = 0; a < 3; a++)
@@ -1879,27 +1913,37 @@ void Engine<D>::plan_rows() {
         float lr2max = 0.f;
         for (int k = 0; k < 16; k++) lr2max = std::max(lr2max, P.pt.list_r2[k]);
         const double reach = std::sqrt((double)lr2max) + 1e-3;
-        uint32_t sub = 8;
-        while (sub > 1 && (uint64_t)P.nc[0] * sub * P.nc[1] * P.nc[2] > (uint64_t(1) << 25)) sub /= 2;
-        const uint64_t keys = (uint64_t)P.nc[0] * sub * P.nc[1] * P.nc[2];
-        if (keys > (uint64_t(1) << 25)) return;
-        const uint32_t nxf = P.nc[0] * sub;
-        if (!(lx > 2.0 * reach + 2.0 * lx / nxf)) return;
-        P.rows = 1u;
-        P.rows_nxf = nxf;
-        P.rows_xscale = (double)nxf / (double)sys_.box_len[0];
-        P.rows_inv_wx = (float)((double)nxf / lx);
-        P.rows_lx = (float)lx;
+        const bool wrap_x = (P.periodic & 1u) != 0u;
+        for (uint32_t sub = 8; sub >= 1 && !P.rows; sub /= 2) {
+            const uint64_t nxf = (uint64_t)P.nc[0] * sub;
+            const double wx = lx / (double)nxf;
+            const uint64_t g = wrap_x ? (uint64_t)std::ceil(reach / wx) + 1u : 0u;
+            const uint64_t rows = (uint64_t)P.nc[1] * P.nc[2];
+            const uint64_t keys = rows * (nxf + 2 * g);
+            // a chord never holds a slot and its ghost, and no slot has two ghosts
+            if (keys > (uint64_t(1) << 25) || nxf <= 2 * g || !(lx > 2.0 * reach + 2.0 * wx)) continue;
+            P.rows = 1u;
+            P.rows_nxf = (uint32_t)nxf;
+            P.rows_g = (uint32_t)g;
+            P.rows_ne = (uint32_t)(nxf + 2 * g);
+            row_keys_ = keys;
+            row_bins_ = rows * nxf;
+        }
+        if (!P.rows) return;
+        P.rows_xscale = (double)P.rows_nxf / (double)sys_.box_len[0];
+        P.rows_inv_wx = (float)((double)P.rows_nxf / lx);
         P.rows_reach = (float)reach;
-        row_keys_ = keys;
     }
 }
 
 template <int D>
 void Engine<D>::alloc_rows() {
     if (P_.rows == 0u || B_.items != nullptr) return;
-    B_.items = (float4*)dalloc((n_ + 32u) * sizeof(float4));  // padded: the build reads up to 31 past a range
-    RH_HIP(hipMemset(B_.items, 0, (n_ + 32u) * sizeof(float4)));
+    // every slot has one copy and at most one ghost; padded, since the build reads past a range
+    const size_t items = 2 * n_ + 32u;
+    B_.items = (float4*)dalloc(items * sizeof(float4));
+    RH_HIP(hipMemset(B_.items, 0, items * sizeof(float4)));
+    B_.row_cnt = (uint32_t*)dalloc(row_bins_ * sizeof(uint32_t));
     B_.row_start = (uint32_t*)dalloc((row_keys_ + 1u) * sizeof(uint32_t));
     B_.row_rank = (uint32_t*)dalloc(n_ * sizeof(uint32_t));
     row_scan_temp_ = dalloc(scan_temp_bytes(row_keys_ + 1u));
@@ -2089,6 +2133,7 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     k_keys<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.keys[0], sort_.vals[0], gate);
     RH_HIP(hipGetLastError());
     int r = sort_.sort(n_, key_bits_, gate, stream_);
+    const bool rows = mode_.kind == 1 && row_path();
     k_gather<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], 1, gate);
     RH_HIP(hipGetLastError());
     fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
@@ -2098,9 +2143,11 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
         if constexpr (D == 3) {
-            if (row_path()) {
-                fill_u32(B_.row_start, 0u, row_keys_ + 1u, gate, stream_);
+            if (rows) {
+                fill_u32(B_.row_cnt, 0u, row_bins_, gate, stream_);
                 k_row_count<<<nb, TPB, 0, stream_>>>(P_, B_, gate);
+                RH_HIP(hipGetLastError());
+                k_row_ext<<<blocks_for(row_keys_ + 1u, TPB), TPB, 0, stream_>>>(P_, B_, (uint32_t)row_keys_, gate);
                 RH_HIP(hipGetLastError());
                 exclusive_scan_u32(B_.row_start, B_.row_start, row_keys_ + 1u, row_scan_temp_, gate, stream_);
                 k_row_scatter<<<nb, TPB, 0, stream_>>>(P_, B_, gate);
@@ -2459,11 +2506,12 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     }
     Params<D> old = P_;
     uint32_t old_bits = key_bits_;
-    uint64_t old_keys = ncell_keys_, old_row_keys = row_keys_;
+    uint64_t old_keys = ncell_keys_, old_row_keys = row_keys_, old_row_bins = row_bins_;
     build_params(sys, mode_);
     key_bits_ = old_bits;
     ncell_keys_ = old_keys;
     row_keys_ = old_row_keys;
+    row_bins_ = old_row_bins;
     // Keep the cell geometry the buffers were sized for; a pair whose list radius grew is listed
     // further out from the next rebuild on.
     for (int a = 0; a < 3; a++) {
@@ -2477,9 +2525,10 @@ void Engine<D>::update_system(const SystemDesc& sys) {
     // the row build's geometry and reach follow the kept cells and list radii
     P_.rows = old.rows;
     P_.rows_nxf = old.rows_nxf;
+    P_.rows_g = old.rows_g;
+    P_.rows_ne = old.rows_ne;
     P_.rows_xscale = old.rows_xscale;
     P_.rows_inv_wx = old.rows_inv_wx;
-    P_.rows_lx = old.rows_lx;
     P_.rows_reach = old.rows_reach;
     P_.half_skin2 = old.half_skin2;
     // Cutoffs only shrink, so a range test that was needed stays safe; keeping it keeps the list
