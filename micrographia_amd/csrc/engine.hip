@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <deque>
+#include <type_traits>
 
 namespace rh {
 
@@ -883,15 +884,26 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
     const uint32_t pp = ctl[CTL_PPOS], pm = ctl[CTL_PMETA];
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
+    // Periodic axes on which the block needs no image: with every particle of the block at
+    // least 8 sigma (the bound on list separations) from both faces, every partner's stored
+    // position is its nearest image, and separation16's correction is exactly zero.
+    __shared__ uint32_t edge;
+    if (t == 0u) edge = 0u;
+    __syncthreads();
+    uint32_t near_face = 0u;
 #pragma unroll
     for (uint32_t r = 0; r < PER; r++) {
         const uint32_t l = t + r * NT;
 #pragma unroll
         for (int a = 0; a < D; a++) {
-            lpos[a][l] = (base + l < P.n ? px[a][base + l] * 0.0625 : 0.0);
+            const bool real = base + l < P.n;
+            const double x16 = real ? px[a][base + l] * 0.0625 : 0.0;
+            lpos[a][l] = x16;
             acc[a][l] = 0u;
+            if (real && ((P.periodic >> a) & 1u) && !(x16 >= 0x1p31 && x16 <= P.box16[a] - 0x1p31)) near_face = 1u;
         }
     }
+    if (ballot64(near_face != 0u) != 0ull && __lane_id() == 0u) atomicOr(&edge, 1u);
     // the chunk counts of the block's NW list-build waves, scanned by the first wave
     constexpr uint32_t NW = (uint32_t)BS / 64u;
     __shared__ uint32_t wfirst[NW + 1];
@@ -907,6 +919,7 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
         if (t == NW - 1u) wfirst[NW] = x;
     }
     __syncthreads();
+    const bool images = __builtin_amdgcn_readfirstlane(edge) != 0u;
     const uint32_t* sp = B.species[pm];
     const PairTable* __restrict__ pt = B.pt;
     {
@@ -924,6 +937,8 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
         // the next chunk is in flight while this one is evaluated
         uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
         if (g < nch) q_next = chunks[slot(g)];
+        // the loop twice over: with the periodic image of each separation, and without it
+        auto chunk_loop = [&](auto with_images) {
         for (; g < nch; g += NT) {
             const uint4 qt = q_next;
             if (g + NT < nch) q_next = chunks[slot(g + NT)];
@@ -973,7 +988,10 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
                 // may reach it runs the full-list kernel, which tests the range)
                 float f[D];
 #pragma unroll
-                for (int a = 0; a < D; a++) f[a] = sep_f16(separation16<D>(P, xi[a], xs[e][a], a));
+                for (int a = 0; a < D; a++) {
+                    if constexpr (decltype(with_images)::value) f[a] = sep_f16(separation16<D>(P, xi[a], xs[e][a], a));
+                    else f[a] = sep_f16(xs[e][a] - xi[a]);
+                }
                 int32_t kj[D];
                 bool hit;
                 if constexpr (MULTI) {
@@ -992,6 +1010,9 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
 #pragma unroll
             for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
         }
+        };
+        if (images) chunk_loop(std::true_type{});
+        else chunk_loop(std::false_type{});
     }
     __syncthreads();
     for (uint32_t r = 0; r < PER; r++) {
