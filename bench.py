@@ -274,8 +274,12 @@ def serving_leg(rate: float, duration: float, meta_seed: int, plain: bool):
     srv = Server(budget_bytes=budget, max_systems=4096, chunk_s=0.02)
     rep = srv.serve(reqs, realtime=True)
     srv.close()
+    offered = sum(r.n * r.steps for r in reqs) / duration if duration > 0 else 0.0
     out = {"workload": f"Poisson {rate}/s over {duration} s (meta-seed {meta_seed}): N log-uniform [1k, 1M], 2D/3D 50/50, steps U[100, 10k], rho U[0.3, 0.9], T U[0.5, 2.0]; admission budget {budget} B of device memory",
-           **rep.summary()}
+           **rep.summary(),
+           # the work arriving per second of the arrival window, against what the GPU served
+           "offered_atom_steps_per_s": offered,
+           "offered_over_served": offered / rep.summary()["atom_steps_per_s"] if rep.completed else None}
     if plain:
         p = serve_plain(reqs, realtime=True)
         out["plain"] = p.summary()

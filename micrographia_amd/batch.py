@@ -79,20 +79,30 @@ class BatchEngine:
         check(load().rh_batch_poll(self._h, ptr(rem), self.max_systems))
         return {i: int(rem[i]) for i in self._members}
 
-    def step_all(self, steps: int) -> None:
-        """Run until every member has finished its steps (and wait).  A system whose list
-        overflowed stops (its steps are gated) until poll() has regrown the list; the loop then
-        runs the steps still left.  Raises if a round of runs makes no progress: an overflowed
-        list that cannot grow in the arena as it stands."""
+    def step_all(self, steps: Optional[int] = None) -> None:
+        """Advance every member by `steps` steps, or to the end of its own when fewer are left
+        (all of them with steps=None), and wait.  A system whose list overflowed stops (its steps
+        are gated) until poll() has regrown the list; the steps it lost are then run for it alone
+        (the others paused at their targets).  Raises if a catch-up round makes no progress: an
+        overflowed list that cannot grow in the arena as it stands."""
+        before = self.poll()
+        if steps is None:
+            steps = max(before.values(), default=0)
+        target = {i: max(0, v - int(steps)) for i, v in before.items()}
         self.run(steps)
         left = self.poll()
-        while any(left.values()):
-            before = left
-            self.run(max(left.values()))
-            left = self.poll()
-            if left == before:
-                stuck = sorted(i for i, v in left.items() if v)
+        while any(left[i] > target[i] for i in left):
+            deficit = {i: left[i] - target[i] for i in left}
+            for i, d in deficit.items():
+                self.set_steps(i, d)  # lagging members run what they lost, the rest pause
+            self.run(max(deficit.values()))
+            now = self.poll()
+            for i in left:
+                self.set_steps(i, target[i] + now[i])
+            if not any(now[i] < deficit[i] for i in deficit if deficit[i]):
+                stuck = sorted(i for i, d in deficit.items() if d)
                 raise RahmanError(RH_ERR_NOMEM, f"systems {stuck} cannot continue: their lists overflowed and the arena has no room to grow them")
+            left = self.poll()
 
     def read_state(self, id_: int) -> Union[State, State3]:
         sys = self._members[id_]
