@@ -231,7 +231,7 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool,
         "systems_per_s_unit": f"systems advanced by one {chunk}-step chunk per second",
         "arena_bytes_needed": need,
         "autotuned": {
-            "table": {k: {"variant": v[0], "cell_factor": v[1]} for k, v in sorted(tuner.table.items())},
+            "table": {k: {"variant": v[0], "cell_factor": v[1], "block": v[2] if len(v) > 2 else 256} for k, v in sorted(tuner.table.items())},
             "tuning_seconds": tuner.seconds,
             "atom_steps_per_s_default_tiles": atoms * steps / el_default if el_default else None,
         },

@@ -8,8 +8,10 @@ all pairs within the cutoff), so tuning only changes speed.
   several species or one).  Each candidate runs a few steps on the caller's own system.
 - ``BatchTuner`` picks, per (dimension, size class), the batched engine's list variant (half list
   within work-groups, or every pair at both ends) and cell occupancy (cells 1, 1.25 or 1.5 times
-  rc + skin wide: fewer, fuller cells or more, emptier ones), timed on a sample of the caller's
-  systems of that bucket, and installs them with ``BatchEngine.set_tuning``.
+  rc + skin wide: fewer, fuller cells or more, emptier ones), then for the half list the block
+  (work-group tile of 256, 512 or 1024 slots: larger blocks keep more pairs inside one work-group's
+  LDS), timed on a sample of the caller's systems of that bucket, and installs them with
+  ``BatchEngine.set_tuning``.
 
 Choices are kept per bucket in memory and, if a path is given, in a JSON file.
 """
@@ -23,6 +25,7 @@ from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
 
 ENGINE_CANDIDATES: Tuple[Tuple[int, int], ...] = ((1, 4096), (1, 2048), (1, 1024), (1, 512), (1, 256), (0, 256))
 BATCH_CANDIDATES: Tuple[Tuple[int, float], ...] = ((1, 1.0), (1, 1.25), (1, 1.5), (0, 1.0), (0, 1.25))
+BATCH_BLOCKS: Tuple[int, ...] = (256, 512, 1024)
 
 
 def size_class(n: int) -> int:
@@ -106,10 +109,15 @@ class EngineTuner:
 
 @dataclass
 class BatchTuner:
-    """Per-(dimension, size class) (variant, cell factor) for the batched engine."""
+    """Per-(dimension, size class) (variant, cell factor, block) for the batched engine: the
+    (variant, cell factor) candidates at block 256 first, then the other blocks for the best of
+    them if it is the half list (a block is tried only where the sample's mean size is at least
+    four blocks, and kept only if it is block_margin faster)."""
 
     path: Optional[str] = None
     candidates: Sequence[Tuple[int, float]] = BATCH_CANDIDATES
+    blocks: Sequence[int] = BATCH_BLOCKS
+    block_margin: float = 0.03
     per_bucket: int = 8
     steps: int = 200
     chunk: int = 100
@@ -150,23 +158,34 @@ class BatchTuner:
                 dim, cls = int(key[0]), int(key.split("/")[1])
                 atoms = sum(st.n() for _, st in sample)
                 trials = []
-                for variant, factor in self.candidates:
-                    eng.set_tuning(dim, cls, variant, factor)
+
+                def trial(variant: int, factor: float, block: int) -> None:
+                    eng.set_tuning(dim, cls, variant, factor, block)
                     ids = [eng.add(s, st, self.steps + self.chunk, skin) for s, st in sample]
                     if any(i < 0 for i in ids):
                         for i in ids:
                             if i >= 0:
                                 eng.remove(i)
-                        continue
+                        return
                     eng.step_all(self.chunk)  # first lists and graph capture
                     t0 = time.perf_counter()
                     eng.step_all(self.steps)
                     el = time.perf_counter() - t0
                     for i in ids:
                         eng.remove(i)
-                    trials.append(Trial((variant, factor), atoms * self.steps / el))
+                    trials.append(Trial((variant, factor, block), atoms * self.steps / el))
+
+                for variant, factor in self.candidates:
+                    trial(variant, factor, self.blocks[0])
                 if trials:
-                    best = max(trials, key=lambda t: t.rate)
+                    first = max(trials, key=lambda t: t.rate)
+                    variant, factor, _ = first.setting
+                    if variant == 1:
+                        for block in self.blocks[1:]:
+                            if atoms >= 4 * block * len(sample):
+                                trial(variant, factor, block)
+                    # a larger block must beat the smallest by a margin above the timing noise
+                    best = max(trials, key=lambda t: t.rate if t.setting[2] == first.setting[2] else t.rate / (1.0 + self.block_margin))
                     self.table[key] = list(best.setting)
                     self.trials[key] = trials
         finally:
@@ -178,10 +197,13 @@ class BatchTuner:
         return self.table
 
     def install(self, engine) -> None:
-        """Apply the table to a BatchEngine (for systems it admits from now on)."""
-        for key, (variant, factor) in self.table.items():
+        """Apply the table to a BatchEngine (for systems it admits from now on); entries written
+        before blocks were tuned keep block 256."""
+        for key, setting in self.table.items():
+            variant, factor = setting[0], setting[1]
+            block = setting[2] if len(setting) > 2 else self.blocks[0]
             dim, cls = int(key[0]), int(key.split("/")[1])
-            engine.set_tuning(dim, cls, int(variant), float(factor))
+            engine.set_tuning(dim, cls, int(variant), float(factor), int(block))
 
 
 def default_path() -> str:

@@ -12,7 +12,8 @@
 
 namespace rh {
 
-constexpr uint32_t BATCH_WG = 256;  // slots per work-group in every batched kernel (and the half-list block)
+constexpr uint32_t BATCH_WG = 256;         // slots per work-group of the per-slot kernels, the slot unit
+constexpr uint32_t BATCH_MAX_BLOCK = 1024;  // largest half-list block (slots per step work-group)
 
 // Per-system device descriptor.
 struct BSys {
@@ -21,6 +22,8 @@ struct BSys {
     uint32_t ncells, cell0;
     uint32_t periodic, nspecies, pinned, soft_walls, range_check, cap;
     uint32_t variant;  // 1: half list within work-groups (default), 0: every pair at both ends
+    uint32_t block;    // slots per half-list block and step work-group: 256, 512 or 1024
+    uint32_t bkind;    // build kind: 2 (dim == 3) + (species > 1)
     uint64_t nbr0;  // list entry k of local slot i at nbr0 + k * nslots + i
     int64_t box[3], half[3];
     double boxf[3], inv_box[3];  // positions are exact f64 integers (2^-32 sigma units)
@@ -59,7 +62,14 @@ struct BBufs {
     uint32_t* cell_count;
     uint32_t* cell_start;
     uint32_t* cell_end;
-    const uint32_t* wg_sys;  // system of each BATCH_WG-slot work-group (~0u: none)
+    const uint32_t* wg_sys;  // system of each BATCH_WG-slot unit (~0u: none)
+    // Work lists.  step_list: per kind of step launch, the block indices of its systems (written
+    // by the host with the tables).  rb_list: per build kind, and for all kinds together, the
+    // units of the systems rebuilding this iteration, appended by k_bb_decide; rb_count holds
+    // their lengths (build kinds 0-3, then all) and is cleared at the end of each iteration.
+    const uint32_t* step_list;
+    uint32_t* rb_list;
+    uint32_t* rb_count;
     const BSys* sys;
     BCtl* ctl;
 };
@@ -98,18 +108,23 @@ class Batch {
     uint32_t active() const { return (uint32_t)members_.size(); }
     void budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) const;
     // Tile settings for systems added from now on, per dimension and size class (floor(log2 n)):
-    // the list variant and the cell occupancy (cells at least cell_factor x (rc + skin) wide).
-    void set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor);
+    // the list variant, the cell occupancy (cells at least cell_factor x (rc + skin) wide) and
+    // the half-list block (slots per step work-group; a system's slots start at a multiple of
+    // it and fill whole blocks).
+    void set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor, uint32_t block);
 
    private:
     struct Tuning {
         uint32_t variant = 1;
         double cell_factor = 1.0;
+        uint32_t block = BATCH_WG;
     };
     Tuning tuning_[2][32];
     void plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const;
     void upload_tables();
     void enqueue_iteration();
+    template <uint32_t BS>
+    void launch_steps(uint32_t b);
     void handle_overflow();
     void invalidate_graph();
 
@@ -124,7 +139,12 @@ class Batch {
     std::vector<BSys> h_sys_;
     std::vector<int> free_ids_;
     uint32_t grid_wgs_ = 0;  // work-groups covering the slot high-water mark
-    uint32_t kinds_ = 0;     // bit 2 (dim == 3) + (species > 1): kinds of system present
+    uint32_t* d_step_list_ = nullptr;
+    uint32_t* d_rb_list_ = nullptr;
+    uint32_t* d_rb_count_ = nullptr;
+    uint32_t units_cap_ = 0;        // BATCH_WG-slot units in the slot arena
+    uint32_t step_off_[18] = {}, step_cnt_[18] = {};  // step launch kind 6 b + kind: its list
+    uint32_t rb_off_[5] = {}, rb_cap_[5] = {};        // rebuild lists: offset, room (units)
     bool tables_dirty_ = true;
     uint32_t graph_chunk_ = 100;
     hipGraphExec_t graph_ = nullptr;

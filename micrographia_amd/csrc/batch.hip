@@ -39,13 +39,28 @@ __device__ __forceinline__ uint32_t lin_cell(const BSys& S, const uint32_t c[D])
     else return c[1] * S.nc[0] + c[0];
 }
 
-__global__ void k_bb_decide(const BSys* S, BCtl* C, uint32_t nsys) {
+// Offsets in rb_list of the rebuild lists: one per build kind, then all kinds together.
+struct RbOff {
+    uint32_t off[5];
+};
+
+// Per system: rebuild now?  A rebuilding system appends its units to its build kind's list and
+// to the list of all kinds (the per-slot rebuild kernels walk those, not the whole arena).
+__global__ void k_bb_decide(const BSys* S, BCtl* C, uint32_t nsys, uint32_t* list, uint32_t* count, RbOff ro) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     BCtl& c = C[s];
     uint32_t go = (S[s].active && c.stale && c.remaining && !c.overflow) ? 1u : 0u;
     c.do_rebuild = go;
-    if (go) c.maxcount = 0u;
+    if (!go) return;
+    c.maxcount = 0u;
+    const BSys& d = S[s];
+    const uint32_t u0 = d.slot0 / BATCH_WG, nu = d.nslots / BATCH_WG;
+    const uint32_t p = atomicAdd(&count[d.bkind], nu), q = atomicAdd(&count[4], nu);
+    for (uint32_t u = 0; u < nu; u++) {
+        list[ro.off[d.bkind] + p + u] = u0 + u;
+        list[ro.off[4] + q + u] = u0 + u;
+    }
 }
 
 template <int D>
@@ -59,16 +74,20 @@ __device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C,
     B.rank[slot] = atomicAdd(&B.cell_count[S.cell0 + key], 1u);
 }
 
-__global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B) {
-    uint32_t s = B.wg_sys[blockIdx.x];
-    if (s == ~0u) return;
-    const BSys& S = B.sys[s];
-    const BCtl& C = B.ctl[s];
-    if (!C.do_rebuild) return;
-    uint32_t slot = blockIdx.x * BATCH_WG + threadIdx.x;
-    if (slot - S.slot0 >= S.n) return;
-    if (S.dim == 3) bb_count<3>(B, S, C, slot);
-    else bb_count<2>(B, S, C, slot);
+// The per-slot rebuild kernels loop over the units of the list they are given (work-group b
+// takes its entries b, b + grid, ...), whatever its length this iteration.
+__global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B, uint32_t off) {
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[4];
+    for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
+        const uint32_t u = B.rb_list[off + b];
+        const uint32_t s = B.wg_sys[u];
+        const BSys& S = B.sys[s];
+        const BCtl& C = B.ctl[s];
+        const uint32_t slot = u * BATCH_WG + threadIdx.x;
+        if (slot - S.slot0 >= S.n) continue;
+        if (S.dim == 3) bb_count<3>(B, S, C, slot);
+        else bb_count<2>(B, S, C, slot);
+    }
 }
 
 __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
@@ -150,21 +169,24 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
     B.ids[pm ^ 1][ns] = B.ids[pm][slot];
 }
 
-__global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B) {
-    uint32_t s = B.wg_sys[blockIdx.x];
-    if (s == ~0u) return;
-    const BSys& S = B.sys[s];
-    const BCtl& C = B.ctl[s];
-    if (!C.do_rebuild) return;
-    uint32_t slot = blockIdx.x * BATCH_WG + threadIdx.x;
-    if (slot - S.slot0 >= S.n) return;
-    if (S.dim == 3) bb_scatter<3>(B, S, C, slot);
-    else bb_scatter<2>(B, S, C, slot);
+__global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B, uint32_t off) {
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[4];
+    for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
+        const uint32_t u = B.rb_list[off + b];
+        const uint32_t s = B.wg_sys[u];
+        const BSys& S = B.sys[s];
+        const BCtl& C = B.ctl[s];
+        const uint32_t slot = u * BATCH_WG + threadIdx.x;
+        if (slot - S.slot0 >= S.n) continue;
+        if (S.dim == 3) bb_scatter<3>(B, S, C, slot);
+        else bb_scatter<2>(B, S, C, slot);
+    }
 }
 
 // The list of local slot i (in the freshly scattered buffers): f32 cell-relative separations
-// against each species pair's (rc + skin)(1 + 1e-4); an in-block pair (same 256-slot block)
-// belongs to its lower-index end, pairs across blocks are listed at both ends; variant 0 lists
+// against each species pair's (rc + skin)(1 + 1e-4); an in-block pair (same block of the
+// system's block size) belongs to its lower-index end, pairs across blocks are listed at both
+// ends; variant 0 lists
 // every pair at both ends.  As in the single-system engine: candidates four at a time in groups
 // of 32 through a distance mask, the candidate and hit loops in step across the wave, and full
 // four-entry chunks (tagged with the owner's index in its block, entries - 1 and in-block, in
@@ -186,7 +208,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     uint32_t c[D];
     cell_of<D>(S, xi, c);
     const uint32_t cap4 = S.cap / 4u, capw = 64u * cap4;
-    const uint32_t own = local & (BATCH_WG - 1u);
+    const uint32_t own = local & (S.block - 1u);
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + (size_t)(local >> 6) * capw;
     const uint64_t lt = lanemask_lt();
     uint32_t fill = 0u;  // changes only where every lane of the wave is present
@@ -208,7 +230,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
 #pragma unroll
     for (int a = 0; a < D; a++) wl[a] = (S.nc[a] == 1u && ((S.periodic >> a) & 1u)) ? S.cell_w[a] : 0.f;
     const bool any_wrap = (wl[0] != 0.f) | (wl[1] != 0.f) | (wl[2] != 0.f);
-    const uint32_t blo = local & ~(BATCH_WG - 1u);
+    const uint32_t blo = local & ~(S.block - 1u);
     uint32_t na = 0, nb = 0;
     for (int o = 0; o < stencil_size(D); o++) {
         const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
@@ -268,7 +290,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
                 // j = base + u: gt: j > local; inb: j in local's block (variant 1)
                 const uint32_t gt = ~bits_below((int)local + 1 - (int)base);
                 if (variant == 1) {
-                    inb = bits_below((int)(blo + BATCH_WG) - (int)base) & ~bits_below((int)blo - (int)base);
+                    inb = bits_below((int)(blo + S.block) - (int)base) & ~bits_below((int)blo - (int)base);
                     mask &= (inb & gt) | ~inb;
                 }
                 const int t0 = (int)local - (int)base;
@@ -305,23 +327,25 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     if (__lane_id() == 0u) B.wave_chunks[slot >> 6] = min(fill, capw);
 }
 
-// One launch per kind of system present, as for the step (each keeps only its own registers).
+// One launch per build kind present (each keeps only its own registers), over that kind's
+// rebuild list.
 template <int D, bool MULTI>
-__global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B) {
-    const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
-    uint32_t s = B.wg_sys[wg];
-    if (s == ~0u) return;
-    const BSys& S = B.sys[s];
-    if (S.dim != (uint32_t)D || (S.nspecies > 1) != MULTI) return;
-    BCtl& C = B.ctl[s];
-    if (!C.do_rebuild) return;
-    uint32_t local = wg * BATCH_WG + threadIdx.x - S.slot0;
-    if (local >= S.n) {
-        // a wave past the system's last particle holds no chunks
-        if ((threadIdx.x & 63u) == 0u) B.wave_chunks[(wg * BATCH_WG + threadIdx.x) >> 6] = 0u;
-        return;
+__global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t off) {
+    constexpr uint32_t K = (D == 3 ? 2u : 0u) + (MULTI ? 1u : 0u);
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[K];
+    for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
+        const uint32_t u = B.rb_list[off + b];
+        const uint32_t s = B.wg_sys[u];
+        const BSys& S = B.sys[s];
+        BCtl& C = B.ctl[s];
+        const uint32_t local = u * BATCH_WG + threadIdx.x - S.slot0;
+        if (local >= S.n) {
+            // a wave past the system's last particle holds no chunks
+            if ((threadIdx.x & 63u) == 0u) B.wave_chunks[(u * BATCH_WG + threadIdx.x) >> 6] = 0u;
+            continue;
+        }
+        bb_build<D, MULTI>(B, S, C, local);
     }
-    bb_build<D, MULTI>(B, S, C, local);
 }
 
 __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
@@ -340,19 +364,20 @@ __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
 
 __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S.active && (C.stale | C.overflow) == 0u && C.remaining != 0u; }
 
-// One 256-slot work-group: as the single-system step kernel, the block's chunks (its four
-// waves' regions, numbered in order) are dealt out evenly over the lanes, every kick goes to the
-// block's LDS accumulators, and positions are staged and gathered as sixteenths.
-template <int D, bool MULTI, bool STOCH>
+// One work-group of BS slots, a block of the system: as the single-system step kernel, the
+// block's chunks (its waves' regions, numbered in order) are dealt out evenly over the lanes,
+// every kick goes to the block's LDS accumulators, and positions are staged and gathered as
+// sixteenths.
+template <int D, bool MULTI, bool STOCH, uint32_t BS>
 __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t wg) {
-    constexpr uint32_t NW = BATCH_WG / 64u;
-    __shared__ uint32_t acc[D][BATCH_WG];
-    __shared__ double lpos[D][BATCH_WG];  // the block's positions in sixteenths, for in-block neighbours
+    constexpr uint32_t NW = BS / 64u;
+    __shared__ uint32_t acc[D][BS];
+    __shared__ double lpos[D][BS];  // the block's positions in sixteenths, for in-block neighbours
     __shared__ uint32_t wfirst[NW + 1];
     const uint32_t t = threadIdx.x;
-    const uint32_t slot = wg * BATCH_WG + t;
+    const uint32_t slot = wg * BS + t;
     const uint32_t local = slot - S.slot0;
-    const uint32_t base = wg * BATCH_WG - S.slot0;  // first local slot of this block
+    const uint32_t base = wg * BS - S.slot0;  // first local slot of this block
     const bool live = local < S.n;
     const uint32_t pp = C.ppos, pm = C.pmeta;
     const double* px[D];
@@ -363,7 +388,7 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         lpos[a][t] = live ? px[a][local] * 0.0625 : 0.0;
     }
     if (t < 64u) {
-        const uint32_t v = t < NW ? B.wave_chunks[(wg * BATCH_WG >> 6) + t] : 0u;
+        const uint32_t v = t < NW ? B.wave_chunks[(wg * BS >> 6) + t] : 0u;
         uint32_t x = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -390,9 +415,9 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         };
         uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
         if (g < nch) q_next = chunks[at(g)];
-        for (; g < nch; g += BATCH_WG) {
+        for (; g < nch; g += BS) {
             const uint4 qt = q_next;
-            if (g + BATCH_WG < nch) q_next = chunks[at(g + BATCH_WG)];
+            if (g + BS < nch) q_next = chunks[at(g + BS)];
             uint32_t own, m;
             bool inblock;
             const uint4 q = untag_chunk(qt, own, m, inblock);
@@ -526,22 +551,24 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
 }
 
 // One launch per kind of system present (dimension, one or more species, stochastic boundaries
-// or none), each compiled with only its own path so the common kernels keep their registers
-// low; work-groups of other kinds exit at once.
-template <int D, bool MULTI, bool STOCH>
-__global__ __launch_bounds__(BATCH_WG) void k_bb_step(BBufs B) {
-    const uint32_t wg = xcd_block(blockIdx.x, gridDim.x);
-    uint32_t s = B.wg_sys[wg];
-    if (s == ~0u) return;
+// or none) and block size, each compiled with only its own path so the common kernels keep
+// their registers low; work-groups of other kinds exit at once.
+// Work-group k takes block step_list[off + k] of the launch's kind: slots [g BS, (g + 1) BS) of
+// a system of block size BS (its slots start at a multiple of its block size and fill whole
+// blocks).  Consecutive blocks of the list go to the same XCD.
+template <int D, bool MULTI, bool STOCH, uint32_t BS>
+__global__ __launch_bounds__(BS) void k_bb_step(BBufs B, uint32_t off) {
+    const uint32_t wg = B.step_list[off + xcd_block(blockIdx.x, gridDim.x)];
+    const uint32_t s = B.wg_sys[wg * (BS / BATCH_WG)];
     const BSys& S = B.sys[s];
-    if (S.dim != (uint32_t)D || (S.nspecies > 1) != MULTI || (S.stoch != 0u) != STOCH) return;
     BCtl& C = B.ctl[s];
     if (!bb_runs(S, C)) return;
-    bb_step<D, MULTI, STOCH>(B, S, C, wg);
+    bb_step<D, MULTI, STOCH, BS>(B, S, C, wg);
 }
 
-__global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys) {
+__global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys, uint32_t* rb_count) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < 5u) rb_count[s] = 0u;  // the next iteration's rebuild lists start empty
     if (s >= nsys) return;
     BCtl& c = C[s];
     if (bb_runs(S[s], c)) {
@@ -581,11 +608,11 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
         RH_HIP(hipMemGetInfo(&fr, &tot));
         budget = fr / 2;
     }
-    uint64_t slot_cap = (uint64_t)(budget * 0.16 / SLOT_BYTES) / BATCH_WG * BATCH_WG;
+    uint64_t slot_cap = (uint64_t)(budget * 0.16 / SLOT_BYTES) / BATCH_MAX_BLOCK * BATCH_MAX_BLOCK;
     uint64_t list_cap = (uint64_t)(budget * 0.80 / 4) / LIST_UNIT * LIST_UNIT;
     uint64_t cell_cap = (uint64_t)(budget * 0.04 / 12) / CELL_UNIT * CELL_UNIT;
-    slot_cap = std::min<uint64_t>(slot_cap, (uint64_t(1) << 31) - BATCH_WG);
-    RH_REQUIRE(slot_cap >= BATCH_WG && list_cap >= LIST_UNIT && cell_cap >= CELL_UNIT, "batch memory budget too small");
+    slot_cap = std::min<uint64_t>(slot_cap, (uint64_t(1) << 31) - BATCH_MAX_BLOCK);
+    RH_REQUIRE(slot_cap >= BATCH_MAX_BLOCK && list_cap >= LIST_UNIT && cell_cap >= CELL_UNIT, "batch memory budget too small");
     slots_.reset(slot_cap / BATCH_WG);
     lists_.reset(list_cap / LIST_UNIT);
     cells_.reset(cell_cap / CELL_UNIT);
@@ -618,6 +645,14 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     d_wg_ = (uint32_t*)dalloc(sizeof(uint32_t) * (slot_cap / BATCH_WG));
     RH_HIP(hipMemset(d_wg_, 0xff, sizeof(uint32_t) * (slot_cap / BATCH_WG)));
     B_.wg_sys = d_wg_;
+    units_cap_ = (uint32_t)(slot_cap / BATCH_WG);
+    d_step_list_ = (uint32_t*)dalloc(sizeof(uint32_t) * units_cap_);
+    d_rb_list_ = (uint32_t*)dalloc(sizeof(uint32_t) * 2 * (size_t)units_cap_);
+    d_rb_count_ = (uint32_t*)dalloc(sizeof(uint32_t) * 5);
+    RH_HIP(hipMemset(d_rb_count_, 0, sizeof(uint32_t) * 5));
+    B_.step_list = d_step_list_;
+    B_.rb_list = d_rb_list_;
+    B_.rb_count = d_rb_count_;
     B_.sys = d_sys_;
     B_.ctl = d_ctl_;
     h_sys_.assign(max_sys_, BSys{});
@@ -678,6 +713,8 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
     while (size_class < 31 && (uint64_t(2) << size_class) <= n) size_class++;
     const Tuning& tune = tuning_[D - 2][size_class];
     d.variant = tune.variant;
+    d.block = tune.block;
+    d.bkind = (D == 3 ? 2u : 0u) + (sys.nspecies > 1 ? 1u : 0u);
     uint64_t ncells = 1;
     for (int a = 0; a < D; a++) {
         d.nc[a] = core::sat_u32(std::floor(b[a] / ((rcm + skin) * tune.cell_factor + margin)));
@@ -750,12 +787,13 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
     *cap = ((uint32_t)std::min(c + 8.0, 65532.0) + 3u) & ~3u;  // whole four-entry chunks
 }
 
-void Batch::set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor) {
+void Batch::set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor, uint32_t block) {
     RH_REQUIRE(dim == 2 || dim == 3, "dimension must be 2 or 3");
     RH_REQUIRE(size_class < 32, "size class is floor(log2 n), below 32");
     RH_REQUIRE(variant <= 1, "list variant must be 0 (full) or 1 (half)");
     RH_REQUIRE(cell_factor >= 1.0 && cell_factor <= 4.0, "cell factor must be in [1, 4]");
-    tuning_[dim - 2][size_class] = Tuning{variant, cell_factor};
+    RH_REQUIRE(block == 256 || block == 512 || block == 1024, "block must be 256, 512 or 1024");
+    tuning_[dim - 2][size_class] = Tuning{variant, cell_factor, block};
 }
 
 uint64_t Batch::bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double skin) const {
@@ -764,7 +802,7 @@ uint64_t Batch::bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double 
     SystemDesc s2 = sys;
     s2.dim = dim;
     plan(s2, n, skin, d, &cap);
-    uint64_t nslots = (n + BATCH_WG - 1) / BATCH_WG * BATCH_WG;
+    uint64_t nslots = (n + d.block - 1) / d.block * d.block;
     return nslots * SLOT_BYTES + (uint64_t)cap * nslots * 4 + (uint64_t)d.ncells * 12;
 }
 
@@ -779,11 +817,12 @@ int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int3
     BSys d;
     uint32_t cap;
     plan(sys, n, skin, d, &cap);
-    uint64_t nslots = (n + BATCH_WG - 1) / BATCH_WG * BATCH_WG;
+    // whole blocks, from a multiple of the block size (the step's work-groups are its blocks)
+    uint64_t nslots = (n + d.block - 1) / d.block * d.block;
     uint64_t s0, l0, c0;
     uint64_t lunits = ((uint64_t)cap * nslots + LIST_UNIT - 1) / LIST_UNIT;
     uint64_t cunits = (d.ncells + CELL_UNIT - 1) / CELL_UNIT;
-    if (!slots_.alloc(nslots / BATCH_WG, &s0)) return -1;
+    if (!slots_.alloc(nslots / BATCH_WG, &s0, d.block / BATCH_WG)) return -1;
     if (!lists_.alloc(lunits, &l0)) {
         slots_.free(s0, nslots / BATCH_WG);
         return -1;
@@ -874,22 +913,49 @@ void Batch::upload_tables() {
     RH_HIP(hipStreamSynchronize(stream_));
     RH_HIP(hipMemcpy(d_sys_, h_sys_.data(), sizeof(BSys) * max_sys_, hipMemcpyHostToDevice));
     uint64_t hw = slots_.high_water();  // units of BATCH_WG
-    std::vector<uint32_t> wg(std::max<uint64_t>(hw, 1), ~0u);
+    // padded to whole blocks of the largest size: a step work-group reads its block's first unit
+    constexpr uint32_t U = BATCH_MAX_BLOCK / BATCH_WG;
+    std::vector<uint32_t> wg((std::max<uint64_t>(hw, 1) + U - 1) / U * U, ~0u);
     for (auto& kv : members_) {
         const BatchMember& m = kv.second;
         for (uint32_t w = m.slot0 / BATCH_WG; w < (m.slot0 + m.nslots) / BATCH_WG; w++) wg[w] = (uint32_t)kv.first;
     }
     RH_HIP(hipMemcpy(d_wg_, wg.data(), wg.size() * 4, hipMemcpyHostToDevice));
     uint32_t new_wgs = (uint32_t)std::max<uint64_t>(hw, 1);
-    uint32_t kinds = 0;
-    // bits 0-3: (3D, several species) without stochastic boundaries; bits 4-5: 3D with them
+    // Work lists: the rebuild lists' room per build kind (their systems' units), and per kind of
+    // step launch (kind j = 4 (stochastic) or 2 (3D), + (species > 1); block 256 << b) the
+    // blocks of its systems.
+    uint32_t rb_cap[5] = {0, 0, 0, 0, 0};
+    std::vector<uint32_t> lists[18];
     for (auto& kv : members_) {
-        const SystemDesc& sy = kv.second.sys;
-        kinds |= 1u << ((stochastic(sy) ? 4u : sy.dim == 3 ? 2u : 0u) + (sy.nspecies > 1 ? 1u : 0u));
+        const BSys& d = h_sys_[kv.first];
+        rb_cap[d.bkind] += d.nslots / BATCH_WG;
+        rb_cap[4] += d.nslots / BATCH_WG;
+        const uint32_t j = (d.stoch ? 4u : d.dim == 3 ? 2u : 0u) + (d.nspecies > 1 ? 1u : 0u);
+        const uint32_t bi = d.block == 1024u ? 2u : d.block == 512u ? 1u : 0u;
+        for (uint32_t g = d.slot0 / d.block; g < (d.slot0 + d.nslots) / d.block; g++) lists[6 * bi + j].push_back(g);
     }
-    if (new_wgs != grid_wgs_ || kinds != kinds_) invalidate_graph();
+    std::vector<uint32_t> flat;
+    bool changed = new_wgs != grid_wgs_;
+    for (int k = 0; k < 18; k++) {
+        changed |= step_cnt_[k] != (uint32_t)lists[k].size() || step_off_[k] != (uint32_t)flat.size();
+        step_off_[k] = (uint32_t)flat.size();
+        step_cnt_[k] = (uint32_t)lists[k].size();
+        flat.insert(flat.end(), lists[k].begin(), lists[k].end());
+    }
+    if (!flat.empty()) RH_HIP(hipMemcpy(d_step_list_, flat.data(), flat.size() * 4, hipMemcpyHostToDevice));
+    uint32_t off = 0;
+    for (int k = 0; k < 4; k++) {
+        changed |= rb_cap_[k] != rb_cap[k] || rb_off_[k] != off;
+        rb_off_[k] = off;
+        rb_cap_[k] = rb_cap[k];
+        off += rb_cap[k];
+    }
+    changed |= rb_cap_[4] != rb_cap[4];
+    rb_off_[4] = units_cap_;  // the list of all kinds takes the second half
+    rb_cap_[4] = rb_cap[4];
+    if (changed) invalidate_graph();
     grid_wgs_ = new_wgs;
-    kinds_ = kinds;
     tables_dirty_ = false;
 }
 
@@ -900,24 +966,39 @@ void Batch::invalidate_graph() {
     }
 }
 
+template <uint32_t BS>
+void Batch::launch_steps(uint32_t b) {
+    const uint32_t* cnt = step_cnt_ + 6 * b;
+    const uint32_t* off = step_off_ + 6 * b;
+    if (cnt[0]) k_bb_step<2, false, false, BS><<<cnt[0], BS, 0, stream_>>>(B_, off[0]);
+    if (cnt[1]) k_bb_step<2, true, false, BS><<<cnt[1], BS, 0, stream_>>>(B_, off[1]);
+    if (cnt[2]) k_bb_step<3, false, false, BS><<<cnt[2], BS, 0, stream_>>>(B_, off[2]);
+    if (cnt[3]) k_bb_step<3, true, false, BS><<<cnt[3], BS, 0, stream_>>>(B_, off[3]);
+    if (cnt[4]) k_bb_step<3, false, true, BS><<<cnt[4], BS, 0, stream_>>>(B_, off[4]);
+    if (cnt[5]) k_bb_step<3, true, true, BS><<<cnt[5], BS, 0, stream_>>>(B_, off[5]);
+}
+
+// Work-groups of a per-slot rebuild kernel: enough to fill the chip, at most one per unit.
+static uint32_t rebuild_grid(uint32_t units) { return std::max<uint32_t>(1u, std::min<uint32_t>(units, 2048u)); }
+
 void Batch::enqueue_iteration() {
     const uint32_t sb = blocks_for(max_sys_, 256);
-    k_bb_decide<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
-    k_bb_count<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    RbOff ro;
+    for (int k = 0; k < 5; k++) ro.off[k] = rb_off_[k];
+    k_bb_decide<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_list_, d_rb_count_, ro);
+    const uint32_t ga = rebuild_grid(rb_cap_[4]);
+    k_bb_count<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
-    k_bb_scatter<<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 1u) k_bb_build<2, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 2u) k_bb_build<2, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & (4u | 16u)) k_bb_build<3, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & (8u | 32u)) k_bb_build<3, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
+    k_bb_scatter<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
+    if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, stream_>>>(B_, rb_off_[0]);
+    if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, stream_>>>(B_, rb_off_[1]);
+    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, stream_>>>(B_, rb_off_[2]);
+    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, stream_>>>(B_, rb_off_[3]);
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
-    if (kinds_ & 1u) k_bb_step<2, false, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 2u) k_bb_step<2, true, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 4u) k_bb_step<3, false, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 8u) k_bb_step<3, true, false><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 16u) k_bb_step<3, false, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    if (kinds_ & 32u) k_bb_step<3, true, true><<<grid_wgs_, BATCH_WG, 0, stream_>>>(B_);
-    k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
+    launch_steps<256>(0);
+    launch_steps<512>(1);
+    launch_steps<1024>(2);
+    k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_count_);
 }
 
 void Batch::run(uint64_t iterations) {

@@ -520,8 +520,8 @@ int rh_batch_list_stats(rh_batch* b, int32_t id, rh_list_stats* out) {
 int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk) {
     return guarded([&] { eng(b)->set_graph_chunk(chunk); });
 }
-int rh_batch_set_tuning(rh_batch* b, uint32_t dim, uint32_t size_class, uint32_t variant, double cell_factor) {
-    return guarded([&] { eng(b)->set_tuning((int)dim, size_class, variant, cell_factor); });
+int rh_batch_set_tuning(rh_batch* b, uint32_t dim, uint32_t size_class, uint32_t variant, double cell_factor, uint32_t block) {
+    return guarded([&] { eng(b)->set_tuning((int)dim, size_class, variant, cell_factor, block); });
 }
 int rh_batch_stream(rh_batch* b, void** stream) {
     return guarded([&] { *stream = (void*)eng(b)->stream(); });

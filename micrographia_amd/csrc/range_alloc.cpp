@@ -13,18 +13,21 @@ void RangeAlloc::reset(uint64_t capacity) {
     if (capacity) free_[0] = capacity;
 }
 
-bool RangeAlloc::alloc(uint64_t size, uint64_t* start) {
+bool RangeAlloc::alloc(uint64_t size, uint64_t* start, uint64_t align) {
     if (size == 0) size = 1;
+    if (align == 0) align = 1;
     for (auto it = free_.begin(); it != free_.end(); ++it) {
-        if (it->second >= size) {
-            uint64_t s = it->first, rest = it->second - size;
-            free_.erase(it);
-            if (rest) free_[s + size] = rest;
-            taken_[s] = size;
-            used_ += size;
-            *start = s;
-            return true;
-        }
+        const uint64_t s = it->first, len = it->second;
+        const uint64_t a = (s + align - 1) / align * align;  // the range's first aligned unit
+        if (a - s > len || len - (a - s) < size) continue;
+        const uint64_t rest = len - (a - s) - size;
+        free_.erase(it);
+        if (a > s) free_[s] = a - s;
+        if (rest) free_[a + size] = rest;
+        taken_[a] = size;
+        used_ += size;
+        *start = a;
+        return true;
     }
     return false;
 }

@@ -1,6 +1,7 @@
 // First-fit range allocator of the batched engine's arenas (slots, list entries, cells): ranges
-// of [0, capacity) in units, the lowest free range that fits is taken, and freed ranges coalesce
-// with their free neighbours.  Plain C++ (no device code), so host tests can drive it directly.
+// of [0, capacity) in units, the lowest free range that fits is taken (starting at a multiple of
+// `align` when one is asked for), and freed ranges coalesce with their free neighbours.  Plain
+// C++ (no device code), so host tests can drive it directly.
 #pragma once
 
 #include <cstddef>
@@ -13,7 +14,7 @@ class RangeAlloc {
    public:
     explicit RangeAlloc(uint64_t capacity = 0) { reset(capacity); }
     void reset(uint64_t capacity);
-    bool alloc(uint64_t size, uint64_t* start);
+    bool alloc(uint64_t size, uint64_t* start, uint64_t align = 1);
     void free(uint64_t start, uint64_t size);
     uint64_t capacity() const { return cap_; }
     uint64_t used() const { return used_; }

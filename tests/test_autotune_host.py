@@ -64,6 +64,6 @@ def test_batch_tuner_installs_its_table():
             calls.append(a)
 
     t = autotune.BatchTuner()
-    t.table = {"3d/12": [0, 1.25], "2d/9": [1, 1.5]}
+    t.table = {"3d/12": [0, 1.25], "2d/9": [1, 1.5, 1024]}
     t.install(E())
-    assert sorted(calls) == [(2, 9, 1, 1.5), (3, 12, 0, 1.25)]
+    assert sorted(calls) == [(2, 9, 1, 1.5, 1024), (3, 12, 0, 1.25, 256)]

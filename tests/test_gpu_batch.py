@@ -36,10 +36,10 @@ def mixed_systems():
     return out
 
 
-@pytest.mark.parametrize("graph,tiles", [(0, None), (50, None), (50, (0, 1.5)), (0, (1, 1.25))])
+@pytest.mark.parametrize("graph,tiles", [(0, None), (50, None), (50, (0, 1.5)), (0, (1, 1.25)), (50, (1, 1.0, 512)), (0, (1, 1.25, 1024))])
 def test_each_system_matches_the_oracle(graph, tiles):
-    """Every system's trajectory equals the oracle's, whatever the tile settings (list variant and
-    cell occupancy, as the autotuner installs them per size bucket)."""
+    """Every system's trajectory equals the oracle's, whatever the tile settings (list variant,
+    cell occupancy and half-list block, as the autotuner installs them per size bucket)."""
     systems = mixed_systems()
     b = BatchEngine(budget_bytes=2 * GB, max_systems=64, graph_chunk=graph)
     if tiles is not None:
@@ -76,14 +76,17 @@ def stochastic_systems():
     return [bath, walls, mix]
 
 
-@pytest.mark.parametrize("graph", [0, 50])
-def test_bath_and_heat_bath_walls_match_the_oracle(graph):
+@pytest.mark.parametrize("graph,block", [(0, 256), (50, 256), (50, 1024)])
+def test_bath_and_heat_bath_walls_match_the_oracle(graph, block):
     """The Andersen bath and heat-bath walls draw the same Threefry numbers as md-core
     (three.rs:232-265), keyed by particle id and the system's own clock, so each system's
     trajectory equals the oracle's bit for bit beside systems without them."""
     stoch = stochastic_systems()
     plain = [make_system(3, 1500, 0.8442, 1.2, 24), make_system(2, 3000, 0.7, 1.0, 25)]
     b = BatchEngine(budget_bytes=2 * GB, max_systems=16, graph_chunk=graph)
+    for dim in (2, 3):
+        for cls in range(8, 16):
+            b.set_tuning(dim, cls, 1, 1.0, block)
     steps = [150, 300, 200, 150, 150]
     systems = stoch + plain
     ids = [b.add(sys, st, k, skin=0.3) for (sys, st), k in zip(systems, steps)]

@@ -1,6 +1,7 @@
 """The batched engine's first-fit range allocator (micrographia_amd/csrc/range_alloc.cpp) on the
 CPU, under the churn a serving loop puts it through: thousands of admit / retire cycles of
-ranges whose sizes spread over three decades, at about 60 % occupancy.  The accounting stays
+ranges whose sizes spread over three decades and whose starts must be multiples of 1, 2 or 4
+units, at about 60 % occupancy.  The accounting stays
 exact, freed ranges coalesce back into one, and fragmentation stays bounded: allocations fail
 while a quarter of the arena is free in well under 0.1 % of attempts, and the largest free
 range never falls below 5 % of the free space."""
@@ -33,12 +34,14 @@ int main(int argc, char** argv) {
         for (int k = 0; k < 3; k++) {  // sizes log-uniform in [100, cap / 16]
             uint64_t size = (uint64_t)std::exp(std::log(100.0) + (std::log(cap / 16.0) - std::log(100.0)) * u(rng));
             uint64_t s = 0;
+            const uint64_t align = uint64_t(1) << (int)(u(rng) * 3);  // 1, 2 or 4, as block sizes ask
             attempts++;
             const uint64_t freeb = cap - a.used();
-            if (a.alloc(size, &s)) {
+            if (a.alloc(size, &s, align)) {
                 for (const Item& it : live)
                     if (s < it.start + it.size && it.start < s + size) { std::printf("OVERLAP\n"); return 1; }
                 if (s + size > cap) { std::printf("OUTSIDE\n"); return 1; }
+                if (s % align) { std::printf("ALIGN\n"); return 1; }
                 live.push_back({s, size, 1 + (int)(u(rng) * 40)});
             } else {
                 fails++;
