@@ -129,9 +129,10 @@ struct Bufs {
     // the reorder and the half-list step: a neighbour gather reads one 32-byte record (one cache
     // line) instead of one 8-byte word from each axis array.
     double* aos[2];
-    // Half list: a stream of tagged four-entry chunks in B.nbr, each 64-particle wave of the build
-    // filling a region of 64 cap/4 slots from its start, and per wave the number of chunks in
-    // its region.
+    // Half list: a stream of 16-byte chunks in B.nbr, each 64-particle wave of the build filling
+    // a region of 64 cap/4 slots, in-block chunks (up to eight offsets, pack_inblock) from its
+    // start and cross-block chunks (up to four slots, tag_chunk) from its end down; per wave the
+    // number of each (wave_chunks[2 w], wave_chunks[2 w + 1]).
     uint32_t* wave_chunks;
     int32_t* vel[2][D];
     uint32_t* species[2];

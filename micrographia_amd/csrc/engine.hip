@@ -436,19 +436,26 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
     const uint64_t lt = lanemask_lt();
     // A lane fills one in-block and one cross-block chunk in registers; full chunks are stored
-    // whole (16 B) at the wave's fill point.  `fill` changes only where every lane is present.
-    uint32_t fill = 0u;
-    auto store = [&](bool want, const uint4& q, uint32_t entries, bool in) {
+    // whole (16 B) in the wave's region, in-block chunks from its start and cross-block ones from
+    // its end down.  The fills change only where every lane is present.
+    uint32_t fill_in = 0u, fill_out = 0u;
+    auto store = [&](bool want, const uint4& q, bool in) {
         const uint64_t bal = ballot64(want);
         if (bal == 0ull) return;
         if (want) {
-            const uint32_t slot = fill + (uint32_t)__popcll(bal & lt);
-            if (slot < capw) chunks[slot] = tag_chunk(q, own, entries, in);
+            const uint32_t k = (in ? fill_in : fill_out) + (uint32_t)__popcll(bal & lt), other = in ? fill_out : fill_in;
+            if (k + other < capw) chunks[in ? k : capw - 1u - k] = q;
             else atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         }
-        fill += (uint32_t)__popcll(bal);
+        if (in) fill_in += (uint32_t)__popcll(bal);
+        else fill_out += (uint32_t)__popcll(bal);
     };
-    uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
+    // the open in-block chunk (eight offsets from the block's first slot) and cross-block chunk
+    uint4 fa0 = make_uint4(0u, 0u, 0u, 0u), fa1 = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
+    auto inblock_chunk = [&](uint32_t entries) {
+        const uint32_t e[8] = {fa0.x, fa0.y, fa0.z, fa0.w, fa1.x, fa1.y, fa1.z, fa1.w};
+        return pack_inblock(e, own, entries);
+    };
     // single-cell periodic axes: the nearest image of the one cell (cell_w is the box length)
     float wl[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -467,9 +474,10 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
             const uint32_t j = b0 + b;
             in = j - blo < P.block;
             if (in) {
-                put4(fa, na, j);
+                if ((na & 7u) < 4u) put4(fa0, na, j - blo);
+                else put4(fa1, na, j - blo);
                 na++;
-                full = (na & 3u) == 0u;
+                full = (na & 7u) == 0u;
             } else {
                 put4(fb, nb, j);
                 nb++;
@@ -480,7 +488,9 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
                 qn--;
             }
         }
-        store(full, in ? fa : fb, 4u, in);
+        const bool fin = full && in, fout = full && !in;
+        store(fin, fin ? inblock_chunk(8u) : make_uint4(0u, 0u, 0u, 0u), true);
+        store(fout, tag_chunk(fb, own, 4u, false), false);
     };
     for (int o = 0; o < stencil_size(D); o++) {
         const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
@@ -559,12 +569,17 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
     }
     while (ballot64(qn != 0u) != 0ull) visit();
     // the partial last chunks; a list longer than the capacity is flagged and regrown
-    store((na & 3u) != 0u, fa, na & 3u, true);
-    store((nb & 3u) != 0u, fb, nb & 3u, false);
-    const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
+    const bool pin = (na & 7u) != 0u, pout = (nb & 3u) != 0u;
+    store(pin, pin ? inblock_chunk(na & 7u) : make_uint4(0u, 0u, 0u, 0u), true);
+    store(pout, tag_chunk(fb, own, pout ? nb & 3u : 1u, false), false);
+    const uint32_t ca = (na + 7u) / 8u, cb = (nb + 3u) / 4u;
     if (ca + cb > cap4) atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
     if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
-    if (__lane_id() == 0u) B.wave_chunks[wave] = min(fill, capw);
+    if (__lane_id() == 0u) {
+        const uint32_t in = min(fill_in, capw);
+        B.wave_chunks[2u * wave] = in;
+        B.wave_chunks[2u * wave + 1u] = min(fill_out, capw - in);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -691,31 +706,39 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
     const uint32_t wave = __builtin_amdgcn_readfirstlane(i >> 6), own = i & (P.block - 1u);
     const uint32_t capw = 64u * cap4;
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr) + (size_t)wave * capw;
-    // A lane fills one in-block and one cross-block chunk in registers; full chunks are stored
-    // whole (16 B) at the wave's fill point.  `fill` changes only where every lane is present.
-    uint32_t fill = 0u;
-    auto store = [&](bool want, const uint4& q, uint32_t entries, bool in) {
+    // Full chunks are stored whole (16 B) in the wave's region: in-block chunks from its start,
+    // cross-block chunks from its end down.  The fills change only where every lane is present.
+    uint32_t fill_in = 0u, fill_out = 0u;
+    auto store = [&](bool want, const uint4& q, bool in) {
         const uint64_t bal = ballot64(want);
         if (bal == 0ull) return;
         if (want) {
             const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-            const uint32_t slot = fill + below;
-            if (slot < capw) chunks[slot] = tag_chunk(q, own, entries, in);
+            const uint32_t k = (in ? fill_in : fill_out) + below, other = in ? fill_out : fill_in;
+            if (k + other < capw) chunks[in ? k : capw - 1u - k] = q;
             else atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
         }
-        fill += (uint32_t)__popcll(bal);
+        if (in) fill_in += (uint32_t)__popcll(bal);
+        else fill_out += (uint32_t)__popcll(bal);
     };
-    // Per lane, in LDS: the open in-block (kind 1) and cross-block (kind 0) chunks, entry s of
-    // kind k at ck[4 k + s][lane], and a ring of up to four hit masks, each with the copy index
-    // of its bit 0 (bit b stands for that index minus b) and its kind in bit 31.  A visit moves
-    // as many hits of the front mask into that kind's open chunk as it has room for (up to
-    // four) and stores the chunk when it is full.  Visits run when some lane's ring has no room for a group's two masks, and at the
-    // end.  (Held in LDS, the ring and chunks leave registers to the candidate loop.)
-    __shared__ uint32_t ck[8][TPB];
+    // Per lane, in LDS: the open in-block chunk (up to eight offsets from the block's first
+    // slot, entry s at ci[s][lane]) and cross-block chunk (four slots, co[s][lane]), and a ring
+    // of up to four hit masks, each with the copy index of its bit 0 (bit b stands for that index
+    // minus b) and its kind in bit 31.  A visit moves as many hits of the front mask into that
+    // kind's open chunk as it has room for (up to four) and stores the chunk when it is full.
+    // Visits run when some lane's ring has no room for a group's two masks, and at the end.
+    // (Held in LDS, the ring and chunks leave registers to the candidate loop.)
+    __shared__ uint32_t ci[8][TPB], co[4][TPB];
     __shared__ uint32_t qm[4][TPB], qt[4][TPB];
     const uint32_t t = threadIdx.x;
     uint32_t na = 0u, nb = 0u, head = 0u, qn = 0u;
-    auto chunk_of = [&](bool in) { return in ? make_uint4(ck[4][t], ck[5][t], ck[6][t], ck[7][t]) : make_uint4(ck[0][t], ck[1][t], ck[2][t], ck[3][t]); };
+    auto inblock_chunk = [&](uint32_t entries) {
+        uint32_t e[8];
+#pragma unroll
+        for (int s2 = 0; s2 < 8; s2++) e[s2] = ci[s2][t];
+        return pack_inblock(e, own, entries);
+    };
+    auto cross_chunk = [&](uint32_t entries) { return tag_chunk(make_uint4(co[0][t], co[1][t], co[2][t], co[3][t]), own, entries, false); };
     auto visit = [&]() {
         bool full = false, in = false;
         if (qn != 0u) {
@@ -724,24 +747,25 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
             const uint32_t tag = qt[h][t];
             in = (tag >> 31) != 0u;
             const uint32_t b1 = tag & 0x7fffffffu;  // the copy index of bit 0
-            const uint32_t n = in ? na : nb;
-            const uint32_t r = n & 3u;
+            const uint32_t size = in ? 8u : 4u, n = in ? na : nb, r = n & (size - 1u);
             uint32_t jv[4];
             bool take[4];
 #pragma unroll
-            for (uint32_t s = 0; s < 4u; s++) {
-                take[s] = s >= r && m != 0u;
-                const uint32_t at = take[s] ? (uint32_t)__builtin_ctz(m) : 0u;
-                m = take[s] ? m & (m - 1u) : m;
-                jv[s] = __float_as_uint(items[b1 - at].w) & 0x0fffffffu;
+            for (uint32_t s2 = 0; s2 < 4u; s2++) {
+                take[s2] = r + s2 < size && m != 0u;
+                const uint32_t at = take[s2] ? (uint32_t)__builtin_ctz(m) : 0u;
+                m = take[s2] ? m & (m - 1u) : m;
+                jv[s2] = __float_as_uint(items[b1 - at].w) & 0x0fffffffu;
             }
-            const uint32_t row = in ? 4u : 0u;
 #pragma unroll
-            for (uint32_t s = 0; s < 4u; s++)
-                if (take[s]) ck[row + s][t] = jv[s];
+            for (uint32_t s2 = 0; s2 < 4u; s2++) {
+                if (!take[s2]) continue;
+                if (in) ci[r + s2][t] = jv[s2] - blo;
+                else co[r + s2][t] = jv[s2];
+            }
             const uint32_t taken = (uint32_t)take[0] + (uint32_t)take[1] + (uint32_t)take[2] + (uint32_t)take[3];
             const uint32_t n2 = n + taken;
-            full = (n2 & 3u) == 0u;  // at least one was taken, so the chunk closed
+            full = (n2 & (size - 1u)) == 0u;  // at least one was taken, so the chunk closed
             if (in) na = n2;
             else nb = n2;
             if (m == 0u) {
@@ -751,10 +775,10 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
                 qm[h][t] = m;
             }
         }
-        // every lane takes part in the store's ballot, so `fill` stays the same across the wave
-        uint4 q = make_uint4(0u, 0u, 0u, 0u);
-        if (full) q = chunk_of(in);
-        store(full, q, 4u, in);
+        // every lane takes part in the stores' ballots, so the fills stay the same across the wave
+        const bool fin = full && in, fout = full && !in;
+        store(fin, fin ? inblock_chunk(8u) : make_uint4(0u, 0u, 0u, 0u), true);
+        store(fout, fout ? cross_chunk(4u) : make_uint4(0u, 0u, 0u, 0u), false);
     };
     auto push = [&](uint32_t m, uint32_t tag) {
         if (m == 0u) return;
@@ -854,12 +878,17 @@ __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, cons
     }
     while (ballot64(qn != 0u) != 0ull) visit();
     // the partial last chunks; a list longer than the capacity is flagged and regrown
-    store((na & 3u) != 0u, chunk_of(true), na & 3u, true);
-    store((nb & 3u) != 0u, chunk_of(false), nb & 3u, false);
-    const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
+    const bool pin = (na & 7u) != 0u, pout = (nb & 3u) != 0u;
+    store(pin, pin ? inblock_chunk(na & 7u) : make_uint4(0u, 0u, 0u, 0u), true);
+    store(pout, pout ? cross_chunk(nb & 3u) : make_uint4(0u, 0u, 0u, 0u), false);
+    const uint32_t ca = (na + 7u) / 8u, cb = (nb + 3u) / 4u;
     if (ca + cb > cap4) atomicOr(&B.ctl[CTL_OVERFLOW], 1u);
     if (4u * (ca + cb) > *(volatile const uint32_t*)&B.ctl[CTL_MAXCOUNT]) atomicMax(&B.ctl[CTL_MAXCOUNT], 4u * (ca + cb));
-    if (__lane_id() == 0u) B.wave_chunks[wave] = min(fill, capw);
+    if (__lane_id() == 0u) {
+        const uint32_t in = min(fill_in, capw);
+        B.wave_chunks[2u * wave] = in;
+        B.wave_chunks[2u * wave + 1u] = min(fill_out, capw - in);
+    }
 }
 
 // A block of BS particles runs on min(BS, 1024) threads (with BS = 2048 each thread advances
@@ -904,19 +933,23 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
         }
     }
     if (ballot64(near_face != 0u) != 0ull && __lane_id() == 0u) atomicOr(&edge, 1u);
-    // the chunk counts of the block's NW list-build waves, scanned by the first wave
+    // the in-block and cross-block chunk counts of the block's NW list-build waves, scanned by
+    // the first wave
     constexpr uint32_t NW = (uint32_t)BS / 64u;
-    __shared__ uint32_t wfirst[NW + 1];
+    __shared__ uint32_t wfirst[2][NW + 1];
     if (t < 64u) {
-        const uint32_t v = t < NW ? B.wave_chunks[blk * NW + t] : 0u;
-        uint32_t x = v;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, 64);
-            if (t >= (uint32_t)d) x += y;
+        for (uint32_t kind = 0; kind < 2u; kind++) {
+            const uint32_t v = t < NW ? B.wave_chunks[2u * (blk * NW + t) + kind] : 0u;
+            uint32_t x = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (t >= (uint32_t)d) x += y;
+            }
+            if (t < NW) wfirst[kind][t] = x - v;
+            if (t == NW - 1u) wfirst[kind][NW] = x;
         }
-        if (t < NW) wfirst[t] = x - v;
-        if (t == NW - 1u) wfirst[NW] = x;
     }
     __syncthreads();
     const bool images = __builtin_amdgcn_readfirstlane(edge) != 0u;
@@ -924,95 +957,138 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
     const PairTable* __restrict__ pt = B.pt;
     {
         const float rc2_0 = pt->rc2[0], sigma2_0 = pt->sigma2[0], six_eps4_0 = pt->six_eps4[0], r2f_0 = pt->r2_fast[0];
-        // The block's chunks, numbered 0 .. nch-1 across its waves' regions in order: chunk g
-        // lies in region w (wfirst[w] <= g < wfirst[w + 1]) at slot g - wfirst[w].
+        // The block's chunks of each kind, numbered 0 .. n-1 across its waves' regions in order:
+        // in-block chunk g lies in region w (wfirst[0][w] <= g < wfirst[0][w + 1]) at slot
+        // g - wfirst[0][w], cross-block chunk g at slot capw - 1 - (g - wfirst[1][w]).
         const size_t capw = 64u * (P.cap / 4u);
         const uint4* __restrict__ chunks = reinterpret_cast<const uint4*>(B.nbr) + (size_t)blk * NW * capw;
-        const uint32_t nch = wfirst[NW];
-        uint32_t g = t, w = 0u;
-        auto slot = [&](uint32_t gg) -> size_t {
-            while (gg >= wfirst[w + 1u]) w++;
-            return (size_t)w * capw + (gg - wfirst[w]);
-        };
-        // the next chunk is in flight while this one is evaluated
-        uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
-        if (g < nch) q_next = chunks[slot(g)];
-        // the loop twice over: with the periodic image of each separation, and without it
-        auto chunk_loop = [&](auto with_images) {
-        for (; g < nch; g += NT) {
-            const uint4 qt = q_next;
-            if (g + NT < nch) q_next = chunks[slot(g + NT)];
-            uint32_t own, m;
-            bool inblock;
-            const uint4 q = untag_chunk(qt, own, m, inblock);
-            const uint32_t i = base + own;
-            double xi[D];
+        // one pair: its separation (with or without the periodic image), its kick on the owner
+        // (summed into k) and its kick on the partner (kj); false beyond the cutoff
+        auto pair = [&](auto with_images, const double xi[D], const double xs[D], uint32_t si, uint32_t j, float c, uint32_t k[D], int32_t kj[D]) {
+            float f[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
-            uint32_t si = 0;
-            float c = P.kick_c[0];
+            for (int a = 0; a < D; a++) {
+                if constexpr (decltype(with_images)::value) f[a] = sep_f16(separation16<D>(P, xi[a], xs[a], a));
+                else f[a] = sep_f16(xs[a] - xi[a]);
+            }
+            if constexpr (MULTI) {
+                const uint32_t sj = sp[j], p = si * 4 + sj;
+                return pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
+            } else {
+                return pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+            }
+        };
+        auto own_of = [&](uint32_t own, uint32_t& si, float& c) {
+            si = 0u;
+            c = P.kick_c[0];
             if (MULTI) {
-                si = sp[i];
+                si = sp[base + own];
                 c = pt->kick_c[si];
             }
-            uint32_t k[D];
+        };
+        // In-block chunks: owner and partners both in LDS, every kick to the LDS accumulators.
+        auto inblock_pass = [&](auto with_images) {
+            const uint32_t nch = wfirst[0][NW];
+            uint32_t g = t, w = 0u;
+            auto slot = [&](uint32_t gg) -> size_t {
+                while (gg >= wfirst[0][w + 1u]) w++;
+                return (size_t)w * capw + (gg - wfirst[0][w]);
+            };
+            // the next chunk is in flight while this one is evaluated
+            uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+            if (g < nch) q_next = chunks[slot(g)];
+            for (; g < nch; g += NT) {
+                const uint4 qc = q_next;
+                if (g + NT < nch) q_next = chunks[slot(g + NT)];
+                uint32_t e[8], own, m;
+                unpack_inblock(qc, e, own, m);
+                double xi[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) k[a] = 0u;
-            const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-            double xs[4][D];
-            if (inblock) {
+                for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
+                uint32_t si;
+                float c;
+                own_of(own, si, c);
+                uint32_t k[D];
 #pragma unroll
-                for (int e = 0; e < 4; e++)
+                for (int a = 0; a < D; a++) k[a] = 0u;
 #pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
-            } else {
+                for (int s2 = 0; s2 < 8; s2++) {
+                    if ((uint32_t)s2 >= m) break;
+                    const uint32_t lj = e[s2];
+                    double xs[D];
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if constexpr (D == 3) {
-                        const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e]];
-                        xs[e][0] = q4.x;
-                        xs[e][1] = q4.y;
-                        xs[e][2] = q4.z;
-                    } else {
-                        const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e]];
-                        xs[e][0] = q2.x;
-                        xs[e][1] = q2.y;
+                    for (int a = 0; a < D; a++) xs[a] = lpos[a][lj];
+                    int32_t kj[D];
+                    if (pair(with_images, xi, xs, si, base + lj, c, k, kj)) {
+#pragma unroll
+                        for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
                     }
                 }
+#pragma unroll
+                for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
             }
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if ((uint32_t)e >= m) break;
-                const uint32_t j = js[e];
-                // list separations stay inside the 8 sigma range (a system whose separations
-                // may reach it runs the full-list kernel, which tests the range)
-                float f[D];
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    if constexpr (decltype(with_images)::value) f[a] = sep_f16(separation16<D>(P, xi[a], xs[e][a], a));
-                    else f[a] = sep_f16(xs[e][a] - xi[a]);
-                }
-                int32_t kj[D];
-                bool hit;
-                if constexpr (MULTI) {
-                    uint32_t sj = sp[j];
-                    uint32_t p = si * 4 + sj;
-                    hit = pair_kick2<D, true>(f, pt->rc2[p], pt->sigma2[p], pt->six_eps4[p], pt->r2_fast[p], c, pt->kick_c[sj], k, kj);
-                } else {
-                    hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                }
-                if (inblock && hit) {
-                    const uint32_t lj = j - base;
-#pragma unroll
-                    for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
-        }
         };
-        if (images) chunk_loop(std::true_type{});
-        else chunk_loop(std::false_type{});
+        // Cross-block chunks: partners gathered from the packed positions, owner kicks only (each
+        // such pair is listed at both its ends).
+        auto cross_pass = [&](auto with_images) {
+            const uint32_t nch = wfirst[1][NW];
+            uint32_t g = t, w = 0u;
+            auto slot = [&](uint32_t gg) -> size_t {
+                while (gg >= wfirst[1][w + 1u]) w++;
+                return (size_t)w * capw + capw - 1u - (gg - wfirst[1][w]);
+            };
+            uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+            if (g < nch) q_next = chunks[slot(g)];
+            for (; g < nch; g += NT) {
+                const uint4 qc = q_next;
+                if (g + NT < nch) q_next = chunks[slot(g + NT)];
+                uint32_t own, m;
+                bool inblock;
+                const uint4 q = untag_chunk(qc, own, m, inblock);
+                const uint32_t i = base + own;
+                double xi[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
+                uint32_t si;
+                float c;
+                own_of(own, si, c);
+                uint32_t k[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) k[a] = 0u;
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+                double xs[4][D];
+#pragma unroll
+                for (int e2 = 0; e2 < 4; e2++) {
+                    if constexpr (D == 3) {
+                        const double4 q4 = reinterpret_cast<const double4*>(B.aos[pp])[js[e2]];
+                        xs[e2][0] = q4.x;
+                        xs[e2][1] = q4.y;
+                        xs[e2][2] = q4.z;
+                    } else {
+                        const double2 q2 = reinterpret_cast<const double2*>(B.aos[pp])[js[e2]];
+                        xs[e2][0] = q2.x;
+                        xs[e2][1] = q2.y;
+                    }
+                }
+#pragma unroll
+                for (int e2 = 0; e2 < 4; e2++) {
+                    if ((uint32_t)e2 >= m) break;
+                    // list separations stay inside the 8 sigma range (a system whose separations
+                    // may reach it runs the full-list kernel, which tests the range)
+                    int32_t kj[D];
+                    (void)pair(with_images, xi, xs[e2], si, js[e2], c, k, kj);
+                }
+#pragma unroll
+                for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
+            }
+        };
+        if (images) {
+            inblock_pass(std::true_type{});
+            cross_pass(std::true_type{});
+        } else {
+            inblock_pass(std::false_type{});
+            cross_pass(std::false_type{});
+        }
     }
     __syncthreads();
     for (uint32_t r = 0; r < PER; r++) {
@@ -1706,7 +1782,7 @@ uint64_t Engine<D>::planned_bytes() const {
     t += 2 * (D * (sz(n * 8) + sz(n * 4)) + sz(n * 4) + sz(n * (D == 3 ? 4 : 2) * 8) + sz(n * 4));  // pos, vel, species, aos, ids
     t += D * (sz(n * 8) + sz(n * 4));                                                               // ref, kicks
     t += sz(n * 4);                                                                                  // nbr_count
-    t += sz(((n + 4095u) & ~uint64_t(4095)) / 64u * 4);                                              // wave_chunks
+    t += sz(((n + 4095u) & ~uint64_t(4095)) / 64u * 8);                                              // wave_chunks
     t += sz((n + 32u) * 16);                                                                         // rel
     if (P_.rows) t += sz((2 * n + 32u) * 16) + sz(row_bins_ * 4) + sz((row_keys_ + 1u) * 4) + sz(n * 4) + sz(scan_temp_bytes(row_keys_ + 1u));
     t += 2 * sz(ncell_keys_ * 4);                                                                    // cell_start, cell_end
@@ -2018,11 +2094,11 @@ Engine<D>::Engine(const SystemDesc& sys, uint64_t n, const int64_t* pos, const i
     }
     B_.nbr_count = (uint32_t*)dalloc(n * sizeof(uint32_t));
     {
-        // one count per 64-particle wave, for every block size up to 4096; waves past the last
-        // particle stay empty
+        // two counts (in-block and cross-block chunks) per 64-particle wave, for every block size
+        // up to 4096; waves past the last particle stay empty
         const size_t waves = ((n + 4095u) & ~size_t(4095)) / 64u;
-        B_.wave_chunks = (uint32_t*)dalloc(waves * sizeof(uint32_t));
-        RH_HIP(hipMemset(B_.wave_chunks, 0, waves * sizeof(uint32_t)));
+        B_.wave_chunks = (uint32_t*)dalloc(2 * waves * sizeof(uint32_t));
+        RH_HIP(hipMemset(B_.wave_chunks, 0, 2 * waves * sizeof(uint32_t)));
     }
     B_.rel = (float4*)dalloc((n + 32u) * sizeof(float4));  // padded: the list build reads up to 31 past a cell
     alloc_rows();

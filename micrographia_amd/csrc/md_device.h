@@ -117,6 +117,23 @@ __device__ __forceinline__ uint4 tag_chunk(uint4 q, uint32_t own, uint32_t entri
                       (q.w & lo) | ((entries - 1u) << 28) | (((own >> 11) & 1u) << 30));
 }
 
+// An in-block chunk of the single-system half list: up to eight partners of one owner, as
+// 12-bit offsets from the block's first slot, two to a word (bits 0-11 and 12-23); the top
+// bytes hold the owner (12 bits, words x and y) and the entry count - 1 (word z).
+__device__ __forceinline__ uint4 pack_inblock(const uint32_t e[8], uint32_t own, uint32_t entries) {
+    const uint32_t m = 0xfffu;
+    return make_uint4((e[0] & m) | ((e[1] & m) << 12) | ((own & 0xffu) << 24), (e[2] & m) | ((e[3] & m) << 12) | ((own >> 8) << 24),
+                      (e[4] & m) | ((e[5] & m) << 12) | ((entries - 1u) << 24), (e[6] & m) | ((e[7] & m) << 12));
+}
+
+__device__ __forceinline__ void unpack_inblock(const uint4& q, uint32_t e[8], uint32_t& own, uint32_t& entries) {
+    own = (q.x >> 24) | ((q.y >> 24) << 8);
+    entries = ((q.z >> 24) & 7u) + 1u;
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 8; k++) e[k] = (w[k >> 1] >> (12 * (k & 1))) & 0xfffu;
+}
+
 // The indices of a tagged chunk, and its owner, entry count and in-block flag.
 __device__ __forceinline__ uint4 untag_chunk(uint4 qt, uint32_t& own, uint32_t& entries, bool& in) {
     own = (qt.x >> 28) | ((qt.y >> 28) << 4) | (((qt.z >> 28) & 7u) << 8) | (((qt.w >> 30) & 1u) << 11);
