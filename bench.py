@@ -28,7 +28,7 @@ BYTES_PER_ATOM_STEP = 72  # read+write 3 x i64 positions and 3 x i32 velocities
 # WRITE_SIZE, each calibrated against a plain 1 GiB copy (profiles/ keeps the summary and the
 # commands).  Keyed by (variant, block); a counter pass cannot run inside this process.
 MEASURED_TRAFFIC = {
-    (1, 2048): (1.970e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
+    (1, 2048): (1.567e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
 }
 
 
@@ -319,7 +319,7 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-table", default=None, help="batched tuning table to read and write (JSON)")
     ap.add_argument("--no-batch-default", dest="batch_default", action="store_false", help="skip the batched run with untuned tiles")
-    ap.add_argument("--serve-duration", type=float, default=0.2, help="seconds of Poisson arrivals in the serving leg (0: skip)")
+    ap.add_argument("--serve-duration", type=float, default=1.15, help="seconds of Poisson arrivals in the serving leg (0: skip); 1.15 s at 200/s gives 202 requests, enough for a p99 below the maximum")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-plain", action="store_true", help="skip the one-system-at-a-time comparisons")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run a plain device copy of this many MiB (counter calibration)")

@@ -46,18 +46,29 @@ struct RbOff {
 
 // Per system: rebuild now?  A rebuilding system appends its units to its build kind's list and
 // to the list of all kinds (the per-slot rebuild kernels walk those, not the whole arena).
-__global__ void k_bb_decide(const BSys* S, BCtl* C, uint32_t nsys, uint32_t* list, uint32_t* count, RbOff ro) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per system: lane 0 decides and reserves room in the rebuild lists, and the wave
+// appends the system's units together.
+__global__ __launch_bounds__(64) void k_bb_decide(const BSys* S, BCtl* C, uint32_t nsys, uint32_t* list, uint32_t* count, RbOff ro) {
+    const uint32_t s = blockIdx.x;
     if (s >= nsys) return;
     BCtl& c = C[s];
-    uint32_t go = (S[s].active && c.stale && c.remaining && !c.overflow) ? 1u : 0u;
-    c.do_rebuild = go;
-    if (!go) return;
-    c.maxcount = 0u;
     const BSys& d = S[s];
+    // every lane reads the same words, none of which this kernel writes
+    const bool go = d.active && c.stale && c.remaining && !c.overflow;
     const uint32_t u0 = d.slot0 / BATCH_WG, nu = d.nslots / BATCH_WG;
-    const uint32_t p = atomicAdd(&count[d.bkind], nu), q = atomicAdd(&count[4], nu);
-    for (uint32_t u = 0; u < nu; u++) {
+    __shared__ uint32_t pq[2];
+    if (threadIdx.x == 0u) {
+        c.do_rebuild = go ? 1u : 0u;
+        if (go) {
+            c.maxcount = 0u;
+            pq[0] = atomicAdd(&count[d.bkind], nu);
+            pq[1] = atomicAdd(&count[4], nu);
+        }
+    }
+    if (!go) return;
+    __syncthreads();
+    const uint32_t p = pq[0], q = pq[1];
+    for (uint32_t u = threadIdx.x; u < nu; u += 64u) {
         list[ro.off[d.bkind] + p + u] = u0 + u;
         list[ro.off[4] + q + u] = u0 + u;
     }
@@ -232,6 +243,38 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     const bool any_wrap = (wl[0] != 0.f) | (wl[1] != 0.f) | (wl[2] != 0.f);
     const uint32_t blo = local & ~(S.block - 1u);
     uint32_t na = 0, nb = 0;
+    // Hit masks wait in a three-entry queue per lane (mask, first candidate) and are visited one
+    // hit per step of a wave-wide loop, run only when some lane's queue is full and at the end:
+    // the loop's length follows the busiest lane's hits over many cells, not the sum over cells
+    // of each cell's busiest lane.
+    uint32_t m0 = 0u, m1 = 0u, m2 = 0u, b0 = 0u, b1 = 0u, b2 = 0u, qn = 0u;
+    auto visit = [&]() __attribute__((always_inline)) {
+        bool full = false, in = false;
+        if (qn != 0u) {
+            const uint32_t b = __builtin_ctz(m0);
+            m0 &= m0 - 1u;
+            const uint32_t j = b0 + b;
+            in = variant == 1 && j - blo < S.block;
+            if (in) {
+                put4(fa, na, j);
+                na++;
+                full = (na & 3u) == 0u;
+            } else {
+                put4(fb, nb, j);
+                nb++;
+                full = (nb & 3u) == 0u;
+            }
+            if (m0 == 0u) {
+                m0 = m1;
+                b0 = b1;
+                m1 = m2;
+                b1 = b2;
+                m2 = 0u;
+                qn--;
+            }
+        }
+        store(full, in ? fa : fb, 4u, in);
+    };
     for (int o = 0; o < stencil_size(D); o++) {
         const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
         uint32_t cc[D];
@@ -296,29 +339,24 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
                 const int t0 = (int)local - (int)base;
                 mask &= ~((t0 >= 0 && t0 < 32) ? (1u << t0) : 0u);
             }
-            for (;;) {
-                const bool hit = mask != 0u;
-                if (ballot64(hit) == 0ull) break;
-                bool full = false, in = false;
-                if (hit) {
-                    const uint32_t b = __builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    const uint32_t j = base + b;
-                    in = (inb >> b) & 1u;
-                    if (in) {
-                        put4(fa, na, j);
-                        na++;
-                        full = (na & 3u) == 0u;
-                    } else {
-                        put4(fb, nb, j);
-                        nb++;
-                        full = (nb & 3u) == 0u;
-                    }
+            const bool push = mask != 0u;
+            while (ballot64(push && qn == 3u) != 0ull) visit();
+            if (push) {
+                if (qn == 0u) {
+                    m0 = mask;
+                    b0 = base;
+                } else if (qn == 1u) {
+                    m1 = mask;
+                    b1 = base;
+                } else {
+                    m2 = mask;
+                    b2 = base;
                 }
-                store(full, in ? fa : fb, 4u, in);
+                qn++;
             }
         }
     }
+    while (ballot64(qn != 0u) != 0ull) visit();
     store((na & 3u) != 0u, fa, na & 3u, true);
     store((nb & 3u) != 0u, fb, nb & 3u, false);
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
@@ -985,7 +1023,7 @@ void Batch::enqueue_iteration() {
     const uint32_t sb = blocks_for(max_sys_, 256);
     RbOff ro;
     for (int k = 0; k < 5; k++) ro.off[k] = rb_off_[k];
-    k_bb_decide<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_list_, d_rb_count_, ro);
+    k_bb_decide<<<max_sys_, 64, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_list_, d_rb_count_, ro);
     const uint32_t ga = rebuild_grid(rb_cap_[4]);
     k_bb_count<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);

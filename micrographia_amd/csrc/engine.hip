@@ -2233,10 +2233,14 @@ void Engine<D>::enqueue_rebuild(bool gated) {
     const bool rows = mode_.kind == 1 && row_path();
     k_gather<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], 1, gate);
     RH_HIP(hipGetLastError());
-    fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
-    fill_u32(B_.cell_end, 0u, ncell_keys_, gate, stream_);
-    k_ranges<<<nb, TPB, 0, stream_>>>(sort_.keys[r], (uint32_t)n_, B_.cell_start, B_.cell_end, gate);
-    RH_HIP(hipGetLastError());
+    if (!rows) {
+        // the cell ranges of the new order (the row build works from its own row-ordered copy,
+        // and readouts that need the ranges recompute them)
+        fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
+        fill_u32(B_.cell_end, 0u, ncell_keys_, gate, stream_);
+        k_ranges<<<nb, TPB, 0, stream_>>>(sort_.keys[r], (uint32_t)n_, B_.cell_start, B_.cell_end, gate);
+        RH_HIP(hipGetLastError());
+    }
     if (mode_.kind == 1) {
         bool multi = P_.nspecies > 1;
         if constexpr (D == 3) {
