@@ -262,9 +262,10 @@ typedef struct rh_batch rh_batch;
 int rh_batch_create(uint64_t budget_bytes, uint32_t max_systems, rh_batch** out);
 int rh_batch_destroy(rh_batch* b);
 /* Admits a system with `steps` steps to run, Verlet skin `skin`; *id = -1 when it does not fit
- * in the free arena space now (the admission rule).  3D systems may have the Andersen bath and
- * heat-bath walls (no ledger is kept); the 2D thermostat region and heat-bath walls,
- * compartments, the membrane, the piston and bonds are refused. */
+ * in the free arena space now (the admission rule).  3D systems may have hard, soft and heat-bath
+ * walls and the Andersen bath, 2D systems hard and soft walls, all with the ledger booked as the
+ * single-system engines book it; the 2D thermostat region and heat-bath walls, compartments, the
+ * membrane, the piston and bonds are refused. */
 int rh_batch_add3(rh_batch* b, const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id);
 int rh_batch_add2(rh_batch* b, const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id);
 int rh_batch_remove(rh_batch* b, int32_t id);
@@ -276,6 +277,10 @@ int rh_batch_sync(rh_batch* b);
 int rh_batch_poll(rh_batch* b, uint64_t* remaining, uint32_t count);
 int rh_batch_read_state(rh_batch* b, int32_t id, int64_t* pos, int32_t* vel, uint32_t* species);
 int rh_batch_list_stats(rh_batch* b, int32_t id, rh_list_stats* out);
+/* The system's ledger since it joined the batch (GpuEngine3::read_ledger, engine3.rs:645-647, and
+ * GpuEngine's for 2D, whose entries 0-3 are the walls): energy and impulse per wall (2 axis +
+ * side) and for the bath (entry 6), and hits.  The same totals as the system run alone. */
+int rh_batch_read_ledger(rh_batch* b, int32_t id, rh_ledger3* out);
 /* Iterations per captured hipGraph (0: launch directly). */
 int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk);
 /* Tile settings for systems added from now on, per dimension and size class (floor(log2 n)):

@@ -208,6 +208,7 @@ SIGNATURES = {
     "rh_batch_poll": ([P, P, u32], i32),
     "rh_batch_read_state": ([P, C.c_int32, P, P, P], i32),
     "rh_batch_list_stats": ([P, C.c_int32, P], i32),
+    "rh_batch_read_ledger": ([P, C.c_int32, P], i32),
     "rh_batch_set_graph_chunk": ([P, u32], i32),
     "rh_batch_set_tuning": ([P, u32, u32, u32, dbl, u32], i32),
     "rh_batch_stream": ([P, P], i32),

@@ -14,9 +14,9 @@ from typing import Dict, List, Optional, Union
 
 import numpy as np
 
-from ._lib import RH_ERR_NOMEM, CListStats, RahmanError, check, load, ptr
+from ._lib import RH_ERR_NOMEM, CLedger3, CListStats, RahmanError, check, load, ptr
 from .core import State, State3, System, System3
-from .engine import ListStats, _stats
+from .engine import Ledger3, ListStats, _stats
 
 
 class BatchEngine:
@@ -113,6 +113,14 @@ class BatchEngine:
         sp = np.zeros(n, np.uint32)
         check(load().rh_batch_read_state(self._h, int(id_), ptr(pos), ptr(vel), ptr(sp)))
         return State3(pos, vel, sp) if d == 3 else State(pos, vel, sp)
+
+    def read_ledger(self, id_: int) -> Ledger3:
+        """The system's ledger since it joined: energy, impulse and hits per wall (2 axis +
+        side) and for the bath (entry 6), as GpuEngine3::read_ledger (engine3.rs:645-647) and
+        GpuEngine's for 2D (entries 0-3 the walls)."""
+        l = CLedger3()
+        check(load().rh_batch_read_ledger(self._h, int(id_), C.byref(l)))
+        return Ledger3(np.array(l.energy[:]), np.array(l.impulse[:]), np.array(l.hits[:], dtype=np.uint64))
 
     def list_stats(self, id_: int) -> ListStats:
         s = CListStats()

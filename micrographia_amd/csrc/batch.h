@@ -33,9 +33,11 @@ struct BSys {
     float kick_c[4];
     int32_t gravity[3];
     float half_skin2;
-    // stochastic boundaries (3D only): the Andersen bath and heat-bath walls
-    uint32_t stoch, bath, bath_species, thermal_walls, seed_lo, seed_hi;
+    // The ledger (walls and bath) is booked when ledger != 0: some axis walled, or the bath.
+    // Stochastic boundaries (3D only): the Andersen bath and heat-bath walls.
+    uint32_t ledger, bath, bath_species, thermal_walls, seed_lo, seed_hi;
     double dt, bath_p;
+    double mass[4];        // per species, for the energies the ledger books
     double bath_sd[4];     // sqrt(T / m) of the bath per species (sigma/tau)
     double wall_sd[6][4];  // sqrt(T / m) per heat-bath wall and species
     PairTable pt;
@@ -72,6 +74,7 @@ struct BBufs {
     uint32_t* rb_count;
     const BSys* sys;
     BCtl* ctl;
+    unsigned long long* ledger;  // LEDGER_WORDS per system slot (fixed point, 2^-32)
 };
 
 struct BatchMember {
@@ -100,6 +103,8 @@ class Batch {
     // Copies every system's control words to the host (synchronises); returns remaining steps.
     void poll(uint64_t* remaining, uint32_t count);
     void read_state(int id, int64_t* pos, int32_t* vel, uint32_t* species);
+    // The system's ledger (walls and bath) since it joined: 7 entries of energy, impulse, hits.
+    void read_ledger(int id, double energy[7], double impulse[7], uint64_t hits[7]);
     rh_list_stats stats(int id);
     void set_graph_chunk(uint32_t chunk) { graph_chunk_ = chunk; }
     hipStream_t stream() const { return stream_; }
@@ -143,7 +148,7 @@ class Batch {
     uint32_t* d_rb_list_ = nullptr;
     uint32_t* d_rb_count_ = nullptr;
     uint32_t units_cap_ = 0;        // BATCH_WG-slot units in the slot arena
-    uint32_t step_off_[18] = {}, step_cnt_[18] = {};  // step launch kind 6 b + kind: its list
+    uint32_t step_off_[24] = {}, step_cnt_[24] = {};  // step launch kind 8 b + kind: its list
     uint32_t rb_off_[5] = {}, rb_cap_[5] = {};        // rebuild lists: offset, room (units)
     bool tables_dirty_ = true;
     uint32_t graph_chunk_ = 100;

@@ -7,9 +7,10 @@
 //                        lower-index end, pairs across 256-slot blocks at both ends, as a tagged
 //                        chunk stream per 64-slot wave
 //   k_bb_commit_rebuild  per system
-//   k_bb_step            per 256-slot work-group (one launch per kind): its chunks dealt
-//                        evenly over the lanes, then per slot the leapfrog (with the 3D bath
-//                        and heat-bath walls in the stochastic kind) and half-skin check
+//   k_bb_step            per block of a system (one launch per kind): its chunks dealt evenly
+//                        over the lanes, then per slot the leapfrog and half-skin check; systems
+//                        with a walled axis or the bath run the booking kind, which books the
+//                        ledger (and draws the 3D bath and heat-bath walls)
 //   k_bb_commit_step     per system
 // Every system takes exactly one step per iteration (its rebuild, when due, runs first in the
 // same iteration), so the host enqueues iterations without reading anything back and a chunk of
@@ -406,8 +407,8 @@ __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S
 // block's chunks (its waves' regions, numbered in order) are dealt out evenly over the lanes,
 // every kick goes to the block's LDS accumulators, and positions are staged and gathered as
 // sixteenths.
-template <int D, bool MULTI, bool STOCH, uint32_t BS>
-__device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t wg) {
+template <int D, bool MULTI, bool BOOK, uint32_t BS>
+__device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t sysid, uint32_t wg) {
     constexpr uint32_t NW = BS / 64u;
     __shared__ uint32_t acc[D][BS];
     __shared__ double lpos[D][BS];  // the block's positions in sixteenths, for in-block neighbours
@@ -546,7 +547,8 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         xi[a] = lpos[a][t] * 16.0;
         k[a] = 0u;
     }
-    if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr);
+    float wf[2 * D] = {};  // each soft wall's force on this particle (booked as impulse)
+    if (S.soft_walls) soft_walls<D>(S, xi, c, k, nullptr, BOOK ? wf : nullptr);
 #pragma unroll
     for (int a = 0; a < D; a++) k[a] += acc[a][t];
     const bool pin = (S.pinned >> si) & 1u;
@@ -554,18 +556,21 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     double x[D];
 #pragma unroll
     for (int a = 0; a < D; a++) v[a] = (int32_t)((uint32_t)B.vel[pm][a][slot] + k[a] + (pin ? 0u : (uint32_t)S.gravity[a]));
-    if constexpr (STOCH && D == 3) {
-        // the Andersen bath, then the move and the walls, heat-bath ones drawing the velocity
-        // (the step's clock is the system's before this step is committed)
+    if constexpr (BOOK) {
+        // as the single-system engine's booked leapfrog: the soft walls' impulses, the Andersen
+        // bath, then the move and the walls (heat-bath ones drawing the velocity), every
+        // exchange booked in the system's ledger (the step's clock is the system's before this
+        // step is committed)
+        unsigned long long* L = B.ledger + (size_t)sysid * LEDGER_WORDS;
         const uint32_t id = B.ids[pm][slot], clock = C.clock;
-        double u[3];
-        if (S.bath && andersen_draw(S, id, clock, si, u)) {
 #pragma unroll
-            for (int a = 0; a < 3; a++) v[a] = to_step(S, u[a]);
-        }
+        for (int w = 0; w < 2 * D; w++)
+            if (wf[w] != 0.0f) ledger_add(L, w, 1, (double)wf[w] * S.dt);
+        if constexpr (D == 3)
+            if (S.bath) bath_redraw<3>(S, L, id, clock, si, v);
 #pragma unroll
-        for (int a = 0; a < 3; a++) x[a] = xi[a] + (double)v[a];
-        bound_thermal3(S, x, v, id, clock, si);
+        for (int a = 0; a < D; a++) x[a] = xi[a] + (double)v[a];
+        bound_booked<D>(S, L, x, v, id, clock, si);
     } else {
 #pragma unroll
         for (int a = 0; a < D; a++) x[a] = xi[a] + (double)v[a];
@@ -588,20 +593,20 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         atomicOr(&C.stale_next, 1u);
 }
 
-// One launch per kind of system present (dimension, one or more species, stochastic boundaries
-// or none) and block size, each compiled with only its own path so the common kernels keep
-// their registers low; work-groups of other kinds exit at once.
+// One launch per kind of system present (dimension, one or more species, booking the ledger or
+// not) and block size, each compiled with only its own path so the common kernels keep their
+// registers low.
 // Work-group k takes block step_list[off + k] of the launch's kind: slots [g BS, (g + 1) BS) of
 // a system of block size BS (its slots start at a multiple of its block size and fill whole
 // blocks).  Consecutive blocks of the list go to the same XCD.
-template <int D, bool MULTI, bool STOCH, uint32_t BS>
+template <int D, bool MULTI, bool BOOK, uint32_t BS>
 __global__ __launch_bounds__(BS) void k_bb_step(BBufs B, uint32_t off) {
     const uint32_t wg = B.step_list[off + xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = B.wg_sys[wg * (BS / BATCH_WG)];
     const BSys& S = B.sys[s];
     BCtl& C = B.ctl[s];
     if (!bb_runs(S, C)) return;
-    bb_step<D, MULTI, STOCH, BS>(B, S, C, wg);
+    bb_step<D, MULTI, BOOK, BS>(B, S, C, s, wg);
 }
 
 __global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys, uint32_t* rb_count) {
@@ -680,6 +685,8 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     d_ctl_ = (BCtl*)dalloc(sizeof(BCtl) * max_sys_);
     RH_HIP(hipMemset(d_sys_, 0, sizeof(BSys) * max_sys_));
     RH_HIP(hipMemset(d_ctl_, 0, sizeof(BCtl) * max_sys_));
+    B_.ledger = (unsigned long long*)dalloc(sizeof(unsigned long long) * LEDGER_WORDS * max_sys_);
+    RH_HIP(hipMemset(B_.ledger, 0, sizeof(unsigned long long) * LEDGER_WORDS * max_sys_));
     d_wg_ = (uint32_t*)dalloc(sizeof(uint32_t) * (slot_cap / BATCH_WG));
     RH_HIP(hipMemset(d_wg_, 0xff, sizeof(uint32_t) * (slot_cap / BATCH_WG)));
     B_.wg_sys = d_wg_;
@@ -711,15 +718,6 @@ void Batch::budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) c
     if (slot_cap) *slot_cap = slots_.capacity() * BATCH_WG;
     if (list_cap) *list_cap = lists_.capacity() * LIST_UNIT;
     if (cell_cap) *cell_cap = cells_.capacity() * CELL_UNIT;
-}
-
-// A 3D system with the Andersen bath or a heat-bath wall: its steps draw random numbers.
-static bool stochastic(const SystemDesc& sys) {
-    if (sys.dim != 3) return false;
-    bool thermal = false;
-    for (int a = 0; a < 3; a++)
-        for (int side = 0; side < 2; side++) thermal |= sys.walled[a] && sys.wall_kind[a][side] == RH_WALL_THERMAL;
-    return sys.has_bath || thermal;
 }
 
 // Cells, list capacity and constants of one system, as Params::new (engine3.rs:70-162).
@@ -783,8 +781,10 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
             }
         d.gravity[a] = core::gravity_kick(sys.dt, sys.gravity[a]);
     }
-    // stochastic boundaries (3D): Threefry keyed by the system seed, as the single-system engine
-    d.stoch = stochastic(sys) ? 1u : 0u;
+    // the ledger, as the single-system engine books it (a walled axis or the bath), and the
+    // stochastic boundaries' Threefry keys (the system seed)
+    d.ledger = (sys.has_bath || d.periodic != (1u << D) - 1u) ? 1u : 0u;
+    for (uint32_t sp = 0; sp < sys.nspecies; sp++) d.mass[sp] = sys.masses[sp];
     d.seed_lo = (uint32_t)sys.seed;
     d.seed_hi = (uint32_t)(sys.seed >> 32);
     d.dt = sys.dt;
@@ -910,6 +910,7 @@ int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int3
         c.stale = 1u;
         c.remaining = (uint32_t)steps;
         RH_HIP(hipMemcpy(d_ctl_ + id, &c, sizeof c, hipMemcpyHostToDevice));
+        RH_HIP(hipMemset(B_.ledger + (size_t)id * LEDGER_WORDS, 0, sizeof(unsigned long long) * LEDGER_WORDS));
     } catch (...) {
         slots_.free(s0, nslots / BATCH_WG);
         lists_.free(l0, lunits);
@@ -961,21 +962,21 @@ void Batch::upload_tables() {
     RH_HIP(hipMemcpy(d_wg_, wg.data(), wg.size() * 4, hipMemcpyHostToDevice));
     uint32_t new_wgs = (uint32_t)std::max<uint64_t>(hw, 1);
     // Work lists: the rebuild lists' room per build kind (their systems' units), and per kind of
-    // step launch (kind j = 4 (stochastic) or 2 (3D), + (species > 1); block 256 << b) the
-    // blocks of its systems.
+    // step launch (kind j = 0 (2D), 2 (3D), 4 (3D with a ledger), 6 (2D with a ledger), + (species
+    // > 1); block 256 << b) the blocks of its systems.
     uint32_t rb_cap[5] = {0, 0, 0, 0, 0};
-    std::vector<uint32_t> lists[18];
+    std::vector<uint32_t> lists[24];
     for (auto& kv : members_) {
         const BSys& d = h_sys_[kv.first];
         rb_cap[d.bkind] += d.nslots / BATCH_WG;
         rb_cap[4] += d.nslots / BATCH_WG;
-        const uint32_t j = (d.stoch ? 4u : d.dim == 3 ? 2u : 0u) + (d.nspecies > 1 ? 1u : 0u);
+        const uint32_t j = (d.ledger ? (d.dim == 3 ? 4u : 6u) : (d.dim == 3 ? 2u : 0u)) + (d.nspecies > 1 ? 1u : 0u);
         const uint32_t bi = d.block == 1024u ? 2u : d.block == 512u ? 1u : 0u;
-        for (uint32_t g = d.slot0 / d.block; g < (d.slot0 + d.nslots) / d.block; g++) lists[6 * bi + j].push_back(g);
+        for (uint32_t g = d.slot0 / d.block; g < (d.slot0 + d.nslots) / d.block; g++) lists[8 * bi + j].push_back(g);
     }
     std::vector<uint32_t> flat;
     bool changed = new_wgs != grid_wgs_;
-    for (int k = 0; k < 18; k++) {
+    for (int k = 0; k < 24; k++) {
         changed |= step_cnt_[k] != (uint32_t)lists[k].size() || step_off_[k] != (uint32_t)flat.size();
         step_off_[k] = (uint32_t)flat.size();
         step_cnt_[k] = (uint32_t)lists[k].size();
@@ -1006,14 +1007,16 @@ void Batch::invalidate_graph() {
 
 template <uint32_t BS>
 void Batch::launch_steps(uint32_t b) {
-    const uint32_t* cnt = step_cnt_ + 6 * b;
-    const uint32_t* off = step_off_ + 6 * b;
+    const uint32_t* cnt = step_cnt_ + 8 * b;
+    const uint32_t* off = step_off_ + 8 * b;
     if (cnt[0]) k_bb_step<2, false, false, BS><<<cnt[0], BS, 0, stream_>>>(B_, off[0]);
     if (cnt[1]) k_bb_step<2, true, false, BS><<<cnt[1], BS, 0, stream_>>>(B_, off[1]);
     if (cnt[2]) k_bb_step<3, false, false, BS><<<cnt[2], BS, 0, stream_>>>(B_, off[2]);
     if (cnt[3]) k_bb_step<3, true, false, BS><<<cnt[3], BS, 0, stream_>>>(B_, off[3]);
     if (cnt[4]) k_bb_step<3, false, true, BS><<<cnt[4], BS, 0, stream_>>>(B_, off[4]);
     if (cnt[5]) k_bb_step<3, true, true, BS><<<cnt[5], BS, 0, stream_>>>(B_, off[5]);
+    if (cnt[6]) k_bb_step<2, false, true, BS><<<cnt[6], BS, 0, stream_>>>(B_, off[6]);
+    if (cnt[7]) k_bb_step<2, true, true, BS><<<cnt[7], BS, 0, stream_>>>(B_, off[7]);
 }
 
 // Work-groups of a per-slot rebuild kernel: enough to fill the chip, at most one per unit.
@@ -1104,6 +1107,18 @@ void Batch::poll(uint64_t* remaining, uint32_t count) {
     handle_overflow();
     if (remaining)
         for (uint32_t i = 0; i < count && i < max_sys_; i++) remaining[i] = members_.count((int)i) ? h_ctl_[i].remaining : 0;
+}
+
+void Batch::read_ledger(int id, double energy[7], double impulse[7], uint64_t hits[7]) {
+    RH_REQUIRE(members_.count(id), "no such system in the batch");
+    RH_HIP(hipStreamSynchronize(stream_));
+    unsigned long long L[LEDGER_WORDS];
+    RH_HIP(hipMemcpy(L, B_.ledger + (size_t)id * LEDGER_WORDS, sizeof L, hipMemcpyDeviceToHost));
+    for (int w = 0; w < 7; w++) {
+        energy[w] = (double)(long long)L[3 * w] / LEDGER_ONE;
+        impulse[w] = (double)(long long)L[3 * w + 1] / LEDGER_ONE;
+        hits[w] = L[3 * w + 2];
+    }
 }
 
 void Batch::read_state(int id, int64_t* pos, int32_t* vel, uint32_t* species) {

@@ -4,6 +4,7 @@
 #include <new>
 #include <string>
 
+#include <cstring>
 #include "batch.h"
 #include "engine.h"
 
@@ -516,6 +517,13 @@ int rh_batch_read_state(rh_batch* b, int32_t id, int64_t* pos, int32_t* vel, uin
 }
 int rh_batch_list_stats(rh_batch* b, int32_t id, rh_list_stats* out) {
     return guarded([&] { *out = eng(b)->stats(id); });
+}
+int rh_batch_read_ledger(rh_batch* b, int32_t id, rh_ledger3* out) {
+    return guarded([&] {
+        RH_REQUIRE(out, "null output");
+        std::memset(out, 0, sizeof *out);
+        eng(b)->read_ledger(id, out->energy, out->impulse, out->hits);
+    });
 }
 int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk) {
     return guarded([&] { eng(b)->set_graph_chunk(chunk); });

@@ -426,6 +426,103 @@ __device__ __forceinline__ void stencil_cell(const PT& P, const uint32_t c[D], i
     }
 }
 
+// ---- The ledger (md_core::walls::Ledger): wall and bath exchanges --------------------------
+// Shared by the single-system and batched engines (the parameter struct adds mass, dt, the bath
+// and wall draws' fields, thermal_walls and periodic).  Entries: 2 axis + side for the walls, 6
+// for the bath; per entry energy, impulse and hits.
+
+constexpr double LEDGER_ONE = 4294967296.0;      // ledger entries in 2^-32 units
+constexpr int LEDGER_WORDS = 21;                 // 7 entries x (energy, impulse, hits)
+
+// The ledger sums fixed-point values with 64-bit integer atomics, so its totals do not depend on
+// the order of the additions.
+__device__ __forceinline__ void ledger_add(unsigned long long* L, int w, int field, double value) {
+    double q = rint(value * LEDGER_ONE);
+    q = q != q ? 0.0 : fmin(fmax(q, -9.2e18), 9.2e18);  // saturate (a runaway wall force)
+    atomicAdd(&L[3 * w + field], (unsigned long long)(long long)q);
+}
+
+// md_core::three::andersen: with probability rate dt a fresh Maxwell velocity, booked as the
+// bath's (ledger entry 6) energy and hit.
+template <int D, class PT>
+__device__ void bath_redraw(const PT& P, unsigned long long* L, uint32_t id, uint32_t clock, uint32_t s, int32_t v[D]) {
+    if constexpr (D == 3) {
+        double u[3];
+        if (!andersen_draw(P, id, clock, s, u)) return;
+        double before = 0.0, after = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double w = velocity(P, v[b]);
+            before += w * w;
+        }
+#pragma unroll
+        for (int b = 0; b < 3; b++) v[b] = to_step(P, u[b]);
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double w = velocity(P, v[b]);
+            after += w * w;
+        }
+        ledger_add(L, 6, 0, 0.5 * P.mass[s] * (after - before));
+        atomicAdd(&L[3 * 6 + 2], 1ull);
+    }
+}
+
+// Wrap (periodic axes) or mirror about -1/2 and L - 1/2 (walls, walls.rs:27-35); a particle that
+// crossed a wall leaves it with a reversed normal velocity (hard and soft) or a velocity drawn
+// from the wall's bath (three.rs:232-243), and the exchange is booked in the ledger
+// (int_engine3.rs:186-206).
+template <int D, class PT>
+__device__ void bound_booked(const PT& P, unsigned long long* L, double x[D], int32_t v[D], uint32_t id, uint32_t clock, uint32_t s) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const double l = P.boxf[a];
+        if ((P.periodic >> a) & 1u) {
+            if (x[a] < 0.0) x[a] += l;
+            else if (x[a] >= l) x[a] -= l;
+            continue;
+        }
+        int side = -1;
+        if (x[a] < 0.0) {
+            x[a] = -1.0 - x[a];
+            side = 0;
+        } else if (x[a] >= l) {
+            x[a] = 2.0 * l - 1.0 - x[a];
+            side = 1;
+        }
+        if (side < 0) continue;
+        const int w = 2 * a + side;
+        double before[D], after[D];
+#pragma unroll
+        for (int b = 0; b < D; b++) before[b] = velocity(P, v[b]);
+        bool thermal = false;
+        if constexpr (D == 3) thermal = (P.thermal_walls >> w) & 1u;
+        if (thermal) {
+            if constexpr (D == 3) {
+                double vn, t0, t1;
+                thermal_wall_draw(P, id, clock, w, side, s, vn, t0, t1);
+                v[a] = to_step(P, vn);
+                v[(a + 1) % 3] = to_step(P, t0);
+                v[(a + 2) % 3] = to_step(P, t1);
+            }
+        } else {
+            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
+        }
+#pragma unroll
+        for (int b = 0; b < D; b++) after[b] = velocity(P, v[b]);
+        const double m = P.mass[s];
+        double ka = after[0] * after[0], kb = before[0] * before[0];
+#pragma unroll
+        for (int b = 1; b < D; b++) {
+            ka += after[b] * after[b];
+            kb += before[b] * before[b];
+        }
+        ledger_add(L, w, 0, 0.5 * m * ka - 0.5 * m * kb);
+        ledger_add(L, w, 1, fabs(m * (before[a] - after[a])));
+        atomicAdd(&L[3 * w + 2], 1ull);
+    }
+}
+
+
 constexpr int stencil_size(int d) { return d == 3 ? 27 : 9; }
 
 // Bits u < t of a 32-bit mask (t may be out of [0, 32]).

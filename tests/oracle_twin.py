@@ -22,11 +22,17 @@ def oracle_system(sys):
         o.bath_t = c.bath_t
         o.bath_rate = c.bath_rate
     else:
+        # field by field: the wall kinds and temperatures, the seed and every boundary travel too
+        # (the bond array stays alive on the copy)
+        import ctypes as C
+
         o = O.System2()
-        for a in range(2):
-            o.box_len[a] = c.box_len[a]
-            o.walled[a] = c.walled[a]
-            o.gravity[a] = c.gravity[a]
+        for name, _ in O.System2._fields_:
+            src, dst = getattr(type(c), name), getattr(O.System2, name)
+            assert src.size == dst.size, name
+            C.memmove(C.addressof(o) + dst.offset, C.addressof(c) + src.offset, src.size)
+        o._keep = c
+        return o
     o.dt = c.dt
     o.nspecies = c.nspecies
     o.pinned = c.pinned

@@ -103,6 +103,66 @@ def test_bath_and_heat_bath_walls_match_the_oracle(graph, block):
     assert b.read_state(ids[1]) != oracle_state(hard, st1, steps[1])
 
 
+ONE = 1 << 32
+
+
+def squeeze(st, box_len, axis):
+    """Move the start off the walls of `axis`: positions scaled into [1, L - 1] sigma."""
+    L = int(box_len[axis])
+    st.pos[:, axis] = ONE + (st.pos[:, axis].astype(np.float64) * (L - 2 * ONE) / L).astype(np.int64)
+
+
+def booked_systems():
+    """Systems whose steps book the ledger: a 3D liquid between hard and soft walls on two axes,
+    a 2D gas between soft walls and hard walls under gravity, and the stochastic 3D systems."""
+    sys, st = make_system(3, 2000, 0.5, 1.5, 31)
+    sys = sys.with_walls(2, rh.Wall.Soft, rh.Wall.Hard).with_walls(0, rh.Wall.Hard, rh.Wall.Soft)
+    squeeze(st, sys.box_len, 0)
+    squeeze(st, sys.box_len, 2)
+    walls3 = (sys, st)
+    sys, st = make_system(2, 3000, 0.5, 1.2, 32)
+    sys = sys.with_walls(0, rh.Wall.Soft, rh.Wall.Soft).with_hard_walls(1).with_gravity((0.0, -0.05))
+    squeeze(st, sys.box_len, 0)
+    squeeze(st, sys.box_len, 1)
+    walls2 = (sys, st)
+    return [walls3, walls2] + stochastic_systems()
+
+
+@pytest.mark.parametrize("graph", [0, 50])
+def test_ledgers_equal_the_systems_run_alone(graph):
+    """Each system books its wall and bath exchanges in its own ledger with the single-system
+    engine's fixed-point, order-independent sums, so the batched ledger equals the one the
+    system books alone, term for term, beside systems that book nothing; the trajectories equal
+    the oracle's."""
+    booked = booked_systems()
+    plain = [make_system(3, 1500, 0.8442, 1.2, 33), make_system(2, 3000, 0.7, 1.0, 34)]
+    b = BatchEngine(budget_bytes=2 * GB, max_systems=16, graph_chunk=graph)
+    steps = [300, 300, 150, 300, 200, 150, 150]
+    systems = booked + plain
+    ids = [b.add(sys, st, k, skin=0.3) for (sys, st), k in zip(systems, steps)]
+    assert all(i >= 0 for i in ids)
+    b.step_all(max(steps))
+    for i, (sys, st), k in zip(ids, systems, steps):
+        assert b.read_state(i) == oracle_state(sys, st, k), f"system {i} (n={st.n()}) differs from the oracle"
+        if isinstance(sys, rh.System3):
+            solo = rh.GpuEngine3(sys, st, rh.verlet3_with_skin(sys, st.n(), 0.3))
+        else:
+            solo = rh.GpuEngine(sys, st, rh.verlet_with_skin(sys, st.n(), 0.3))
+        solo.step(k)
+        got, want = b.read_ledger(i), solo.read_ledger()
+        solo.close()
+        for field in ("energy", "impulse", "hits"):
+            assert np.array_equal(getattr(got, field), getattr(want, field)), f"system {i}: ledger {field} {getattr(got, field)} vs {getattr(want, field)}"
+    # the walls and the bath were hit and booked; the periodic systems booked nothing
+    led3 = b.read_ledger(ids[0])  # hard walls are crossed (hits), soft ones push back (impulse)
+    assert led3.hits[[0, 5]].min() > 0 and np.count_nonzero(led3.impulse[[1, 4]]) == 2
+    led2 = b.read_ledger(ids[1])
+    assert led2.hits[:4].sum() > 0 and np.count_nonzero(led2.impulse[:4]) >= 2
+    assert b.read_ledger(ids[2]).hits[6] > 0 and b.read_ledger(ids[3]).hits[4:6].min() > 0
+    for i in ids[-2:]:
+        assert not b.read_ledger(i).hits.any()
+
+
 def test_alone_and_in_the_mix_agree():
     specs = ensemble(24, meta_seed=99, n_hi=20000)
     built = [s.build() for s in specs]
