@@ -772,7 +772,6 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         if (!sys.walled[a]) d.periodic |= 1u << a;
         else
             for (int side = 0; side < 2; side++) {
-                RH_REQUIRE(D == 3 || sys.wall_kind[a][side] != RH_WALL_THERMAL, "2D heat-bath walls are not supported by the batched engine");
                 if (sys.wall_kind[a][side] == RH_WALL_SOFT) d.soft_walls |= 1u << (2 * a + side);
                 if (sys.wall_kind[a][side] == RH_WALL_THERMAL) {
                     d.thermal_walls |= 1u << (2 * a + side);

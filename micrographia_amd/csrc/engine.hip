@@ -1531,17 +1531,10 @@ __global__ __launch_bounds__(TPB) void k_advance2(Params<2> P, Bufs<2> B, uint32
             const int w = 2 * a + side;
             const double b0 = velocity(P, v[0]), b1 = velocity(P, v[1]);
             if ((P.thermal_walls >> w) & 1u) {
-                // walls::thermal_velocity (walls.rs:91-104)
-                const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_WALL;
-                uint32_t r[2], r2[2];
-                threefry2x32(id, clock, k0, k1, r);
-                threefry2x32(id, clock, k0, k1 ^ 1u, r2);
-                const double u1 = 1.0 - unit_f64(r[0], r[1]);
-                const double u2 = 1.0 - unit_f64(r2[0], 0u), u3 = unit_f64(r2[1], 0u);
-                const double sd = P.wall_sd[w][si];
-                const double inward = side == 0 ? 1.0 : -1.0;
-                v[a] = to_step(P, inward * sd * sqrt(-2.0 * log(u1)));
-                v[1 - a] = to_step(P, sd * sqrt(-2.0 * log(u2)) * cos(TAU_D * u3));
+                double vn, t;
+                thermal_wall_draw2(P, id, clock, w, side, si, vn, t);
+                v[a] = to_step(P, vn);
+                v[1 - a] = to_step(P, t);
             } else {
                 v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
             }

@@ -249,39 +249,20 @@ __device__ __forceinline__ void thermal_wall_draw(const PT& P, uint32_t id, uint
     t1 = rad * sin(ang);
 }
 
-// The 3D wrap / mirror with heat-bath walls and no ledger: a particle that crossed a heat-bath
-// wall leaves it with a drawn velocity, one that crossed a hard or soft wall with its normal
-// velocity reversed.
+// walls::thermal_velocity in 2D (walls.rs:91-104): the normal component from the wall-flux law,
+// the tangential one a normal draw, under the system seed's wall stream.
 template <class PT>
-__device__ __forceinline__ void bound_thermal3(const PT& P, double x[3], int32_t v[3], uint32_t id, uint32_t clock, uint32_t s) {
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const double l = P.boxf[a];
-        if ((P.periodic >> a) & 1u) {
-            if (x[a] < 0.0) x[a] += l;
-            else if (x[a] >= l) x[a] -= l;
-            continue;
-        }
-        int side = -1;
-        if (x[a] < 0.0) {
-            x[a] = -1.0 - x[a];
-            side = 0;
-        } else if (x[a] >= l) {
-            x[a] = 2.0 * l - 1.0 - x[a];
-            side = 1;
-        }
-        if (side < 0) continue;
-        const int w = 2 * a + side;
-        if ((P.thermal_walls >> w) & 1u) {
-            double vn, t0, t1;
-            thermal_wall_draw(P, id, clock, w, side, s, vn, t0, t1);
-            v[a] = to_step(P, vn);
-            v[(a + 1) % 3] = to_step(P, t0);
-            v[(a + 2) % 3] = to_step(P, t1);
-        } else {
-            v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
-        }
-    }
+__device__ __forceinline__ void thermal_wall_draw2(const PT& P, uint32_t id, uint32_t clock, int w, int side, uint32_t s, double& vn, double& t) {
+    const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_WALL;
+    uint32_t r[2], r2[2];
+    threefry2x32(id, clock, k0, k1, r);
+    threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+    const double u1 = 1.0 - unit_f64(r[0], r[1]);
+    const double u2 = 1.0 - unit_f64(r2[0], 0u), u3 = unit_f64(r2[1], 0u);
+    const double sd = P.wall_sd[w][s];
+    const double inward = side == 0 ? 1.0 : -1.0;
+    vn = inward * sd * sqrt(-2.0 * log(u1));
+    t = sd * sqrt(-2.0 * log(u2)) * cos(TAU_D * u3);
 }
 
 template <int D, class PT>
@@ -494,15 +475,18 @@ __device__ void bound_booked(const PT& P, unsigned long long* L, double x[D], in
         double before[D], after[D];
 #pragma unroll
         for (int b = 0; b < D; b++) before[b] = velocity(P, v[b]);
-        bool thermal = false;
-        if constexpr (D == 3) thermal = (P.thermal_walls >> w) & 1u;
-        if (thermal) {
+        if ((P.thermal_walls >> w) & 1u) {
             if constexpr (D == 3) {
                 double vn, t0, t1;
                 thermal_wall_draw(P, id, clock, w, side, s, vn, t0, t1);
                 v[a] = to_step(P, vn);
                 v[(a + 1) % 3] = to_step(P, t0);
                 v[(a + 2) % 3] = to_step(P, t1);
+            } else {
+                double vn, t;
+                thermal_wall_draw2(P, id, clock, w, side, s, vn, t);
+                v[a] = to_step(P, vn);
+                v[1 - a] = to_step(P, t);
             }
         } else {
             v[a] = (int32_t)((uint32_t)0 - (uint32_t)v[a]);
