@@ -263,8 +263,9 @@ int rh_batch_create(uint64_t budget_bytes, uint32_t max_systems, rh_batch** out)
 int rh_batch_destroy(rh_batch* b);
 /* Admits a system with `steps` steps to run, Verlet skin `skin`; *id = -1 when it does not fit
  * in the free arena space now (the admission rule).  Systems may have hard, soft and heat-bath
- * walls, 3D ones the Andersen bath, all with the ledger booked as the single-system engines book
- * it; the 2D thermostat region, compartments, the membrane, the piston and bonds are refused. */
+ * walls, 3D ones the Andersen bath and 2D ones the thermostat region (fan, bath layer), all with
+ * the ledger booked as the single-system engines book it; compartments, the membrane, the piston
+ * and bonds are refused. */
 int rh_batch_add3(rh_batch* b, const rh_system3* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id);
 int rh_batch_add2(rh_batch* b, const rh_system2* sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps, int32_t* id);
 int rh_batch_remove(rh_batch* b, int32_t id);

@@ -38,6 +38,10 @@ struct BSys {
     uint32_t ledger, bath, bath_species, thermal_walls, seed_lo, seed_hi;
     double dt, bath_p;
     double mass[4];        // per species, for the energies the ledger books
+    // the 2D thermostat region: along bath_axis in [bath_lo, bath_hi), the fan's drift along x
+    uint32_t bath_axis;
+    int32_t bath_drift;
+    double bath_lo, bath_hi;
     double bath_sd[4];     // sqrt(T / m) of the bath per species (sigma/tau)
     double wall_sd[6][4];  // sqrt(T / m) per heat-bath wall and species
     PairTable pt;

@@ -558,7 +558,8 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     for (int a = 0; a < D; a++) v[a] = (int32_t)((uint32_t)B.vel[pm][a][slot] + k[a] + (pin ? 0u : (uint32_t)S.gravity[a]));
     if constexpr (BOOK) {
         // as the single-system engine's booked leapfrog: the soft walls' impulses, the Andersen
-        // bath, then the move and the walls (heat-bath ones drawing the velocity), every
+        // bath (3D) or the thermostat region (2D), then the move and the walls (heat-bath ones
+        // drawing the velocity), every
         // exchange booked in the system's ledger (the step's clock is the system's before this
         // step is committed)
         unsigned long long* L = B.ledger + (size_t)sysid * LEDGER_WORDS;
@@ -566,8 +567,10 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
 #pragma unroll
         for (int w = 0; w < 2 * D; w++)
             if (wf[w] != 0.0f) ledger_add(L, w, 1, (double)wf[w] * S.dt);
-        if constexpr (D == 3)
-            if (S.bath) bath_redraw<3>(S, L, id, clock, si, v);
+        if (S.bath) {
+            if constexpr (D == 3) bath_redraw<3>(S, L, id, clock, si, v);
+            else thermostat_redraw2(S, L, id, clock, si, xi, v);
+        }
 #pragma unroll
         for (int a = 0; a < D; a++) x[a] = xi[a] + (double)v[a];
         bound_booked<D>(S, L, x, v, id, clock, si);
@@ -722,7 +725,6 @@ void Batch::budget(uint64_t* slot_cap, uint64_t* list_cap, uint64_t* cell_cap) c
 
 // Cells, list capacity and constants of one system, as Params::new (engine3.rs:70-162).
 void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32_t* cap) const {
-    RH_REQUIRE(!sys.has_bath || sys.dim == 3, "the 2D thermostat region is not supported by the batched engine");
     RH_REQUIRE(!sys.compartments[0] && !sys.compartments[1] && !sys.has_membrane && !sys.has_piston && sys.bonds.empty(),
                "compartments, membranes, pistons and bonds are not supported by the batched engine");
     const int D = sys.dim;
@@ -792,6 +794,12 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         d.bath_species = sys.bath_species;
         d.bath_p = sys.bath_rate * sys.dt;
         for (uint32_t sp = 0; sp < sys.nspecies; sp++) d.bath_sd[sp] = std::sqrt(sys.bath_t / sys.masses[sp]);
+        if (D == 2) {
+            d.bath_axis = sys.bath_axis;
+            d.bath_drift = core::to_step_displacement(sys.dt, sys.bath_drift);
+            d.bath_lo = (double)sys.bath_region[0];
+            d.bath_hi = (double)sys.bath_region[1];
+        }
     }
     RH_REQUIRE(ncells < (uint64_t(1) << 31), "too many cells");
     d.ncells = (uint32_t)ncells;

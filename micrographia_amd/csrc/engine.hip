@@ -1460,20 +1460,11 @@ __global__ __launch_bounds__(TPB) void k_advance2(Params<2> P, Bufs<2> B, uint32
     }
     const double xb = P.bath_axis ? xi[1] : xi[0];
     if (P.bath && !pin && ((P.bath_species >> si) & 1u) && xb >= P.bath_lo && xb < P.bath_hi) {
-        const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_THERMO;
-        uint32_t r[2];
-        threefry2x32(id, clock, k0, k1, r);
-        if (unit_f64(r[0], 0u) < P.bath_p) {
-            uint32_t r2[2];
-            threefry2x32(id, clock, k0, k1 ^ 1u, r2);
-            const double u1 = 1.0 - unit_f64(r[1], 0u);
-            const double u2 = unit_f64(r2[0], 0u);
-            const double sd = P.bath_sd[si];
-            const double rad = sqrt(-2.0 * log(u1));
-            const double ang = TAU_D * u2;
+        double u[2];
+        if (thermostat_draw2(P, id, clock, si, u)) {
             const double b0 = velocity(P, v[0]), b1 = velocity(P, v[1]);
-            v[0] = to_step(P, sd * rad * cos(ang));
-            v[1] = to_step(P, sd * rad * sin(ang));
+            v[0] = to_step(P, u[0]);
+            v[1] = to_step(P, u[1]);
             v[0] = (int32_t)((uint32_t)v[0] + (uint32_t)P.bath_drift);
             if (P.piston && !P.piston_free) {
                 const double f = fmin(fmax((xi[1] * 0x1p-32) / (pw * 0x1p-32), 0.0), 1.0);

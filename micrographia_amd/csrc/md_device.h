@@ -249,6 +249,27 @@ __device__ __forceinline__ void thermal_wall_draw(const PT& P, uint32_t id, uint
     t1 = rad * sin(ang);
 }
 
+// md_core::System's thermostat draw in 2D (the thermostat region of int_engine.rs's step): with
+// probability rate dt a fresh Maxwell velocity u (sigma/tau) from one Box-Muller pair; false
+// otherwise.  The caller checks the region, the species and pinning.
+template <class PT>
+__device__ __forceinline__ bool thermostat_draw2(const PT& P, uint32_t id, uint32_t clock, uint32_t s, double u[2]) {
+    const uint32_t k0 = P.seed_lo, k1 = P.seed_hi ^ STREAM_THERMO;
+    uint32_t r[2];
+    threefry2x32(id, clock, k0, k1, r);
+    if (!(unit_f64(r[0], 0u) < P.bath_p)) return false;
+    uint32_t r2[2];
+    threefry2x32(id, clock, k0, k1 ^ 1u, r2);
+    const double u1 = 1.0 - unit_f64(r[1], 0u);
+    const double u2 = unit_f64(r2[0], 0u);
+    const double sd = P.bath_sd[s];
+    const double rad = sqrt(-2.0 * log(u1));
+    const double ang = TAU_D * u2;
+    u[0] = sd * rad * cos(ang);
+    u[1] = sd * rad * sin(ang);
+    return true;
+}
+
 // walls::thermal_velocity in 2D (walls.rs:91-104): the normal component from the wall-flux law,
 // the tangential one a normal draw, under the system seed's wall stream.
 template <class PT>
@@ -446,6 +467,26 @@ __device__ void bath_redraw(const PT& P, unsigned long long* L, uint32_t id, uin
         ledger_add(L, 6, 0, 0.5 * P.mass[s] * (after - before));
         atomicAdd(&L[3 * 6 + 2], 1ull);
     }
+}
+
+// The 2D thermostat region with its fan: a particle of a coupled, unpinned species whose
+// position along bath_axis is in [bath_lo, bath_hi) may take a fresh velocity plus the fan's
+// drift along x, booked as the thermostat's (entry 6) energy and hit.  (Without a piston: the
+// batched engine runs no piston.)
+template <class PT>
+__device__ void thermostat_redraw2(const PT& P, unsigned long long* L, uint32_t id, uint32_t clock, uint32_t s, const double xi[2], int32_t v[2]) {
+    const double xb = P.bath_axis ? xi[1] : xi[0];
+    if (((P.pinned >> s) & 1u) || !((P.bath_species >> s) & 1u) || !(xb >= P.bath_lo && xb < P.bath_hi)) return;
+    double u[2];
+    if (!thermostat_draw2(P, id, clock, s, u)) return;
+    const double b0 = velocity(P, v[0]), b1 = velocity(P, v[1]);
+    v[0] = to_step(P, u[0]);
+    v[1] = to_step(P, u[1]);
+    v[0] = (int32_t)((uint32_t)v[0] + (uint32_t)P.bath_drift);
+    const double a0 = velocity(P, v[0]), a1 = velocity(P, v[1]);
+    const double m = P.mass[s];
+    ledger_add(L, 6, 0, 0.5 * m * (a0 * a0 + a1 * a1 - b0 * b0 - b1 * b1));
+    atomicAdd(&L[3 * 6 + 2], 1ull);
 }
 
 // Wrap (periodic axes) or mirror about -1/2 and L - 1/2 (walls, walls.rs:27-35); a particle that

@@ -7,7 +7,7 @@ and cell ranges fit in the free memory (the admission rule is the device-memory 
 inside the shared launches with every other admitted request and leave when their steps are
 done, freeing their ranges for waiting requests.  Requests large enough to fill the GPU alone run
 in the single-system engine on a second lane, and so do requests whose systems the batched engine
-does not run (the 2D thermostat region, compartments, membrane, piston, bonds).  State stays on the device from admission to
+does not run (compartments, membrane, piston, bonds).  State stays on the device from admission to
 completion; only the final state is read back.  Each request's result is bit-identical to running
 it alone.
 """
@@ -217,8 +217,8 @@ class Server:
                     except RahmanError as err:
                         if err.code != RH_ERR_UNSUPPORTED:
                             raise  # a real failure, not a refusal
-                        # a system the batched engine does not run (the 2D thermostat region,
-                        # compartments, membrane, piston, bonds): the single-system lane takes it
+                        # a system the batched engine does not run (compartments, membrane,
+                        # piston, bonds): the single-system lane takes it
                         with big_cv:
                             big_queue.append((r, fut))
                             big_cv.notify()

@@ -114,8 +114,9 @@ def squeeze(st, box_len, axis):
 
 def booked_systems():
     """Systems whose steps book the ledger: a 3D liquid between hard and soft walls on two axes,
-    a 2D gas between soft walls and hard walls under gravity, the stochastic 3D systems, and a
-    2D gas between heat-bath walls at 2.0 and 0.5 under gravity."""
+    a 2D gas between soft walls and hard walls under gravity, the stochastic 3D systems, a 2D
+    gas between heat-bath walls at 2.0 and 0.5 under gravity, and 2D thermostat regions: a fan
+    (a slab along x with a drift) and a bath layer (a slab along y)."""
     sys, st = make_system(3, 2000, 0.5, 1.5, 31)
     sys = sys.with_walls(2, rh.Wall.Soft, rh.Wall.Hard).with_walls(0, rh.Wall.Hard, rh.Wall.Soft)
     squeeze(st, sys.box_len, 0)
@@ -130,7 +131,11 @@ def booked_systems():
     sys = sys.with_walls(1, rh.thermal_wall(2.0), rh.thermal_wall(0.5)).with_gravity((0.0, -0.1)).with_seed(80)
     squeeze(st, sys.box_len, 1)
     heat2 = (sys, st)
-    return [walls3, walls2] + stochastic_systems() + [heat2]
+    sys, st = make_system(2, 3000, 0.6, 0.6, 36)
+    fan = (sys.with_thermostat(1.5, 4.0).with_fan(0b1, 2.0, 20.0, 0.3).with_seed(81), st)  # species mask
+    sys, st = make_system(2, 3000, 0.6, 0.6, 37)
+    layer = (sys.with_thermostat(1.8, 3.0).with_bath_layer(0.0, 30.0).with_seed(82), st)
+    return [walls3, walls2] + stochastic_systems() + [heat2, fan, layer]
 
 
 @pytest.mark.parametrize("graph", [0, 50])
@@ -142,7 +147,7 @@ def test_ledgers_equal_the_systems_run_alone(graph):
     booked = booked_systems()
     plain = [make_system(3, 1500, 0.8442, 1.2, 33), make_system(2, 3000, 0.7, 1.0, 34)]
     b = BatchEngine(budget_bytes=2 * GB, max_systems=16, graph_chunk=graph)
-    steps = [300, 300, 150, 300, 200, 300, 150, 150]
+    steps = [300, 300, 150, 300, 200, 300, 200, 200, 150, 150]
     systems = booked + plain
     ids = [b.add(sys, st, k, skin=0.3) for (sys, st), k in zip(systems, steps)]
     assert all(i >= 0 for i in ids)
@@ -165,6 +170,7 @@ def test_ledgers_equal_the_systems_run_alone(graph):
     assert led2.hits[:4].sum() > 0 and np.count_nonzero(led2.impulse[:4]) >= 2
     assert b.read_ledger(ids[2]).hits[6] > 0 and b.read_ledger(ids[3]).hits[4:6].min() > 0
     assert b.read_ledger(ids[5]).hits[2:4].min() > 0  # the 2D heat-bath walls
+    assert b.read_ledger(ids[6]).hits[6] > 0 and b.read_ledger(ids[7]).hits[6] > 0  # the 2D thermostat regions
     for i in ids[-2:]:
         assert not b.read_ledger(i).hits.any()
 

@@ -99,20 +99,23 @@ def test_serving_at_full_request_sizes_against_the_oracle():
 
 
 @dataclass
-class ThermostatRequest(Request):
-    """A request whose system has a thermostat: the batched engine runs it in 3D and refuses it
-    in 2D (the thermostat region belongs to the 2D boundary path)."""
+class BoundaryRequest(Request):
+    """A request whose system carries a boundary: in 3D the Andersen bath (the batched engine
+    runs it), in 2D a harmonic bond between its first two particles (neighbours on the start
+    lattice; the batched engine refuses bonds)."""
 
     def build(self):
         sys, st = super().build()
+        if self.dim == 2:
+            return sys.with_bonds([rh.BondType.harmonic(40.0, 1.0)], [(0, 1, 0)]), st
         return sys.with_thermostat(1.2, 2.0), st
 
 
 def test_requests_the_batch_refuses_take_the_single_system_lane():
     reqs = request_stream(600.0, 0.03, meta_seed=31, n_lo=500, n_hi=6000, steps_lo=40, steps_hi=150)
-    reqs = [ThermostatRequest(**{f: getattr(r, f) for f in r.__dataclass_fields__}) if k % 2 == 0 else r for k, r in enumerate(reqs)]
-    assert any(isinstance(r, ThermostatRequest) and r.dim == 2 for r in reqs)
-    assert any(isinstance(r, ThermostatRequest) and r.dim == 3 for r in reqs)
+    reqs = [BoundaryRequest(**{f: getattr(r, f) for f in r.__dataclass_fields__}) if k % 2 == 0 else r for k, r in enumerate(reqs)]
+    assert any(isinstance(r, BoundaryRequest) and r.dim == 2 for r in reqs)
+    assert any(isinstance(r, BoundaryRequest) and r.dim == 3 for r in reqs)
     srv = Server(budget_bytes=1 << 30, max_systems=64, chunk_s=0.005, big_n=1 << 30)
     rep = srv.serve(reqs, realtime=False)
     srv.close()
