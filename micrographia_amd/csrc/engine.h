@@ -289,6 +289,7 @@ class Engine {
     uint64_t row_keys_ = 0;   // extended bins of all rows (row_start has one more entry)
     uint64_t row_bins_ = 0;   // main bins of all rows (row_cnt)
     void* row_scan_temp_ = nullptr;
+    void* key_scan_temp_ = nullptr;  // the row path's counting sort of the cell keys
     void enqueue_extras(uint32_t flags);
     void write_partners();
     void flip(uint32_t sel, int by_species, uint32_t axis, uint32_t bit);

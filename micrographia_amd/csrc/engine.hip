@@ -142,6 +142,74 @@ __global__ __launch_bounds__(TPB) void k_keys(Params<D> P, Bufs<D> B, uint32_t* 
     vals[i] = i;
 }
 
+// Main bin of a position: row (z cell * cells along y + y cell), fine x cell.
+__device__ __forceinline__ void row_bin(const Params<3>& P, const double x[3], uint32_t& row, uint32_t& xf) {
+    uint32_t c[3];
+    cell_of<3>(P, x, c);
+    row = c[2] * P.nc[1] + c[1];
+    const double f = x[0] * P.rows_xscale;
+    const uint32_t v = f > 0.0 ? (uint32_t)f : 0u;
+    xf = v < P.rows_nxf - 1u ? v : P.rows_nxf - 1u;
+}
+
+__global__ __launch_bounds__(TPB) void k_row_count(Params<3> P, Bufs<3> B, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= P.n) return;
+    const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u;
+    double x[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) x[a] = B.pos[pp][a][k];
+    uint32_t row, xf;
+    row_bin(P, x, row, xf);
+    B.row_rank[k] = atomicAdd(&B.row_cnt[row * P.rows_nxf + xf], 1u);
+}
+
+// The rebuild's reorder on the row-build path: a counting sort of the Morton cell keys (one digit
+// of all the key's bits), in place of the radix sort's passes.  Particles of one cell land in
+// the order their atomic ranks fall; no result depends on the order within a cell (kicks are
+// integer sums over the same pairs).
+// The particles arrive in the previous cell order, so a wave's lanes mostly hold runs of equal
+// keys: each run takes its ranks with one atomic (made by the run's first lane) instead of one
+// per particle contending for the same counter.
+template <int D>
+__global__ __launch_bounds__(TPB) void k_keys_count(Params<D> P, Bufs<D> B, uint32_t* keys, uint32_t* ranks, uint32_t* count, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    const bool live = i < P.n;
+    uint32_t key = 0xffffffffu;  // lanes past the end form a run of their own (keys are below 2^30)
+    if (live) {
+        const uint32_t pp = B.ctl[CTL_PPOS];
+        double x[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) x[a] = B.pos[pp][a][i];
+        uint32_t c[D];
+        cell_of<D>(P, x, c);
+        key = morton<D>(c);
+    }
+    const uint32_t lane = __lane_id();
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const uint64_t heads = ballot64(lane == 0u || prev != key);
+    const uint64_t upto = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);  // lanes 0 .. lane
+    const uint32_t start = 63u - (uint32_t)__clzll(heads & upto);
+    const uint64_t after = heads & ~upto;
+    const uint32_t end = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+    uint32_t base = 0u;
+    if (live && lane == start) base = atomicAdd(&count[key], end - start);
+    base = __shfl(base, (int)start, 64);
+    if (!live) return;
+    keys[i] = key;
+    ranks[i] = base + (lane - start);
+}
+
+// perm[first slot of the key + rank] = the particle's current slot
+__global__ __launch_bounds__(TPB) void k_keys_place(uint32_t n, const uint32_t* keys, const uint32_t* ranks, const uint32_t* first, uint32_t* perm, const uint32_t* gate) {
+    if (gate_closed(gate)) return;
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    perm[first[keys[i]] + ranks[i]] = i;
+}
+
 __global__ void k_ranges(const uint32_t* keys, uint32_t n, uint32_t* start, uint32_t* end, const uint32_t* gate) {
     if (gate_closed(gate)) return;
     uint32_t k = blockIdx.x * TPB + threadIdx.x;
@@ -508,28 +576,6 @@ __global__ __launch_bounds__(TPB) void k_build_half(Params<D> P, Bufs<D> B, cons
 // relative to the cell's corner.  The chunk stream written is the stencil build's: the same
 // pairs, tags and wave regions, in another order.
 
-// Main bin of a position: row (z cell * cells along y + y cell), fine x cell.
-__device__ __forceinline__ void row_bin(const Params<3>& P, const double x[3], uint32_t& row, uint32_t& xf) {
-    uint32_t c[3];
-    cell_of<3>(P, x, c);
-    row = c[2] * P.nc[1] + c[1];
-    const double f = x[0] * P.rows_xscale;
-    const uint32_t v = f > 0.0 ? (uint32_t)f : 0u;
-    xf = v < P.rows_nxf - 1u ? v : P.rows_nxf - 1u;
-}
-
-__global__ __launch_bounds__(TPB) void k_row_count(Params<3> P, Bufs<3> B, const uint32_t* gate) {
-    if (gate_closed(gate)) return;
-    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
-    if (k >= P.n) return;
-    const uint32_t pp = B.ctl[CTL_PPOS] ^ 1u;
-    double x[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) x[a] = B.pos[pp][a][k];
-    uint32_t row, xf;
-    row_bin(P, x, row, xf);
-    B.row_rank[k] = atomicAdd(&B.row_cnt[row * P.rows_nxf + xf], 1u);
-}
 
 // Counts of the extended bins (ghost bins take their source bin's count), for the scan; the
 // entry past the last is zeroed, so the scan leaves the total there.
@@ -1677,7 +1723,7 @@ uint64_t Engine<D>::planned_bytes() const {
     t += sz(n * 4);                                                                                  // nbr_count
     t += sz(((n + 4095u) & ~uint64_t(4095)) / 64u * 8);                                              // wave_chunks
     t += sz((n + 32u) * 16);                                                                         // rel
-    if (P_.rows) t += sz((2 * n + 32u) * 16) + sz(row_bins_ * 4) + sz((row_keys_ + 1u) * 4) + sz(n * 4) + sz(scan_temp_bytes(row_keys_ + 1u));
+    if (P_.rows) t += sz((2 * n + 32u) * 16) + sz(row_bins_ * 4) + sz((row_keys_ + 1u) * 4) + sz(n * 4) + sz(scan_temp_bytes(row_keys_ + 1u)) + sz(scan_temp_bytes(ncell_keys_));
     t += 2 * sz(ncell_keys_ * 4);                                                                    // cell_start, cell_end
     t += 2 * sz(n * 8);                                                                              // pe, ke
     t += sz(CTL_WORDS * 4) + sz(LEDGER_WORDS * 8);
@@ -1937,6 +1983,7 @@ void Engine<D>::alloc_rows() {
     B_.row_start = (uint32_t*)dalloc((row_keys_ + 1u) * sizeof(uint32_t));
     B_.row_rank = (uint32_t*)dalloc(n_ * sizeof(uint32_t));
     row_scan_temp_ = dalloc(scan_temp_bytes(row_keys_ + 1u));
+    key_scan_temp_ = dalloc(scan_temp_bytes(ncell_keys_));
 }
 
 template <int D>
@@ -2120,15 +2167,26 @@ void Engine<D>::enqueue_rebuild(bool gated) {
         RH_HIP(hipEventRecord(t0, stream_));
     }
     uint32_t nb = blocks_for(n_, TPB);
-    k_keys<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.keys[0], sort_.vals[0], gate);
-    RH_HIP(hipGetLastError());
-    int r = sort_.sort(n_, key_bits_, gate, stream_);
     const bool rows = mode_.kind == 1 && row_path();
-    k_gather<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], 1, gate);
-    RH_HIP(hipGetLastError());
-    if (!rows) {
-        // the cell ranges of the new order (the row build works from its own row-ordered copy,
-        // and readouts that need the ranges recompute them)
+    if (rows) {
+        // counting sort of the cell keys (the cell-start array holds the counts, then the first
+        // slot of each key), and the reorder
+        fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
+        k_keys_count<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.keys[0], sort_.vals[0], B_.cell_start, gate);
+        RH_HIP(hipGetLastError());
+        exclusive_scan_u32(B_.cell_start, B_.cell_start, ncell_keys_, key_scan_temp_, gate, stream_);
+        k_keys_place<<<nb, TPB, 0, stream_>>>((uint32_t)n_, sort_.keys[0], sort_.vals[0], B_.cell_start, sort_.vals[1], gate);
+        RH_HIP(hipGetLastError());
+        k_gather<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[1], 1, gate);
+        RH_HIP(hipGetLastError());
+    } else {
+        k_keys<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.keys[0], sort_.vals[0], gate);
+        RH_HIP(hipGetLastError());
+        int r = sort_.sort(n_, key_bits_, gate, stream_);
+        k_gather<D><<<nb, TPB, 0, stream_>>>(P_, B_, sort_.vals[r], 1, gate);
+        RH_HIP(hipGetLastError());
+        // the cell ranges of the new order, for the stencil and full-list builds (readouts that
+        // need the ranges recompute them)
         fill_u32(B_.cell_start, 0u, ncell_keys_, gate, stream_);
         fill_u32(B_.cell_end, 0u, ncell_keys_, gate, stream_);
         k_ranges<<<nb, TPB, 0, stream_>>>(sort_.keys[r], (uint32_t)n_, B_.cell_start, B_.cell_end, gate);
