@@ -75,15 +75,33 @@ __global__ __launch_bounds__(64) void k_bb_decide(const BSys* S, BCtl* C, uint32
     }
 }
 
+// A slot's cell and its rank in the cell.  The slots arrive in the previous cell order, so a
+// wave's lanes mostly hold runs of equal cells: each run takes its ranks with one atomic (made by
+// the run's first lane) instead of one per slot contending for the same counter.  Every lane of
+// the wave takes part; lanes past the system's end count nothing.
 template <int D>
-__device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C, uint32_t slot) {
-    double x[D];
-    for (int a = 0; a < D; a++) x[a] = B.pos[C.ppos][a][slot];
-    uint32_t c[D];
-    cell_of<D>(S, x, c);
-    uint32_t key = lin_cell<D>(S, c);
+__device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C, uint32_t slot, bool live) {
+    uint32_t key = 0xffffffffu;  // lanes past the end form a run of their own
+    if (live) {
+        double x[D];
+        for (int a = 0; a < D; a++) x[a] = B.pos[C.ppos][a][slot];
+        uint32_t c[D];
+        cell_of<D>(S, x, c);
+        key = lin_cell<D>(S, c);
+    }
+    const uint32_t lane = __lane_id();
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const uint64_t heads = ballot64(lane == 0u || prev != key);
+    const uint64_t upto = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);  // lanes 0 .. lane
+    const uint32_t start = 63u - (uint32_t)__clzll(heads & upto);
+    const uint64_t after = heads & ~upto;
+    const uint32_t end = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+    uint32_t base = 0u;
+    if (live && lane == start) base = atomicAdd(&B.cell_count[S.cell0 + key], end - start);
+    base = __shfl(base, (int)start, 64);
+    if (!live) return;
     B.key[slot] = key;
-    B.rank[slot] = atomicAdd(&B.cell_count[S.cell0 + key], 1u);
+    B.rank[slot] = base + (lane - start);
 }
 
 // The per-slot rebuild kernels loop over the units of the list they are given (work-group b
@@ -96,9 +114,9 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B, uint32_t off) {
         const BSys& S = B.sys[s];
         const BCtl& C = B.ctl[s];
         const uint32_t slot = u * BATCH_WG + threadIdx.x;
-        if (slot - S.slot0 >= S.n) continue;
-        if (S.dim == 3) bb_count<3>(B, S, C, slot);
-        else bb_count<2>(B, S, C, slot);
+        const bool live = slot - S.slot0 < S.n;
+        if (S.dim == 3) bb_count<3>(B, S, C, slot, live);
+        else bb_count<2>(B, S, C, slot, live);
     }
 }
 
