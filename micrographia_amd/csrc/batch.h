@@ -60,7 +60,7 @@ struct BBufs {
     uint32_t* ids[2];
     double* ref[3];
     double4* aos[2];  // positions in sixteenths packed per slot (x, y, z or 0, 0), for the step's neighbour gathers
-    uint32_t* wave_chunks;  // list chunks in each 64-slot wave's region
+    uint32_t* wave_chunks;  // in-block and cross-block list chunks in each 64-slot wave's region (2 per wave)
     float4* rel;  // position relative to its cell's corner (sigma, f32), for list builds
     uint32_t* key;
     uint32_t* rank;

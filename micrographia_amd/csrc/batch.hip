@@ -28,7 +28,7 @@ namespace rh {
 
 constexpr uint32_t LIST_UNIT = 4096;  // list-arena allocation unit (entries)
 constexpr uint32_t CELL_UNIT = 256;   // cell-arena allocation unit (cells)
-constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 1;  // per slot: pos, vel, species+ids, ref, aos, rel, key+rank, wave counts (1/16, rounded up)
+constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 1;  // per slot: pos, vel, species+ids, ref, aos, rel, key+rank, wave counts (1/8, rounded up)
 
 // ---------------------------------------------------------------------------------------------
 // Kernels
@@ -216,12 +216,12 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B, uint32_t off) 
 // The list of local slot i (in the freshly scattered buffers): f32 cell-relative separations
 // against each species pair's (rc + skin)(1 + 1e-4); an in-block pair (same block of the
 // system's block size) belongs to its lower-index end, pairs across blocks are listed at both
-// ends; variant 0 lists
-// every pair at both ends.  As in the single-system engine: candidates four at a time in groups
-// of 32 through a distance mask, the candidate and hit loops in step across the wave, and full
-// four-entry chunks (tagged with the owner's index in its block, entries - 1 and in-block, in
-// the top bits of their indices) stored densely, in fill order, in the wave's region of
-// 64 cap/4 chunks at nbr0; the wave's chunk count goes to wave_chunks.
+// ends; variant 0 lists every pair at both ends.  As in the single-system engine: candidates
+// four at a time in groups of 32 through a distance mask, the candidate and hit loops in step
+// across the wave, and full chunks stored densely in the wave's region of 64 cap/4 chunks at
+// nbr0 (four indices tagged with the owner's index in its block and entries - 1): in-block
+// chunks from the region's start, cross-block chunks from its end down; the wave's two chunk
+// counts go to wave_chunks[2 w], wave_chunks[2 w + 1].
 template <int D, bool MULTI>
 __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local) {
     const uint32_t variant = S.variant;
@@ -241,17 +241,22 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     const uint32_t own = local & (S.block - 1u);
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + (size_t)(local >> 6) * capw;
     const uint64_t lt = lanemask_lt();
-    uint32_t fill = 0u;  // changes only where every lane of the wave is present
-    auto store = [&](bool want, const uint4& q, uint32_t entries, bool in) {
+    // the fills change only where every lane of the wave is present; a store that would meet the
+    // other end's chunks flags the overflow instead
+    uint32_t fill_in = 0u, fill_out = 0u;
+    auto store = [&](auto in_c, bool want, const uint4& q) __attribute__((always_inline)) {
+        constexpr bool in = decltype(in_c)::value;
+        uint32_t& fill = in ? fill_in : fill_out;
         const uint64_t bal = ballot64(want);
         if (bal == 0ull) return;
         if (want) {
-            const uint32_t at = fill + (uint32_t)__popcll(bal & lt);
-            if (at < capw) chunks[at] = tag_chunk(q, own, entries, in);
+            const uint32_t k = fill + (uint32_t)__popcll(bal & lt), other = in ? fill_out : fill_in;
+            if (k + other < capw) chunks[in ? k : capw - 1u - k] = q;
             else atomicOr(&C.overflow, 1u);
         }
         fill += (uint32_t)__popcll(bal);
     };
+    // the open in-block and cross-block chunks
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
     const uint32_t* cs = B.cell_start + S.cell0;
     const uint32_t* ce = B.cell_end + S.cell0;
@@ -274,15 +279,13 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
             m0 &= m0 - 1u;
             const uint32_t j = b0 + b;
             in = variant == 1 && j - blo < S.block;
-            if (in) {
-                put4(fa, na, j);
-                na++;
-                full = (na & 3u) == 0u;
-            } else {
-                put4(fb, nb, j);
-                nb++;
-                full = (nb & 3u) == 0u;
-            }
+            // the two counts step by selects (a branch on `in` around them is turned into a
+            // per-lane indexed pair in scratch memory)
+            if (in) put4(fa, na, j);
+            else put4(fb, nb, j);
+            na += in ? 1u : 0u;
+            nb += in ? 0u : 1u;
+            full = ((in ? na : nb) & 3u) == 0u;
             if (m0 == 0u) {
                 m0 = m1;
                 b0 = b1;
@@ -292,7 +295,9 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
                 qn--;
             }
         }
-        store(full, in ? fa : fb, 4u, in);
+        const bool fin = full && in, fout = full && !in;
+        store(std::true_type{}, fin, tag_chunk(fa, own, 4u, true));
+        store(std::false_type{}, fout, tag_chunk(fb, own, 4u, false));
     };
     for (int o = 0; o < stencil_size(D); o++) {
         const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
@@ -376,18 +381,23 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
         }
     }
     while (ballot64(qn != 0u) != 0ull) visit();
-    store((na & 3u) != 0u, fa, na & 3u, true);
-    store((nb & 3u) != 0u, fb, nb & 3u, false);
+    const bool pin = (na & 3u) != 0u, pout = (nb & 3u) != 0u;
+    store(std::true_type{}, pin, tag_chunk(fa, own, pin ? na & 3u : 1u, true));
+    store(std::false_type{}, pout, tag_chunk(fb, own, pout ? nb & 3u : 1u, false));
     const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
     if (ca + cb > cap4) atomicOr(&C.overflow, 1u);
     if (4u * (ca + cb) > *(volatile const uint32_t*)&C.maxcount) atomicMax(&C.maxcount, 4u * (ca + cb));
-    if (__lane_id() == 0u) B.wave_chunks[slot >> 6] = min(fill, capw);
+    if (__lane_id() == 0u) {
+        const uint32_t in = min(fill_in, capw);
+        B.wave_chunks[2u * (slot >> 6)] = in;
+        B.wave_chunks[2u * (slot >> 6) + 1u] = min(fill_out, capw - in);
+    }
 }
 
 // One launch per build kind present (each keeps only its own registers), over that kind's
 // rebuild list.
 template <int D, bool MULTI>
-__global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t off) {
+__global__ __launch_bounds__(BATCH_WG) __attribute__((amdgpu_waves_per_eu(5))) void k_bb_build(BBufs B, uint32_t off) {
     constexpr uint32_t K = (D == 3 ? 2u : 0u) + (MULTI ? 1u : 0u);
     const uint32_t len = *(volatile const uint32_t*)&B.rb_count[K];
     for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
@@ -398,7 +408,11 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_build(BBufs B, uint32_t off) {
         const uint32_t local = u * BATCH_WG + threadIdx.x - S.slot0;
         if (local >= S.n) {
             // a wave past the system's last particle holds no chunks
-            if ((threadIdx.x & 63u) == 0u) B.wave_chunks[(u * BATCH_WG + threadIdx.x) >> 6] = 0u;
+            if ((threadIdx.x & 63u) == 0u) {
+                const uint32_t w = (u * BATCH_WG + threadIdx.x) >> 6;
+                B.wave_chunks[2u * w] = 0u;
+                B.wave_chunks[2u * w + 1u] = 0u;
+            }
             continue;
         }
         bb_build<D, MULTI>(B, S, C, local);
@@ -422,15 +436,17 @@ __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
 __device__ __forceinline__ bool bb_runs(const BSys& S, const BCtl& C) { return S.active && (C.stale | C.overflow) == 0u && C.remaining != 0u; }
 
 // One work-group of BS slots, a block of the system: as the single-system step kernel, the
-// block's chunks (its waves' regions, numbered in order) are dealt out evenly over the lanes,
-// every kick goes to the block's LDS accumulators, and positions are staged and gathered as
-// sixteenths.
+// block's chunks of each kind (its waves' regions, numbered in order) are dealt out evenly over
+// the lanes, in-block chunks first (partners read from LDS), then cross-block ones (partners
+// gathered); every kick goes to the block's LDS accumulators, and positions are staged and
+// gathered as sixteenths.
 template <int D, bool MULTI, bool BOOK, uint32_t BS>
 __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32_t sysid, uint32_t wg) {
     constexpr uint32_t NW = BS / 64u;
     __shared__ uint32_t acc[D][BS];
     __shared__ double lpos[D][BS];  // the block's positions in sixteenths, for in-block neighbours
-    __shared__ uint32_t wfirst[NW + 1];
+    __shared__ uint32_t wfirst[2][NW + 1];
+    __shared__ uint32_t edge;
     const uint32_t t = threadIdx.x;
     const uint32_t slot = wg * BS + t;
     const uint32_t local = slot - S.slot0;
@@ -439,23 +455,36 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     const uint32_t pp = C.ppos, pm = C.pmeta;
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a] + S.slot0;
+    if (t == 0u) edge = S.range_check;
+    // Periodic axes on which the block needs no image: with every particle of the block at least
+    // 8 sigma (the bound on list separations) from both faces, every partner's stored position is
+    // its nearest image and separation16's correction is exactly zero.
+    uint32_t near_face = 0u;
 #pragma unroll
     for (int a = 0; a < D; a++) {
+        const double x16 = live ? px[a][local] * 0.0625 : 0.0;
         acc[a][t] = 0u;
-        lpos[a][t] = live ? px[a][local] * 0.0625 : 0.0;
+        lpos[a][t] = x16;
+        if (live && ((S.periodic >> a) & 1u) && !(x16 >= 0x1p31 && x16 <= S.box16[a] - 0x1p31)) near_face = 1u;
     }
     if (t < 64u) {
-        const uint32_t v = t < NW ? B.wave_chunks[(wg * BS >> 6) + t] : 0u;
-        uint32_t x = v;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, 64);
-            if (t >= (uint32_t)d) x += y;
+        for (uint32_t kind = 0; kind < 2u; kind++) {
+            const uint32_t v = t < NW ? B.wave_chunks[2u * ((wg * BS >> 6) + t) + kind] : 0u;
+            uint32_t x = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (t >= (uint32_t)d) x += y;
+            }
+            if (t < NW) wfirst[kind][t] = x - v;
+            if (t == NW - 1u) wfirst[kind][NW] = x;
         }
-        if (t < NW) wfirst[t] = x - v;
-        if (t == NW - 1u) wfirst[NW] = x;
     }
     __syncthreads();
+    if (ballot64(near_face != 0u) != 0ull && __lane_id() == 0u) atomicOr(&edge, 1u);
+    __syncthreads();
+    const bool images = __builtin_amdgcn_readfirstlane(edge) != 0u;
     const uint32_t* sp = B.species[pm] + S.slot0;
     const PairTable& pt = S.pt;
     {
@@ -464,41 +493,114 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
         const uint4* __restrict__ chunks = reinterpret_cast<const uint4*>(B.nbr + S.nbr0) + (size_t)(base >> 6) * capw;
         const double4* __restrict__ pq = B.aos[pp] + S.slot0;
         const bool rc = S.range_check != 0;
-        const uint32_t nch = wfirst[NW];
-        uint32_t g = t, w = 0u;
-        auto at = [&](uint32_t gg) -> size_t {
-            while (gg >= wfirst[w + 1u]) w++;
-            return (size_t)w * capw + (gg - wfirst[w]);
-        };
-        uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
-        if (g < nch) q_next = chunks[at(g)];
-        for (; g < nch; g += BS) {
-            const uint4 qt = q_next;
-            if (g + BS < nch) q_next = chunks[at(g + BS)];
-            uint32_t own, m;
-            bool inblock;
-            const uint4 q = untag_chunk(qt, own, m, inblock);
-            const uint32_t i = base + own;
-            double xi[D];
+        // one pair: its separation (with or without the periodic image), its kick on the owner
+        // (summed into k) and its kick on the partner (kj); false beyond the cutoff or, on a
+        // system whose list separations may reach 8 sigma, outside the reference's range
+        auto pair = [&](auto with_images, const double xi[D], const double xs[D], uint32_t si, uint32_t j, float c, uint32_t k[D], int32_t kj[D]) -> bool {
+            float f[D];
+            double d[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
-            uint32_t si = 0;
-            float c = S.kick_c[0];
+            for (int a = 0; a < D; a++) {
+                d[a] = decltype(with_images)::value ? separation16<D>(S, xi[a], xs[a], a) : xs[a] - xi[a];
+                f[a] = sep_f16(d[a]);
+            }
+            if constexpr (decltype(with_images)::value) {
+                if (rc) {
+                    // the reference's range test on the separation (|d| < 8 sigma, in sixteenths)
+                    bool ok = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) ok &= d[a] >= -2147483648.0 && d[a] < 2147483648.0;
+                    if (!ok) return false;
+                }
+            }
+            if constexpr (MULTI) {
+                const uint32_t sj = sp[j], p = si * 4 + sj;
+                return pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
+            } else {
+                return pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
+            }
+        };
+        auto own_of = [&](uint32_t own, uint32_t& si, float& c) {
+            si = 0u;
+            c = S.kick_c[0];
             if (MULTI) {
-                si = sp[i];
+                si = sp[base + own];
                 c = pt.kick_c[si];
             }
-            uint32_t k[D];
+        };
+        // In-block chunks: owner and partners both in LDS, every kick to the LDS accumulators.
+        auto inblock_pass = [&](auto with_images) {
+            const uint32_t nch = wfirst[0][NW];
+            uint32_t g = t, w = 0u;
+            auto at = [&](uint32_t gg) -> size_t {
+                while (gg >= wfirst[0][w + 1u]) w++;
+                return (size_t)w * capw + (gg - wfirst[0][w]);
+            };
+            // the next chunk is in flight while this one is evaluated
+            uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+            if (g < nch) q_next = chunks[at(g)];
+            for (; g < nch; g += BS) {
+                const uint4 qc = q_next;
+                if (g + BS < nch) q_next = chunks[at(g + BS)];
+                uint32_t own, m;
+                bool inblock;
+                const uint4 q = untag_chunk(qc, own, m, inblock);
+                const uint32_t e[4] = {q.x - base, q.y - base, q.z - base, q.w - base};
+                double xi[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) k[a] = 0u;
-            const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
-            double xs[4][D];
-            if (inblock) {
+                for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
+                uint32_t si;
+                float c;
+                own_of(own, si, c);
+                uint32_t k[D];
 #pragma unroll
-                for (int e = 0; e < 4; e++)
+                for (int a = 0; a < D; a++) k[a] = 0u;
 #pragma unroll
-                    for (int a = 0; a < D; a++) xs[e][a] = lpos[a][js[e] - base];
-            } else {
+                for (int s2 = 0; s2 < 4; s2++) {
+                    if ((uint32_t)s2 >= m) break;
+                    const uint32_t lj = e[s2];
+                    double xs[D];
+#pragma unroll
+                    for (int a = 0; a < D; a++) xs[a] = lpos[a][lj];
+                    int32_t kj[D];
+                    if (pair(with_images, xi, xs, si, base + lj, c, k, kj)) {
+#pragma unroll
+                        for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
+            }
+        };
+        // Cross-block chunks (and every chunk of a full list): partners gathered from the packed
+        // positions, owner kicks only (each such pair is listed at both its ends).
+        auto cross_pass = [&](auto with_images) {
+            const uint32_t nch = wfirst[1][NW];
+            uint32_t g = t, w = 0u;
+            auto at = [&](uint32_t gg) -> size_t {
+                while (gg >= wfirst[1][w + 1u]) w++;
+                return (size_t)w * capw + capw - 1u - (gg - wfirst[1][w]);
+            };
+            uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+            if (g < nch) q_next = chunks[at(g)];
+            for (; g < nch; g += BS) {
+                const uint4 qc = q_next;
+                if (g + BS < nch) q_next = chunks[at(g + BS)];
+                uint32_t own, m;
+                bool inblock;
+                const uint4 q = untag_chunk(qc, own, m, inblock);
+                const uint32_t i = base + own;
+                double xi[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) xi[a] = lpos[a][own];
+                uint32_t si;
+                float c;
+                own_of(own, si, c);
+                uint32_t k[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) k[a] = 0u;
+                const uint32_t js[4] = {q.x, m > 1u ? q.y : i, m > 2u ? q.z : i, m > 3u ? q.w : i};
+                double xs[4][D];
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     if constexpr (D == 3) {
@@ -512,42 +614,22 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
                         xs[e][1] = q2.y;
                     }
                 }
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if ((uint32_t)e >= m) break;
+                    int32_t kj[D];
+                    (void)pair(with_images, xi, xs[e], si, js[e], c, k, kj);
+                }
+#pragma unroll
+                for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
             }
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if ((uint32_t)e >= m) break;
-                const uint32_t j = js[e];
-                float f[D];
-                double d[D];
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    d[a] = separation16<D>(S, xi[a], xs[e][a], a);
-                    f[a] = sep_f16(d[a]);
-                }
-                if (rc) {
-                    // the reference's range test on the separation (|d| < 8 sigma, in sixteenths)
-                    bool ok = true;
-#pragma unroll
-                    for (int a = 0; a < D; a++) ok &= d[a] >= -2147483648.0 && d[a] < 2147483648.0;
-                    if (!ok) continue;
-                }
-                int32_t kj[D];
-                bool hit;
-                if constexpr (MULTI) {
-                    uint32_t sj = sp[j];
-                    uint32_t p = si * 4 + sj;
-                    hit = pair_kick2<D, true>(f, pt.rc2[p], pt.sigma2[p], pt.six_eps4[p], pt.r2_fast[p], c, pt.kick_c[sj], k, kj);
-                } else {
-                    hit = pair_kick2<D, false>(f, rc2_0, sigma2_0, six_eps4_0, r2f_0, c, c, k, kj);
-                }
-                if (inblock && hit) {
-                    const uint32_t lj = j - base;
-#pragma unroll
-                    for (int a = 0; a < D; a++) atomicAdd(&acc[a][lj], (uint32_t)kj[a]);
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < D; a++) atomicAdd(&acc[a][own], k[a]);
+        };
+        if (images) {
+            inblock_pass(std::true_type{});
+            cross_pass(std::true_type{});
+        } else {
+            inblock_pass(std::false_type{});
+            cross_pass(std::false_type{});
         }
     }
     __syncthreads();
@@ -693,8 +775,8 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     }
     for (int a = 0; a < 3; a++) B_.ref[a] = (double*)dalloc(slot_cap * 8);
     B_.rel = (float4*)dalloc((slot_cap + 32) * 16);  // padded: the list build reads up to 31 past a cell
-    B_.wave_chunks = (uint32_t*)dalloc(slot_cap / 64 * 4);
-    RH_HIP(hipMemset(B_.wave_chunks, 0, slot_cap / 64 * 4));
+    B_.wave_chunks = (uint32_t*)dalloc(slot_cap / 64 * 8);
+    RH_HIP(hipMemset(B_.wave_chunks, 0, slot_cap / 64 * 8));
     B_.key = (uint32_t*)dalloc(slot_cap * 4);
     B_.rank = (uint32_t*)dalloc(slot_cap * 4);
     B_.nbr = (uint32_t*)dalloc(list_cap * 4);
