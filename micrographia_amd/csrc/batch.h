@@ -19,6 +19,10 @@ constexpr uint32_t BATCH_MAX_BLOCK = 1024;  // largest half-list block (slots pe
 struct BSys {
     uint32_t active, dim, n, slot0, nslots;
     uint32_t nc[3];
+    // Cell keys: tiles of 2^tile_sh cells a side in linear order (nt[] tiles per axis, the last
+    // ones padded), Morton order inside a tile, so a block of consecutive slots covers a compact
+    // region and keeps more of its pairs inside the block.  ncells counts the padded keys.
+    uint32_t tile_sh, nt[3];
     uint32_t ncells, cell0;
     uint32_t periodic, nspecies, pinned, soft_walls, range_check, cap;
     uint32_t variant;  // 1: half list within work-groups (default), 0: every pair at both ends

@@ -34,10 +34,44 @@ constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 1;  // per slot:
 // Kernels
 // ---------------------------------------------------------------------------------------------
 
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
+
+// The key of cell c: its tile's linear index, then its Morton index inside the tile (bit b of
+// the in-tile coordinate on axis a at bit D b + a).  The key is a sum of one term per axis.
+template <int D>
+__device__ __forceinline__ uint32_t axis_key(const BSys& S, int a, uint32_t v) {
+    const uint32_t sh = S.tile_sh, l = v & ((1u << sh) - 1u);
+    uint32_t stride = 1u, m = 0u;
+    for (int b = 0; b < a; b++) stride *= S.nt[b];
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; b++) m |= ((l >> b) & 1u) << (D * b + a);
+    return (((v >> sh) * stride) << (D * sh)) + m;
+}
+
 template <int D>
 __device__ __forceinline__ uint32_t lin_cell(const BSys& S, const uint32_t c[D]) {
-    if constexpr (D == 3) return (c[2] * S.nc[1] + c[1]) * S.nc[0] + c[0];
-    else return c[1] * S.nc[0] + c[0];
+    uint32_t key = 0u;
+#pragma unroll
+    for (int a = 0; a < D; a++) key += axis_key<D>(S, a, c[a]);
+    return key;
+}
+
+// The cell of a key (lin_cell's inverse).
+template <int D>
+__device__ __forceinline__ void cell_of_key(const BSys& S, uint32_t key, uint32_t c[3]) {
+    const uint32_t sh = S.tile_sh, in = key & ((1u << (D * sh)) - 1u);
+    uint32_t tile = key >> (D * sh);
+#pragma unroll
+    for (int a = 0; a < 3; a++) c[a] = 0u;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        uint32_t l = 0u;
+#pragma unroll
+        for (uint32_t b = 0; b < 4u; b++) l |= ((in >> (D * b + a)) & 1u) << b;
+        c[a] = ((tile % S.nt[a]) << sh) | l;
+        tile /= S.nt[a];
+    }
 }
 
 // Offsets in rb_list of the rebuild lists: one per build kind, then all kinds together.
@@ -109,8 +143,10 @@ __device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C,
 __global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B, uint32_t off) {
     const uint32_t len = *(volatile const uint32_t*)&B.rb_count[4];
     for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
-        const uint32_t u = B.rb_list[off + b];
-        const uint32_t s = B.wg_sys[u];
+        // the unit and its system are the same for the whole work-group: held in scalar registers,
+        // the system's fields are read once into scalar registers too
+        const uint32_t u = __builtin_amdgcn_readfirstlane(B.rb_list[off + b]);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[u]);
         const BSys& S = B.sys[s];
         const BCtl& C = B.ctl[s];
         const uint32_t slot = u * BATCH_WG + threadIdx.x;
@@ -181,7 +217,8 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
     const uint32_t pp = C.ppos, pm = C.pmeta;
     uint32_t key = B.key[slot];
     uint32_t ns = S.slot0 + B.cell_start[S.cell0 + key] + B.rank[slot];
-    uint32_t c[3] = {key % S.nc[0], (key / S.nc[0]) % S.nc[1], D == 3 ? key / (S.nc[0] * S.nc[1]) : 0u};
+    uint32_t c[3];
+    cell_of_key<D>(S, key, c);
     float r[3] = {0.f, 0.f, 0.f};
     double xs[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -202,8 +239,10 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
 __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B, uint32_t off) {
     const uint32_t len = *(volatile const uint32_t*)&B.rb_count[4];
     for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
-        const uint32_t u = B.rb_list[off + b];
-        const uint32_t s = B.wg_sys[u];
+        // the unit and its system are the same for the whole work-group: held in scalar registers,
+        // the system's fields are read once into scalar registers too
+        const uint32_t u = __builtin_amdgcn_readfirstlane(B.rb_list[off + b]);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[u]);
         const BSys& S = B.sys[s];
         const BCtl& C = B.ctl[s];
         const uint32_t slot = u * BATCH_WG + threadIdx.x;
@@ -224,21 +263,30 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B, uint32_t off) 
 // counts go to wave_chunks[2 w], wave_chunks[2 w + 1].
 template <int D, bool MULTI>
 __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint32_t local) {
-    const uint32_t variant = S.variant;
+    // the system's fields used in the loops, in scalar registers (the same for the whole wave)
+    const uint32_t variant = uni(S.variant), block = uni(S.block), slot0 = uni(S.slot0), cell0 = uni(S.cell0);
+    const uint32_t periodic = uni(S.periodic), cap = uni(S.cap);
+    uint32_t nc[3];
+    float cw[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nc[a] = uni(S.nc[a]);
+        cw[a] = uni(S.cell_w[a]);
+    }
     const uint32_t pm = C.pmeta ^ 1u;
-    const uint32_t slot = S.slot0 + local;
-    const uint32_t* sp = B.species[pm] + S.slot0;
+    const uint32_t slot = slot0 + local;
+    const uint32_t* sp = B.species[pm] + slot0;
     const float4 me = B.rel[slot];
     const float ri[3] = {me.x, me.y, me.z};
     double xi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) xi[a] = B.pos[C.ppos ^ 1u][a][slot];
     const uint32_t si = MULTI ? sp[local] : 0u;
-    const float list_r2 = S.pt.list_r2[0];
+    const float list_r2 = uni(S.pt.list_r2[0]);
     uint32_t c[D];
     cell_of<D>(S, xi, c);
-    const uint32_t cap4 = S.cap / 4u, capw = 64u * cap4;
-    const uint32_t own = local & (S.block - 1u);
+    const uint32_t cap4 = cap / 4u, capw = 64u * cap4;
+    const uint32_t own = local & (block - 1u);
     uint4* chunks = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + (size_t)(local >> 6) * capw;
     const uint64_t lt = lanemask_lt();
     // the fills change only where every lane of the wave is present; a store that would meet the
@@ -258,14 +306,14 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
     };
     // the open in-block and cross-block chunks
     uint4 fa = make_uint4(0u, 0u, 0u, 0u), fb = make_uint4(0u, 0u, 0u, 0u);
-    const uint32_t* cs = B.cell_start + S.cell0;
-    const uint32_t* ce = B.cell_end + S.cell0;
-    const float4* rel = B.rel + S.slot0;
+    const uint32_t* cs = B.cell_start + cell0;
+    const uint32_t* ce = B.cell_end + cell0;
+    const float4* rel = B.rel + slot0;
     float wl[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; a++) wl[a] = (S.nc[a] == 1u && ((S.periodic >> a) & 1u)) ? S.cell_w[a] : 0.f;
+    for (int a = 0; a < D; a++) wl[a] = (nc[a] == 1u && ((periodic >> a) & 1u)) ? cw[a] : 0.f;
     const bool any_wrap = (wl[0] != 0.f) | (wl[1] != 0.f) | (wl[2] != 0.f);
-    const uint32_t blo = local & ~(S.block - 1u);
+    const uint32_t blo = local & ~(block - 1u);
     uint32_t na = 0, nb = 0;
     // Hit masks wait in a three-entry queue per lane (mask, first candidate) and are visited one
     // hit per step of a wave-wide loop, run only when some lane's queue is full and at the end:
@@ -278,7 +326,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
             const uint32_t b = __builtin_ctz(m0);
             m0 &= m0 - 1u;
             const uint32_t j = b0 + b;
-            in = variant == 1 && j - blo < S.block;
+            in = variant == 1 && j - blo < block;
             // the two counts step by selects (a branch on `in` around them is turned into a
             // per-lane indexed pair in scratch memory)
             if (in) put4(fa, na, j);
@@ -295,29 +343,43 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
                 qn--;
             }
         }
-        const bool fin = full && in, fout = full && !in;
-        store(std::true_type{}, fin, tag_chunk(fa, own, 4u, true));
-        store(std::false_type{}, fout, tag_chunk(fb, own, 4u, false));
+        // a chunk fills on one step in four: one ballot skips both stores on the others
+        if (ballot64(full) != 0ull) {
+            const bool fin = full && in, fout = full && !in;
+            store(std::true_type{}, fin, tag_chunk(fa, own, 4u, true));
+            store(std::false_type{}, fout, tag_chunk(fb, own, 4u, false));
+        }
     };
+    // each axis's three neighbour coordinates (wrapped on periodic axes) as key terms, and which
+    // of them to skip (past a walled face, or off a single-cell axis)
+    uint32_t kt[3][3] = {};
+    uint32_t skipm = 0u;
+#pragma unroll
+    for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int o3 = 0; o3 < 3; o3++) {
+            int v = (int)c[a] + o3 - 1;
+            bool sk = nc[a] == 1u && o3 != 1;
+            if (v < 0 || v >= (int)nc[a]) {
+                if (!((periodic >> a) & 1u)) sk = true;
+                v = v < 0 ? v + (int)nc[a] : v - (int)nc[a];
+            }
+            kt[a][o3] = axis_key<D>(S, a, (uint32_t)v);
+            skipm |= sk ? 1u << (3 * a + o3) : 0u;
+        }
     for (int o = 0; o < stencil_size(D); o++) {
-        const int off[3] = {o % 3 - 1, o / 3 % 3 - 1, o / 9 - 1};
-        uint32_t cc[D];
+        const int o3[3] = {o % 3, o / 3 % 3, o / 9};
         float shift[3] = {0.f, 0.f, 0.f};
         bool skip = false;
+        uint32_t key = 0u;
 #pragma unroll
         for (int a = 0; a < D; a++) {
-            int v = (int)c[a] + off[a];
-            if (S.nc[a] == 1u && off[a] != 0) skip = true;
-            if (v < 0 || v >= (int)S.nc[a]) {
-                if (!((S.periodic >> a) & 1u)) skip = true;
-                v = v < 0 ? v + (int)S.nc[a] : v - (int)S.nc[a];
-            }
-            cc[a] = (uint32_t)v;
-            shift[a] = (float)off[a] * S.cell_w[a] - ri[a];
+            key += o3[a] == 0 ? kt[a][0] : (o3[a] == 1 ? kt[a][1] : kt[a][2]);
+            skip |= ((skipm >> (3 * a + o3[a])) & 1u) != 0u;
+            shift[a] = (float)(o3[a] - 1) * cw[a] - ri[a];
         }
         uint32_t base = 0u, e0 = 0u;
         if (!skip) {
-            const uint32_t key = lin_cell<D>(S, cc);
             const uint32_t s0 = cs[key];
             e0 = ce[key];
             // in-block candidates up to local are not this end's: scan such a cell from past it
@@ -357,7 +419,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
                 // j = base + u: gt: j > local; inb: j in local's block (variant 1)
                 const uint32_t gt = ~bits_below((int)local + 1 - (int)base);
                 if (variant == 1) {
-                    inb = bits_below((int)(blo + S.block) - (int)base) & ~bits_below((int)blo - (int)base);
+                    inb = bits_below((int)(blo + block) - (int)base) & ~bits_below((int)blo - (int)base);
                     mask &= (inb & gt) | ~inb;
                 }
                 const int t0 = (int)local - (int)base;
@@ -401,8 +463,10 @@ __global__ __launch_bounds__(BATCH_WG) __attribute__((amdgpu_waves_per_eu(5))) v
     constexpr uint32_t K = (D == 3 ? 2u : 0u) + (MULTI ? 1u : 0u);
     const uint32_t len = *(volatile const uint32_t*)&B.rb_count[K];
     for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
-        const uint32_t u = B.rb_list[off + b];
-        const uint32_t s = B.wg_sys[u];
+        // the unit and its system are the same for the whole work-group: held in scalar registers,
+        // the system's fields are read once into scalar registers too
+        const uint32_t u = __builtin_amdgcn_readfirstlane(B.rb_list[off + b]);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[u]);
         const BSys& S = B.sys[s];
         BCtl& C = B.ctl[s];
         const uint32_t local = u * BATCH_WG + threadIdx.x - S.slot0;
@@ -704,8 +768,8 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
 // blocks).  Consecutive blocks of the list go to the same XCD.
 template <int D, bool MULTI, bool BOOK, uint32_t BS>
 __global__ __launch_bounds__(BS) void k_bb_step(BBufs B, uint32_t off) {
-    const uint32_t wg = B.step_list[off + xcd_block(blockIdx.x, gridDim.x)];
-    const uint32_t s = B.wg_sys[wg * (BS / BATCH_WG)];
+    const uint32_t wg = __builtin_amdgcn_readfirstlane(B.step_list[off + xcd_block(blockIdx.x, gridDim.x)]);
+    const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[wg * (BS / BATCH_WG)]);
     const BSys& S = B.sys[s];
     BCtl& C = B.ctl[s];
     if (!bb_runs(S, C)) return;
@@ -900,6 +964,25 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
             d.bath_lo = (double)sys.bath_region[0];
             d.bath_hi = (double)sys.bath_region[1];
         }
+    }
+    // Tiles: the smallest power-of-two edge whose cells hold about two blocks of particles (at
+    // most 8 cells in 3D, 16 in 2D), halved while padding the grid to whole tiles would more
+    // than double the cell count.
+    {
+        const double per_cell = (double)n / (double)ncells;
+        const uint32_t tmax = D == 3 ? 3u : 4u;
+        uint32_t sh = 0;
+        while (sh < tmax && std::pow((double)(1u << sh), D) * per_cell < 2.0 * d.block) sh++;
+        for (;; sh--) {
+            uint64_t padded = 1;
+            for (int a = 0; a < D; a++) padded *= (uint64_t)((d.nc[a] + (1u << sh) - 1u) >> sh) << sh;
+            if (sh == 0 || padded <= 2 * ncells) {
+                ncells = padded;
+                break;
+            }
+        }
+        d.tile_sh = sh;
+        for (int a = 0; a < 3; a++) d.nt[a] = a < D ? (d.nc[a] + (1u << sh) - 1u) >> sh : 1u;
     }
     RH_REQUIRE(ncells < (uint64_t(1) << 31), "too many cells");
     d.ncells = (uint32_t)ncells;
