@@ -28,7 +28,7 @@ BYTES_PER_ATOM_STEP = 72  # read+write 3 x i64 positions and 3 x i32 velocities
 # WRITE_SIZE, each calibrated against a plain 1 GiB copy (profiles/ keeps the summary and the
 # commands).  Keyed by (variant, block); a counter pass cannot run inside this process.
 MEASURED_TRAFFIC = {
-    (1, 2048): (1.567e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
+    (1, 2048): (1.577e9, "profiles/k_step_half_lj3d_n4194304_hbm_traffic.md"),
 }
 
 
