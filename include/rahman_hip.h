@@ -284,7 +284,9 @@ int rh_batch_read_ledger(rh_batch* b, int32_t id, rh_ledger3* out);
 /* Iterations per captured hipGraph (0: launch directly). */
 int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk);
 /* Tile settings for systems added from now on, per dimension and size class (floor(log2 n)):
- * list variant (1 half list within work-groups, the default; 0 every pair at both ends), cell
+ * list variant (1 half list within work-groups, the default, built in 3D over a row-ordered
+ * copy where the box allows; 2 the same half list built over the cell stencil; 0 every pair at
+ * both ends), cell
  * occupancy (cells at least cell_factor x (rc + skin) wide, 1 <= cell_factor <= 4) and block
  * (slots per half-list block and step work-group: 256, the default, 512 or 1024; the system's
  * slots are whole blocks).  Results are identical for every setting; micrographia_amd.autotune

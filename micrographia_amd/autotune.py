@@ -180,7 +180,7 @@ class BatchTuner:
                 if trials:
                     first = max(trials, key=lambda t: t.rate)
                     variant, factor, _ = first.setting
-                    if variant == 1:
+                    if variant != 0:
                         for block in self.blocks[1:]:
                             if atoms >= 4 * block * len(sample):
                                 trial(variant, factor, block)

@@ -131,9 +131,10 @@ class BatchEngine:
         check(load().rh_batch_set_graph_chunk(self._h, int(chunk)))
 
     def set_tuning(self, dim: int, size_class: int, variant: int, cell_factor: float, block: int = 256) -> None:
-        """Tile settings (list variant, cell occupancy, half-list block of 256, 512 or 1024 slots)
-        for systems of this dimension and size class (floor(log2 n)) added from now on;
-        trajectories do not depend on them."""
+        """Tile settings (list variant: 1 half list, built in 3D over the row-ordered copy where
+        the box allows, 2 the half list over the cell stencil, 0 every pair at both ends; cell
+        occupancy; half-list block of 256, 512 or 1024 slots) for systems of this dimension and
+        size class (floor(log2 n)) added from now on; trajectories do not depend on them."""
         check(load().rh_batch_set_tuning(self._h, int(dim), int(size_class), int(variant), float(cell_factor), int(block)))
 
     def capacity(self):

@@ -14,6 +14,10 @@ namespace rh {
 
 constexpr uint32_t BATCH_WG = 256;         // slots per work-group of the per-slot kernels, the slot unit
 constexpr uint32_t BATCH_MAX_BLOCK = 1024;  // largest half-list block (slots per step work-group)
+// Build kinds: 0-1 2D, 2-3 3D over the cell stencil, 4-5 3D over the row-ordered copy (+1: more
+// than one species); the rebuild lists hold one list per kind, then one of all kinds together.
+constexpr uint32_t BUILD_KINDS = 6;
+constexpr uint32_t RB_ALL = BUILD_KINDS;
 
 // Per-system device descriptor.
 struct BSys {
@@ -24,10 +28,18 @@ struct BSys {
     // region and keeps more of its pairs inside the block.  ncells counts the padded keys.
     uint32_t tile_sh, nt[3];
     uint32_t ncells, cell0;
+    // The row-ordered copy for the 3D list build (rows != 0): each (y, z) cell's row of rows_nxf
+    // fine x bins, with rows_g ghost bins at both ends on a periodic x axis (rows_ne = nxf + 2 g
+    // per row).  The fine bins' counts and the extended bins' starts follow the cells in the
+    // system's cell range (cell_count and cell_start from cell0 + ncells); the copy itself (every
+    // slot once, plus at most one ghost) is at items + 2 slot0.  cell_words: the whole cell range.
+    uint32_t rows, rows_nxf, rows_g, rows_ne, cell_words;
+    float rows_inv_wx, rows_reach;
+    double rows_xscale;
     uint32_t periodic, nspecies, pinned, soft_walls, range_check, cap;
-    uint32_t variant;  // 1: half list within work-groups (default), 0: every pair at both ends
+    uint32_t variant;  // 1: half list within work-groups (default; built over the rows in 3D), 2: the same built over the cell stencil, 0: every pair at both ends
     uint32_t block;    // slots per half-list block and step work-group: 256, 512 or 1024
-    uint32_t bkind;    // build kind: 2 (dim == 3) + (species > 1)
+    uint32_t bkind;    // build kind: 0 (2D), 2 (3D, cell stencil) or 4 (3D, rows), + (species > 1)
     uint64_t nbr0;  // list entry k of local slot i at nbr0 + k * nslots + i
     int64_t box[3], half[3];
     double boxf[3], inv_box[3];  // positions are exact f64 integers (2^-32 sigma units)
@@ -66,6 +78,7 @@ struct BBufs {
     double4* aos[2];  // positions in sixteenths packed per slot (x, y, z or 0, 0), for the step's neighbour gathers
     uint32_t* wave_chunks;  // in-block and cross-block list chunks in each 64-slot wave's region (2 per wave)
     float4* rel;  // position relative to its cell's corner (sigma, f32), for list builds
+    float4* items;  // the row-ordered copies (x absolute, y and z cell-relative, local slot)
     uint32_t* key;
     uint32_t* rank;
     uint32_t* nbr;
@@ -76,7 +89,7 @@ struct BBufs {
     // Work lists.  step_list: per kind of step launch, the block indices of its systems (written
     // by the host with the tables).  rb_list: per build kind, and for all kinds together, the
     // units of the systems rebuilding this iteration, appended by k_bb_decide; rb_count holds
-    // their lengths (build kinds 0-3, then all) and is cleared at the end of each iteration.
+    // their lengths (build kinds, then all) and is cleared at the end of each iteration.
     const uint32_t* step_list;
     uint32_t* rb_list;
     uint32_t* rb_count;
@@ -142,6 +155,8 @@ class Batch {
     void invalidate_graph();
 
     hipStream_t stream_ = nullptr;
+    hipStream_t aux_ = nullptr;  // the stencil builds' branch beside the row pipeline
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     BBufs B_;
     BSys* d_sys_ = nullptr;
     BCtl* d_ctl_ = nullptr;
@@ -157,7 +172,7 @@ class Batch {
     uint32_t* d_rb_count_ = nullptr;
     uint32_t units_cap_ = 0;        // BATCH_WG-slot units in the slot arena
     uint32_t step_off_[24] = {}, step_cnt_[24] = {};  // step launch kind 8 b + kind: its list
-    uint32_t rb_off_[5] = {}, rb_cap_[5] = {};        // rebuild lists: offset, room (units)
+    uint32_t rb_off_[RB_ALL + 1] = {}, rb_cap_[RB_ALL + 1] = {};  // rebuild lists: offset, room (units)
     bool tables_dirty_ = true;
     uint32_t graph_chunk_ = 100;
     hipGraphExec_t graph_ = nullptr;

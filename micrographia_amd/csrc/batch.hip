@@ -28,7 +28,7 @@ namespace rh {
 
 constexpr uint32_t LIST_UNIT = 4096;  // list-arena allocation unit (entries)
 constexpr uint32_t CELL_UNIT = 256;   // cell-arena allocation unit (cells)
-constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 8 + 1;  // per slot: pos, vel, species+ids, ref, aos, rel, key+rank, wave counts (1/8, rounded up)
+constexpr size_t SLOT_BYTES = 48 + 24 + 16 + 24 + 64 + 16 + 32 + 8 + 1;  // per slot: pos, vel, species+ids, ref, aos, rel, row copies, key+rank, wave counts (1/8, rounded up)
 
 // ---------------------------------------------------------------------------------------------
 // Kernels
@@ -76,7 +76,7 @@ __device__ __forceinline__ void cell_of_key(const BSys& S, uint32_t key, uint32_
 
 // Offsets in rb_list of the rebuild lists: one per build kind, then all kinds together.
 struct RbOff {
-    uint32_t off[5];
+    uint32_t off[RB_ALL + 1];
 };
 
 // Per system: rebuild now?  A rebuilding system appends its units to its build kind's list and
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void k_bb_decide(const BSys* S, BCtl* C, uint32
         if (go) {
             c.maxcount = 0u;
             pq[0] = atomicAdd(&count[d.bkind], nu);
-            pq[1] = atomicAdd(&count[4], nu);
+            pq[1] = atomicAdd(&count[RB_ALL], nu);
         }
     }
     if (!go) return;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(64) void k_bb_decide(const BSys* S, BCtl* C, uint32
     const uint32_t p = pq[0], q = pq[1];
     for (uint32_t u = threadIdx.x; u < nu; u += 64u) {
         list[ro.off[d.bkind] + p + u] = u0 + u;
-        list[ro.off[4] + q + u] = u0 + u;
+        list[ro.off[RB_ALL] + q + u] = u0 + u;
     }
 }
 
@@ -141,7 +141,7 @@ __device__ __forceinline__ void bb_count(BBufs& B, const BSys& S, const BCtl& C,
 // The per-slot rebuild kernels loop over the units of the list they are given (work-group b
 // takes its entries b, b + grid, ...), whatever its length this iteration.
 __global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B, uint32_t off) {
-    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[4];
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[RB_ALL];
     for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
         // the unit and its system are the same for the whole work-group: held in scalar registers,
         // the system's fields are read once into scalar registers too
@@ -237,7 +237,7 @@ __device__ __forceinline__ void bb_scatter(BBufs& B, const BSys& S, const BCtl& 
 }
 
 __global__ __launch_bounds__(BATCH_WG) void k_bb_scatter(BBufs B, uint32_t off) {
-    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[4];
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[RB_ALL];
     for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
         // the unit and its system are the same for the whole work-group: held in scalar registers,
         // the system's fields are read once into scalar registers too
@@ -326,7 +326,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
             const uint32_t b = __builtin_ctz(m0);
             m0 &= m0 - 1u;
             const uint32_t j = b0 + b;
-            in = variant == 1 && j - blo < block;
+            in = variant != 0 && j - blo < block;
             // the two counts step by selects (a branch on `in` around them is turned into a
             // per-lane indexed pair in scratch memory)
             if (in) put4(fa, na, j);
@@ -383,7 +383,7 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
             const uint32_t s0 = cs[key];
             e0 = ce[key];
             // in-block candidates up to local are not this end's: scan such a cell from past it
-            base = (variant == 1 && s0 >= blo) ? max(s0, local + 1u) : s0;
+            base = (variant != 0 && s0 >= blo) ? max(s0, local + 1u) : s0;
         }
         const float sx = shift[0], sy = shift[1], sz = shift[2];
         for (;; base += 32u) {
@@ -416,9 +416,9 @@ __device__ __forceinline__ void bb_build(BBufs& B, const BSys& S, BCtl& C, uint3
                     for (uint32_t k = 0; k < 4u; k++) mask |= near(u0 + k) ? 1u << (u0 + k) : 0u;
                 }
                 mask &= bits_below((int)cnt);
-                // j = base + u: gt: j > local; inb: j in local's block (variant 1)
+                // j = base + u: gt: j > local; inb: j in local's block (half list)
                 const uint32_t gt = ~bits_below((int)local + 1 - (int)base);
-                if (variant == 1) {
+                if (variant != 0) {
                     inb = bits_below((int)(blo + block) - (int)base) & ~bits_below((int)blo - (int)base);
                     mask &= (inb & gt) | ~inb;
                 }
@@ -494,6 +494,367 @@ __global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
         c.pmeta ^= 1u;
         c.rebuilds += 1u;
         c.do_rebuild = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The 3D list build over the row-ordered copy (systems with rows != 0), as the single-system
+// engine's: after the cell sort, each slot's rank in its fine x bin (k_bb_row_count), the
+// extended bins' starts per system (k_bb_row_scan, which also clears the counts), the copies
+// with their ghosts (k_bb_row_scatter), then the build (k_bb_build_rows).  The slot order stays
+// the tiled cell order the step's blocks want; only the build reads the rows.
+// ---------------------------------------------------------------------------------------------
+
+// A slot's row (its (y, z) cell) and fine x bin.
+__device__ __forceinline__ void bb_row_bin(const BSys& S, const double x[3], uint32_t& row, uint32_t& xf) {
+    uint32_t c[3];
+    cell_of<3>(S, x, c);
+    row = c[2] * S.nc[1] + c[1];
+    const double f = x[0] * S.rows_xscale;
+    const uint32_t v = f > 0.0 ? (uint32_t)f : 0u;
+    xf = v < S.rows_nxf - 1u ? v : S.rows_nxf - 1u;
+}
+
+// Over the units of rebuild list K (a rows kind): the slot's rank in its fine bin.
+__global__ __launch_bounds__(BATCH_WG) void k_bb_row_count(BBufs B, uint32_t off, uint32_t K) {
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[K];
+    for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
+        const uint32_t u = __builtin_amdgcn_readfirstlane(B.rb_list[off + b]);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[u]);
+        const BSys& S = B.sys[s];
+        const BCtl& C = B.ctl[s];
+        const uint32_t slot = u * BATCH_WG + threadIdx.x;
+        if (slot - S.slot0 >= S.n) continue;
+        double x[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) x[a] = B.pos[C.ppos ^ 1u][a][slot];
+        uint32_t row, xf;
+        bb_row_bin(S, x, row, xf);
+        B.rank[slot] = atomicAdd(&B.cell_count[S.cell0 + S.ncells + row * S.rows_nxf + xf], 1u);
+    }
+}
+
+// One work-group per rebuilding rows system: the start of every extended bin (a ghost bin takes
+// its source bin's count) and, past the last, the total; then the counts are cleared.
+__global__ __launch_bounds__(BATCH_WG) void k_bb_row_scan(BBufs B, uint32_t nsys) {
+    const uint32_t s = blockIdx.x;
+    if (s >= nsys) return;
+    const BSys& S = B.sys[s];
+    if (!S.active || !S.rows || !B.ctl[s].do_rebuild) return;
+    __shared__ uint32_t lds[BATCH_WG / WAVE];
+    uint32_t* cnt = B.cell_count + S.cell0 + S.ncells;
+    uint32_t* st = B.cell_start + S.cell0 + S.ncells;
+    const uint32_t ne = S.rows_ne, nxf = S.rows_nxf, g = S.rows_g;
+    const uint32_t nrows = S.nc[1] * S.nc[2], nk = nrows * ne;
+    // sixteen consecutive bins per lane: their sixteen count reads are in flight together
+    constexpr uint32_t PER = 16;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nk; base += PER * BATCH_WG) {
+        const uint32_t c0 = base + PER * threadIdx.x;
+        uint32_t row = c0 / ne, e = c0 - row * ne;
+        uint32_t v[PER];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            int b = (int)e - (int)g;  // the source bin of extended bin e (a ghost: the other end's)
+            if (b < 0) b += (int)nxf;
+            else if (b >= (int)nxf) b -= (int)nxf;
+            v[k] = c0 + k < nk ? cnt[row * nxf + (uint32_t)b] : 0u;
+            sum += v[k];
+            if (++e == ne) {
+                e = 0u;
+                row++;
+            }
+        }
+        uint32_t total;
+        uint32_t ex = carry + wg_exclusive_scan(sum, lds, &total);
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            if (c0 + k < nk) st[c0 + k] = ex;
+            ex += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0u) st[nk] = carry;
+    __syncthreads();  // every count is read before any is cleared
+    for (uint32_t k = threadIdx.x; k < nrows * nxf; k += BATCH_WG) cnt[k] = 0u;
+}
+
+// The copy of each slot (x absolute in sigma, y and z relative to its cell's corner, its local
+// slot with the species in bits 30-31) and its ghost past the row's other end on a periodic x
+// axis.
+__global__ __launch_bounds__(BATCH_WG) void k_bb_row_scatter(BBufs B, uint32_t off, uint32_t K) {
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[K];
+    for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
+        const uint32_t u = __builtin_amdgcn_readfirstlane(B.rb_list[off + b]);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[u]);
+        const BSys& S = B.sys[s];
+        const BCtl& C = B.ctl[s];
+        const uint32_t slot = u * BATCH_WG + threadIdx.x;
+        const uint32_t local = slot - S.slot0;
+        if (local >= S.n) continue;
+        double x[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) x[a] = B.pos[C.ppos ^ 1u][a][slot];
+        uint32_t row, xf;
+        bb_row_bin(S, x, row, xf);
+        const float4 r = B.rel[slot];
+        const uint32_t* rs = B.cell_start + S.cell0 + S.ncells;
+        float4* it = B.items + 2 * (size_t)S.slot0;
+        const uint32_t rank = B.rank[slot], base = row * S.rows_ne, g = S.rows_g, nxf = S.rows_nxf;
+        const uint32_t sp = S.nspecies > 1 ? B.species[C.pmeta ^ 1u][slot] << 30 : 0u;
+        const float w = __uint_as_float(local | sp);
+        it[rs[base + g + xf] + rank] = make_float4((float)(x[0] * 0x1p-32), r.y, r.z, w);
+        if (xf < g)  // a ghost past the row's end
+            it[rs[base + g + nxf + xf] + rank] = make_float4((float)((x[0] + S.boxf[0]) * 0x1p-32), r.y, r.z, w);
+        if (xf + g >= nxf)  // a ghost before the row's start
+            it[rs[base + xf + g - nxf] + rank] = make_float4((float)((x[0] - S.boxf[0]) * 0x1p-32), r.y, r.z, w);
+    }
+}
+
+// The half list of local slot `local` from the rows: the list sphere meets at most the nine rows
+// around the particle's own, and a row's candidates within the sphere's chord are one range of
+// the copy (ghost bins make this true across a periodic x face).  Candidates four at a time; the
+// three tests (within the radius, in the block, of higher index) are wave masks, shifted into
+// per-lane hit masks.  Hit masks wait in a per-lane ring in LDS; a visit moves up to four hits of
+// the front mask into that kind's open chunk (also in LDS) and stores chunks as they fill, at
+// wave-uniform places of the wave's region: in-block chunks from its start, cross-block chunks
+// from its end down, both four indices tagged with the owner (as the stencil build).
+template <bool MULTI>
+__device__ __forceinline__ void bb_build_rows(BBufs& B, const BSys& S, BCtl& C, uint32_t local) {
+    const uint32_t block = uni(S.block), slot0 = uni(S.slot0), cap = uni(S.cap);
+    const uint32_t periodic = uni(S.periodic), ny = uni(S.nc[1]), nz = uni(S.nc[2]);
+    const uint32_t rows_ne = uni(S.rows_ne), rows_g = uni(S.rows_g), rows_nxf = uni(S.rows_nxf);
+    const float wy = uni(S.cell_w[1]), wz = uni(S.cell_w[2]), inv_wx = uni(S.rows_inv_wx), reach = uni(S.rows_reach);
+    const uint32_t pm = C.pmeta ^ 1u;
+    const uint32_t slot = slot0 + local;
+    uint32_t c[3];
+    float Xi;
+    {
+        double xi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) xi[a] = B.pos[C.ppos ^ 1u][a][slot];
+        cell_of<3>(S, xi, c);
+        Xi = (float)(xi[0] * 0x1p-32);
+    }
+    float mey, mez;  // y and z relative to the cell corner
+    {
+        const float4 me = B.rel[slot];
+        mey = me.y;
+        mez = me.z;
+    }
+    float lr2[4];
+    if constexpr (MULTI) {
+        const uint32_t si = B.species[pm][slot];
+#pragma unroll
+        for (uint32_t s = 0; s < 4; s++) lr2[s] = S.pt.list_r2[si * 4 + s];
+    } else {
+        lr2[0] = uni(S.pt.list_r2[0]);
+    }
+    const float reach2 = reach * reach;
+    const float4* __restrict__ items = B.items + 2 * (size_t)slot0;
+    const uint32_t* __restrict__ rstart = B.cell_start + uni(S.cell0) + uni(S.ncells);
+    const uint32_t cap4 = cap / 4u, capw = 64u * cap4;
+    const uint32_t blo = local & ~(block - 1u), own = local & (block - 1u);
+    uint4* chunks = reinterpret_cast<uint4*>(B.nbr + S.nbr0) + (size_t)(local >> 6) * capw;
+    const uint64_t lt = lanemask_lt();
+    // the fills change only where every lane of the wave is present; a store that would meet the
+    // other end's chunks flags the overflow instead
+    uint32_t fill_in = 0u, fill_out = 0u;
+    auto store = [&](auto in_c, bool want, const uint4& q) __attribute__((always_inline)) {
+        constexpr bool in = decltype(in_c)::value;
+        uint32_t& fill = in ? fill_in : fill_out;
+        const uint64_t bal = ballot64(want);
+        if (bal == 0ull) return;
+        if (want) {
+            const uint32_t k = fill + (uint32_t)__popcll(bal & lt), other = in ? fill_out : fill_in;
+            if (k + other < capw) chunks[in ? k : capw - 1u - k] = q;
+            else atomicOr(&C.overflow, 1u);
+        }
+        fill += (uint32_t)__popcll(bal);
+    };
+    // Per lane, in LDS: the open in-block and cross-block chunks (entry s at ci[s][t], co[s][t])
+    // and a ring of up to four hit masks, each with the copy index of its bit 0 (bit b stands for
+    // that index minus b) and its kind in bit 31.
+    __shared__ uint32_t ci[4][BATCH_WG], co[4][BATCH_WG];
+    __shared__ uint32_t qm[4][BATCH_WG], qt[4][BATCH_WG];
+    const uint32_t t = threadIdx.x;
+    uint32_t na = 0u, nb = 0u, head = 0u, qn = 0u;
+    auto chunk_of = [&](bool in, uint32_t entries) {
+        return in ? tag_chunk(make_uint4(ci[0][t], ci[1][t], ci[2][t], ci[3][t]), own, entries, true)
+                  : tag_chunk(make_uint4(co[0][t], co[1][t], co[2][t], co[3][t]), own, entries, false);
+    };
+    auto visit = [&]() __attribute__((always_inline)) {
+        bool full = false, in = false;
+        if (qn != 0u) {
+            const uint32_t h = head & 3u;
+            uint32_t m = qm[h][t];
+            const uint32_t tag = qt[h][t];
+            in = (tag >> 31) != 0u;
+            const uint32_t b1 = tag & 0x7fffffffu;  // the copy index of bit 0
+            const uint32_t n = in ? na : nb, r = n & 3u;
+            uint32_t jv[4];
+            bool take[4];
+#pragma unroll
+            for (uint32_t s2 = 0; s2 < 4u; s2++) {
+                take[s2] = r + s2 < 4u && m != 0u;
+                const uint32_t at = take[s2] ? (uint32_t)__builtin_ctz(m) : 0u;
+                m = take[s2] ? m & (m - 1u) : m;
+                jv[s2] = __float_as_uint(items[b1 - at].w) & 0x0fffffffu;
+            }
+#pragma unroll
+            for (uint32_t s2 = 0; s2 < 4u; s2++) {
+                if (!take[s2]) continue;
+                if (in) ci[r + s2][t] = jv[s2];
+                else co[r + s2][t] = jv[s2];
+            }
+            const uint32_t taken = (uint32_t)take[0] + (uint32_t)take[1] + (uint32_t)take[2] + (uint32_t)take[3];
+            const uint32_t n2 = n + taken;
+            full = (n2 & 3u) == 0u;  // at least one was taken, so the chunk closed
+            na = in ? n2 : na;
+            nb = in ? nb : n2;
+            if (m == 0u) {
+                head++;
+                qn--;
+            } else {
+                qm[h][t] = m;
+            }
+        }
+        // every lane takes part in the stores' ballots, so the fills stay the same across the wave
+        if (ballot64(full) != 0ull) {
+            const bool fin = full && in, fout = full && !in;
+            store(std::true_type{}, fin, fin ? chunk_of(true, 4u) : make_uint4(0u, 0u, 0u, 0u));
+            store(std::false_type{}, fout, fout ? chunk_of(false, 4u) : make_uint4(0u, 0u, 0u, 0u));
+        }
+    };
+    auto push = [&](uint32_t m, uint32_t tag) {
+        if (m == 0u) return;
+        const uint32_t h = (head + qn) & 3u;
+        qm[h][t] = m;
+        qt[h][t] = tag;
+        qn++;
+    };
+    // the chord's bins are clamped to the row's extended bins (its ghosts on a periodic x axis)
+    const int glo = -(int)rows_g, ghi = (int)(rows_nxf + rows_g) - 1;
+    const float sx = -Xi;
+    for (int o = 0; o < 9; o++) {
+        const int oy = o % 3 - 1, oz = o / 3 - 1;
+        int yy = (int)c[1] + oy, zz = (int)c[2] + oz;
+        bool skip = false;
+        if (yy < 0 || yy >= (int)ny) {
+            if (!((periodic >> 1) & 1u)) skip = true;
+            yy = yy < 0 ? yy + (int)ny : yy - (int)ny;
+        }
+        if (zz < 0 || zz >= (int)nz) {
+            if (!((periodic >> 2) & 1u)) skip = true;
+            zz = zz < 0 ? zz + (int)nz : zz - (int)nz;
+        }
+        // distance from the particle to the row's (y, z) rectangle, and the chord along x
+        const float dyl = oy == 0 ? 0.f : fmaxf(0.f, oy > 0 ? wy - mey : mey);
+        const float dzl = oz == 0 ? 0.f : fmaxf(0.f, oz > 0 ? wz - mez : mez);
+        const float d2 = dyl * dyl + dzl * dzl;
+        skip |= !(d2 < reach2);
+        const float ext = skip ? 0.f : __builtin_sqrtf(reach2 - d2);
+        const float sy = (float)oy * wy - mey, sz = (float)oz * wz - mez;
+        int a = (int)__builtin_floorf((Xi - ext) * inv_wx);
+        int b = (int)__builtin_floorf((Xi + ext) * inv_wx);
+        a = a > glo ? a : glo;
+        b = b < ghi ? b : ghi;
+        uint32_t s0 = 0u, e0 = 0u;
+        if (!skip && a <= b) {
+            const uint32_t rowbase = ((uint32_t)zz * ny + (uint32_t)yy) * rows_ne + rows_g;
+            s0 = rstart[rowbase + (uint32_t)a];
+            e0 = rstart[rowbase + (uint32_t)b + 1u];
+        }
+        // Candidates in groups of up to 32, tested without a branch (each candidate is one
+        // 16-byte load) into two masks: in-block partners of higher index than the particle (an
+        // in-block pair is kept at its lower-index end only) and partners outside its block,
+        // both within the list radius.
+        for (uint32_t base = s0;; base += 32u) {
+            const bool act = base < e0;
+            if (ballot64(act) == 0ull) break;
+            // Hits are shifted in as they are tested: candidate base + u ends at bit n - 1 - u of
+            // its mask, n being the number of candidates tested.
+            uint32_t m_in = 0u, m_out = 0u, last = base;
+            if (act) {
+                const uint32_t cnt = min(32u, e0 - base);
+                // four at a time; a lane past its range's end tests the copy's entries that follow
+                // (the copy is padded by 32), whose bits are masked off after
+                const float4* __restrict__ rq = items + base;
+#pragma unroll 1
+                for (uint32_t u0 = 0; u0 < 32u; u0 += 4u) {
+                    if (u0 >= cnt) break;
+#pragma unroll
+                    for (uint32_t k = 0; k < 4u; k++) {
+                        const float4 q = rq[u0 + k];
+                        const float dx = q.x + sx, dy = q.y + sy, dz = q.z + sz;
+                        float r2 = dx * dx;
+                        r2 = __builtin_fmaf(dy, dy, r2);
+                        r2 = __builtin_fmaf(dz, dz, r2);
+                        const uint32_t w = __float_as_uint(q.w);
+                        uint32_t j;
+                        float lr;
+                        if constexpr (MULTI) {
+                            j = w & 0x0fffffffu;
+                            const uint32_t sj = w >> 30;
+                            lr = sj == 0u ? lr2[0] : (sj == 1u ? lr2[1] : (sj == 2u ? lr2[2] : lr2[3]));
+                        } else {
+                            j = w;  // one species: the copy holds the slot alone
+                            lr = lr2[0];
+                        }
+                        // the three tests as wave masks, combined on the scalar unit
+                        const uint64_t near = ballot64(r2 < lr), inb = ballot64(j - blo < block), gt = ballot64(j > local);
+                        m_in = shift_in(m_in, near & inb & gt);
+                        m_out = shift_in(m_out, near & ~inb);
+                    }
+                }
+                const uint32_t n = (cnt + 3u) & ~3u;
+                const uint32_t live = ~bits_below((int)(n - cnt));
+                m_in &= live;
+                m_out &= live;
+                last = base + n - 1u;
+            }
+            const uint32_t need = (m_in != 0u ? 1u : 0u) + (m_out != 0u ? 1u : 0u);
+            while (ballot64(qn + need > 4u) != 0ull) visit();
+            push(m_in, last | 0x80000000u);
+            push(m_out, last);
+        }
+    }
+    while (ballot64(qn != 0u) != 0ull) visit();
+    // the partial last chunks; a list longer than the capacity is flagged and regrown
+    const bool pin = (na & 3u) != 0u, pout = (nb & 3u) != 0u;
+    store(std::true_type{}, pin, pin ? chunk_of(true, na & 3u) : make_uint4(0u, 0u, 0u, 0u));
+    store(std::false_type{}, pout, pout ? chunk_of(false, nb & 3u) : make_uint4(0u, 0u, 0u, 0u));
+    const uint32_t ca = (na + 3u) / 4u, cb = (nb + 3u) / 4u;
+    if (ca + cb > cap4) atomicOr(&C.overflow, 1u);
+    if (4u * (ca + cb) > *(volatile const uint32_t*)&C.maxcount) atomicMax(&C.maxcount, 4u * (ca + cb));
+    if (__lane_id() == 0u) {
+        const uint32_t in = min(fill_in, capw);
+        B.wave_chunks[2u * (slot >> 6)] = in;
+        B.wave_chunks[2u * (slot >> 6) + 1u] = min(fill_out, capw - in);
+    }
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(BATCH_WG) void k_bb_build_rows(BBufs B, uint32_t off) {
+    constexpr uint32_t K = 4u + (MULTI ? 1u : 0u);
+    const uint32_t len = *(volatile const uint32_t*)&B.rb_count[K];
+    for (uint32_t b = blockIdx.x; b < len; b += gridDim.x) {
+        const uint32_t u = __builtin_amdgcn_readfirstlane(B.rb_list[off + b]);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(B.wg_sys[u]);
+        const BSys& S = B.sys[s];
+        BCtl& C = B.ctl[s];
+        const uint32_t local = u * BATCH_WG + threadIdx.x - S.slot0;
+        if (local >= S.n) {
+            // a wave past the system's last particle holds no chunks
+            if ((threadIdx.x & 63u) == 0u) {
+                const uint32_t w = (u * BATCH_WG + threadIdx.x) >> 6;
+                B.wave_chunks[2u * w] = 0u;
+                B.wave_chunks[2u * w + 1u] = 0u;
+            }
+            continue;
+        }
+        bb_build_rows<MULTI>(B, S, C, local);
     }
 }
 
@@ -778,7 +1139,7 @@ __global__ __launch_bounds__(BS) void k_bb_step(BBufs B, uint32_t off) {
 
 __global__ void k_bb_commit_step(const BSys* S, BCtl* C, uint32_t nsys, uint32_t* rb_count) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < 5u) rb_count[s] = 0u;  // the next iteration's rebuild lists start empty
+    if (s <= RB_ALL) rb_count[s] = 0u;  // the next iteration's rebuild lists start empty
     if (s >= nsys) return;
     BCtl& c = C[s];
     if (bb_runs(S[s], c)) {
@@ -827,6 +1188,9 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     lists_.reset(list_cap / LIST_UNIT);
     cells_.reset(cell_cap / CELL_UNIT);
     RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    RH_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+    RH_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    RH_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     std::memset(&B_, 0, sizeof B_);
     for (int p = 0; p < 2; p++) {
         for (int a = 0; a < 3; a++) {
@@ -839,6 +1203,8 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     }
     for (int a = 0; a < 3; a++) B_.ref[a] = (double*)dalloc(slot_cap * 8);
     B_.rel = (float4*)dalloc((slot_cap + 32) * 16);  // padded: the list build reads up to 31 past a cell
+    B_.items = (float4*)dalloc((2 * slot_cap + 32) * 16);  // two per slot; padded as rel
+    RH_HIP(hipMemset(B_.items, 0, (2 * slot_cap + 32) * 16));
     B_.wave_chunks = (uint32_t*)dalloc(slot_cap / 64 * 8);
     RH_HIP(hipMemset(B_.wave_chunks, 0, slot_cap / 64 * 8));
     B_.key = (uint32_t*)dalloc(slot_cap * 4);
@@ -860,8 +1226,8 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     units_cap_ = (uint32_t)(slot_cap / BATCH_WG);
     d_step_list_ = (uint32_t*)dalloc(sizeof(uint32_t) * units_cap_);
     d_rb_list_ = (uint32_t*)dalloc(sizeof(uint32_t) * 2 * (size_t)units_cap_);
-    d_rb_count_ = (uint32_t*)dalloc(sizeof(uint32_t) * 5);
-    RH_HIP(hipMemset(d_rb_count_, 0, sizeof(uint32_t) * 5));
+    d_rb_count_ = (uint32_t*)dalloc(sizeof(uint32_t) * (RB_ALL + 1));
+    RH_HIP(hipMemset(d_rb_count_, 0, sizeof(uint32_t) * (RB_ALL + 1)));
     B_.step_list = d_step_list_;
     B_.rb_list = d_rb_list_;
     B_.rb_count = d_rb_count_;
@@ -878,6 +1244,9 @@ Batch::~Batch() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
     for (void* p : allocs_) (void)hipFree(p);
     if (h_ctl_) (void)hipHostFree(h_ctl_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (aux_) (void)hipStreamDestroy(aux_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1008,6 +1377,36 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
     double hs = 0.5 * skin * (1.0 - 1e-3);
     d.half_skin2 = (float)(hs * hs);
     d.range_check = ((rcm + skin) * (1.0 + 1e-4) + skin >= 7.99) ? 1u : 0u;
+    // The row-ordered copy for the 3D half list (variant 1), as the single-system engine plans it:
+    // fine x bins a power-of-two fraction of the cell wide, each row padded with ghost bins that
+    // cover the list radius plus one on a periodic x axis; x absolute in f32 needs x below 512.
+    d.rows = 0u;
+    d.cell_words = d.ncells;
+    if (D == 3 && d.variant == 1 && !d.range_check && d.nc[0] >= 3 && d.nc[1] >= 3 && d.nc[2] >= 3 && b[0] <= 512.0) {
+        float lr2max = 0.f;
+        for (int k = 0; k < 16; k++) lr2max = std::max(lr2max, d.pt.list_r2[k]);
+        const double reach = std::sqrt((double)lr2max) + 1e-3;
+        const bool wrap_x = (d.periodic & 1u) != 0u;
+        for (uint32_t sub = 8; sub >= 1 && !d.rows; sub /= 2) {
+            const uint64_t nxf = (uint64_t)d.nc[0] * sub;
+            const double wx = b[0] / (double)nxf;
+            const uint64_t g = wrap_x ? (uint64_t)std::ceil(reach / wx) + 1u : 0u;
+            const uint64_t words = (uint64_t)d.nc[1] * d.nc[2] * (nxf + 2 * g) + 1u;
+            // a chord never holds a slot and its ghost, and no slot has two ghosts
+            if (words > (uint64_t(1) << 25) || nxf <= 2 * g || !(b[0] > 2.0 * reach + 2.0 * wx)) continue;
+            d.rows = 1u;
+            d.rows_nxf = (uint32_t)nxf;
+            d.rows_g = (uint32_t)g;
+            d.rows_ne = (uint32_t)(nxf + 2 * g);
+            d.cell_words = (uint32_t)(d.ncells + words);
+        }
+        if (d.rows) {
+            d.rows_xscale = (double)d.rows_nxf / (double)sys.box_len[0];
+            d.rows_inv_wx = (float)((double)d.rows_nxf / b[0]);
+            d.rows_reach = (float)reach;
+            d.bkind = 4u + (sys.nspecies > 1 ? 1u : 0u);
+        }
+    }
     const double pi = 3.141592653589793;
     double r = rcm + skin;
     double dens = (double)n / vol;
@@ -1018,7 +1417,7 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
 void Batch::set_tuning(int dim, uint32_t size_class, uint32_t variant, double cell_factor, uint32_t block) {
     RH_REQUIRE(dim == 2 || dim == 3, "dimension must be 2 or 3");
     RH_REQUIRE(size_class < 32, "size class is floor(log2 n), below 32");
-    RH_REQUIRE(variant <= 1, "list variant must be 0 (full) or 1 (half)");
+    RH_REQUIRE(variant <= 2, "list variant must be 0 (full), 1 (half, built over the rows in 3D) or 2 (half, over the cell stencil)");
     RH_REQUIRE(cell_factor >= 1.0 && cell_factor <= 4.0, "cell factor must be in [1, 4]");
     RH_REQUIRE(block == 256 || block == 512 || block == 1024, "block must be 256, 512 or 1024");
     tuning_[dim - 2][size_class] = Tuning{variant, cell_factor, block};
@@ -1031,7 +1430,7 @@ uint64_t Batch::bytes_needed(int dim, uint64_t n, const SystemDesc& sys, double 
     s2.dim = dim;
     plan(s2, n, skin, d, &cap);
     uint64_t nslots = (n + d.block - 1) / d.block * d.block;
-    return nslots * SLOT_BYTES + (uint64_t)cap * nslots * 4 + (uint64_t)d.ncells * 12;
+    return nslots * SLOT_BYTES + (uint64_t)cap * nslots * 4 + (uint64_t)d.cell_words * 12;
 }
 
 int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int32_t* vel, const uint32_t* species, double skin, uint64_t steps) {
@@ -1049,7 +1448,7 @@ int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int3
     uint64_t nslots = (n + d.block - 1) / d.block * d.block;
     uint64_t s0, l0, c0;
     uint64_t lunits = ((uint64_t)cap * nslots + LIST_UNIT - 1) / LIST_UNIT;
-    uint64_t cunits = (d.ncells + CELL_UNIT - 1) / CELL_UNIT;
+    uint64_t cunits = (d.cell_words + CELL_UNIT - 1) / CELL_UNIT;
     if (!slots_.alloc(nslots / BATCH_WG, &s0, d.block / BATCH_WG)) return -1;
     if (!lists_.alloc(lunits, &l0)) {
         slots_.free(s0, nslots / BATCH_WG);
@@ -1074,7 +1473,7 @@ int Batch::add(const SystemDesc& sys, uint64_t n, const int64_t* pos, const int3
     m.slot0 = d.slot0;
     m.nslots = d.nslots;
     m.cell0 = d.cell0;
-    m.ncells = d.ncells;
+    m.ncells = d.cell_words;
     m.cap = cap;
     m.nbr0 = d.nbr0;
     m.nbr_entries = lunits * LIST_UNIT;
@@ -1154,12 +1553,12 @@ void Batch::upload_tables() {
     // Work lists: the rebuild lists' room per build kind (their systems' units), and per kind of
     // step launch (kind j = 0 (2D), 2 (3D), 4 (3D with a ledger), 6 (2D with a ledger), + (species
     // > 1); block 256 << b) the blocks of its systems.
-    uint32_t rb_cap[5] = {0, 0, 0, 0, 0};
+    uint32_t rb_cap[RB_ALL + 1] = {};
     std::vector<uint32_t> lists[24];
     for (auto& kv : members_) {
         const BSys& d = h_sys_[kv.first];
         rb_cap[d.bkind] += d.nslots / BATCH_WG;
-        rb_cap[4] += d.nslots / BATCH_WG;
+        rb_cap[RB_ALL] += d.nslots / BATCH_WG;
         const uint32_t j = (d.ledger ? (d.dim == 3 ? 4u : 6u) : (d.dim == 3 ? 2u : 0u)) + (d.nspecies > 1 ? 1u : 0u);
         const uint32_t bi = d.block == 1024u ? 2u : d.block == 512u ? 1u : 0u;
         for (uint32_t g = d.slot0 / d.block; g < (d.slot0 + d.nslots) / d.block; g++) lists[8 * bi + j].push_back(g);
@@ -1174,15 +1573,15 @@ void Batch::upload_tables() {
     }
     if (!flat.empty()) RH_HIP(hipMemcpy(d_step_list_, flat.data(), flat.size() * 4, hipMemcpyHostToDevice));
     uint32_t off = 0;
-    for (int k = 0; k < 4; k++) {
+    for (uint32_t k = 0; k < BUILD_KINDS; k++) {
         changed |= rb_cap_[k] != rb_cap[k] || rb_off_[k] != off;
         rb_off_[k] = off;
         rb_cap_[k] = rb_cap[k];
         off += rb_cap[k];
     }
-    changed |= rb_cap_[4] != rb_cap[4];
-    rb_off_[4] = units_cap_;  // the list of all kinds takes the second half
-    rb_cap_[4] = rb_cap[4];
+    changed |= rb_cap_[RB_ALL] != rb_cap[RB_ALL];
+    rb_off_[RB_ALL] = units_cap_;  // the list of all kinds takes the second half
+    rb_cap_[RB_ALL] = rb_cap[RB_ALL];
     if (changed) invalidate_graph();
     grid_wgs_ = new_wgs;
     tables_dirty_ = false;
@@ -1215,16 +1614,42 @@ static uint32_t rebuild_grid(uint32_t units) { return std::max<uint32_t>(1u, std
 void Batch::enqueue_iteration() {
     const uint32_t sb = blocks_for(max_sys_, 256);
     RbOff ro;
-    for (int k = 0; k < 5; k++) ro.off[k] = rb_off_[k];
+    for (uint32_t k = 0; k <= RB_ALL; k++) ro.off[k] = rb_off_[k];
     k_bb_decide<<<max_sys_, 64, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_list_, d_rb_count_, ro);
-    const uint32_t ga = rebuild_grid(rb_cap_[4]);
-    k_bb_count<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
+    const uint32_t ga = rebuild_grid(rb_cap_[RB_ALL]);
+    k_bb_count<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
-    k_bb_scatter<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
-    if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, stream_>>>(B_, rb_off_[0]);
-    if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, stream_>>>(B_, rb_off_[1]);
-    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, stream_>>>(B_, rb_off_[2]);
-    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, stream_>>>(B_, rb_off_[3]);
+    k_bb_scatter<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
+    // The stencil builds (2D, and 3D systems too small for rows) and the row pipeline work on
+    // different systems: the stencil builds run on the second stream beside the row pipeline
+    // (in a captured graph, a branch of their own), joined before the commit.  Their launches
+    // are few and latency-bound (long serial candidate loops in few waves), so side by side
+    // they hide behind the row build.
+    const bool stencil = rb_cap_[0] + rb_cap_[1] + rb_cap_[2] + rb_cap_[3] != 0u;
+    const bool rows = rb_cap_[4] + rb_cap_[5] != 0u;
+    hipStream_t sten = rows ? aux_ : stream_;
+    if (stencil && rows) {
+        RH_HIP(hipEventRecord(ev_fork_, stream_));
+        RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
+    }
+    if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, sten>>>(B_, rb_off_[0]);
+    if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, sten>>>(B_, rb_off_[1]);
+    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, sten>>>(B_, rb_off_[2]);
+    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, sten>>>(B_, rb_off_[3]);
+    // the row-ordered copies of the rebuilding rows systems, then their builds
+    if (rows) {
+        for (uint32_t k = 4; k < 6; k++)
+            if (rb_cap_[k]) k_bb_row_count<<<rebuild_grid(rb_cap_[k]), BATCH_WG, 0, stream_>>>(B_, rb_off_[k], k);
+        k_bb_row_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
+        for (uint32_t k = 4; k < 6; k++)
+            if (rb_cap_[k]) k_bb_row_scatter<<<rebuild_grid(rb_cap_[k]), BATCH_WG, 0, stream_>>>(B_, rb_off_[k], k);
+        if (rb_cap_[4]) k_bb_build_rows<false><<<rebuild_grid(rb_cap_[4]), BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
+        if (rb_cap_[5]) k_bb_build_rows<true><<<rebuild_grid(rb_cap_[5]), BATCH_WG, 0, stream_>>>(B_, rb_off_[5]);
+    }
+    if (stencil && rows) {
+        RH_HIP(hipEventRecord(ev_join_, aux_));
+        RH_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
+    }
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
     launch_steps<256>(0);
     launch_steps<512>(1);

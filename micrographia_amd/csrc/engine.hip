@@ -616,14 +616,6 @@ __global__ __launch_bounds__(TPB) void k_row_scatter(Params<3> P, Bufs<3> B, con
         B.items[B.row_start[base + xf + g - nxf] + rank] = make_float4((float)((x[0] - P.boxf[0]) * 0x1p-32), ry, rz, w);
 }
 
-// 2 q + (bit of this lane in the wave mask m): one add with the mask as its carry-in.
-__device__ __forceinline__ uint32_t shift_in(uint32_t q, uint64_t m) {
-    uint32_t r;
-    uint64_t carry_out;
-    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(q), "s"(m));
-    return r;
-}
-
 template <bool MULTI>
 __global__ __launch_bounds__(TPB) void k_build_rows(Params<3> P, Bufs<3> B, const uint32_t* gate) {
     if (gate_closed(gate)) return;

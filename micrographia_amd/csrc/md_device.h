@@ -555,6 +555,14 @@ __device__ __forceinline__ uint32_t bits_below(int t) {
     return t <= 0 ? 0u : (t >= 32 ? 0xffffffffu : ((1u << t) - 1u));
 }
 
+// 2 q + (bit of this lane in the wave mask m): one add with the mask as its carry-in.
+__device__ __forceinline__ uint32_t shift_in(uint32_t q, uint64_t m) {
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(q), "s"(m));
+    return r;
+}
+
 // Entry k & 3 of a four-entry list chunk, without dynamic register indexing.
 __device__ __forceinline__ void put4(uint4& q, uint32_t k, uint32_t j) {
     const uint32_t s = k & 3u;

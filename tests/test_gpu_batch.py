@@ -36,10 +36,13 @@ def mixed_systems():
     return out
 
 
-@pytest.mark.parametrize("graph,tiles", [(0, None), (50, None), (50, (0, 1.5)), (0, (1, 1.25)), (50, (1, 1.0, 512)), (0, (1, 1.25, 1024))])
+@pytest.mark.parametrize("graph,tiles", [(0, None), (50, None), (50, (0, 1.5)), (0, (1, 1.25)), (50, (1, 1.0, 512)), (0, (1, 1.25, 1024)),
+                                         (0, (2, 1.0)), (50, (2, 1.25, 512))])
 def test_each_system_matches_the_oracle(graph, tiles):
-    """Every system's trajectory equals the oracle's, whatever the tile settings (list variant,
-    cell occupancy and half-list block, as the autotuner installs them per size bucket)."""
+    """Every system's trajectory equals the oracle's, whatever the tile settings (list variant:
+    the half list built over the row-ordered copy (1, in 3D) or over the cell stencil (2), or
+    every pair at both ends (0); cell occupancy and half-list block, as the autotuner installs
+    them per size bucket)."""
     systems = mixed_systems()
     b = BatchEngine(budget_bytes=2 * GB, max_systems=64, graph_chunk=graph)
     if tiles is not None:
