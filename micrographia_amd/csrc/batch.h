@@ -155,8 +155,8 @@ class Batch {
     void invalidate_graph();
 
     hipStream_t stream_ = nullptr;
-    hipStream_t aux_ = nullptr;  // the stencil builds' branch beside the row pipeline
-    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    hipStream_t aux_ = nullptr, aux2_ = nullptr;  // the 2D and 3D stencil builds' branches beside the row pipeline
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_join2_ = nullptr;
     BBufs B_;
     BSys* d_sys_ = nullptr;
     BCtl* d_ctl_ = nullptr;

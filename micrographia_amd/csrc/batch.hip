@@ -156,6 +156,7 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_count(BBufs B, uint32_t off) {
     }
 }
 
+template <uint32_t NT = BATCH_WG>
 __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     uint32_t x = v;
@@ -168,7 +169,7 @@ __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t* lds,
     __syncthreads();
     uint32_t pre = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < (int)(BATCH_WG / WAVE); k++) {
+    for (int k = 0; k < (int)(NT / WAVE); k++) {
         uint32_t s = lds[k];
         pre += (k < (int)w) ? s : 0u;
         tot += s;
@@ -534,14 +535,16 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_row_count(BBufs B, uint32_t off
     }
 }
 
-// One work-group per rebuilding rows system: the start of every extended bin (a ghost bin takes
-// its source bin's count) and, past the last, the total; then the counts are cleared.
-__global__ __launch_bounds__(BATCH_WG) void k_bb_row_scan(BBufs B, uint32_t nsys) {
+// One work-group of ROW_SCAN_WG lanes per rebuilding rows system: the start of every extended
+// bin (a ghost bin takes its source bin's count) and, past the last, the total; then the counts
+// are cleared.
+constexpr uint32_t ROW_SCAN_WG = 256;
+__global__ __launch_bounds__(ROW_SCAN_WG) void k_bb_row_scan(BBufs B, uint32_t nsys) {
     const uint32_t s = blockIdx.x;
     if (s >= nsys) return;
     const BSys& S = B.sys[s];
     if (!S.active || !S.rows || !B.ctl[s].do_rebuild) return;
-    __shared__ uint32_t lds[BATCH_WG / WAVE];
+    __shared__ uint32_t lds[ROW_SCAN_WG / WAVE];
     uint32_t* cnt = B.cell_count + S.cell0 + S.ncells;
     uint32_t* st = B.cell_start + S.cell0 + S.ncells;
     const uint32_t ne = S.rows_ne, nxf = S.rows_nxf, g = S.rows_g;
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_row_scan(BBufs B, uint32_t nsys
     // sixteen consecutive bins per lane: their sixteen count reads are in flight together
     constexpr uint32_t PER = 16;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < nk; base += PER * BATCH_WG) {
+    for (uint32_t base = 0; base < nk; base += PER * ROW_SCAN_WG) {
         const uint32_t c0 = base + PER * threadIdx.x;
         uint32_t row = c0 / ne, e = c0 - row * ne;
         uint32_t v[PER];
@@ -567,7 +570,7 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_row_scan(BBufs B, uint32_t nsys
             }
         }
         uint32_t total;
-        uint32_t ex = carry + wg_exclusive_scan(sum, lds, &total);
+        uint32_t ex = carry + wg_exclusive_scan<ROW_SCAN_WG>(sum, lds, &total);
 #pragma unroll
         for (uint32_t k = 0; k < PER; k++) {
             if (c0 + k < nk) st[c0 + k] = ex;
@@ -577,7 +580,7 @@ __global__ __launch_bounds__(BATCH_WG) void k_bb_row_scan(BBufs B, uint32_t nsys
     }
     if (threadIdx.x == 0u) st[nk] = carry;
     __syncthreads();  // every count is read before any is cleared
-    for (uint32_t k = threadIdx.x; k < nrows * nxf; k += BATCH_WG) cnt[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < nrows * nxf; k += ROW_SCAN_WG) cnt[k] = 0u;
 }
 
 // The copy of each slot (x absolute in sigma, y and z relative to its cell's corner, its local
@@ -1189,8 +1192,10 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     cells_.reset(cell_cap / CELL_UNIT);
     RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     RH_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+    RH_HIP(hipStreamCreateWithFlags(&aux2_, hipStreamNonBlocking));
     RH_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    RH_HIP(hipEventCreateWithFlags(&ev_join2_, hipEventDisableTiming));
     std::memset(&B_, 0, sizeof B_);
     for (int p = 0; p < 2; p++) {
         for (int a = 0; a < 3; a++) {
@@ -1246,7 +1251,9 @@ Batch::~Batch() {
     if (h_ctl_) (void)hipHostFree(h_ctl_);
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (ev_join2_) (void)hipEventDestroy(ev_join2_);
     if (aux_) (void)hipStreamDestroy(aux_);
+    if (aux2_) (void)hipStreamDestroy(aux2_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1621,34 +1628,46 @@ void Batch::enqueue_iteration() {
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
     k_bb_scatter<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
     // The stencil builds (2D, and 3D systems too small for rows) and the row pipeline work on
-    // different systems: the stencil builds run on the second stream beside the row pipeline
-    // (in a captured graph, a branch of their own), joined before the commit.  Their launches
-    // are few and latency-bound (long serial candidate loops in few waves), so side by side
-    // they hide behind the row build.
-    const bool stencil = rb_cap_[0] + rb_cap_[1] + rb_cap_[2] + rb_cap_[3] != 0u;
+    // different systems: the 2D and the 3D stencil builds run on streams of their own beside the
+    // row pipeline (in a captured graph, branches of their own), joined before the commit.
+    // Their launches are latency-bound (long serial candidate loops in few waves), so side by
+    // side they hide behind the row build.
+    const bool b2 = rb_cap_[0] + rb_cap_[1] != 0u, b3 = rb_cap_[2] + rb_cap_[3] != 0u;
     const bool rows = rb_cap_[4] + rb_cap_[5] != 0u;
-    hipStream_t sten = rows ? aux_ : stream_;
-    if (stencil && rows) {
+    // a branch per kind of build present, the row pipeline on the main stream
+    hipStream_t s2 = stream_, s3 = stream_;
+    if (rows && (b2 || b3)) {
         RH_HIP(hipEventRecord(ev_fork_, stream_));
-        RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
+        if (b2) {
+            s2 = aux_;
+            RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
+        }
+        if (b3) {
+            s3 = aux2_;
+            RH_HIP(hipStreamWaitEvent(aux2_, ev_fork_, 0));
+        }
     }
-    if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, sten>>>(B_, rb_off_[0]);
-    if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, sten>>>(B_, rb_off_[1]);
-    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, sten>>>(B_, rb_off_[2]);
-    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, sten>>>(B_, rb_off_[3]);
+    if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, s2>>>(B_, rb_off_[0]);
+    if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, s2>>>(B_, rb_off_[1]);
+    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, s3>>>(B_, rb_off_[2]);
+    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, s3>>>(B_, rb_off_[3]);
     // the row-ordered copies of the rebuilding rows systems, then their builds
     if (rows) {
         for (uint32_t k = 4; k < 6; k++)
             if (rb_cap_[k]) k_bb_row_count<<<rebuild_grid(rb_cap_[k]), BATCH_WG, 0, stream_>>>(B_, rb_off_[k], k);
-        k_bb_row_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
+        k_bb_row_scan<<<max_sys_, ROW_SCAN_WG, 0, stream_>>>(B_, max_sys_);
         for (uint32_t k = 4; k < 6; k++)
             if (rb_cap_[k]) k_bb_row_scatter<<<rebuild_grid(rb_cap_[k]), BATCH_WG, 0, stream_>>>(B_, rb_off_[k], k);
         if (rb_cap_[4]) k_bb_build_rows<false><<<rebuild_grid(rb_cap_[4]), BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
         if (rb_cap_[5]) k_bb_build_rows<true><<<rebuild_grid(rb_cap_[5]), BATCH_WG, 0, stream_>>>(B_, rb_off_[5]);
     }
-    if (stencil && rows) {
+    if (s2 != stream_) {
         RH_HIP(hipEventRecord(ev_join_, aux_));
         RH_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
+    }
+    if (s3 != stream_) {
+        RH_HIP(hipEventRecord(ev_join2_, aux2_));
+        RH_HIP(hipStreamWaitEvent(stream_, ev_join2_, 0));
     }
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
     launch_steps<256>(0);
