@@ -1196,6 +1196,8 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     RH_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join2_, hipEventDisableTiming));
+    RH_HIP(hipEventCreateWithFlags(&ev_fork2_, hipEventDisableTiming));
+    RH_HIP(hipEventCreateWithFlags(&ev_join3_, hipEventDisableTiming));
     std::memset(&B_, 0, sizeof B_);
     for (int p = 0; p < 2; p++) {
         for (int a = 0; a < 3; a++) {
@@ -1252,6 +1254,8 @@ Batch::~Batch() {
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (ev_join2_) (void)hipEventDestroy(ev_join2_);
+    if (ev_fork2_) (void)hipEventDestroy(ev_fork2_);
+    if (ev_join3_) (void)hipEventDestroy(ev_join3_);
     if (aux_) (void)hipStreamDestroy(aux_);
     if (aux2_) (void)hipStreamDestroy(aux2_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -1602,17 +1606,17 @@ void Batch::invalidate_graph() {
 }
 
 template <uint32_t BS>
-void Batch::launch_steps(uint32_t b) {
+void Batch::launch_steps(uint32_t b, hipStream_t s2, hipStream_t s3) {
     const uint32_t* cnt = step_cnt_ + 8 * b;
     const uint32_t* off = step_off_ + 8 * b;
-    if (cnt[0]) k_bb_step<2, false, false, BS><<<cnt[0], BS, 0, stream_>>>(B_, off[0]);
-    if (cnt[1]) k_bb_step<2, true, false, BS><<<cnt[1], BS, 0, stream_>>>(B_, off[1]);
-    if (cnt[2]) k_bb_step<3, false, false, BS><<<cnt[2], BS, 0, stream_>>>(B_, off[2]);
-    if (cnt[3]) k_bb_step<3, true, false, BS><<<cnt[3], BS, 0, stream_>>>(B_, off[3]);
-    if (cnt[4]) k_bb_step<3, false, true, BS><<<cnt[4], BS, 0, stream_>>>(B_, off[4]);
-    if (cnt[5]) k_bb_step<3, true, true, BS><<<cnt[5], BS, 0, stream_>>>(B_, off[5]);
-    if (cnt[6]) k_bb_step<2, false, true, BS><<<cnt[6], BS, 0, stream_>>>(B_, off[6]);
-    if (cnt[7]) k_bb_step<2, true, true, BS><<<cnt[7], BS, 0, stream_>>>(B_, off[7]);
+    if (cnt[0]) k_bb_step<2, false, false, BS><<<cnt[0], BS, 0, s2>>>(B_, off[0]);
+    if (cnt[1]) k_bb_step<2, true, false, BS><<<cnt[1], BS, 0, s2>>>(B_, off[1]);
+    if (cnt[2]) k_bb_step<3, false, false, BS><<<cnt[2], BS, 0, s3>>>(B_, off[2]);
+    if (cnt[3]) k_bb_step<3, true, false, BS><<<cnt[3], BS, 0, s3>>>(B_, off[3]);
+    if (cnt[4]) k_bb_step<3, false, true, BS><<<cnt[4], BS, 0, s3>>>(B_, off[4]);
+    if (cnt[5]) k_bb_step<3, true, true, BS><<<cnt[5], BS, 0, s3>>>(B_, off[5]);
+    if (cnt[6]) k_bb_step<2, false, true, BS><<<cnt[6], BS, 0, s2>>>(B_, off[6]);
+    if (cnt[7]) k_bb_step<2, true, true, BS><<<cnt[7], BS, 0, s2>>>(B_, off[7]);
 }
 
 // Work-groups of a per-slot rebuild kernel: enough to fill the chip, at most one per unit.
@@ -1670,9 +1674,25 @@ void Batch::enqueue_iteration() {
         RH_HIP(hipStreamWaitEvent(stream_, ev_join2_, 0));
     }
     k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
-    launch_steps<256>(0);
-    launch_steps<512>(1);
-    launch_steps<1024>(2);
+    // the 2D systems' steps on a branch of their own beside the 3D systems' (different systems;
+    // the 3D step waits on its gathers, the 2D step on issue, so the two overlap well)
+    bool k2 = false, k3 = false;
+    for (uint32_t b = 0; b < 3; b++)
+        for (uint32_t k = 0; k < 8; k++)
+            if (step_cnt_[8 * b + k]) (k == 0 || k == 1 || k == 6 || k == 7 ? k2 : k3) = true;
+    hipStream_t st2 = stream_;
+    if (k2 && k3) {
+        st2 = aux_;
+        RH_HIP(hipEventRecord(ev_fork2_, stream_));
+        RH_HIP(hipStreamWaitEvent(aux_, ev_fork2_, 0));
+    }
+    launch_steps<256>(0, st2, stream_);
+    launch_steps<512>(1, st2, stream_);
+    launch_steps<1024>(2, st2, stream_);
+    if (st2 != stream_) {
+        RH_HIP(hipEventRecord(ev_join3_, aux_));
+        RH_HIP(hipStreamWaitEvent(stream_, ev_join3_, 0));
+    }
     k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_count_);
 }
 
