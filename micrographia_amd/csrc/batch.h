@@ -150,14 +150,13 @@ class Batch {
     void upload_tables();
     void enqueue_iteration();
     template <uint32_t BS>
-    void launch_steps(uint32_t b, hipStream_t s2, hipStream_t s3);
+    void launch_steps(uint32_t b, hipStream_t st, int dim);
     void handle_overflow();
     void invalidate_graph();
 
     hipStream_t stream_ = nullptr;
-    hipStream_t aux_ = nullptr, aux2_ = nullptr;  // the 2D and 3D stencil builds' branches beside the row pipeline
-    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_join2_ = nullptr;  // the builds' branches
-    hipEvent_t ev_fork2_ = nullptr, ev_join3_ = nullptr;                    // the 2D steps' branch
+    hipStream_t aux_ = nullptr, aux2_ = nullptr;  // the 2D branch and the 3D stencil builds' branch
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_join2_ = nullptr;  // the branches of an iteration
     BBufs B_;
     BSys* d_sys_ = nullptr;
     BCtl* d_ctl_ = nullptr;

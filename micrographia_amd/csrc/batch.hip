@@ -484,9 +484,10 @@ __global__ __launch_bounds__(BATCH_WG) __attribute__((amdgpu_waves_per_eu(5))) v
     }
 }
 
-__global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys) {
+// The systems of dimension `dim` that rebuilt this iteration.
+__global__ void k_bb_commit_rebuild(const BSys* S, BCtl* C, uint32_t nsys, uint32_t dim) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys) return;
+    if (s >= nsys || S[s].dim != dim) return;
     BCtl& c = C[s];
     if (c.do_rebuild) {
         c.stale = 0u;
@@ -1196,8 +1197,6 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     RH_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join2_, hipEventDisableTiming));
-    RH_HIP(hipEventCreateWithFlags(&ev_fork2_, hipEventDisableTiming));
-    RH_HIP(hipEventCreateWithFlags(&ev_join3_, hipEventDisableTiming));
     std::memset(&B_, 0, sizeof B_);
     for (int p = 0; p < 2; p++) {
         for (int a = 0; a < 3; a++) {
@@ -1254,8 +1253,6 @@ Batch::~Batch() {
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (ev_join2_) (void)hipEventDestroy(ev_join2_);
-    if (ev_fork2_) (void)hipEventDestroy(ev_fork2_);
-    if (ev_join3_) (void)hipEventDestroy(ev_join3_);
     if (aux_) (void)hipStreamDestroy(aux_);
     if (aux2_) (void)hipStreamDestroy(aux2_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -1606,17 +1603,20 @@ void Batch::invalidate_graph() {
 }
 
 template <uint32_t BS>
-void Batch::launch_steps(uint32_t b, hipStream_t s2, hipStream_t s3) {
+void Batch::launch_steps(uint32_t b, hipStream_t st, int dim) {
     const uint32_t* cnt = step_cnt_ + 8 * b;
     const uint32_t* off = step_off_ + 8 * b;
-    if (cnt[0]) k_bb_step<2, false, false, BS><<<cnt[0], BS, 0, s2>>>(B_, off[0]);
-    if (cnt[1]) k_bb_step<2, true, false, BS><<<cnt[1], BS, 0, s2>>>(B_, off[1]);
-    if (cnt[2]) k_bb_step<3, false, false, BS><<<cnt[2], BS, 0, s3>>>(B_, off[2]);
-    if (cnt[3]) k_bb_step<3, true, false, BS><<<cnt[3], BS, 0, s3>>>(B_, off[3]);
-    if (cnt[4]) k_bb_step<3, false, true, BS><<<cnt[4], BS, 0, s3>>>(B_, off[4]);
-    if (cnt[5]) k_bb_step<3, true, true, BS><<<cnt[5], BS, 0, s3>>>(B_, off[5]);
-    if (cnt[6]) k_bb_step<2, false, true, BS><<<cnt[6], BS, 0, s2>>>(B_, off[6]);
-    if (cnt[7]) k_bb_step<2, true, true, BS><<<cnt[7], BS, 0, s2>>>(B_, off[7]);
+    if (dim == 2) {
+        if (cnt[0]) k_bb_step<2, false, false, BS><<<cnt[0], BS, 0, st>>>(B_, off[0]);
+        if (cnt[1]) k_bb_step<2, true, false, BS><<<cnt[1], BS, 0, st>>>(B_, off[1]);
+        if (cnt[6]) k_bb_step<2, false, true, BS><<<cnt[6], BS, 0, st>>>(B_, off[6]);
+        if (cnt[7]) k_bb_step<2, true, true, BS><<<cnt[7], BS, 0, st>>>(B_, off[7]);
+    } else {
+        if (cnt[2]) k_bb_step<3, false, false, BS><<<cnt[2], BS, 0, st>>>(B_, off[2]);
+        if (cnt[3]) k_bb_step<3, true, false, BS><<<cnt[3], BS, 0, st>>>(B_, off[3]);
+        if (cnt[4]) k_bb_step<3, false, true, BS><<<cnt[4], BS, 0, st>>>(B_, off[4]);
+        if (cnt[5]) k_bb_step<3, true, true, BS><<<cnt[5], BS, 0, st>>>(B_, off[5]);
+    }
 }
 
 // Work-groups of a per-slot rebuild kernel: enough to fill the chip, at most one per unit.
@@ -1631,32 +1631,32 @@ void Batch::enqueue_iteration() {
     k_bb_count<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
     k_bb_scatter<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
-    // The stencil builds (2D, and 3D systems too small for rows) and the row pipeline work on
-    // different systems: the 2D and the 3D stencil builds run on streams of their own beside the
-    // row pipeline (in a captured graph, branches of their own), joined before the commit.
-    // Their launches are latency-bound (long serial candidate loops in few waves), so side by
-    // side they hide behind the row build.
-    const bool b2 = rb_cap_[0] + rb_cap_[1] != 0u, b3 = rb_cap_[2] + rb_cap_[3] != 0u;
-    const bool rows = rb_cap_[4] + rb_cap_[5] != 0u;
-    // a branch per kind of build present, the row pipeline on the main stream
-    hipStream_t s2 = stream_, s3 = stream_;
-    if (rows && (b2 || b3)) {
-        RH_HIP(hipEventRecord(ev_fork_, stream_));
-        if (b2) {
-            s2 = aux_;
-            RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
-        }
-        if (b3) {
-            s3 = aux2_;
-            RH_HIP(hipStreamWaitEvent(aux2_, ev_fork_, 0));
-        }
+    // From here on the dimensions go separate ways (they are different systems): the 2D systems'
+    // builds, commit and steps on one branch, the 3D systems' cell-stencil builds (those too small
+    // for rows) on another, and on the main stream the 3D row pipeline and builds, then the 3D
+    // commit and steps; the branches join before the step commit.  In a captured graph these are
+    // branches the GPU runs side by side: the stencil builds are latency-bound (long serial
+    // candidate loops in few waves) and the 3D step waits on its gathers, so the work of one
+    // branch fills the others' stalls.
+    const bool d2 = rb_cap_[0] + rb_cap_[1] != 0u, d3s = rb_cap_[2] + rb_cap_[3] != 0u, d3r = rb_cap_[4] + rb_cap_[5] != 0u;
+    const bool d3 = d3s || d3r;
+    const bool fork2 = d2 && d3, fork3 = d3s && d3r;
+    hipStream_t s2 = fork2 ? aux_ : stream_, s3 = fork3 ? aux2_ : stream_;
+    if (fork2 || fork3) RH_HIP(hipEventRecord(ev_fork_, stream_));
+    if (fork2) RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
+    if (fork3) RH_HIP(hipStreamWaitEvent(aux2_, ev_fork_, 0));
+    if (d2) {
+        if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, s2>>>(B_, rb_off_[0]);
+        if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, s2>>>(B_, rb_off_[1]);
+        k_bb_commit_rebuild<<<sb, 256, 0, s2>>>(d_sys_, d_ctl_, max_sys_, 2u);
+        launch_steps<256>(0, s2, 2);
+        launch_steps<512>(1, s2, 2);
+        launch_steps<1024>(2, s2, 2);
     }
-    if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, s2>>>(B_, rb_off_[0]);
-    if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, s2>>>(B_, rb_off_[1]);
     if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, s3>>>(B_, rb_off_[2]);
     if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, s3>>>(B_, rb_off_[3]);
     // the row-ordered copies of the rebuilding rows systems, then their builds
-    if (rows) {
+    if (d3r) {
         for (uint32_t k = 4; k < 6; k++)
             if (rb_cap_[k]) k_bb_row_count<<<rebuild_grid(rb_cap_[k]), BATCH_WG, 0, stream_>>>(B_, rb_off_[k], k);
         k_bb_row_scan<<<max_sys_, ROW_SCAN_WG, 0, stream_>>>(B_, max_sys_);
@@ -1665,33 +1665,19 @@ void Batch::enqueue_iteration() {
         if (rb_cap_[4]) k_bb_build_rows<false><<<rebuild_grid(rb_cap_[4]), BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
         if (rb_cap_[5]) k_bb_build_rows<true><<<rebuild_grid(rb_cap_[5]), BATCH_WG, 0, stream_>>>(B_, rb_off_[5]);
     }
-    if (s2 != stream_) {
-        RH_HIP(hipEventRecord(ev_join_, aux_));
-        RH_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
-    }
-    if (s3 != stream_) {
+    if (fork3) {
         RH_HIP(hipEventRecord(ev_join2_, aux2_));
         RH_HIP(hipStreamWaitEvent(stream_, ev_join2_, 0));
     }
-    k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_);
-    // the 2D systems' steps on a branch of their own beside the 3D systems' (different systems;
-    // the 3D step waits on its gathers, the 2D step on issue, so the two overlap well)
-    bool k2 = false, k3 = false;
-    for (uint32_t b = 0; b < 3; b++)
-        for (uint32_t k = 0; k < 8; k++)
-            if (step_cnt_[8 * b + k]) (k == 0 || k == 1 || k == 6 || k == 7 ? k2 : k3) = true;
-    hipStream_t st2 = stream_;
-    if (k2 && k3) {
-        st2 = aux_;
-        RH_HIP(hipEventRecord(ev_fork2_, stream_));
-        RH_HIP(hipStreamWaitEvent(aux_, ev_fork2_, 0));
+    if (d3) {
+        k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, 3u);
+        launch_steps<256>(0, stream_, 3);
+        launch_steps<512>(1, stream_, 3);
+        launch_steps<1024>(2, stream_, 3);
     }
-    launch_steps<256>(0, st2, stream_);
-    launch_steps<512>(1, st2, stream_);
-    launch_steps<1024>(2, st2, stream_);
-    if (st2 != stream_) {
-        RH_HIP(hipEventRecord(ev_join3_, aux_));
-        RH_HIP(hipStreamWaitEvent(stream_, ev_join3_, 0));
+    if (fork2) {
+        RH_HIP(hipEventRecord(ev_join_, aux_));
+        RH_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
     }
     k_bb_commit_step<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, d_rb_count_);
 }
