@@ -214,11 +214,16 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool,
         el = time.perf_counter() - t0
         left = b.poll()
         assert all(v == 0 for v in left.values())
+        # list rebuilds over all steps run (warm-up chunk included), weighted by system size
+        rebuilt = {3: 0, 2: 0}
+        for i, (sys_, st) in zip(ids, built):
+            rebuilt[len(sys_.box_len)] += b.list_stats(i).rebuilds * st.n()
         b.close()
-        return el, need
+        return el, need, rebuilt
 
     el_default = run_ensemble(False)[0] if default_tiles else None
-    el, need = run_ensemble(True)
+    el, need, rebuilt = run_ensemble(True)
+    atoms_by_dim = {d: sum(st.n() for sys_, st in built if len(sys_.box_len) == d) for d in (2, 3)}
     out = {
         "workload": f"{count} independent LJ systems (2D/3D 50/50, N log-uniform [256, 65536], rho U[0.3,0.9], T U[0.5,2.0], rc 2.5, skin 0.3, dt 0.005), meta-seed {meta_seed}, hipGraph chunks of {chunk} steps",
         "systems": count,
@@ -230,6 +235,7 @@ def batched_leg(count: int, steps: int, chunk: int, meta_seed: int, plain: bool,
         "systems_per_s": count * steps / chunk / el,
         "systems_per_s_unit": f"systems advanced by one {chunk}-step chunk per second",
         "arena_bytes_needed": need,
+        "rebuilt_fraction_per_step": {f"{d}d": rebuilt[d] / max(atoms_by_dim[d], 1) / (steps + chunk) for d in (2, 3)},
         "autotuned": {
             "table": {k: {"variant": v[0], "cell_factor": v[1], "block": v[2] if len(v) > 2 else 256} for k, v in sorted(tuner.table.items())},
             "tuning_seconds": tuner.seconds,
