@@ -232,13 +232,15 @@ def test_the_full_512_system_ensemble_matches_the_oracle():
     assert not bad, f"systems {bad[:10]} differ from the oracle"
 
 
-def test_a_list_that_overflows_regrows_and_stays_exact():
-    """A dense cluster (FCC at density 1.6) in a dilute box (mean density 0.15): the capacity
-    rule sizes the list for the box's density, so the first build overflows.  The system stops,
-    poll() regrows its list, step_all() runs the steps still left, and the trajectory equals
-    the oracle's, beside a system that never overflowed."""
+@pytest.mark.parametrize("variant,density", [(0, 1.6), (1, 2.2)])
+def test_a_list_that_overflows_regrows_and_stays_exact(variant, density):
+    """A dense cluster (FCC) in a dilute box (mean density 0.15): the capacity rule sizes the
+    list for the box's density, so the first build overflows (every pair at both ends at cluster
+    density 1.6: ~147 neighbours against a capacity of 120; the half list built over the rows at
+    2.2: ~160 entries).  The system stops, poll() regrows its list, step_all() runs the steps
+    still left, and the trajectory equals the oracle's, beside a system that never overflowed."""
     n = 2000
-    small = rh.cube_for(n, 1.6)
+    small = rh.cube_for(n, density)
     big = rh.cube_for(n, 0.15)
     pos = rh.fcc(n, small, rh.fcc_cells(n, small), 0.02, 5)
     sys = rh.System3.new(big, 0.002, rh.Pair.lj(2.5), [1.0])
@@ -246,7 +248,7 @@ def test_a_list_that_overflows_regrows_and_stays_exact():
     st = rh.State3(pos, rh.maxwell(sys, sp, 0.5, 5), sp)
     other = make_system(3, 3000, 0.8, 1.0, 6)
     b = BatchEngine(budget_bytes=2 * GB, max_systems=8, graph_chunk=0)
-    b.set_tuning(3, 10, 0, 1.0)  # every pair at both ends: ~147 neighbours against a capacity of 120
+    b.set_tuning(3, 10, variant, 1.0)
     i = b.add(sys, st, 60, skin=0.3)
     cap0 = b.list_stats(i).cap
     j = b.add(*other, steps=60)
