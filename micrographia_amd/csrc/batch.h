@@ -155,7 +155,7 @@ class Batch {
     void invalidate_graph();
 
     hipStream_t stream_ = nullptr;
-    hipStream_t aux_ = nullptr, aux2_ = nullptr;  // the 2D branch and the 3D stencil builds' branch
+    hipStream_t aux_ = nullptr;  // an iteration's side branch (small 3D systems' builds, 2D systems)
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_join2_ = nullptr;  // the branches of an iteration
     BBufs B_;
     BSys* d_sys_ = nullptr;

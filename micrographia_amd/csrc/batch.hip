@@ -1193,7 +1193,6 @@ Batch::Batch(uint64_t budget, uint32_t max_systems) : max_sys_(max_systems) {
     cells_.reset(cell_cap / CELL_UNIT);
     RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     RH_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
-    RH_HIP(hipStreamCreateWithFlags(&aux2_, hipStreamNonBlocking));
     RH_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     RH_HIP(hipEventCreateWithFlags(&ev_join2_, hipEventDisableTiming));
@@ -1254,7 +1253,6 @@ Batch::~Batch() {
     if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (ev_join2_) (void)hipEventDestroy(ev_join2_);
     if (aux_) (void)hipStreamDestroy(aux_);
-    if (aux2_) (void)hipStreamDestroy(aux2_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1631,20 +1629,26 @@ void Batch::enqueue_iteration() {
     k_bb_count<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
     k_bb_scan<<<max_sys_, BATCH_WG, 0, stream_>>>(B_, max_sys_);
     k_bb_scatter<<<ga, BATCH_WG, 0, stream_>>>(B_, rb_off_[RB_ALL]);
-    // From here on the dimensions go separate ways (they are different systems): the 2D systems'
-    // builds, commit and steps on one branch, the 3D systems' cell-stencil builds (those too small
-    // for rows) on another, and on the main stream the 3D row pipeline and builds, then the 3D
-    // commit and steps; the branches join before the step commit.  In a captured graph these are
-    // branches the GPU runs side by side: the stencil builds are latency-bound (long serial
-    // candidate loops in few waves) and the 3D step waits on its gathers, so the work of one
-    // branch fills the others' stalls.
+    // From here on the dimensions go separate ways (they are different systems): on a side branch
+    // the 3D systems' cell-stencil builds (those too small for rows), then the 2D systems' builds,
+    // commit and steps; on the main stream the 3D row pipeline and builds, then (once the small
+    // 3D systems' lists are built) the 3D commit and steps; the branches join before the step
+    // commit.  In a captured graph the GPU runs the two side by side: the stencil builds are
+    // latency-bound (long serial candidate loops in few waves) and the 3D step waits on its
+    // gathers, so the work of one branch fills the other's stalls.
     const bool d2 = rb_cap_[0] + rb_cap_[1] != 0u, d3s = rb_cap_[2] + rb_cap_[3] != 0u, d3r = rb_cap_[4] + rb_cap_[5] != 0u;
     const bool d3 = d3s || d3r;
-    const bool fork2 = d2 && d3, fork3 = d3s && d3r;
-    hipStream_t s2 = fork2 ? aux_ : stream_, s3 = fork3 ? aux2_ : stream_;
-    if (fork2 || fork3) RH_HIP(hipEventRecord(ev_fork_, stream_));
-    if (fork2) RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
-    if (fork3) RH_HIP(hipStreamWaitEvent(aux2_, ev_fork_, 0));
+    // one side branch (a second hardware queue, no more: a serving process runs other streams
+    // beside the batch): the small 3D systems' stencil builds first, then the 2D systems' work
+    const bool fork = (d2 && d3) || (d3s && d3r);
+    hipStream_t s2 = fork ? aux_ : stream_, s3 = d3s && d3r ? aux_ : stream_;
+    if (fork) {
+        RH_HIP(hipEventRecord(ev_fork_, stream_));
+        RH_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
+    }
+    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, s3>>>(B_, rb_off_[2]);
+    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, s3>>>(B_, rb_off_[3]);
+    if (s3 != stream_) RH_HIP(hipEventRecord(ev_join2_, aux_));  // the small 3D systems' lists are built
     if (d2) {
         if (rb_cap_[0]) k_bb_build<2, false><<<rebuild_grid(rb_cap_[0]), BATCH_WG, 0, s2>>>(B_, rb_off_[0]);
         if (rb_cap_[1]) k_bb_build<2, true><<<rebuild_grid(rb_cap_[1]), BATCH_WG, 0, s2>>>(B_, rb_off_[1]);
@@ -1653,8 +1657,6 @@ void Batch::enqueue_iteration() {
         launch_steps<512>(1, s2, 2);
         launch_steps<1024>(2, s2, 2);
     }
-    if (rb_cap_[2]) k_bb_build<3, false><<<rebuild_grid(rb_cap_[2]), BATCH_WG, 0, s3>>>(B_, rb_off_[2]);
-    if (rb_cap_[3]) k_bb_build<3, true><<<rebuild_grid(rb_cap_[3]), BATCH_WG, 0, s3>>>(B_, rb_off_[3]);
     // the row-ordered copies of the rebuilding rows systems, then their builds
     if (d3r) {
         for (uint32_t k = 4; k < 6; k++)
@@ -1665,17 +1667,14 @@ void Batch::enqueue_iteration() {
         if (rb_cap_[4]) k_bb_build_rows<false><<<rebuild_grid(rb_cap_[4]), BATCH_WG, 0, stream_>>>(B_, rb_off_[4]);
         if (rb_cap_[5]) k_bb_build_rows<true><<<rebuild_grid(rb_cap_[5]), BATCH_WG, 0, stream_>>>(B_, rb_off_[5]);
     }
-    if (fork3) {
-        RH_HIP(hipEventRecord(ev_join2_, aux2_));
-        RH_HIP(hipStreamWaitEvent(stream_, ev_join2_, 0));
-    }
+    if (s3 != stream_) RH_HIP(hipStreamWaitEvent(stream_, ev_join2_, 0));
     if (d3) {
         k_bb_commit_rebuild<<<sb, 256, 0, stream_>>>(d_sys_, d_ctl_, max_sys_, 3u);
         launch_steps<256>(0, stream_, 3);
         launch_steps<512>(1, stream_, 3);
         launch_steps<1024>(2, stream_, 3);
     }
-    if (fork2) {
+    if (fork) {
         RH_HIP(hipEventRecord(ev_join_, aux_));
         RH_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
     }
