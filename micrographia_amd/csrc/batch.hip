@@ -1,17 +1,23 @@
 // Batched engine (see batch.h).  One iteration, for every active system at once:
 //   k_bb_decide          per system: rebuild now? (stale list, steps left, no overflow)
-//   k_bb_count           per slot of a rebuilding system: linear cell, rank in the cell (atomic)
+//   k_bb_count           per slot of a rebuilding system: its cell (Morton order inside tiles),
+//                        rank in the cell (one atomic per run of equal cells in a wave)
 //   k_bb_scan            per system (one work-group): cell counts -> cell ranges, counts cleared
 //   k_bb_scatter         per slot: move to its cell-sorted slot in the other buffers
-//   k_bb_build           per slot (one launch per kind of system): in-block pairs at their
-//                        lower-index end, pairs across 256-slot blocks at both ends, as a tagged
-//                        chunk stream per 64-slot wave
-//   k_bb_commit_rebuild  per system
-//   k_bb_step            per block of a system (one launch per kind): its chunks dealt evenly
-//                        over the lanes, then per slot the leapfrog and half-skin check; systems
-//                        with a walled axis or the bath run the booking kind, which books the
-//                        ledger (and draws the 3D bath and heat-bath walls)
-//   k_bb_commit_step     per system
+//   then, per dimension on graph branches of their own (different systems):
+//   k_bb_row_count, k_bb_row_scan, k_bb_row_scatter
+//                        3D systems at least three cells wide: the row-ordered copy
+//   k_bb_build_rows      their half list over the rows (one launch per kind)
+//   k_bb_build           the others' lists over the cell stencil (2D, small 3D boxes, variant 2
+//                        and the full list): in-block pairs at their lower-index end, pairs
+//                        across blocks at both ends, as tagged chunks per 64-slot wave
+//   k_bb_commit_rebuild  per system of the branch's dimension
+//   k_bb_step            per block of a system (one launch per kind): its in-block then its
+//                        cross-block chunks dealt evenly over the lanes, then per slot the
+//                        leapfrog and half-skin check; systems with a walled axis or the bath
+//                        run the booking kind, which books the ledger (and draws the 3D bath and
+//                        heat-bath walls)
+//   k_bb_commit_step     per system, after the branches join
 // Every system takes exactly one step per iteration (its rebuild, when due, runs first in the
 // same iteration), so the host enqueues iterations without reading anything back and a chunk of
 // iterations can be captured once as a hipGraph and replayed.  Systems never interact, and each
