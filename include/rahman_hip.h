@@ -286,12 +286,14 @@ int rh_batch_set_graph_chunk(rh_batch* b, uint32_t chunk);
 /* Tile settings for systems added from now on, per dimension and size class (floor(log2 n)):
  * list variant (1 half list within work-groups, the default, built in 3D over a row-ordered
  * copy where the box allows; 2 the same half list built over the cell stencil; 0 every pair at
- * both ends), cell
- * occupancy (cells at least cell_factor x (rc + skin) wide, 1 <= cell_factor <= 4) and block
+ * both ends), cell occupancy (cells at least cell_factor x (rc + skin) wide,
+ * 1 <= cell_factor <= 4) and block
  * (slots per half-list block and step work-group: 256, the default, 512 or 1024; the system's
  * slots are whole blocks).  Results are identical for every setting; micrographia_amd.autotune
  * picks them by timing. */
 int rh_batch_set_tuning(rh_batch* b, uint32_t dim, uint32_t size_class, uint32_t variant, double cell_factor, uint32_t block);
+/* The batch's stream: every iteration's work (its side branch included) is joined back into it,
+ * so work queued on it after rh_batch_run follows the whole run. */
 int rh_batch_stream(rh_batch* b, void** stream);
 int rh_batch_capacity(rh_batch* b, uint64_t* slots, uint64_t* list_entries, uint64_t* cells, uint32_t* active);
 int rh_batch_bytes_needed3(rh_batch* b, const rh_system3* sys, uint64_t n, double skin, uint64_t* bytes);
