@@ -44,6 +44,7 @@ struct BSys {
     int64_t box[3], half[3];
     double boxf[3], inv_box[3];  // positions are exact f64 integers (2^-32 sigma units)
     double box16[3], inv_box16[3];  // box / 16 and 16 / box: the image on positions in sixteenths
+    double face16;                  // a block this far (sixteenths) from every periodic face needs no image
     double cell_scale[3], cell_w64[3];
     float cell_w[3];
     float kick_c[4];

@@ -892,15 +892,16 @@ __device__ __forceinline__ void bb_step(BBufs& B, const BSys& S, BCtl& C, uint32
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a] + S.slot0;
     if (t == 0u) edge = S.range_check;
     // Periodic axes on which the block needs no image: with every particle of the block at least
-    // 8 sigma (the bound on list separations) from both faces, every partner's stored position is
-    // its nearest image and separation16's correction is exactly zero.
+    // face16 from both faces (the bound on a listed partner's distance at step time: the largest
+    // list radius plus the two particles' moves since the build), every partner's stored position
+    // is its nearest image and separation16's correction is exactly zero.
     uint32_t near_face = 0u;
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const double x16 = live ? px[a][local] * 0.0625 : 0.0;
         acc[a][t] = 0u;
         lpos[a][t] = x16;
-        if (live && ((S.periodic >> a) & 1u) && !(x16 >= 0x1p31 && x16 <= S.box16[a] - 0x1p31)) near_face = 1u;
+        if (live && ((S.periodic >> a) & 1u) && !(x16 >= S.face16 && x16 <= S.box16[a] - S.face16)) near_face = 1u;
     }
     if (t < 64u) {
 #pragma unroll
@@ -1388,6 +1389,7 @@ void Batch::plan(const SystemDesc& sys, uint64_t n, double skin, BSys& d, uint32
         }
     double hs = 0.5 * skin * (1.0 - 1e-3);
     d.half_skin2 = (float)(hs * hs);
+    d.face16 = face_margin16(d.pt, d.half_skin2);
     d.range_check = ((rcm + skin) * (1.0 + 1e-4) + skin >= 7.99) ? 1u : 0u;
     // The row-ordered copy for the 3D half list (variant 1), as the single-system engine plans it:
     // fine x bins a power-of-two fraction of the cell wide, each row padded with ghost bins that

@@ -56,6 +56,7 @@ struct Params {
     double inv_box[3];     // 1 / box on periodic axes, 0 on walled ones
     double box16[3];       // box / 16 and 16 / box: the same image on positions held in sixteenths
     double inv_box16[3];
+    double face16;         // a block this far (sixteenths) from every periodic face needs no image
     double cell_scale[3];  // cells per position unit
     float kick_c[4];
     int32_t gravity[3];

@@ -861,7 +861,8 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
     const double* px[D];
     for (int a = 0; a < D; a++) px[a] = B.pos[pp][a];
     // Periodic axes on which the block needs no image: with every particle of the block at
-    // least 8 sigma (the bound on list separations) from both faces, every partner's stored
+    // least face16 from both faces (the bound on a listed partner's distance at step time: the
+    // largest list radius plus the two particles' moves since the build), every partner's stored
     // position is its nearest image, and separation16's correction is exactly zero.
     __shared__ uint32_t edge;
     if (t == 0u) edge = 0u;
@@ -876,7 +877,7 @@ __global__ __launch_bounds__(BS > 1024 ? 1024 : BS, BS == 2048 && !BOOK ? 8 : 1)
             const double x16 = real ? px[a][base + l] * 0.0625 : 0.0;
             lpos[a][l] = x16;
             acc[a][l] = 0u;
-            if (real && ((P.periodic >> a) & 1u) && !(x16 >= 0x1p31 && x16 <= P.box16[a] - 0x1p31)) near_face = 1u;
+            if (real && ((P.periodic >> a) & 1u) && !(x16 >= P.face16 && x16 <= P.box16[a] - P.face16)) near_face = 1u;
         }
     }
     if (ballot64(near_face != 0u) != 0ull && __lane_id() == 0u) atomicOr(&edge, 1u);
@@ -1824,6 +1825,7 @@ void Engine<D>::build_params(const SystemDesc& sys, const rh_mode& mode) {
             P.pt.shift_64[k] = core::pair_shift(p);
         }
     for (uint32_t s = 0; s < 4; s++) P.pt.kick_c[s] = P.kick_c[s];
+    P.face16 = face_margin16(P.pt, P.half_skin2);
     P.cap = mode.kind == 1 ? mode.cap : 0;
     RH_REQUIRE(P.cap < 65536, "list capacity must be below 65536");
     for (int a = 0; a < D; a++) {
@@ -2583,6 +2585,7 @@ void Engine<D>::update_system(const SystemDesc& sys) {
         grew |= P_.pt.list_r2[k] > old.pt.list_r2[k];
         P_.pt.list_r2[k] = std::max(P_.pt.list_r2[k], old.pt.list_r2[k]);
     }
+    P_.face16 = face_margin16(P_.pt, P_.half_skin2);  // the kept list radii and skin
     if (grew) write_ctl(CTL_STALE, 1u);
     P_.arith = old.arith;
     sys_ = sys;

@@ -5,6 +5,8 @@
 // soft_walls).
 #pragma once
 
+#include <cmath>
+
 #include "prims.h"
 #include "rh_common.h"
 
@@ -561,6 +563,17 @@ __device__ __forceinline__ uint32_t shift_in(uint32_t q, uint64_t m) {
     uint64_t carry_out;
     asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(q), "s"(m));
     return r;
+}
+
+// The distance from both faces of a periodic axis, in sixteenths, beyond which a particle's
+// listed partners need no periodic image: a list separation at step time is below the largest
+// list radius plus the two particles' moves since the build (each under half the skin, or the
+// list is rebuilt first), and 0.05 sigma covers the f32 tests.  (Host side.)
+template <class PTab>
+inline double face_margin16(const PTab& pt, float half_skin2) {
+    float lr2 = 0.f;
+    for (int k = 0; k < 16; k++) lr2 = lr2 > pt.list_r2[k] ? lr2 : pt.list_r2[k];
+    return (std::sqrt((double)lr2) + 2.0 * std::sqrt((double)half_skin2) + 0.05) * 0x1p28;
 }
 
 // Entry k & 3 of a four-entry list chunk, without dynamic register indexing.
